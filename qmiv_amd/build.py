@@ -1,0 +1,62 @@
+"""Builds the native libraries of qmiv_amd in-tree.
+
+  qmiv_amd/libivqm_hip.so    gfx950 kernels + C ABI (hipcc)
+  qmiv_amd/libivqm_synth.so  seeded synthetic frame generator (g++)
+
+Run ``python -m qmiv_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_runtime.hip", "ivqm_batch.hip", "ivqm_serve.hip"]
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 VALU forms stay off (hipcc -O3 would otherwise emit v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_hip(verbose_isa: bool = False) -> str:
+    out = os.path.join(HERE, "libivqm_hip.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-I" + os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_synth() -> str:
+    out = os.path.join(HERE, "libivqm_synth.so")
+    _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-fPIC", "-shared", os.path.join(CSRC, "synth.cpp"), "-o", out])
+    return out
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"])
+
+
+def build(oracle: bool = True) -> None:
+    build_synth()
+    build_hip()
+    if oracle:
+        build_oracle()
+
+
+if __name__ == "__main__":
+    build(oracle="--no-oracle" not in sys.argv)

@@ -1,0 +1,184 @@
+"""Python handle over one C-ABI context (tests, bench and smoke use it).
+
+Pictures are given as three numpy planes (host, uploaded) or three torch CUDA
+tensors (device-resident, bound without a copy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, FrameResult, Picture, check
+
+
+def default_config(width, height, bit_depth=10, **kw) -> Config:
+    L = _lib.load()
+    cfg = Config()
+    check(L.ivqm_config_init(C.byref(cfg)))
+    cfg.width, cfg.height, cfg.bit_depth = width, height, bit_depth
+    for k, v in kw.items():
+        if k in ("cmp_weights_search", "cmp_weights_average"):
+            getattr(cfg, k)[:] = list(v)
+        elif k == "unnoticeable_coef":
+            cfg.unnoticeable_coef[:] = [float(x) for x in v]
+        else:
+            setattr(cfg, k, int(v))
+    return cfg
+
+
+def _picture_from_numpy(planes, chroma_420: bool):
+    arrs = [np.ascontiguousarray(p) for p in planes]
+    bps = arrs[0].dtype.itemsize
+    pic = Picture()
+    for k in range(3):
+        pic.plane[k] = arrs[k].ctypes.data
+        pic.stride[k] = arrs[k].shape[1]
+    pic.chroma_420 = int(chroma_420)
+    pic.bytes_per_sample = bps
+    return pic, arrs
+
+
+def _picture_from_torch(planes, chroma_420: bool):
+    pic = Picture()
+    for k in range(3):
+        t = planes[k]
+        assert t.is_cuda and t.dim() == 2 and t.stride(1) == 1
+        pic.plane[k] = t.data_ptr()
+        pic.stride[k] = t.stride(0)
+    pic.chroma_420 = int(chroma_420)
+    pic.bytes_per_sample = planes[0].element_size()
+    return pic
+
+
+class IvqmContext:
+    def __init__(self, width, height, bit_depth=10, **kw):
+        self.L = _lib.load()
+        self.cfg = default_config(width, height, bit_depth, **kw)
+        h = C.c_void_p()
+        check(self.L.ivqm_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.width, self.height, self.bit_depth = width, height, bit_depth
+        self._keep = [None, None]
+
+    def close(self):
+        if self.h:
+            self.L.ivqm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- pictures ------------------------------------------------------------
+    def upload(self, slot, planes, chroma_420=True):
+        pic, arrs = _picture_from_numpy(planes, chroma_420)
+        check(self.L.ivqm_upload(self.h, slot, C.byref(pic)))
+
+    def bind(self, slot, planes, chroma_420=True):
+        pic = _picture_from_torch(planes, chroma_420)
+        self._keep[slot] = planes
+        check(self.L.ivqm_bind(self.h, slot, C.byref(pic)))
+
+    def upload_pair(self, p0, p1, chroma_420=True):
+        self.upload(0, p0, chroma_420)
+        self.upload(1, p1, chroma_420)
+
+    # -- per-metric calls ---------------------------------------------------------
+    def gcd(self):
+        g = (C.c_int32 * 4)()
+        check(self.L.ivqm_gcd(self.h, g))
+        return list(g)
+
+    def ssd(self):
+        s = (C.c_uint64 * 4)()
+        check(self.L.ivqm_ssd(self.h, s))
+        return list(s)
+
+    def _v4(self, fn):
+        v = (C.c_double * 4)()
+        check(getattr(self.L, fn)(self.h, v))
+        return list(v)
+
+    def mse(self):
+        return self._v4("ivqm_mse")
+
+    def psnr(self):
+        return self._v4("ivqm_psnr")
+
+    def wspsnr(self):
+        return self._v4("ivqm_wspsnr")
+
+    def ssim(self):
+        return self._v4("ivqm_ssim")
+
+    def msssim(self):
+        return self._v4("ivqm_msssim")
+
+    def ivpsnr(self, gcd=None):
+        v, a, b = C.c_double(), C.c_double(), C.c_double()
+        g = (C.c_int32 * 4)(*gcd) if gcd is not None else None
+        check(self.L.ivqm_ivpsnr(self.h, g, C.byref(v), C.byref(a), C.byref(b)))
+        return v.value, a.value, b.value
+
+    def gen_scp(self, gcd=None):
+        g = (C.c_int32 * 4)(*gcd) if gcd is not None else None
+        check(self.L.ivqm_gen_scp(self.h, g))
+
+    def ivssim(self):
+        v, a, b = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.ivqm_ivssim(self.h, C.byref(v), C.byref(a), C.byref(b)))
+        return v.value, a.value, b.value
+
+    def ivmsssim(self):
+        v, a, b = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.ivqm_ivmsssim(self.h, C.byref(v), C.byref(a), C.byref(b)))
+        return v.value, a.value, b.value
+
+    def frame(self, metrics=_lib.IVQM_M_ALL) -> FrameResult:
+        r = FrameResult()
+        check(self.L.ivqm_frame(self.h, metrics, C.byref(r)))
+        return r
+
+    def enqueue(self, metrics, ring):
+        check(self.L.ivqm_frame_enqueue(self.h, metrics, ring))
+
+    def collect(self, ring) -> FrameResult:
+        r = FrameResult()
+        check(self.L.ivqm_frame_collect(self.h, ring, C.byref(r)))
+        return r
+
+    def profile(self, kernel, iters=20):
+        """Mean duration (ms) of one kernel family over `iters` launches (HIP events, context stream)."""
+        ms = C.c_double()
+        check(self.L.ivqm_profile(self.h, kernel, iters, C.byref(ms)))
+        return ms.value
+
+    # -- debug ------------------------------------------------------------------
+    def rows(self, direction):
+        out = np.zeros((self.height, 4), np.uint64)
+        check(self.L.ivqm_get_rows(self.h, direction, out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def scp(self, idx):
+        planes = [np.zeros((self.height, self.width), np.uint16) for _ in range(3)]
+        ptrs = (C.POINTER(C.c_uint16) * 3)(*[p.ctypes.data_as(C.POINTER(C.c_uint16)) for p in planes])
+        check(self.L.ivqm_get_scp(self.h, idx, ptrs))
+        return planes
+
+    def enable_best_map(self, on=True):
+        check(self.L.ivqm_enable_best_map(self.h, int(on)))
+
+    def best_map(self, direction):
+        out = np.zeros((self.height, self.width), np.uint8)
+        check(self.L.ivqm_get_best_map(self.h, direction, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out

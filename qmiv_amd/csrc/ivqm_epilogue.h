@@ -1,0 +1,93 @@
+// Host FP64 epilogue of the metric hot path.
+//
+// The device produces exact integers (row distances, row SSDs, SD totals) and
+// FP64 SSIM totals; the floating-point steps that decide the printed dB
+// digits run here, on the host, in the reference's operation order, with the
+// host libm (log10, cos, pow), without contraction, so PSNR, WS-PSNR and
+// IV-PSNR match the reference bit for bit.
+#pragma once
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace ivqm {
+namespace host {
+
+// Kahan-Babuska-Neumaier: libs/pmbb-core/src/xKBNS.h:36-52 (and the SSE / AVX
+// twins in xKBNS.cpp:14-26, :85-96, which compute the same values).
+struct Kbns {
+  double s = 0.0, c = 0.0;
+  void acc(double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += ((s - t) + v);
+    else c += ((v - t) + s);
+    s = t;
+  }
+  double result() const { return s + c; }
+};
+
+// libs/pmbb-core/src/xHelpersFLT.h:22 (value + copysign(0.5f, value), truncate).
+inline int32_t round_f32(float v) {
+  const float a = v + copysignf(0.5f, v);
+  return (int32_t)a;
+}
+
+// libs/pmbb-ivqm/src/xPSNR.cpp:186-192.
+inline double psnr_from_ssd(double ssd, int64_t area, int bit_depth) {
+  const uint64_t maxv = (uint64_t)((1 << bit_depth) - 1);
+  const uint64_t MAX = (uint64_t)area * (maxv * maxv);
+  return ssd > 0 ? 10.0 * log10((double)MAX / ssd) : DBL_MAX;
+}
+
+// libs/pmbb-ivqm/src/xWeightedSpherically.cpp:13-29.
+inline void ws_weights(int h, int lat_range_deg, std::vector<double>& w, double& correction) {
+  w.resize(h);
+  const double eh = 180.0 * (double)h / (double)lat_range_deg;
+  const double eo = (eh - h) / 2.0;
+  const double pi = 3.14159265358979323846;
+  for (int y = 0; y < h; y++) w[y] = cos((y + eo - (eh / 2 - 0.5)) * pi / eh);
+  Kbns k;
+  for (int y = 0; y < h; y++) k.acc(w[y]);
+  correction = (double)h / k.result();
+}
+
+// (v * w).getSum() * (1 / sum(w)): libs/pmbb-ivqm/src/xIVPSNR.cpp:120-123.
+inline double weighted_mean4(const double v[4], const int32_t cw[4]) {
+  const double inv = 1.0 / (double)(cw[0] + cw[1] + cw[2] + cw[3]);
+  return (v[0] * (double)cw[0] + v[1] * (double)cw[1] + v[2] * (double)cw[2] + v[3] * (double)cw[3]) * inv;
+}
+// Picture value of a per-component metric: apps/QMIV/src/xMetricQMIV.cpp:91-97.
+inline double pic_value(const double v[4], const int32_t cw[4]) {
+  const double inv = 1.0 / (double)(cw[0] + cw[1] + cw[2] + cw[3]);
+  return (v[0] * cw[0] + v[1] * cw[1] + v[2] * cw[2]) * inv;
+}
+
+// Asymmetric IV-PSNR quality from per-row distances (device uint32 rows,
+// [3][h]; the reference's int32 row sums are sign-extended to uint64,
+// libs/pmbb-ivqm/src/xCorrespPixelShiftSSE.cpp:38-40):
+// libs/pmbb-ivqm/src/xIVPSNR.cpp:99-125.
+inline double ivpsnr_quality(const uint32_t* rows, int w, int h, int bit_depth, const double* erp_w,
+                             const int32_t cw_avg[4]) {
+  double err[4] = {0, 0, 0, 0};
+  if (erp_w) {
+    Kbns k[3];
+    for (int y = 0; y < h; y++)
+      for (int c = 0; c < 3; c++) k[c].acc((double)(uint64_t)(int64_t)(int32_t)rows[(size_t)c * h + y] * erp_w[y]);
+    for (int c = 0; c < 3; c++) err[c] = k[c].result();
+  } else {
+    uint64_t s[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++)
+      for (int y = 0; y < h; y++) s[c] += (uint64_t)(int64_t)(int32_t)rows[(size_t)c * h + y];
+    for (int c = 0; c < 3; c++) err[c] = (double)s[c];
+  }
+  double q[4] = {0, 0, 0, 0};
+  for (int c = 0; c < 3; c++) q[c] = psnr_from_ssd(err[c] > 0 ? err[c] : 1.0, (int64_t)w * h, bit_depth);
+  return weighted_mean4(q, cw_avg);
+}
+
+}  // namespace host
+}  // namespace ivqm

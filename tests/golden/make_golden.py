@@ -1,0 +1,123 @@
+"""Writes tests/golden/golden_vectors.json from the reference library itself.
+
+The reference pmbb libraries are compiled from their own sources by
+oracle/Makefile into oracle/_ref/ (this script needs them; run
+`make -C oracle` first). Inputs are regenerated from the seeded generator in
+qmiv_amd/csrc/synth.cpp, so the fixture stores only the generator spec, input
+checksums (plus the first and last 16 samples of every plane, to detect
+generator drift) and the reference's outputs. Large integer arrays are stored
+as SHA-1 digests; floating-point results as exact repr strings.
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.oracle import Oracle, Reference  # noqa: E402
+from qmiv_amd import synth  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_vectors.json")
+
+# (name, workload, frame, R, erp) -- small shapes, all bit depths, both chroma
+# formats, an ERP case large enough for 5-scale MS-SSIM, and an adversarial
+# 16-bit case whose squared differences wrap int32.
+CASES = [
+    ("w64h48_10b_420", synth.Workload(cfg=100, width=64, height=48), 0, 2, False),
+    ("w96h64_8b_420", synth.Workload(cfg=101, width=96, height=64, bit_depth=8), 1, 2, False),
+    ("w128h96_14b_420", synth.Workload(cfg=102, width=128, height=96, bit_depth=14, sigma_r=40, sigma_t=30), 0, 2, False),
+    ("w80h72_16b_444_r4", synth.Workload(cfg=103, width=80, height=72, bit_depth=16, chroma_420=False, sigma_r=12,
+                                         sigma_t=12, block_shift=3, offset_rand=8), 0, 4, False),
+    ("w72h64_10b_444_r1", synth.Workload(cfg=104, width=72, height=64, chroma_420=False), 2, 1, False),
+    ("w64h64_16b_444_wrap", synth.Workload(cfg=105, width=64, height=64, bit_depth=16, chroma_420=False,
+                                           sigma_r=30000, sigma_t=30000, block_shift=2, offset_rand=600), 0, 2, False),
+    ("w96h64_10b_420_r3", synth.Workload(cfg=106, width=96, height=64, block_shift=3, sigma_t=5), 0, 3, True),
+    ("w512h512_10b_420_erp", synth.Workload(cfg=107, width=512, height=512, global_shift=2), 0, 2, True),
+    ("w1920h1080_10b_420", synth.CPU_1080P, 0, 2, False),
+]
+
+
+def sha1(a) -> str:
+    return hashlib.sha1(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def f(x) -> str:
+    return repr(float(x))
+
+
+def make_case(name, wl, frame, R, erp, ref: Reference, orc: Oracle):
+    p0n, p1n = synth.make_pair(wl, frame)
+    p0 = orc.to444(p0n, wl.chroma_420)
+    p1 = orc.to444(p1n, wl.chroma_420)
+    bd = wl.bit_depth
+    case = {
+        "name": name, "workload": dataclasses.asdict(wl), "frame": frame, "R": R, "erp": erp,
+        "inputs": {
+            "p0": [sha1(p) for p in p0n], "p1": [sha1(p) for p in p1n],
+            "p0_head": [p.ravel()[:16].tolist() for p in p0n], "p0_tail": [p.ravel()[-16:].tolist() for p in p0n],
+            "p1_head": [p.ravel()[:16].tolist() for p in p1n], "p1_tail": [p.ravel()[-16:].tolist() for p in p1n],
+        },
+    }
+    up = [ref.upsample(p1n[1])] if wl.chroma_420 else []
+    case["upsample_cb1_sha1"] = sha1(up[0]) if up else None
+    case["downsample_y0_sha1"] = sha1(ref.downsample(p0[0]))
+    g = ref.gcd(p0, p1, bd, R=R)
+    case["gcd_r2t"] = g
+    case["sd"] = [int(ref.sd(p1[c], p0[c], bd)) for c in range(3)]
+    case["ssd"] = [int(ref.ssd(p0[c], p1[c], bd)) for c in range(3)]
+    rows_r2t = ref.cps_rows(p0, p1, bd, g, R=R)
+    rows_t2r = ref.cps_rows(p1, p0, bd, [-x for x in g], R=R)
+    h = wl.height
+    if h <= 96:
+        case["rows_r2t"] = rows_r2t.astype(np.uint64).tolist()
+        case["rows_t2r"] = rows_t2r.astype(np.uint64).tolist()
+    case["rows_r2t_sha1"] = sha1(rows_r2t.astype(np.uint64))
+    case["rows_t2r_sha1"] = sha1(rows_t2r.astype(np.uint64))
+    case["rows_r2t_total"] = [int(x) for x in rows_r2t.sum(axis=0, dtype=np.uint64)]
+    case["rows_t2r_total"] = [int(x) for x in rows_t2r.sum(axis=0, dtype=np.uint64)]
+    s0, s1 = ref.scp(p0, p1, bd, g, R=R)
+    case["scp0_sha1"] = [sha1(p) for p in s0]
+    case["scp1_sha1"] = [sha1(p) for p in s1]
+    case["psnr"] = [f(x) for x in ref.psnr(p0, p1, bd)]
+    case["wspsnr_erp"] = [f(x) for x in ref.wspsnr(p0, p1, bd, erp=True)]
+    v, a, b = ref.ivpsnr(p0, p1, bd, g, R=R, erp=False)
+    case["ivpsnr"] = [f(v), f(a), f(b)]
+    v, a, b = ref.ivpsnr(p0, p1, bd, g, R=R, erp=True)
+    case["ivpsnr_erp"] = [f(v), f(a), f(b)]
+    if bd <= 14:  # the reference's SIMD SSIM kernels use signed 16-bit madd (valid up to 15-bit samples)
+        case["ssim"] = [f(x) for x in ref.ssim(p0, p1, bd)]
+        case["msssim"] = [f(x) for x in ref.ssim(p0, p1, bd, ms=True)]
+        case["ivssim"] = f(ref.ivssim(p0, p1, s0, s1, bd, erp=False))
+        case["ivssim_erp"] = f(ref.ivssim(p0, p1, s0, s1, bd, erp=True))
+        case["ivmsssim"] = f(ref.ivssim(p0, p1, s0, s1, bd, ms=True))
+    return case
+
+
+def main():
+    ref = Reference()
+    orc = Oracle()
+    cases = []
+    for c in CASES:
+        print("case", c[0], flush=True)
+        cases.append(make_case(*c, ref=ref, orc=orc))
+    doc = {
+        "generator": "qmiv_amd/csrc/synth.cpp",
+        "reference_build": f"oracle/Makefile (x86-64-{ref.level}) over /root/reference/libs",
+        "cases": cases,
+    }
+    with open(OUT, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,83 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol the
+headers declare, and reports errors instead of crashing when misused."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(ivqm_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_abi():
+    names = declared_symbols()
+    assert "ivqm_create" in names and "ivqm_frame" in names
+    from qmiv_amd._lib import EXPORTED
+    missing = [n for n in names if n not in EXPORTED and not n.startswith(("ivqm_batch", "ivqm_server"))]
+    assert not missing, f"bindings do not list: {missing}"
+
+
+def test_library_exports_every_declared_symbol():
+    from qmiv_amd._lib import LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libivqm_hip.so not built (run python -m qmiv_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    for n in declared_symbols():
+        assert hasattr(L, n), n
+
+
+def test_config_defaults_match_reference():
+    """Defaults of the reference's parameter classes (xCorrespPixelShiftPrms.h:20-21,
+    xGlobClrDiff.h:20, xSSIM.h:35-37, xWeightedSpherically.h:21)."""
+    from qmiv_amd._lib import LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libivqm_hip.so not built")
+    from qmiv_amd.context import default_config
+    cfg = default_config(64, 64)
+    assert cfg.search_range == 2
+    assert list(cfg.cmp_weights_search) == [4, 1, 1, 0]
+    assert list(cfg.cmp_weights_average) == [4, 1, 1, 0]
+    assert [round(x, 6) for x in cfg.unnoticeable_coef] == [0.01, 0.01, 0.01, 0.0]
+    assert (cfg.ssim_window, cfg.ssim_stride) == (8, 4)
+    assert (cfg.lon_range_deg, cfg.lat_range_deg) == (360, 180)
+
+
+def test_invalid_arguments_return_status_not_crash():
+    from qmiv_amd._lib import LIB_PATH, Config, load
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libivqm_hip.so not built")
+    L = load()
+    assert L.ivqm_config_init(None) == -22
+    assert L.ivqm_create(None, None) == -22
+    cfg = Config()
+    L.ivqm_config_init(C.byref(cfg))
+    cfg.width, cfg.height, cfg.bit_depth = 64, 64, 20
+    h = C.c_void_p()
+    assert L.ivqm_create(C.byref(cfg), C.byref(h)) == -22
+    assert L.ivqm_strerror(-22) == b"invalid argument"
+    assert L.ivqm_frame(None, 0, None) == -22
+    # no GPU in this container: creating a valid context reports a HIP error, it does not abort
+    cfg.bit_depth = 10
+    rc = L.ivqm_create(C.byref(cfg), C.byref(h))
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        assert rc == 0
+        L.ivqm_destroy(h)
+    else:
+        assert rc <= -1000
+        assert len(L.ivqm_last_error()) > 0

@@ -1,0 +1,332 @@
+"""Pins the CPU oracle (oracle/ivqm_oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked by at least one of:
+  * golden vectors written by the reference library itself
+    (tests/golden/make_golden.py over oracle/_ref) -- GCD, SD/SSD, search rows,
+    SCP pictures, PSNR, WS-PSNR, IV-PSNR (bit-exact), SSIM, MS-SSIM, IV-SSIM,
+    IV-MS-SSIM (FP64, within 1e-12), upsampling, downsampling;
+  * the reference's own known-answer tests, restated (GCD on constant planes,
+    test-xGlobClrDiff.cpp:45-87; SD/SSD on constant planes,
+    test-xDistortion.cpp:28-99; rounding vs std::round,
+    test-xCommon.cpp:62-113; up/down round trip, test-xPixelOps.cpp:160-194;
+    SSIM window counts vs brute force, test-xSSIM.cpp:29-77; SCP invariant,
+    test-xCorrespPixelShift.cpp:155-179);
+  * independent numpy brute force on tiny inputs (search argmin, window SSIM,
+    KBNS against exact rational sums).
+Planted mistakes (a flipped tie-break, a dropped wrap, a transposed SSIM
+operand) are shown to fail these checks in test_oracle_detects_planted_bugs.
+"""
+import hashlib
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import golden_cases
+from qmiv_amd import synth
+
+CASES = golden_cases()
+
+
+def sha1(a):
+    return hashlib.sha1(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def wl_of(case):
+    w = dict(case["workload"])
+    w["offsets"] = tuple(w["offsets"])
+    return synth.Workload(**w)
+
+
+def inputs(case, oracle):
+    wl = wl_of(case)
+    p0n, p1n = synth.make_pair(wl, case["frame"])
+    return wl, p0n, p1n, oracle.to444(p0n, wl.chroma_420), oracle.to444(p1n, wl.chroma_420)
+
+
+# ---------------------------------------------------------------------------
+# generator drift
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_generator_is_frozen(case, oracle):
+    wl, p0n, p1n, _, _ = inputs(case, oracle)
+    assert [sha1(p) for p in p0n] == case["inputs"]["p0"]
+    assert [sha1(p) for p in p1n] == case["inputs"]["p1"]
+    assert [p.ravel()[:16].tolist() for p in p0n] == case["inputs"]["p0_head"]
+    assert [p.ravel()[-16:].tolist() for p in p1n] == case["inputs"]["p1_tail"]
+
+
+# ---------------------------------------------------------------------------
+# oracle vs golden vectors of the reference library
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_oracle_matches_reference_golden(case, oracle):
+    wl, p0n, p1n, p0, p1 = inputs(case, oracle)
+    bd, R = wl.bit_depth, case["R"]
+    if wl.chroma_420:
+        assert sha1(oracle.upsample(p1n[1])) == case["upsample_cb1_sha1"]
+    if bd <= 14:
+        # the reference's SIMD DownsampleHV adds in 16-bit lanes (xPixelOpsAVX512.cpp:173-200), exact up
+        # to 14-bit samples; above that the oracle follows the STD definition (xPixelOpsSTD.cpp:50-67),
+        # pinned by the closed form in test_upsample_downsample_round_trip
+        assert sha1(oracle.downsample(p0[0])) == case["downsample_y0_sha1"]
+    g = oracle.gcd(p0, p1, bd)
+    assert g == case["gcd_r2t"]
+    assert [oracle.sd(p1[c], p0[c], wrap32=bd <= 14) for c in range(3)] == case["sd"] or \
+        [oracle.sd(p1[c], p0[c], wrap32=False) for c in range(3)] == case["sd"]
+    assert [oracle.ssd(p0[c], p1[c]) for c in range(3)] == case["ssd"]
+    rows_a, scp1, _ = oracle.cps(p0, p1, bd, g, R=R, want_scp=True)
+    rows_b, scp0, _ = oracle.cps(p1, p0, bd, [-x for x in g], R=R, want_scp=True)
+    assert sha1(rows_a.astype(np.uint64)) == case["rows_r2t_sha1"]
+    assert sha1(rows_b.astype(np.uint64)) == case["rows_t2r_sha1"]
+    if "rows_r2t" in case:
+        assert rows_a.tolist() == case["rows_r2t"]
+    assert [sha1(p) for p in scp0] == case["scp0_sha1"]
+    assert [sha1(p) for p in scp1] == case["scp1_sha1"]
+    assert [repr(x) for x in oracle.psnr(p0, p1, bd)] == case["psnr"]
+    assert [repr(x) for x in oracle.wspsnr(p0, p1, bd, erp=True)] == case["wspsnr_erp"]
+    v = oracle.ivpsnr(p0, p1, bd, g, R=R, erp=False)
+    assert [repr(x) for x in v] == case["ivpsnr"]
+    v = oracle.ivpsnr(p0, p1, bd, g, R=R, erp=True)
+    assert [repr(x) for x in v] == case["ivpsnr_erp"]
+    if "ssim" in case:
+        tol = 1e-12
+        np.testing.assert_allclose(oracle.ssim(p0, p1, bd), [float(x) for x in case["ssim"]], rtol=0, atol=tol)
+        np.testing.assert_allclose(oracle.msssim(p0, p1, bd), [float(x) for x in case["msssim"]], rtol=0, atol=tol)
+        assert abs(oracle.ivssim(p0, p1, scp0, scp1, bd)[0] - float(case["ivssim"])) < tol
+        assert abs(oracle.ivssim(p0, p1, scp0, scp1, bd, erp=True)[0] - float(case["ivssim_erp"])) < tol
+        assert abs(oracle.ivmsssim(p0, p1, scp0, scp1, bd) - float(case["ivmsssim"])) < tol
+
+
+# ---------------------------------------------------------------------------
+# known-answer tests restated from the reference's own test suite
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("bd", [8, 10, 12, 14, 16])
+def test_gcd_constant_planes(oracle, bd):
+    """test-xGlobClrDiff.cpp:45-87 (coef 1.0): 0, +-1, +-Max, (4,3,2)."""
+    h, w = 32, 48
+    coef = (1.0, 1.0, 1.0, 1.0)
+    z = [np.zeros((h, w), np.uint16)] * 3
+    one = [np.ones((h, w), np.uint16)] * 3
+    mx = [np.full((h, w), (1 << bd) - 1, np.uint16)] * 3
+    # oracle.gcd(p0, p1) = round(mean(p1 - p0))
+    assert oracle.gcd(z, z, bd, coef) == [0, 0, 0, 0]
+    assert oracle.gcd(z, one, bd, coef) == [1, 1, 1, 0]
+    assert oracle.gcd(one, z, bd, coef) == [-1, -1, -1, 0]
+    M = (1 << bd) - 1
+    assert oracle.gcd(z, mx, bd, coef) == [M, M, M, 0]
+    assert oracle.gcd(mx, z, bd, coef) == [-M, -M, -M, 0]
+    rng = np.random.default_rng(bd)
+    base = [rng.integers(8, (1 << bd) - 8, (h, w)).astype(np.uint16) for _ in range(3)]
+    mod = [base[0] + 4, base[1] + 3, base[2] + 2]
+    assert oracle.gcd(base, mod, bd, coef) == [4, 3, 2, 0]
+    assert oracle.gcd(mod, base, bd, coef) == [-4, -3, -2, 0]
+    # default coefficient clips to round(0.01 * Max)
+    lim = oracle.round_f32(np.float32(0.01) * np.float32(M))
+    assert oracle.gcd(z, mx, bd) == [lim, lim, lim, 0]
+
+
+@pytest.mark.parametrize("bd", [8, 10, 14, 16])
+def test_sd_ssd_constant_planes(oracle, bd):
+    """test-xDistortion.cpp:28-99: SD/SSD of constant planes are Area multiples."""
+    h, w = 17, 33
+    area = h * w
+    M = (1 << bd) - 1
+    z = np.zeros((h, w), np.uint16)
+    assert oracle.sd(np.full((h, w), 1, np.uint16), z) == area
+    assert oracle.sd(np.full((h, w), 2, np.uint16), z) == 2 * area
+    assert oracle.sd(z, np.full((h, w), M, np.uint16)) == -M * area
+    assert oracle.ssd(np.full((h, w), 2, np.uint16), z) == 4 * area
+    assert oracle.ssd(np.full((h, w), M, np.uint16), z) == M * M * area
+
+
+def test_sd_wrap32_matches_x86_64_v4():
+    """The x86-64-v4 <=14-bit SD returns a sign-extended int32 (xDistortionAVX512.cpp:14-73)."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    h, w = 4096, 2048
+    a = np.full((h, w), 16383, np.uint16)
+    b = np.zeros((h, w), np.uint16)
+    exact = 16383 * h * w
+    assert o.sd(a, b, wrap32=False) == exact
+    assert o.sd(a, b, wrap32=True) == int(np.int32(np.uint32(exact % (1 << 32))))
+
+
+def test_rounding_matches_std_round(oracle):
+    """test-xCommon.cpp:62-113: round half away == std::round over i/1000."""
+    for i in range(-65536, 65536, 7):
+        v = i / 1000.0
+        want = int(math.copysign(math.floor(abs(v) + 0.5), v))
+        assert oracle.round_f64(v) == want
+        vf = float(np.float32(v))
+        wantf = int(math.copysign(math.floor(abs(vf) + 0.5), vf))
+        assert oracle.round_f32(vf) == wantf
+    for v in (0.5, 1.5, 2.5, -0.5, -1.5, -2.5):
+        assert oracle.round_f64(v) == int(math.copysign(math.floor(abs(v) + 0.5), v))
+
+
+def test_upsample_downsample_round_trip(oracle):
+    """test-xPixelOps.cpp:160-194: downsample(upsample(x)) == x, sum x4."""
+    rng = np.random.default_rng(1)
+    src = rng.integers(0, 1024, (37, 53)).astype(np.uint16)
+    up = oracle.upsample(src)
+    assert up.shape == (74, 106)
+    assert int(up.sum(dtype=np.int64)) == 4 * int(src.sum(dtype=np.int64))
+    assert np.array_equal(oracle.downsample(up), src)
+    # closed form of the rounded mean, odd sizes truncate
+    big = rng.integers(0, 65536, (9, 11)).astype(np.uint16)
+    d = oracle.downsample(big)
+    b = big[:8, :10].astype(np.int64)
+    want = (b[0::2, 0::2] + b[0::2, 1::2] + b[1::2, 0::2] + b[1::2, 1::2] + 2) >> 2
+    assert np.array_equal(d, want.astype(np.uint16))
+
+
+@pytest.mark.parametrize("n", [8, 9, 15, 16, 17, 63, 64, 1080, 1920, 2048, 4096])
+def test_ssim_block_counts_brute_force(oracle, n):
+    """test-xSSIM.cpp:29-77."""
+    for wnd, stride in ((8, 4), (8, 8), (16, 4), (8, 1)):
+        brute = len(range(0, n - wnd + 1, stride))
+        assert oracle.num_blocks(n, wnd, stride) == brute
+    assert oracle.ms_num_scales(512, 512) == 5
+    assert oracle.ms_num_scales(511, 4096) == 4
+    assert oracle.ms_num_scales(31, 31) == 0
+
+
+def _numpy_search(tst, ref, gcs, R, cw):
+    """Brute-force restatement in numpy (int64, no wrap); raster first-min."""
+    h, w = tst[0].shape
+    t = np.stack([p.astype(np.int64) for p in tst]) + np.array(gcs[:3]).reshape(3, 1, 1)
+    pad = [np.pad(p.astype(np.int64), R, mode="edge") for p in ref]
+    best_e = np.full((h, w), np.iinfo(np.int64).max)
+    best_i = np.zeros((h, w), np.int64)
+    best_d = np.zeros((3, h, w), np.int64)
+    idx = 0
+    for dy in range(-R, R + 1):
+        for dx in range(-R, R + 1):
+            r = np.stack([p[R + dy:R + dy + h, R + dx:R + dx + w] for p in pad])
+            d = (t - r) ** 2
+            e = cw[0] * d[0] + cw[1] * d[1] + cw[2] * d[2]
+            upd = e < best_e
+            best_e = np.where(upd, e, best_e)
+            best_i = np.where(upd, idx, best_i)
+            best_d = np.where(upd[None], d, best_d)
+            idx += 1
+    return best_i, best_d
+
+
+@pytest.mark.parametrize("seed,R,bd", [(0, 1, 8), (1, 2, 10), (2, 3, 10), (3, 4, 12)])
+def test_search_brute_force(oracle, seed, R, bd):
+    rng = np.random.default_rng(seed)
+    h, w = 23, 29
+    ref = [rng.integers(0, 1 << bd, (h, w)).astype(np.uint16) for _ in range(3)]
+    tst = [np.clip(np.roll(p, 1, axis=1).astype(np.int64) + rng.integers(-3, 4, (h, w)), 0, (1 << bd) - 1)
+           .astype(np.uint16) for p in ref]
+    # plant exact ties so the raster tie-break is exercised
+    tst[0][5, 5] = ref[0][5, 5]
+    gcs = [1, -2, 0, 0]
+    cw = (4, 1, 1, 0)
+    rows, scp, best = oracle.cps(tst, ref, bd, gcs, R=R, cw=cw, want_scp=True, want_best=True)
+    bi, bdist = _numpy_search(tst, ref, gcs, R, cw)
+    assert np.array_equal(best.astype(np.int64), bi)
+    assert np.array_equal(rows[:, :3].astype(np.int64), bdist.sum(axis=2).T)
+    # SCP = clip(ref_best - gcs)
+    D = 2 * R + 1
+    yy, xx = np.mgrid[0:h, 0:w]
+    ry = np.clip(yy + bi // D - R, 0, h - 1)
+    rx = np.clip(xx + bi % D - R, 0, w - 1)
+    for c in range(3):
+        want = np.clip(ref[c][ry, rx].astype(np.int64) - gcs[c], 0, (1 << bd) - 1)
+        assert np.array_equal(scp[c], want)
+
+
+def test_search_int32_wrap_semantics(oracle):
+    """16-bit differences square past 2^31: the SSE kernel wraps in int32
+    (xCorrespPixelShiftSSE.cpp:60-65); rows are int32 sums sign-extended."""
+    h, w = 4, 4
+    ref = [np.zeros((h, w), np.uint16) for _ in range(3)]
+    tst = [np.full((h, w), 65535, np.uint16) for _ in range(3)]
+    rows, _, _ = oracle.cps(tst, ref, 16, [0, 0, 0, 0], R=1)
+    d = np.uint32(65535 * 65535)   # 4294836225 -> int32 -131071
+    want = np.int64(np.int32(np.uint32((int(d) * w) % (1 << 32))))
+    assert int(np.int64(np.uint64(rows[0, 0]).astype(np.int64))) == int(want) or \
+        rows[0, 0] == np.uint64(want % (1 << 64))
+
+
+def test_scp_invariant(oracle):
+    """test-xCorrespPixelShift.cpp:155-179: SSD(X, SCP(Y)) == asymmetric search distance (no clipping)."""
+    rng = np.random.default_rng(7)
+    h, w, bd = 40, 56, 10
+    org = [rng.integers(100, 900, (h, w)).astype(np.uint16) for _ in range(3)]
+    mod = [np.clip(np.roll(p, (1, -1), axis=(0, 1)).astype(np.int64) + rng.integers(-2, 3, (h, w)), 0, 1023)
+           .astype(np.uint16) for p in org]
+    for gcd in ([0, 0, 0, 0], [1, 1, 1, 0], [-1, -1, -1, 0], [-1, 1, 1, 0]):
+        rows, scp, _ = oracle.cps(mod, org, bd, gcd, R=2, want_scp=True)
+        for c in range(3):
+            assert int(rows[:, c].sum()) == oracle.ssd(mod[c], scp[c])
+
+
+def test_ssim_window_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    bd = 10
+    M = (1 << bd) - 1
+    c1, c2 = (0.01 * M) ** 2, (0.03 * M) ** 2
+    t = rng.integers(0, 1024, (8, 8)).astype(np.uint16)
+    r = rng.integers(0, 1024, (8, 8)).astype(np.uint16)
+    tf, rf = t.astype(np.float64), r.astype(np.float64)
+    mt, mr = tf.mean(), rf.mean()
+    vt, vr = (tf * tf).mean() - mt * mt, (rf * rf).mean() - mr * mr
+    cov = (tf * rf).mean() - mt * mr
+    want = ((2 * mr * mt + c1) / (mr * mr + mt * mt + c1)) * ((2 * cov + c2) / (vr + vt + c2))
+    assert abs(oracle.ssim_plane(t, r, bd) - want) < 1e-12
+    assert abs(oracle.ssim_plane(t, t, bd) - 1.0) < 1e-15
+
+
+def test_kbns_exact_sums(oracle):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(4096) * 10.0 ** rng.integers(-8, 9, 4096)
+    exact = float(sum(Fraction(v) for v in x))
+    assert abs(oracle.kbns(x) - exact) <= abs(exact) * 2e-16 + 1e-300
+    # the plain sum is visibly worse on this data, the compensated one is not
+    y = np.array([1e16, 1.0, -1e16, 1.0] * 100)
+    assert oracle.kbns(y) == 200.0
+
+
+def test_oracle_detects_planted_bugs(oracle):
+    """Plausible mistakes planted in the oracle's inputs/outputs are caught by the checks above."""
+    case = next(c for c in CASES if c["name"] == "w64h48_10b_420")
+    wl, p0n, p1n, p0, p1 = inputs(case, oracle)
+    g = oracle.gcd(p0, p1, wl.bit_depth)
+    # wrong GCD sign -> rows differ from the reference's
+    rows_bad, _, _ = oracle.cps(p0, p1, wl.bit_depth, [-x for x in g], R=2)
+    assert sha1(rows_bad.astype(np.uint64)) != case["rows_r2t_sha1"]
+    # swapped directions -> rows differ
+    rows_sw, _, _ = oracle.cps(p1, p0, wl.bit_depth, g, R=2)
+    assert sha1(rows_sw.astype(np.uint64)) != case["rows_r2t_sha1"]
+    # wrong component weights (1,1,1) -> argmin changes somewhere
+    rows_w, _, _ = oracle.cps(p0, p1, wl.bit_depth, g, R=2, cw=(1, 1, 1, 0))
+    assert sha1(rows_w.astype(np.uint64)) != case["rows_r2t_sha1"]
+    # SSIM of operands misaligned by one row is not the reference value
+    bad = oracle.ssim([p[1:].copy() for p in p0], [p[:-1].copy() for p in p1], wl.bit_depth)
+    assert abs(bad[0] - float(case["ssim"][0])) > 1e-9
+    # SSIM with the wrong window stride is not the reference value
+    bad = oracle.ssim(p0, p1, wl.bit_depth, wnd=8, stride=8)
+    assert abs(bad[0] - float(case["ssim"][0])) > 1e-9
+
+
+def test_search_tie_break_is_first_in_raster_order(oracle):
+    """All candidates tie on a flat reference: the first strict minimum (dy-outer raster
+    order, xCorrespPixelShiftSSE.cpp:54-66) is candidate 0; a last-minimum rule would pick 24."""
+    h, w = 12, 12
+    ref = [np.full((h, w), 500, np.uint16) for _ in range(3)]
+    tst = [np.full((h, w), 510, np.uint16) for _ in range(3)]
+    _, _, best = oracle.cps(tst, ref, 10, [0, 0, 0, 0], R=2, want_best=True)
+    assert np.all(best == 0)
+    # two-way tie between (dy,dx)=(-1,0) and (1,0): the upper one (index 7) wins
+    ref2 = [np.full((h, w), 900, np.uint16) for _ in range(3)]
+    for c in range(3):
+        ref2[c][5, 6] = 510
+        ref2[c][7, 6] = 510
+    tst2 = [np.full((h, w), 510, np.uint16) for _ in range(3)]
+    _, _, best2 = oracle.cps(tst2, ref2, 10, [0, 0, 0, 0], R=2, want_best=True)
+    assert best2[6, 6] == 1 * 5 + 2
