@@ -51,8 +51,9 @@ typedef struct ivqm_config {
   int32_t ssim_window, ssim_stride;     /* block-averaged SSIM, default 8 and 4 (xSSIM.h:35-37) */
   int32_t fake_vals_for_exact;    /* PSNR of an exact component = PSNR(SSD=1) (xPSNR.cpp:147, xAppQMIV.cpp:974-981) */
   int32_t gcd_sd_wrap32;          /* reproduce the x86-64-v4 <=14-bit int32 SD accumulation (xDistortionAVX512.cpp:14-73) */
-  int32_t search_tile_rows;       /* search tile height per wave (4 or 8); 0 = autotuned */
-  int32_t reserved[7];
+  int32_t search_tile_rows;       /* search tile height per wave (4 or 8); 0 = default (8) */
+  int32_t ssim_wide_sums;         /* 64-bit squared sums in SSIM at any bit depth (always on above 14-bit) */
+  int32_t reserved[6];
 } ivqm_config;
 
 /* One picture: three planes of 8-bit (bytes_per_sample = 1) or 16-bit samples.
@@ -125,6 +126,9 @@ int ivqm_msssim(ivqm_ctx* ctx, double msssim[4]);
 /* xIVSSIM::calcPicIVSSIM / calcPicIVMSSSIM(P0, P1, SCP0, SCP1): libs/pmbb-ivqm/src/xIVSSIM.h:39-40.
  * Uses the SCP pictures of the last ivqm_gen_scp. */
 int ivqm_ivssim(ivqm_ctx* ctx, double* ivssim, double* r2t, double* t2r);
+/* Loads shift-compensated picture idx (0 = SCP0, 1 = SCP1) from host 4:4:4 planes,
+ * for callers that hold their own SCP pictures (xIVSSIM.h:39 takes them as arguments). */
+int ivqm_upload_scp(ivqm_ctx* ctx, int32_t idx, const ivqm_picture* host_pic);
 int ivqm_ivmsssim(ivqm_ctx* ctx, double* ivmsssim, double* r2t, double* t2r);
 
 /* ---- fused frame pass ----------------------------------------------------- */

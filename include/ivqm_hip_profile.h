@@ -9,16 +9,19 @@
 #define IVQM_HIP_PROFILE_H
 #include <stdint.h>
 #include "ivqm_hip.h"
+#include "ivqm_batch.h"
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 #define IVQM_K_STATS 0  /* k_stats: SD / SSD rows (streaming) */
 #define IVQM_K_SEARCH 1 /* k_search: both directions + SCP */
-#define IVQM_K_SSIM 2   /* k_ssim: the two IV-SSIM jobs (streaming) */
+#define IVQM_K_SSIM 2   /* k_ssim: all SSIM jobs of the pass (streaming) */
 
-/* Runs `iters` launches of `kernel` on the bound pair; *avg_ms = mean duration. */
+/* Relaunches `kernel` `iters` times for the last enqueued pair / batch;
+ * *avg_ms = mean duration (HIP events on the context's stream). */
 int ivqm_profile(ivqm_ctx* ctx, int32_t kernel, int32_t iters, double* avg_ms);
+int ivqm_batch_profile(ivqm_batch* b, int32_t kernel, int32_t iters, double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,10 @@ EXPORTED = [
     "ivqm_upload", "ivqm_bind", "ivqm_gcd", "ivqm_ssd", "ivqm_mse", "ivqm_psnr", "ivqm_wspsnr", "ivqm_ivpsnr",
     "ivqm_gen_scp", "ivqm_ssim", "ivqm_msssim", "ivqm_ivssim", "ivqm_ivmsssim", "ivqm_frame",
     "ivqm_frame_enqueue", "ivqm_frame_collect", "ivqm_get_rows", "ivqm_get_scp", "ivqm_enable_best_map",
-    "ivqm_get_best_map", "ivqm_upsample420", "ivqm_downsample", "ivqm_profile",
+    "ivqm_get_best_map", "ivqm_upsample420", "ivqm_downsample", "ivqm_profile", "ivqm_upload_scp",
+    "ivqm_batch_create", "ivqm_batch_destroy", "ivqm_batch_set_stream", "ivqm_batch_set_tuning",
+    "ivqm_batch_get_tuning", "ivqm_batch_autotune", "ivqm_batch_run", "ivqm_batch_enqueue", "ivqm_batch_collect",
+    "ivqm_batch_profile",
 ]
 
 
@@ -38,13 +41,18 @@ class Config(C.Structure):
         ("unnoticeable_coef", C.c_float * 4), ("erp", C.c_int32), ("lon_range_deg", C.c_int32),
         ("lat_range_deg", C.c_int32), ("ssim_window", C.c_int32), ("ssim_stride", C.c_int32),
         ("fake_vals_for_exact", C.c_int32), ("gcd_sd_wrap32", C.c_int32), ("search_tile_rows", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("ssim_wide_sums", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 
 class Picture(C.Structure):
     _fields_ = [("plane", C.c_void_p * 3), ("stride", C.c_int32 * 3), ("chroma_420", C.c_int32),
                 ("bytes_per_sample", C.c_int32)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("p0", Picture), ("p1", Picture), ("width", C.c_int32), ("height", C.c_int32),
+                ("bit_depth", C.c_int32)]
 
 
 class FrameResult(C.Structure):
@@ -82,7 +90,7 @@ def load():
     L.ivqm_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
     L.ivqm_destroy.argtypes = [C.c_void_p]
     L.ivqm_set_stream.argtypes = [C.c_void_p, C.c_void_p]
-    for fn in ("ivqm_upload", "ivqm_bind"):
+    for fn in ("ivqm_upload", "ivqm_bind", "ivqm_upload_scp"):
         getattr(L, fn).argtypes = [C.c_void_p, C.c_int32, C.POINTER(Picture)]
     L.ivqm_gcd.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.ivqm_ssd.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
@@ -103,6 +111,17 @@ def load():
     L.ivqm_upsample420.argtypes = [C.POINTER(Picture), C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
     L.ivqm_downsample.argtypes = [C.POINTER(Picture), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     L.ivqm_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.ivqm_batch_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.ivqm_batch_destroy.argtypes = [C.c_void_p]
+    L.ivqm_batch_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+    L.ivqm_batch_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+    L.ivqm_batch_get_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.ivqm_batch_autotune.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]
+    L.ivqm_batch_run.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.POINTER(FrameResult)]
+    L.ivqm_batch_enqueue.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.c_int32]
+    L.ivqm_batch_collect.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FrameResult)]
+    L.ivqm_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     _lib = L
     return L
 

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_runtime.hip", "ivqm_batch.hip", "ivqm_serve.hip"]
+HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_engine.hip", "ivqm_runtime.hip", "ivqm_serve.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -36,7 +36,7 @@ def build_hip(verbose_isa: bool = False) -> str:
     out = os.path.join(HERE, "libivqm_hip.so")
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-I" + os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-Wl,-soname,libivqm_hip.so", "-I" + os.path.join(ROOT, "include"), *srcs, "-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -47,13 +47,25 @@ def build_synth() -> str:
     return out
 
 
+def build_dropin_check(lib: str) -> str:
+    """C++ program that drives include/ivqm_dropin.hpp (run by tests/test_gpu_dropin.py)."""
+    out_dir = os.path.join(HERE, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "ivqm_dropin_check")
+    _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp"), "-L" + os.path.dirname(lib), "-livqm_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-o", out])
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"])
 
 
 def build(oracle: bool = True) -> None:
     build_synth()
-    build_hip()
+    lib = build_hip()
+    build_dropin_check(lib)
     if oracle:
         build_oracle()
 

@@ -182,3 +182,78 @@ class IvqmContext:
         out = np.zeros((self.height, self.width), np.uint8)
         check(self.L.ivqm_get_best_map(self.h, direction, out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+
+def _torch_picture(planes, chroma_420: bool) -> Picture:
+    return _picture_from_torch(planes, chroma_420)
+
+
+class IvqmBatch:
+    """Handle over one batch object (include/ivqm_batch.h); pairs are torch CUDA planes."""
+
+    def __init__(self, bit_depth_unused=None, **kw):
+        self.L = _lib.load()
+        self.cfg = default_config(64, 64, 10, **kw)
+        h = C.c_void_p()
+        check(self.L.ivqm_batch_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            self.L.ivqm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def make_pairs(items):
+        """items: list of (p0_planes, p1_planes, width, height, bit_depth, chroma_420) with torch CUDA planes."""
+        arr = (_lib.Pair * len(items))()
+        for i, (p0, p1, w, h, bd, c420) in enumerate(items):
+            arr[i].p0 = _picture_from_torch(p0, c420)
+            arr[i].p1 = _picture_from_torch(p1, c420)
+            arr[i].width, arr[i].height, arr[i].bit_depth = w, h, bd
+        return arr
+
+    def set_tuning(self, search_tile_rows=0, ssim_band_rows=0, use_graph=False):
+        check(self.L.ivqm_batch_set_tuning(self.h, search_tile_rows, ssim_band_rows, int(use_graph)))
+
+    def tuning(self):
+        a, b = C.c_int32(), C.c_int32()
+        check(self.L.ivqm_batch_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, pairs, metrics):
+        a, b = C.c_int32(), C.c_int32()
+        check(self.L.ivqm_batch_autotune(self.h, pairs, len(pairs), metrics, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def run(self, pairs, metrics):
+        res = (FrameResult * len(pairs))()
+        check(self.L.ivqm_batch_run(self.h, pairs, len(pairs), metrics, res))
+        return list(res)
+
+    def enqueue(self, pairs, metrics, ring):
+        self._keep[ring] = pairs
+        check(self.L.ivqm_batch_enqueue(self.h, pairs, len(pairs), metrics, ring))
+
+    def collect(self, ring, n):
+        res = (FrameResult * n)()
+        check(self.L.ivqm_batch_collect(self.h, ring, res))
+        return list(res)
+
+    def profile(self, kernel, iters=10):
+        ms = C.c_double()
+        check(self.L.ivqm_batch_profile(self.h, kernel, iters, C.byref(ms)))
+        return ms.value

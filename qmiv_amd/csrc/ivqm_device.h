@@ -1,4 +1,8 @@
 // Shared host/device definitions of the gfx950 metric kernels.
+//
+// Every kernel works on a table of frame pairs (or SSIM / pyramid jobs) in
+// device memory: one launch per phase covers all pairs of a batch, whatever
+// their sizes and sample formats. A single pair is a batch of one.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -15,42 +19,67 @@ struct PicDev {
   int32_t bps;  // bytes per sample: 1 or 2
 };
 
+// One frame pair of a batch and where its outputs live.
+struct PairDev {
+  PicDev p0, p1;
+  int32_t w, h, maxv;
+  int32_t narrow;         // search variant (host-checked ranges)
+  int32_t md[3];          // GCD clip limits round_f32(coef * Max)
+  int32_t wrap32;         // x86-64-v4 int32 SD accumulation
+  int32_t stat_blk0;      // first k_stats block of this pair
+  int32_t tile0;          // first k_search tile of this pair
+  int32_t gcd_given;      // 1: gcd[] already holds the caller's GCD
+  int32_t scp_pitch;
+  int64_t rowsd_off;      // [3][h] in row_sd (int64) and row_ssd (uint64)
+  int64_t rows_off;       // [2][3][h] uint32
+  int64_t scp_off;        // [2][3][h][scp_pitch] uint16, or -1
+  int64_t best_off;       // [2][h][w] uint8, or -1
+  int32_t res_idx;        // index of gcd[4], sd[3], ssd[3]
+  int32_t gcd_in[4];      // the caller's GCD when gcd_given
+  int32_t pad;
+};
+
 // One SSIM evaluation (a picture pair at one pyramid level).
-struct SsimJob {
+struct SsimJobDev {
   PicDev a, b;
-  int32_t w, h;         // luma-resolution size of this level
-  int32_t nux, nuy;     // windows along x and y
-  int32_t nchunk;       // 63-window column chunks
-  int32_t calc_l;       // luminance term on/off
-  int32_t part_off;     // offset (in doubles) of this job's partial block [3][nuy][nchunk]
-  int32_t erp;          // weight rows by the ERP weights
+  int32_t w, h;
+  int32_t nux, nuy, nchunk, nbg;  // windows, 63-window chunks, band groups (4 bands of NB rows)
+  int32_t calc_l, erp, wsq;       // luminance term, ERP row weights, 64-bit squared sums
+  int32_t blk0;                   // first block of this job (all 3 components)
+  int64_t part_off;               // [3][nuy][nchunk] doubles in the partial buffer
+  int64_t erp_off;                // offset of this height's weights, or -1
+  int32_t out_idx;                // [3][2] doubles at out[6 * out_idx]
+  float c1n, c2n;                 // C1 * n^2, C2 * n^2 of this job's bit depth (xSSIM.cpp:50-52)
+  int32_t pad;
 };
 
-constexpr int kMaxSsimJobs = 16;
-struct SsimJobTable {
-  SsimJob job[kMaxSsimJobs];
-  int32_t njobs;
+// One downsampling job of a pyramid level.
+struct DownJob {
+  PicDev src;
+  int32_t w, h;      // source size
+  int32_t pitch;     // destination pitch
+  int32_t blk0;
+  uint16_t* dst;     // three planes, plane size pitch * (h >> 1)
 };
 
-// Launchers (ivqm_kernels.hip). All are asynchronous on `st`.
-hipError_t launch_stats(const PicDev& p0, const PicDev& p1, int w, int h, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* zero_buf, int n_zero, hipStream_t st);
-hipError_t launch_gcd_finalize(const int64_t* row_sd, const uint64_t* row_ssd, int w, int h, int cs, int bit_depth,
-                               const int32_t maxdiff[3], int wrap32, int32_t* gcd_out, int64_t* sd_out,
-                               uint64_t* ssd_out, hipStream_t st);
-// rows: [2][3][h] uint32; scp: [2][3][h][scp_pitch] uint16 (picture 0 = SCP0, 1 = SCP1), may be null.
-// best: optional [2][h][w] uint8 raster index of the winning candidate.
-hipError_t launch_search(const PicDev& p0, const PicDev& p1, int w, int h, int bit_depth, const int32_t* gcd,
-                         const int32_t cw[4], int R, bool narrow, uint32_t* rows, uint16_t* scp, int scp_pitch,
-                         uint8_t* best, int tile_py, hipStream_t st);
-hipError_t launch_ssim(const SsimJobTable& jobs, int max_nchunk, int max_bands, float c1n, float c2n, bool wide_sq,
-                       double* partial, hipStream_t st);
-// out: per job, per component: [job][3][2] = {sum, weight-sum}.
-hipError_t launch_ssim_finalize(const SsimJobTable& jobs, const double* partial, const double* erp_w, int wnd,
-                                int stride, double* out, hipStream_t st);
-// dst: three planes (dw x dh, row pitch dst_pitch, plane size dst_pitch * dh) of the next pyramid level.
-hipError_t launch_downsample(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st);
-hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
+// Launchers (ivqm_kernels.hip). All are asynchronous on `st`; tables live in
+// device memory. `nblk` is the total block count of the flattened grid.
+hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
+                        uint32_t* rows, hipStream_t st);
+hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
+                               int32_t* gcd, int64_t* sd, uint64_t* ssd, hipStream_t st);
+hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
+                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
+hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st);
+hipError_t launch_ssim_finalize(const SsimJobDev* jobs, int njobs, const double* partial, const double* erp_w,
+                                int half_wnd, int stride, double* out, hipStream_t st);
+hipError_t launch_downsample(const DownJob* jobs, int njobs, int nblk, hipStream_t st);
+// Standalone pixel operations on one picture.
+hipError_t launch_downsample_one(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st);
 hipError_t launch_upsample420(const PicDev& src, int w, int h, uint16_t* dst, int dst_stride, hipStream_t st);
+
+constexpr int kSearchTileW = 64;
+inline int search_tile_h(int py) { return 4 * py; }
+constexpr int kDownBlkW = 64, kDownBlkH = 4;
 
 }  // namespace ivqm

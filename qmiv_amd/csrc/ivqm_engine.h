@@ -1,0 +1,178 @@
+// Batch engine of the gfx950 metric hot path (host side).
+//
+// An Engine owns one HIP stream and the device buffers of a batch of frame
+// pairs. enqueue() plans a batch (offsets, block counts, job tables), writes
+// the tables into a pinned staging area of the chosen ring slot and enqueues,
+// in stream order: table upload -> stats -> GCD -> search (both directions,
+// fused SCP) -> pyramid levels -> SSIM jobs -> FP64 totals -> one copy per
+// result region into the slot's pinned arena. collect() waits for the slot
+// and runs the host FP64 epilogue (ivqm_epilogue.h) per pair. Every phase is
+// one launch for the whole batch; a batch of one pair is the plain API.
+// Optionally the enqueued sequence of a slot is captured once as a hipGraph
+// and replayed while the batch shape stays the same.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ivqm_hip.h"
+#include "ivqm_device.h"
+
+namespace ivqm {
+
+int set_error(int code, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+
+constexpr int kRing = 8;
+constexpr int kPitchAlign = 64;  // samples
+constexpr uint32_t kReuseScp = 1u << 31;  // IV-SSIM metrics use the stored SCP pictures
+constexpr uint32_t kScpOnly = 1u << 30;   // run the search for the SCP pictures only
+
+struct PairSpec {
+  PicDev p0, p1;
+  int w = 0, h = 0, bd = 8;
+  bool has_gcd = false;
+  int32_t gcd[4] = {0, 0, 0, 0};
+};
+
+// Per-pair bookkeeping the epilogue needs.
+struct PairPlan {
+  int w, h, bd, cs;
+  int64_t rows_off, rowsd_off, scp_off;
+  int ns;
+  int ssim = -1, iv_t2r = -1, iv_r2t = -1;
+  int ms[5] = {-1, -1, -1, -1, -1}, ivms_t2r[5] = {-1, -1, -1, -1, -1}, ivms_r2t[5] = {-1, -1, -1, -1, -1};
+  int erp_h = -1;  // height whose ERP weights apply
+};
+
+// A picture of a plan: a caller's picture, or three planes in one of the
+// engine's arenas (turned into device pointers, with a bounds check, when the
+// batch is issued).
+struct PicRef {
+  int arena = 0;        // 0: caller picture, 1: SCP arena, 2: pyramid arena
+  int64_t off = 0;      // element offset of plane 0 in the arena
+  int pitch = 0, w = 0, h = 0;
+  PicDev user;
+};
+
+struct BatchPlan {
+  uint32_t metrics = 0;
+  bool need_stats = false, need_search = false, need_scp = false;
+  int n = 0;
+  std::vector<PairPlan> pp;
+  std::vector<PairDev> pairs;
+  std::vector<SsimJobDev> jobs;      // a / b filled from job_a / job_b when issued
+  std::vector<PicRef> job_a, job_b;
+  std::vector<DownJob> down[5];      // src / dst filled from down_src / down_dst when issued
+  std::vector<PicRef> down_src[5], down_dst[5];
+  int stat_blocks = 0, tiles = 0, ssim_blocks = 0, down_blocks[5] = {0, 0, 0, 0, 0};
+  int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
+  int py = 8, nb = 8;
+  std::string signature;
+};
+
+struct RingSlot {
+  // pinned host staging (tables) and results
+  PairDev* t_pairs = nullptr;
+  SsimJobDev* t_jobs = nullptr;
+  DownJob* t_down = nullptr;
+  int32_t* r_gcd = nullptr;
+  int64_t* r_sd = nullptr;
+  uint64_t* r_ssd = nullptr;
+  uint64_t* r_row_ssd = nullptr;
+  uint32_t* r_rows = nullptr;
+  double* r_ssim = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  BatchPlan plan;
+  hipGraphExec_t graph = nullptr;
+  std::string graph_sig;
+};
+
+class Engine {
+ public:
+  explicit Engine(const ivqm_config& cfg);
+  ~Engine();
+  int init();
+  int set_stream(hipStream_t s);
+  hipStream_t stream() const { return stream_; }
+  const ivqm_config& cfg() const { return cfg_; }
+
+  // Plans and enqueues one batch into ring slot `ring`.
+  int enqueue(const PairSpec* pairs, int n, uint32_t metrics, int ring, bool use_graph = false);
+  // Waits for ring slot `ring` and writes one result per pair.
+  int collect(int ring, ivqm_frame_result* out);
+  int sync();
+
+  void set_tuning(int py, int nb) {
+    if (py == 4 || py == 8) py_ = py;
+    if (nb == 4 || nb == 8 || nb == 16) nb_ = nb;
+  }
+  int tuning_py() const { return py_; }
+  int tuning_nb() const { return nb_; }
+  void enable_best(bool on) { best_on_ = on; }
+  // Times `iters` launches of one phase of the last enqueued batch (HIP events).
+  int profile(int kernel, int iters, double* avg_ms);
+
+  // device buffers (last batch) for debug readers
+  uint32_t* d_rows() const { return d_rows_; }
+  uint16_t* d_scp() const { return d_scp_; }
+  uint8_t* d_best() const { return d_best_; }
+  const BatchPlan& last_plan() const { return last_; }
+  bool scp_valid() const { return scp_valid_; }
+  void set_scp_valid(bool v) { scp_valid_ = v; }
+  int ensure_scp(int64_t elems);
+  static int scp_pitch(int w) { return (w + kPitchAlign - 1) / kPitchAlign * kPitchAlign; }
+
+ private:
+  int plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P);
+  int ensure_capacity(const BatchPlan& P);
+  int ensure_ring_capacity();
+  int issue(const BatchPlan& P, RingSlot& S);
+  int fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_first[5]);
+  int materialize(const PicRef& r, PicDev& out) const;
+  void drop_graphs();
+  int erp_offset(int h, int64_t* off);
+  bool narrow_ok(int maxv, int32_t gcd_abs_max) const;
+
+  ivqm_config cfg_;
+  hipStream_t stream_ = nullptr;
+  bool own_stream_ = false;
+  int py_ = 8, nb_ = 8;
+  bool best_on_ = false;
+  bool scp_valid_ = false;
+
+  // capacities (elements)
+  int64_t cap_pairs_ = 0, cap_rows_ = 0, cap_rowsd_ = 0, cap_scp_ = 0, cap_pyr_ = 0, cap_partial_ = 0,
+          cap_jobs_ = 0, cap_down_ = 0, cap_best_ = 0;
+  int64_t ring_pairs_ = 0, ring_rows_ = 0, ring_rowsd_ = 0, ring_jobs_ = 0, ring_down_ = 0;
+
+  PairDev* d_pairs_ = nullptr;
+  SsimJobDev* d_jobs_ = nullptr;
+  DownJob* d_down_ = nullptr;
+  int64_t* d_row_sd_ = nullptr;
+  uint64_t* d_row_ssd_ = nullptr;
+  uint32_t* d_rows_ = nullptr;
+  int32_t* d_gcd_ = nullptr;
+  int64_t* d_sd_ = nullptr;
+  uint64_t* d_ssd_ = nullptr;
+  uint16_t* d_scp_ = nullptr;
+  uint8_t* d_best_ = nullptr;
+  uint16_t* d_pyr_ = nullptr;
+  double* d_partial_ = nullptr;
+  double* d_ssim_out_ = nullptr;
+  double* d_erp_ = nullptr;
+  int64_t erp_used_ = 0, erp_cap_ = 0;
+  std::map<int, int64_t> erp_off_;
+
+ public:
+  std::map<int, std::pair<std::vector<double>, double>> erp_host;  // height -> weights, correction
+ private:
+  RingSlot ring_[kRing];
+  BatchPlan last_;
+};
+
+}  // namespace ivqm

@@ -1,0 +1,717 @@
+// Batch engine of the gfx950 metric hot path: planning, buffers, launches,
+// result ring and the host FP64 epilogue. See ivqm_engine.h.
+#include "ivqm_engine.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "ivqm_epilogue.h"
+
+namespace ivqm {
+
+namespace {
+thread_local std::string g_last_error;
+
+int key_bits(int R) {
+  const int n = (2 * R + 1) * (2 * R + 1);
+  return n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
+}
+int ms_num_scales(int w, int h) {  // libs/pmbb-ivqm/src/xSSIM.cpp:405-417
+  for (int i = 0; i < 5; i++) {
+    const int lim = 32 << i;
+    if (w < lim || h < lim) return i;
+  }
+  return 5;
+}
+int num_blocks(int len, int wnd, int stride) {  // libs/pmbb-ivqm/src/xSSIM.cpp:381-404
+  int cnt = 0;
+  for (int i = 0; i < len - wnd + 1; i += stride) cnt++;
+  return cnt;
+}
+const double kMsExp[5][5] = {{1.0000, 0.0000, 0.0000, 0.0000, 0.0000}, {0.1356, 0.8644, 0.0000, 0.0000, 0.0000},
+                             {0.0711, 0.4530, 0.4760, 0.0000, 0.0000}, {0.0517, 0.3295, 0.3462, 0.2726, 0.0000},
+                             {0.0448, 0.2856, 0.3001, 0.2363, 0.1333}};  // xStructSimConsts.h:42-49
+
+template <typename T> int dmalloc(T** p, int64_t n) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (n <= 0) return IVQM_OK;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return hip_error(e, "hipMalloc");
+  return IVQM_OK;
+}
+template <typename T> int hmalloc(T** p, int64_t n) {
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipHostMalloc((void**)p, sizeof(T) * (size_t)n, hipHostMallocDefault);
+  if (e != hipSuccess) return hip_error(e, "hipHostMalloc");
+  return IVQM_OK;
+}
+int64_t grow(int64_t need, int64_t have) { return need <= have ? have : std::max<int64_t>(need, have + have / 2); }
+
+}  // namespace
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int hip_error(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return IVQM_EHIP - (int)e;
+}
+const char* last_error_message() { return g_last_error.c_str(); }
+
+#define HIPCHK(expr)                                   \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_error(_e, #expr); \
+  } while (0)
+#define RCCHK(expr)          \
+  do {                       \
+    int _rc = (expr);        \
+    if (_rc) return _rc;     \
+  } while (0)
+
+Engine::Engine(const ivqm_config& cfg) : cfg_(cfg) {
+  if (cfg.search_tile_rows == 4 || cfg.search_tile_rows == 8) py_ = cfg.search_tile_rows;
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& S : ring_) {
+    if (S.graph) (void)hipGraphExecDestroy(S.graph);
+    if (S.ev) (void)hipEventDestroy(S.ev);
+    void* hs[] = {S.t_pairs, S.t_jobs, S.t_down, S.r_gcd, S.r_sd, S.r_ssd, S.r_row_ssd, S.r_rows, S.r_ssim};
+    for (void* h : hs)
+      if (h) (void)hipHostFree(h);
+  }
+  void* ds[] = {d_pairs_, d_jobs_, d_down_, d_row_sd_, d_row_ssd_, d_rows_, d_gcd_, d_sd_, d_ssd_,
+                d_scp_,   d_best_, d_pyr_,  d_partial_, d_ssim_out_, d_erp_};
+  for (void* d : ds)
+    if (d) (void)hipFree(d);
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::init() {
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) return set_error(IVQM_EHIP - (int)(e ? e : hipErrorNoDevice), "no HIP device");
+  HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  own_stream_ = true;
+  for (auto& S : ring_) HIPCHK(hipEventCreateWithFlags(&S.ev, hipEventDisableTiming));
+  return IVQM_OK;
+}
+
+void Engine::drop_graphs() {
+  for (auto& S : ring_)
+    if (S.graph) { (void)hipGraphExecDestroy(S.graph); S.graph = nullptr; S.graph_sig.clear(); }
+}
+
+int Engine::set_stream(hipStream_t s) {
+  RCCHK(sync());
+  drop_graphs();
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+  own_stream_ = false;
+  stream_ = s;
+  if (!s) {
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  return IVQM_OK;
+}
+
+int Engine::sync() {
+  HIPCHK(hipStreamSynchronize(stream_));
+  return IVQM_OK;
+}
+
+bool Engine::narrow_ok(int maxv, int32_t gmax) const {
+  const int32_t* w = cfg_.cmp_weights_search;
+  for (int k = 0; k < 3; k++)
+    if (w[k] < 0) return false;
+  const int64_t A = (int64_t)maxv + gmax;
+  const int64_t S = 1 << key_bits(cfg_.search_range);
+  const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
+  const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
+  if (3 * S * wsum * A * A >= (1ll << 31)) return false;
+  if (2 * S * wmax * A >= (1ll << 23)) return false;
+  return true;
+}
+
+int Engine::erp_offset(int h, int64_t* off) {
+  auto it = erp_off_.find(h);
+  if (it != erp_off_.end()) { *off = it->second; return IVQM_OK; }
+  auto& hw = erp_host[h];
+  host::ws_weights(h, cfg_.lat_range_deg, hw.first, hw.second);
+  if (erp_used_ + h > erp_cap_) {
+    // grow: keep existing weights (copy old contents)
+    const int64_t nc = grow(erp_used_ + h, std::max<int64_t>(erp_cap_, 4096));
+    double* nd = nullptr;
+    HIPCHK(hipMalloc((void**)&nd, sizeof(double) * nc));
+    if (d_erp_) {
+      HIPCHK(hipStreamSynchronize(stream_));
+      HIPCHK(hipMemcpy(nd, d_erp_, sizeof(double) * erp_used_, hipMemcpyDeviceToDevice));
+      (void)hipFree(d_erp_);
+    }
+    d_erp_ = nd;
+    drop_graphs();  // captured SSIM totals read the old weights
+    erp_cap_ = nc;
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  HIPCHK(hipMemcpy(d_erp_ + erp_used_, hw.first.data(), sizeof(double) * h, hipMemcpyHostToDevice));
+  erp_off_[h] = erp_used_;
+  *off = erp_used_;
+  erp_used_ += h;
+  return IVQM_OK;
+}
+
+int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
+  const bool reuse = (metrics & kReuseScp) != 0, scp_only = (metrics & kScpOnly) != 0;
+  const uint32_t m = metrics & 0xFFu;
+  const uint32_t ivs = IVQM_M_IVSSIM | IVQM_M_IVMSSSIM;
+  P = BatchPlan();
+  P.metrics = m;
+  P.n = n;
+  P.py = py_;
+  P.nb = nb_;
+  P.need_search = (m & IVQM_M_IVPSNR) || (!reuse && (m & ivs)) || scp_only;
+  P.need_scp = (!reuse && (m & ivs)) || scp_only;
+  P.need_stats = (m & (IVQM_M_PSNR | IVQM_M_MSE | IVQM_M_WSPSNR)) || P.need_search;
+  if ((m & ivs) && reuse && !(n == 1 && scp_valid_))
+    return set_error(IVQM_ESTATE, "IV-SSIM needs shift-compensated pictures (ivqm_gen_scp / ivqm_upload_scp)");
+  const int R = cfg_.search_range;
+  const int TH = search_tile_h(P.py);
+  const int wnd = cfg_.ssim_window, str = cfg_.ssim_stride;
+  P.pp.resize(n);
+  P.pairs.resize(n);
+  char sigbuf[96];
+  std::snprintf(sigbuf, sizeof sigbuf, "m%x py%d nb%d n%d|", m | (reuse ? 0x100u : 0u) | (scp_only ? 0x200u : 0u),
+                P.py, P.nb, n);
+  P.signature = sigbuf;
+  const bool ms_any = (m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) != 0;
+  for (int i = 0; i < n; i++) {
+    const PairSpec& s = pairs[i];
+    PairPlan& pp = P.pp[i];
+    PairDev& d = P.pairs[i];
+    std::memset(&d, 0, sizeof(d));
+    pp.w = s.w; pp.h = s.h; pp.bd = s.bd; pp.cs = s.p0.cs;
+    d.p0 = s.p0;
+    d.p1 = s.p1;
+    d.w = s.w;
+    d.h = s.h;
+    d.maxv = (1 << s.bd) - 1;
+    int32_t gmax = 0;
+    for (int k = 0; k < 3; k++) {
+      d.md[k] = host::round_f32(cfg_.unnoticeable_coef[k] * (float)d.maxv);  // xGlobClrDiff.cpp:29
+      gmax = std::max<int32_t>(gmax, s.has_gcd ? std::abs(s.gcd[k]) : std::abs(d.md[k]));
+    }
+    d.narrow = narrow_ok(d.maxv, gmax) ? 1 : 0;
+    d.wrap32 = (cfg_.gcd_sd_wrap32 && s.bd <= 14) ? 1 : 0;
+    d.gcd_given = s.has_gcd ? 1 : 0;
+    for (int k = 0; k < 4; k++) d.gcd_in[k] = s.has_gcd ? s.gcd[k] : 0;
+    d.res_idx = i;
+    d.stat_blk0 = P.stat_blocks;
+    P.stat_blocks += s.h + 2 * (s.h >> s.p0.cs);
+    d.tile0 = P.tiles;
+    P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
+    d.rowsd_off = P.rowsd_total;
+    pp.rowsd_off = d.rowsd_off;
+    P.rowsd_total += 3 * (int64_t)s.h;
+    d.rows_off = P.rows_total;
+    pp.rows_off = d.rows_off;
+    P.rows_total += 6 * (int64_t)s.h;
+    d.scp_pitch = scp_pitch(s.w);
+    if (P.need_scp || (reuse && (m & ivs))) {
+      d.scp_off = P.scp_total;
+      P.scp_total += 6 * (int64_t)d.scp_pitch * s.h;
+    } else {
+      d.scp_off = -1;
+    }
+    pp.scp_off = d.scp_off;
+    d.best_off = (best_on_ && n == 1) ? 0 : -1;
+    pp.ns = ms_num_scales(s.w, s.h);
+    if (cfg_.erp) pp.erp_h = s.h;
+    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps);
+    P.signature += sigbuf;
+  }
+  // pyramid offsets: per pair, pictures 0..3 (P0, P1, SCP0, SCP1), levels 1..ns-1
+  std::vector<std::array<std::array<int64_t, 5>, 4>> pyr(n);
+  for (int i = 0; i < n; i++) {
+    PairPlan& pp = P.pp[i];
+    if (!ms_any || pp.ns <= 1) continue;
+    for (int which = 0; which < 4; which++) {
+      if (which >= 2 && !(m & IVQM_M_IVMSSSIM)) continue;
+      if (which < 2 && !(m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM))) continue;
+      for (int l = 1; l < pp.ns; l++) {
+        pyr[i][which][l] = P.pyr_total;
+        P.pyr_total += 3 * (int64_t)scp_pitch(pp.w >> l) * (pp.h >> l);
+      }
+    }
+  }
+  // Pictures of level l: caller pictures at level 0, SCP arena for SCP0/SCP1 at
+  // level 0, pyramid arena above.
+  auto ref_for = [&](int i, int which, int l) -> PicRef {
+    const PairDev& d = P.pairs[i];
+    PicRef r;
+    r.w = d.w >> l;
+    r.h = d.h >> l;
+    if (l == 0 && which < 2) {
+      r.arena = 0;
+      r.user = which == 0 ? d.p0 : d.p1;
+    } else if (l == 0) {
+      r.arena = 1;
+      r.pitch = d.scp_pitch;
+      r.off = d.scp_off + (int64_t)(which - 2) * 3 * d.scp_pitch * d.h;
+    } else {
+      r.arena = 2;
+      r.pitch = scp_pitch(d.w >> l);
+      r.off = pyr[i][which][l];
+    }
+    return r;
+  };
+  // downsample jobs per level
+  for (int i = 0; i < n; i++) {
+    const PairPlan& pp = P.pp[i];
+    if (!ms_any || pp.ns <= 1) continue;
+    for (int which = 0; which < 4; which++) {
+      if (which >= 2 && !(m & IVQM_M_IVMSSSIM)) continue;
+      for (int l = 1; l < pp.ns; l++) {
+        DownJob dj;
+        std::memset(&dj, 0, sizeof(dj));
+        dj.w = pp.w >> (l - 1);
+        dj.h = pp.h >> (l - 1);
+        dj.pitch = scp_pitch(pp.w >> l);
+        const int dw = dj.w >> 1, dh = dj.h >> 1;
+        dj.blk0 = P.down_blocks[l];
+        P.down_blocks[l] += 3 * ((dw + kDownBlkW - 1) / kDownBlkW) * ((dh + kDownBlkH - 1) / kDownBlkH);
+        P.down[l].push_back(dj);
+        P.down_src[l].push_back(ref_for(i, which, l - 1));
+        P.down_dst[l].push_back(ref_for(i, which, l));
+      }
+    }
+  }
+  // SSIM jobs
+  auto add_job = [&](int i, int* slot, const PicRef& a, const PicRef& b, int w, int h, int calc_l, int erp) -> int {
+    SsimJobDev J;
+    std::memset(&J, 0, sizeof(J));
+    J.w = w;
+    J.h = h;
+    J.nux = num_blocks(w, wnd, str);
+    J.nuy = num_blocks(h, wnd, str);
+    if (J.nux <= 0 || J.nuy <= 0) { *slot = -1; return IVQM_OK; }
+    J.nchunk = (J.nux + 62) / 63;
+    J.nbg = ((J.nuy + P.nb - 1) / P.nb + 3) / 4;
+    J.calc_l = calc_l;
+    J.erp = erp;
+    J.wsq = (P.pp[i].bd > 14 || cfg_.ssim_wide_sums) ? 1 : 0;
+    J.erp_off = -1;
+    if (erp) RCCHK(erp_offset(h, &J.erp_off));
+    J.blk0 = P.ssim_blocks;
+    P.ssim_blocks += 3 * J.nchunk * J.nbg;
+    J.part_off = P.partial_total;
+    P.partial_total += 3 * (int64_t)J.nuy * J.nchunk;
+    J.out_idx = (int)P.jobs.size();
+    const double maxv = (double)((1 << P.pp[i].bd) - 1);
+    const double c1 = (0.01 * maxv) * (0.01 * maxv), c2 = (0.03 * maxv) * (0.03 * maxv);  // xSSIM.cpp:50-52
+    const double n2 = (double)(wnd * wnd) * (double)(wnd * wnd);
+    J.c1n = (float)(c1 * n2);
+    J.c2n = (float)(c2 * n2);
+    *slot = J.out_idx;
+    P.jobs.push_back(J);
+    P.job_a.push_back(a);
+    P.job_b.push_back(b);
+    return IVQM_OK;
+  };
+  for (int i = 0; i < n; i++) {
+    PairPlan& pp = P.pp[i];
+    if (m & IVQM_M_SSIM) RCCHK(add_job(i, &pp.ssim, ref_for(i, 0, 0), ref_for(i, 1, 0), pp.w, pp.h, 1, 0));
+    if (m & IVQM_M_IVSSIM) {
+      RCCHK(add_job(i, &pp.iv_t2r, ref_for(i, 0, 0), ref_for(i, 3, 0), pp.w, pp.h, 1, cfg_.erp));
+      RCCHK(add_job(i, &pp.iv_r2t, ref_for(i, 1, 0), ref_for(i, 2, 0), pp.w, pp.h, 1, cfg_.erp));
+    }
+    if (ms_any) {
+      for (int l = 0; l < pp.ns; l++) {
+        const int w = pp.w >> l, h = pp.h >> l, cl = (l == pp.ns - 1) ? 1 : 0;
+        if (m & IVQM_M_MSSSIM) RCCHK(add_job(i, &pp.ms[l], ref_for(i, 0, l), ref_for(i, 1, l), w, h, cl, 0));
+        if (m & IVQM_M_IVMSSSIM) {
+          RCCHK(add_job(i, &pp.ivms_t2r[l], ref_for(i, 0, l), ref_for(i, 3, l), w, h, cl, 0));
+          RCCHK(add_job(i, &pp.ivms_r2t[l], ref_for(i, 1, l), ref_for(i, 2, l), w, h, cl, 0));
+        }
+      }
+    }
+  }
+  return IVQM_OK;
+}
+
+int Engine::ensure_scp(int64_t elems) {
+  if (elems <= cap_scp_) return IVQM_OK;
+  RCCHK(sync());
+  cap_scp_ = grow(elems, cap_scp_);
+  RCCHK(dmalloc(&d_scp_, cap_scp_));
+  scp_valid_ = false;
+  drop_graphs();
+  return IVQM_OK;
+}
+
+int Engine::ensure_capacity(const BatchPlan& P) {
+  const int64_t njobs = (int64_t)P.jobs.size();
+  int64_t ndown = 0;
+  for (int l = 1; l < 5; l++) ndown += (int64_t)P.down[l].size();
+  const bool need = P.n > cap_pairs_ || P.rows_total > cap_rows_ || P.rowsd_total > cap_rowsd_ ||
+                    P.scp_total > cap_scp_ || P.pyr_total > cap_pyr_ || P.partial_total > cap_partial_ ||
+                    njobs > cap_jobs_ || ndown > cap_down_;
+  const int64_t best_need = best_on_ && P.n == 1 ? 2 * (int64_t)P.pp[0].w * P.pp[0].h : 0;
+  if (!need && best_need <= cap_best_) return IVQM_OK;
+  RCCHK(sync());
+  if (P.n > cap_pairs_) {
+    cap_pairs_ = grow(P.n, cap_pairs_);
+    RCCHK(dmalloc(&d_pairs_, cap_pairs_));
+    RCCHK(dmalloc(&d_gcd_, 4 * cap_pairs_));
+    RCCHK(dmalloc(&d_sd_, 3 * cap_pairs_));
+    RCCHK(dmalloc(&d_ssd_, 3 * cap_pairs_));
+  }
+  if (P.rows_total > cap_rows_) { cap_rows_ = grow(P.rows_total, cap_rows_); RCCHK(dmalloc(&d_rows_, cap_rows_)); }
+  if (P.rowsd_total > cap_rowsd_) {
+    cap_rowsd_ = grow(P.rowsd_total, cap_rowsd_);
+    RCCHK(dmalloc(&d_row_sd_, cap_rowsd_));
+    RCCHK(dmalloc(&d_row_ssd_, cap_rowsd_));
+  }
+  if (P.scp_total > cap_scp_) RCCHK(ensure_scp(P.scp_total));
+  if (P.pyr_total > cap_pyr_) { cap_pyr_ = grow(P.pyr_total, cap_pyr_); RCCHK(dmalloc(&d_pyr_, cap_pyr_)); }
+  if (P.partial_total > cap_partial_) {
+    cap_partial_ = grow(P.partial_total, cap_partial_);
+    RCCHK(dmalloc(&d_partial_, cap_partial_));
+  }
+  if (njobs > cap_jobs_) {
+    cap_jobs_ = grow(njobs, cap_jobs_);
+    RCCHK(dmalloc(&d_jobs_, cap_jobs_));
+    RCCHK(dmalloc(&d_ssim_out_, 6 * cap_jobs_));
+  }
+  if (ndown > cap_down_) { cap_down_ = grow(ndown, cap_down_); RCCHK(dmalloc(&d_down_, cap_down_)); }
+  if (best_need > cap_best_) { cap_best_ = best_need; RCCHK(dmalloc(&d_best_, cap_best_)); }
+  // device buffers moved: captured graphs are stale
+  drop_graphs();
+  return ensure_ring_capacity();
+}
+
+int Engine::ensure_ring_capacity() {
+  if (ring_pairs_ >= cap_pairs_ && ring_rows_ >= cap_rows_ && ring_rowsd_ >= cap_rowsd_ && ring_jobs_ >= cap_jobs_ &&
+      ring_down_ >= cap_down_)
+    return IVQM_OK;
+  for (auto& S : ring_) {
+    if (S.busy) { HIPCHK(hipEventSynchronize(S.ev)); S.busy = false; }
+    RCCHK(hmalloc(&S.t_pairs, cap_pairs_));
+    RCCHK(hmalloc(&S.t_jobs, cap_jobs_));
+    RCCHK(hmalloc(&S.t_down, cap_down_));
+    RCCHK(hmalloc(&S.r_gcd, 4 * cap_pairs_));
+    RCCHK(hmalloc(&S.r_sd, 3 * cap_pairs_));
+    RCCHK(hmalloc(&S.r_ssd, 3 * cap_pairs_));
+    RCCHK(hmalloc(&S.r_row_ssd, cap_rowsd_));
+    RCCHK(hmalloc(&S.r_rows, cap_rows_));
+    RCCHK(hmalloc(&S.r_ssim, 6 * cap_jobs_));
+  }
+  ring_pairs_ = cap_pairs_;
+  ring_rows_ = cap_rows_;
+  ring_rowsd_ = cap_rowsd_;
+  ring_jobs_ = cap_jobs_;
+  ring_down_ = cap_down_;
+  return IVQM_OK;
+}
+
+int Engine::materialize(const PicRef& r, PicDev& out) const {
+  if (r.arena == 0) {
+    out = r.user;
+    return IVQM_OK;
+  }
+  uint16_t* base = r.arena == 1 ? d_scp_ : d_pyr_;
+  const int64_t cap = r.arena == 1 ? cap_scp_ : cap_pyr_;
+  const int64_t plane = (int64_t)r.pitch * r.h;
+  if (!base || r.off < 0 || r.w > r.pitch || r.off + 3 * plane > cap)
+    return set_error(IVQM_EINVAL, "internal: arena picture outside its allocation");
+  for (int k = 0; k < 3; k++) {
+    out.p[k] = base + r.off + (int64_t)k * plane;
+    out.stride[k] = r.pitch;
+  }
+  out.cs = 0;
+  out.bps = 2;
+  return IVQM_OK;
+}
+
+// Writes the slot's pinned tables (pairs, SSIM jobs, pyramid jobs) with device pointers.
+int Engine::fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_first[5]) {
+  std::memcpy(S.t_pairs, P.pairs.data(), sizeof(PairDev) * P.n);
+  const int nj = (int)P.jobs.size();
+  for (int j = 0; j < nj; j++) {
+    S.t_jobs[j] = P.jobs[j];
+    RCCHK(materialize(P.job_a[j], S.t_jobs[j].a));
+    RCCHK(materialize(P.job_b[j], S.t_jobs[j].b));
+  }
+  int nd = 0;
+  for (int l = 0; l < 5; l++) down_first[l] = 0;
+  for (int l = 1; l < 5; l++) {
+    down_first[l] = nd;
+    for (size_t k = 0; k < P.down[l].size(); k++) {
+      DownJob d = P.down[l][k];
+      RCCHK(materialize(P.down_src[l][k], d.src));
+      PicDev dst;
+      RCCHK(materialize(P.down_dst[l][k], dst));
+      if (P.down_dst[l][k].arena != 2 || dst.stride[0] != d.pitch)
+        return set_error(IVQM_EINVAL, "internal: pyramid destination mismatch");
+      d.dst = (uint16_t*)dst.p[0];
+      S.t_down[nd++] = d;
+    }
+  }
+  *ndown = nd;
+  return IVQM_OK;
+}
+
+int Engine::issue(const BatchPlan& P, RingSlot& S) {
+  const hipStream_t st = stream_;
+  const int n = P.n;
+  const int nj = (int)P.jobs.size();
+  int nd = 0, down_first[5];
+  RCCHK(fill_tables(P, S, &nd, down_first));
+  HIPCHK(hipMemcpyAsync(d_pairs_, S.t_pairs, sizeof(PairDev) * n, hipMemcpyHostToDevice, st));
+  if (nj) HIPCHK(hipMemcpyAsync(d_jobs_, S.t_jobs, sizeof(SsimJobDev) * nj, hipMemcpyHostToDevice, st));
+  if (nd) HIPCHK(hipMemcpyAsync(d_down_, S.t_down, sizeof(DownJob) * nd, hipMemcpyHostToDevice, st));
+  if (P.need_stats) {
+    HIPCHK(launch_stats(d_pairs_, n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, st));
+    HIPCHK(launch_gcd_finalize(d_pairs_, n, d_row_sd_, d_row_ssd_, d_gcd_, d_sd_, d_ssd_, st));
+  }
+  if (P.need_search)
+    HIPCHK(launch_search(d_pairs_, n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
+                         d_scp_, d_best_, st));
+  for (int l = 1; l < 5; l++)
+    if (!P.down[l].empty())
+      HIPCHK(launch_downsample(d_down_ + down_first[l], (int)P.down[l].size(), P.down_blocks[l], st));
+  if (nj) {
+    HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, st));
+    HIPCHK(launch_ssim_finalize(d_jobs_, nj, d_partial_, d_erp_, cfg_.ssim_window >> 1, cfg_.ssim_stride,
+                                d_ssim_out_, st));
+  }
+  if (P.need_stats) {
+    HIPCHK(hipMemcpyAsync(S.r_gcd, d_gcd_, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(S.r_sd, d_sd_, sizeof(int64_t) * 3 * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(S.r_ssd, d_ssd_, sizeof(uint64_t) * 3 * n, hipMemcpyDeviceToHost, st));
+    if ((P.metrics & IVQM_M_WSPSNR) && cfg_.erp)
+      HIPCHK(hipMemcpyAsync(S.r_row_ssd, d_row_ssd_, sizeof(uint64_t) * P.rowsd_total, hipMemcpyDeviceToHost, st));
+  }
+  if (P.metrics & IVQM_M_IVPSNR)
+    HIPCHK(hipMemcpyAsync(S.r_rows, d_rows_, sizeof(uint32_t) * P.rows_total, hipMemcpyDeviceToHost, st));
+  if (nj) HIPCHK(hipMemcpyAsync(S.r_ssim, d_ssim_out_, sizeof(double) * 6 * nj, hipMemcpyDeviceToHost, st));
+  return IVQM_OK;
+}
+
+int Engine::enqueue(const PairSpec* pairs, int n, uint32_t metrics, int ring, bool use_graph) {
+  if (n <= 0) return set_error(IVQM_EINVAL, "empty batch");
+  if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+  RingSlot& S = ring_[ring];
+  if (S.busy) {
+    HIPCHK(hipEventSynchronize(S.ev));
+    S.busy = false;
+  }
+  BatchPlan P;
+  RCCHK(plan(pairs, n, metrics, P));
+  RCCHK(ensure_capacity(P));
+  // pointer identity enters the graph signature: a replay reuses the captured tables' device pointers
+  // only through the staging copy, so the signature needs shapes only
+  if (use_graph) {
+    if (S.graph && S.graph_sig != P.signature) {
+      (void)hipGraphExecDestroy(S.graph);
+      S.graph = nullptr;
+    }
+    if (!S.graph) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      int rc = issue(P, S);
+      hipError_t e = hipStreamEndCapture(stream_, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { S.graph = nullptr; return hip_error(e, "hipGraphInstantiate"); }
+      S.graph_sig = P.signature;
+    } else {
+      // same shapes: refresh the staging tables the graph's upload nodes read
+      int nd = 0, down_first[5];
+      RCCHK(fill_tables(P, S, &nd, down_first));
+    }
+    HIPCHK(hipGraphLaunch(S.graph, stream_));
+  } else {
+    RCCHK(issue(P, S));
+  }
+  HIPCHK(hipEventRecord(S.ev, stream_));
+  S.busy = true;
+  S.plan = P;
+  last_ = P;
+  if (P.need_scp) scp_valid_ = (n == 1);
+  return IVQM_OK;
+}
+
+namespace {
+double ssim_value(const BatchPlan& P, const double* out, int idx, int comp) {
+  const SsimJobDev& J = P.jobs[idx];
+  const double sum = out[(size_t)idx * 6 + comp * 2 + 0];
+  const double wsum = out[(size_t)idx * 6 + comp * 2 + 1];
+  const double active = (double)((int64_t)J.nuy * J.nux);
+  if (J.erp) return sum / (active * (wsum / J.nuy));  // xIVSSIM.cpp:158-163
+  return sum / active;                                 // xSSIM.cpp:233-236
+}
+// libs/pmbb-ivqm/src/xSSIM.cpp:132-140: product of ReLU(score)^exponent
+void ms_combine(const BatchPlan& P, const double* out, int ns, const int* idx, double v4[4]) {
+  for (int comp = 0; comp < 3; comp++) {
+    double v = 1.0;
+    for (int l = 0; l < ns; l++) {
+      const double s = idx[l] >= 0 ? ssim_value(P, out, idx[l], comp) : 0.0;
+      v *= pow(s > 0 ? s : 0.0, kMsExp[ns - 1][l]);
+    }
+    v4[comp] = v;
+  }
+  double v3 = 1.0;  // lane 3 starts at 1 and is multiplied by pow(ReLU(0), w_i)
+  for (int l = 0; l < ns; l++) v3 *= pow(0.0, kMsExp[ns - 1][l]);
+  v4[3] = v3;
+}
+}  // namespace
+
+int Engine::collect(int ring, ivqm_frame_result* out) {
+  if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+  RingSlot& S = ring_[ring];
+  if (!S.busy) return set_error(IVQM_ESTATE, "ring slot has no pending batch");
+  HIPCHK(hipEventSynchronize(S.ev));
+  S.busy = false;
+  const BatchPlan& P = S.plan;
+  const uint32_t m = P.metrics;
+  const int32_t* cwa = cfg_.cmp_weights_average;
+  for (int i = 0; i < P.n; i++) {
+    const PairPlan& pp = P.pp[i];
+    const int W = pp.w, H = pp.h, bd = pp.bd;
+    const int64_t area = (int64_t)W * H;
+    ivqm_frame_result r;
+    std::memset(&r, 0, sizeof(r));
+    r.metrics = m;
+    if (P.need_stats) {
+      for (int k = 0; k < 4; k++) r.gcd_r2t[k] = S.r_gcd[4 * i + k];
+      for (int k = 0; k < 3; k++) {
+        r.ssd[k] = S.r_ssd[3 * i + k];
+        r.exact[k] = r.ssd[k] == 0;
+      }
+    }
+    const double fake = host::psnr_from_ssd(1.0, area, bd);
+    if (m & (IVQM_M_PSNR | IVQM_M_MSE)) {
+      // libs/pmbb-ivqm/src/xPSNR.cpp:131-151; exact components -> fake PSNR (xAppQMIV.cpp:974-981)
+      for (int k = 0; k < 4; k++) { r.mse[k] = DBL_MAX; r.psnr[k] = DBL_MAX; }
+      for (int k = 0; k < 3; k++) {
+        const double ssd = (double)r.ssd[k];
+        r.mse[k] = ssd / (double)area;
+        r.psnr[k] = host::psnr_from_ssd(ssd, area, bd);
+        if (cfg_.fake_vals_for_exact && r.exact[k]) r.psnr[k] = fake;
+      }
+      r.psnr_pic = host::pic_value(r.psnr, cwa);
+    }
+    if (m & IVQM_M_WSPSNR) {
+      for (int k = 0; k < 4; k++) r.wspsnr[k] = DBL_MAX;
+      if (!cfg_.erp) {
+        for (int k = 0; k < 3; k++) r.wspsnr[k] = host::psnr_from_ssd((double)r.ssd[k], area, bd);
+      } else {
+        // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a 4:2:0 chroma plane row counts twice per luma row
+        const auto& hw = erp_host.at(H);
+        const uint64_t* rs = S.r_row_ssd + pp.rowsd_off;
+        for (int k = 0; k < 3; k++) {
+          host::Kbns kb;
+          for (int y = 0; y < H; y++) {
+            const uint64_t row = (k && pp.cs) ? 2 * rs[(size_t)k * H + (y >> 1)] : rs[(size_t)k * H + y];
+            kb.acc((double)row * hw.first[y]);
+          }
+          const double err = kb.result() * hw.second;
+          r.wspsnr[k] = host::psnr_from_ssd(err, area, bd);
+          if (cfg_.fake_vals_for_exact && err == 0) r.wspsnr[k] = fake;
+        }
+      }
+      for (int k = 0; k < 3; k++)
+        if (cfg_.fake_vals_for_exact && r.exact[k]) r.wspsnr[k] = fake;
+      r.wspsnr_pic = host::pic_value(r.wspsnr, cwa);
+    }
+    if (m & IVQM_M_IVPSNR) {
+      const uint32_t* rows = S.r_rows + pp.rows_off;
+      const double* ew = nullptr;
+      if (cfg_.erp) {
+        auto it = erp_host.find(H);
+        if (it == erp_host.end()) {
+          auto& hw = erp_host[H];
+          host::ws_weights(H, cfg_.lat_range_deg, hw.first, hw.second);
+          it = erp_host.find(H);
+        }
+        ew = it->second.first.data();
+      }
+      r.ivpsnr_r2t = host::ivpsnr_quality(rows, W, H, bd, ew, cwa);
+      r.ivpsnr_t2r = host::ivpsnr_quality(rows + (size_t)3 * H, W, H, bd, ew, cwa);
+      r.ivpsnr = std::min(r.ivpsnr_r2t, r.ivpsnr_t2r);  // xIVPSNR.cpp:55 (xMin = std::min)
+    }
+    const double* so = S.r_ssim;
+    if ((m & IVQM_M_SSIM) && pp.ssim >= 0) {
+      for (int k = 0; k < 3; k++) r.ssim[k] = ssim_value(P, so, pp.ssim, k);
+      r.ssim_pic = host::pic_value(r.ssim, cwa);
+    }
+    if ((m & IVQM_M_MSSSIM) && pp.ns >= 1) {
+      ms_combine(P, so, pp.ns, pp.ms, r.msssim);
+      r.msssim_pic = host::pic_value(r.msssim, cwa);
+    }
+    if ((m & IVQM_M_IVSSIM) && pp.iv_t2r >= 0) {
+      double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 3; k++) {
+        a[k] = ssim_value(P, so, pp.iv_t2r, k);
+        b[k] = ssim_value(P, so, pp.iv_r2t, k);
+      }
+      r.ivssim_t2r = host::weighted_mean4(a, cwa);
+      r.ivssim_r2t = host::weighted_mean4(b, cwa);
+      r.ivssim = std::min(r.ivssim_t2r, r.ivssim_r2t);  // xIVSSIM.cpp:56
+    }
+    if ((m & IVQM_M_IVMSSSIM) && pp.ns >= 1) {
+      double a[4], b[4];
+      ms_combine(P, so, pp.ns, pp.ivms_t2r, a);
+      ms_combine(P, so, pp.ns, pp.ivms_r2t, b);
+      a[3] = 0.0;
+      b[3] = 0.0;
+      r.ivmsssim_t2r = host::weighted_mean4(a, cwa);
+      r.ivmsssim_r2t = host::weighted_mean4(b, cwa);
+      r.ivmsssim = std::min(r.ivmsssim_t2r, r.ivmsssim_r2t);  // xIVSSIM.cpp:87
+    }
+    if (out) out[i] = r;
+  }
+  return IVQM_OK;
+}
+
+int Engine::profile(int kernel, int iters, double* avg_ms) {
+  const BatchPlan& P = last_;
+  if (P.n <= 0) return set_error(IVQM_ESTATE, "no batch enqueued yet");
+  RCCHK(sync());
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, stream_));
+  const int nj = (int)P.jobs.size();
+  for (int i = 0; i < iters; i++) {
+    switch (kernel) {
+      case 0: HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, stream_)); break;
+      case 1:
+        HIPCHK(launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
+                             d_scp_, d_best_, stream_));
+        break;
+      case 2: HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, stream_)); break;
+      default: return set_error(IVQM_EINVAL, "unknown kernel id");
+    }
+  }
+  HIPCHK(hipEventRecord(e1, stream_));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / iters;
+  return IVQM_OK;
+}
+
+}  // namespace ivqm

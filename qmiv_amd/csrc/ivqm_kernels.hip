@@ -1,32 +1,36 @@
 // gfx950 (CDNA4) kernels of the QMIV metric hot path.
 //
-// Kernels:
+// Every kernel reads a table of frame pairs (or jobs) from device memory and
+// flattens all of them into one grid, so one launch per phase serves a whole
+// batch of pairs of mixed sizes and sample formats; blocks find their table
+// entry by a binary search over the entries' first-block indices (uniform per
+// block, so it runs on the scalar unit).
+//
 //   k_stats          per plane row: sum(P1 - P0) and sum((P0 - P1)^2)
-//                    (GCD input, PSNR / WS-PSNR row SSDs); zeroes the search
-//                    row accumulators as a side job.
-//   k_gcd_finalize   frame totals + global colour difference (FP64 mean,
-//                    round half away, clip), written to device memory.
-//   k_search         corresponding-pixel-shift search, both directions in one
-//                    grid, LDS-staged reference tile with a +-R halo, per-row
-//                    int32-wrapping distance sums and the shift-compensated
-//                    pictures as a fused epilogue.
+//                    (GCD input, PSNR / WS-PSNR row SSDs); zeroes the pair's
+//                    search row accumulators as a side job.
+//   k_gcd_finalize   per pair: frame totals + global colour difference (FP64
+//                    mean, round half away, clip), kept in device memory.
+//   k_search         corresponding-pixel-shift search, both directions, LDS
+//                    tile with a +-R halo, int32-wrapping row sums, fused
+//                    shift-compensated-picture epilogue.
 //   k_ssim           block-averaged 8x8 / stride-4 SSIM: exact integer 4x4
 //                    sub-block moments, 2x2 combination, FP32 ratios, FP64
-//                    per-window-row partial sums; many (picture pair, level)
-//                    jobs per launch.
+//                    per-window-row partial sums.
 //   k_ssim_finalize  FP64 frame totals per job and component (ERP-weighted
 //                    for IV-SSIM).
 //   k_downsample     2x2 rounded-mean pyramid level (MS-SSIM).
-//   k_upsample420    4:2:0 -> 4:4:4 nearest replication.
+//   k_upsample420    4:2:0 -> 4:4:4 nearest replication (standalone).
 //
 // 4:2:0 input is never materialised at 4:4:4 by the metric kernels: a chroma
-// sample at luma-grid (x, y) is read at (x >> 1, y >> 1), which is exactly the
-// value 2x2 replication (libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49) would have
-// stored there. The reference's Nearest margin (xMarginOps.h:40-73) is
-// reproduced by clamping coordinates.
+// sample at luma-grid (x, y) is read at (x >> 1, y >> 1), exactly the value
+// 2x2 replication (libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49) stores there.
+// The reference's Nearest margin (xMarginOps.h:40-73) is reproduced by
+// clamping coordinates.
 #include "ivqm_device.h"
 
 #include <limits.h>
+
 #include <type_traits>
 
 namespace ivqm {
@@ -53,26 +57,31 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
+// Index of the last table entry whose first-block index is <= blk.
+template <typename E, typename F>
+__device__ __forceinline__ int find_entry(const E* __restrict__ t, int n, int blk, F first) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first(t[mid]) <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------------------
-// k_stats: one 256-thread block per plane row.
+// k_stats: one 256-thread block per plane row of some pair.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_stats(PicDev P0, PicDev P1, int W, int H, int64_t* __restrict__ row_sd,
-                                               uint64_t* __restrict__ row_ssd, uint32_t* __restrict__ zero_buf,
-                                               int n_zero) {
+__device__ void stats_row(const PairDev& P, int lb, int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd) {
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  {
-    const int z = b * 256 + tid;
-    if (z < n_zero) zero_buf[z] = 0u;
-  }
-  const int Hc = H >> P0.cs, Wc = W >> P0.cs;
+  const int H = P.h, W = P.w, Hc = H >> P.p0.cs, Wc = W >> P.p0.cs;
   int c, y, wp;
-  if (b < H) { c = 0; y = b; wp = W; }
-  else if (b < H + Hc) { c = 1; y = b - H; wp = Wc; }
-  else { c = 2; y = b - H - Hc; wp = Wc; }
-  const T* a = (const T*)P0.p[c] + (size_t)y * P0.stride[c];
-  const T* r = (const T*)P1.p[c] + (size_t)y * P1.stride[c];
+  if (lb < H) { c = 0; y = lb; wp = W; }
+  else if (lb < H + Hc) { c = 1; y = lb - H; wp = Wc; }
+  else { c = 2; y = lb - H - Hc; wp = Wc; }
+  const T* a = (const T*)P.p0.p[c] + (size_t)y * P.p0.stride[c];
+  const T* r = (const T*)P.p1.p[c] + (size_t)y * P.p1.stride[c];
   int64_t sd = 0;
   uint64_t ssd = 0;
   for (int x = tid; x < wp; x += 256) {
@@ -90,29 +99,46 @@ __global__ __launch_bounds__(256) void k_stats(PicDev P0, PicDev P1, int W, int 
   if ((tid & 63) == 0) { s_sd[tid >> 6] = sd; s_ssd[tid >> 6] = ssd; }
   __syncthreads();
   if (tid == 0) {
-    row_sd[(size_t)c * H + y] = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
-    row_ssd[(size_t)c * H + y] = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
+    row_sd[P.rowsd_off + (int64_t)c * H + y] = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
+    row_ssd[P.rowsd_off + (int64_t)c * H + y] = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
   }
 }
 
+__global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs,
+                                               int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
+                                               uint32_t* __restrict__ rows) {
+  const int b = blockIdx.x;
+  const int pi = find_entry(pairs, npairs, b, [](const PairDev& p) { return p.stat_blk0; });
+  const PairDev P = pairs[pi];
+  const int lb = b - P.stat_blk0;
+  {
+    const int z = lb * 256 + (int)threadIdx.x;
+    if (z < 6 * P.h) rows[P.rows_off + z] = 0u;
+  }
+  if (P.p0.bps == 2) stats_row<uint16_t>(P, lb, row_sd, row_ssd);
+  else stats_row<uint8_t>(P, lb, row_sd, row_ssd);
+}
+
 // ---------------------------------------------------------------------------
-// k_gcd_finalize: one block. libs/pmbb-ivqm/src/xGlobClrDiff.cpp:23-56 with
-// the app's argument order (sum of P1 - P0), chroma sums of 4:2:0 input x4.
+// k_gcd_finalize: one block per pair. libs/pmbb-ivqm/src/xGlobClrDiff.cpp:23-56
+// with the app's argument order (sum of P1 - P0); chroma sums of 4:2:0 x4.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gcd_finalize(const int64_t* __restrict__ row_sd,
-                                                      const uint64_t* __restrict__ row_ssd, int W, int H, int cs,
-                                                      int md0, int md1, int md2, int wrap32, int32_t* __restrict__ gcd,
+__global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict__ pairs,
+                                                      const int64_t* __restrict__ row_sd,
+                                                      const uint64_t* __restrict__ row_ssd, int32_t* __restrict__ gcd,
                                                       int64_t* __restrict__ sd_out, uint64_t* __restrict__ ssd_out) {
+  const PairDev P = pairs[blockIdx.x];
   __shared__ int64_t s_sd[4];
   __shared__ uint64_t s_ssd[4];
   const int tid = threadIdx.x;
+  const int W = P.w, H = P.h, cs = P.p0.cs;
   for (int c = 0; c < 3; c++) {
     const int rows = c ? (H >> cs) : H;
     int64_t sd = 0;
     uint64_t ssd = 0;
     for (int y = tid; y < rows; y += 256) {
-      sd += row_sd[(size_t)c * H + y];
-      ssd += row_ssd[(size_t)c * H + y];
+      sd += row_sd[P.rowsd_off + (int64_t)c * H + y];
+      ssd += row_ssd[P.rowsd_off + (int64_t)c * H + y];
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -125,16 +151,21 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const int64_t* __restrict_
       int64_t t = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
       uint64_t q = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
       if (c && cs) { t *= 4; q *= 4; }
-      sd_out[c] = t;
-      ssd_out[c] = q;
-      if (wrap32) t = (int64_t)(int32_t)(uint32_t)(uint64_t)t;
-      const double avg = (double)t / (double)((int64_t)W * (int64_t)H);
-      const double rnd = avg + copysign(0.5, avg);
-      int g = (int)rnd;
-      const int md = c == 0 ? md0 : (c == 1 ? md1 : md2);
-      g = g < -md ? -md : (g > md ? md : g);
-      gcd[c] = g;
-      if (c == 2) gcd[3] = 0;
+      sd_out[3 * P.res_idx + c] = t;
+      ssd_out[3 * P.res_idx + c] = q;
+      if (P.gcd_given) {
+        gcd[4 * P.res_idx + c] = P.gcd_in[c];
+        if (c == 2) gcd[4 * P.res_idx + 3] = 0;
+      } else {
+        if (P.wrap32) t = (int64_t)(int32_t)(uint32_t)(uint64_t)t;
+        const double avg = (double)t / (double)((int64_t)W * (int64_t)H);
+        const double rnd = avg + copysign(0.5, avg);
+        int g = (int)rnd;
+        const int md = P.md[c];
+        g = g < -md ? -md : (g > md ? md : g);
+        gcd[4 * P.res_idx + c] = g;
+        if (c == 2) gcd[4 * P.res_idx + 3] = 0;
+      }
     }
     __syncthreads();
   }
@@ -162,28 +193,31 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const int64_t* __restrict_
 // precomputed per reference pixel. Nothing wraps in this range, so
 // argmin E == argmin E'; the candidate's raster index is packed into the low
 // KB bits of key = 2^KB * E' + idx, which makes a plain integer min equal to
-// "first strict minimum in raster order". Per candidate: three v_mad_i32_i24
-// and one min.
+// "first strict minimum in raster order".
 // ---------------------------------------------------------------------------
-template <int R> struct KeyBits { static constexpr int v = (2 * R + 1) * (2 * R + 1) <= 16 ? 4 : (2 * R + 1) * (2 * R + 1) <= 32 ? 5 : (2 * R + 1) * (2 * R + 1) <= 64 ? 6 : 7; };
+template <int R> struct KeyBits {
+  static constexpr int n = (2 * R + 1) * (2 * R + 1);
+  static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
+};
+
+template <int R, int PY> struct SearchTile {
+  static constexpr int TW = kSearchTileW, TH = 4 * PY, LW = TW + 2 * R, LH = TH + 2 * R;
+};
 
 template <typename T, int R, int PY, bool NARROW>
-__global__ __launch_bounds__(256) void k_search(PicDev P0, PicDev P1, int W, int H, int maxv,
-                                                const int32_t* __restrict__ gcd, int wY, int wCb, int wCr,
-                                                uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
-                                                int scp_pitch, uint8_t* __restrict__ best) {
-  constexpr int TW = 64, TH = 4 * PY, D = 2 * R + 1;
-  constexpr int LW = TW + 2 * R, LH = TH + 2 * R;
+__device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*lds)[SearchTile<R, PY>::LW],
+                            int wY, int wCb, int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
+                            uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
+  constexpr int D = 2 * R + 1;
+  constexpr int LW = SearchTile<R, PY>::LW, LH = SearchTile<R, PY>::LH;
   constexpr int KB = KeyBits<R>::v, S = 1 << KB;
-  __shared__ int4 lds[LH][LW];
-
-  const int dir = blockIdx.z;  // 0: R2T (Tst = P0, Ref = P1, +gcd); 1: T2R (Tst = P1, Ref = P0, -gcd)
-  const PicDev Tst = dir ? P1 : P0;
-  const PicDev Ref = dir ? P0 : P1;
-  const int g0 = dir ? -gcd[0] : gcd[0];
-  const int g1 = dir ? -gcd[1] : gcd[1];
-  const int g2 = dir ? -gcd[2] : gcd[2];
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int W = Pr.w, H = Pr.h;
+  const PicDev& Tst = dir ? Pr.p1 : Pr.p0;  // dir 0: R2T (Tst = P0, Ref = P1, +gcd); dir 1: T2R (-gcd)
+  const PicDev& Ref = dir ? Pr.p0 : Pr.p1;
+  const int32_t* g4 = gcd + 4 * Pr.res_idx;
+  const int g0 = dir ? -g4[0] : g4[0];
+  const int g1 = dir ? -g4[1] : g4[1];
+  const int g2 = dir ? -g4[2] : g4[2];
 
   for (int i = threadIdx.x; i < LW * LH; i += 256) {
     const int ly = i / LW, lx = i - ly * LW;
@@ -265,10 +299,12 @@ __global__ __launch_bounds__(256) void k_search(PicDev P0, PicDev P1, int W, int
     }
   }
 
-  // Epilogue: winner's squared differences, row sums, SCP, debug map.
-  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0); planes [c][H][scp_pitch].
-  const size_t plane_sz = (size_t)scp_pitch * H;
-  uint16_t* scp_pic = scp ? scp + (size_t)(1 - dir) * 3 * plane_sz : nullptr;
+  // Epilogue: winner's squared differences, row sums, SCP, winner map.
+  uint32_t* prow = rows + Pr.rows_off + (size_t)dir * 3 * H;
+  const size_t plane_sz = (size_t)Pr.scp_pitch * H;
+  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0)
+  uint16_t* scp_pic = Pr.scp_off >= 0 ? scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz : nullptr;
+  uint8_t* bmap = Pr.best_off >= 0 ? best + Pr.best_off + (size_t)dir * W * H : nullptr;
 #pragma unroll
   for (int i = 0; i < PY; i++) {
     const int y = yb + i;
@@ -283,19 +319,42 @@ __global__ __launch_bounds__(256) void k_search(PicDev P0, PicDev P1, int W, int
     q1 = wave_sum_u32(q1);
     q2 = wave_sum_u32(q2);
     if (tx == 0 && y < H) {
-      atomicAdd(&rows[((size_t)dir * 3 + 0) * H + y], q0);
-      atomicAdd(&rows[((size_t)dir * 3 + 1) * H + y], q1);
-      atomicAdd(&rows[((size_t)dir * 3 + 2) * H + y], q2);
+      atomicAdd(&prow[0 * (size_t)H + y], q0);
+      atomicAdd(&prow[1 * (size_t)H + y], q1);
+      atomicAdd(&prow[2 * (size_t)H + y], q2);
     }
     if (ok) {
       if (scp_pic) {
-        const size_t o = (size_t)y * scp_pitch + x;
-        scp_pic[o] = (uint16_t)clampi(r.y - g0, 0, maxv);
-        scp_pic[plane_sz + o] = (uint16_t)clampi(r.z - g1, 0, maxv);
-        scp_pic[2 * plane_sz + o] = (uint16_t)clampi(r.w - g2, 0, maxv);
+        const size_t o = (size_t)y * Pr.scp_pitch + x;
+        scp_pic[o] = (uint16_t)clampi(r.y - g0, 0, Pr.maxv);
+        scp_pic[plane_sz + o] = (uint16_t)clampi(r.z - g1, 0, Pr.maxv);
+        scp_pic[2 * plane_sz + o] = (uint16_t)clampi(r.w - g2, 0, Pr.maxv);
       }
-      if (best) best[(size_t)dir * W * H + (size_t)y * W + x] = (uint8_t)bi[i];
+      if (bmap) bmap[(size_t)y * W + x] = (uint8_t)bi[i];
     }
+  }
+}
+
+template <int R, int PY>
+__global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pairs, int npairs, int wY, int wCb,
+                                                int wCr, const int32_t* __restrict__ gcd,
+                                                uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
+                                                uint8_t* __restrict__ best) {
+  using TT = SearchTile<R, PY>;
+  __shared__ int4 lds[TT::LH][TT::LW];
+  const int tile = blockIdx.x, dir = blockIdx.y;
+  const int pi = find_entry(pairs, npairs, tile, [](const PairDev& p) { return p.tile0; });
+  const PairDev P = pairs[pi];
+  const int lt = tile - P.tile0;
+  const int ntx = (P.w + TT::TW - 1) / TT::TW;
+  const int ty = lt / ntx, tx = lt - ty * ntx;
+  const int x0 = tx * TT::TW, y0 = ty * TT::TH;
+  if (P.p0.bps == 2) {
+    if (P.narrow) search_tile<uint16_t, R, PY, true>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else search_tile<uint16_t, R, PY, false>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+  } else {
+    if (P.narrow) search_tile<uint8_t, R, PY, true>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else search_tile<uint8_t, R, PY, false>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   }
 }
 
@@ -326,13 +385,12 @@ __device__ __forceinline__ void load4(const T* __restrict__ p, int stride, int x
   }
 }
 
-template <typename TA, int CSA, typename TB, int CSB, bool WSQ>
-__device__ void ssim_wave(const SsimJob& J, int c, int chunk, int band, int lane, float c1n, float c2n,
+template <typename TA, int CSA, typename TB, int CSB, bool WSQ, int NB>
+__device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
                           double* __restrict__ partial) {
-  constexpr int NB = 8;
   using SQ = typename std::conditional<WSQ, uint64_t, uint32_t>::type;
   const int wy0 = band * NB;
-  if (wy0 >= J.nuy || chunk >= J.nchunk) return;
+  if (wy0 >= J.nuy) return;
   const int bx = chunk * 63 + lane;
   const bool col_ok = bx <= J.nux;
   const TA* pa = (const TA*)J.a.p[c];
@@ -382,50 +440,63 @@ __device__ void ssim_wave(const SsimJob& J, int c, int chunk, int band, int lane
         s = (double)cs;
       }
       s = wave_sum_f64(s);
-      if (lane == 0) partial[(size_t)J.part_off + ((size_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
+      if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
     }
     pA = A; pB = B; pAA = AA; pBB = BB; pAB = AB;
   }
 }
 
-template <bool WSQ>
-__global__ __launch_bounds__(256) void k_ssim(SsimJobTable T, float c1n, float c2n, double* __restrict__ partial) {
-  const int j = blockIdx.z / 3, c = blockIdx.z - 3 * (blockIdx.z / 3);
-  const SsimJob& J = T.job[j];
-  const int lane = threadIdx.x & 63, band = blockIdx.y * 4 + (threadIdx.x >> 6);
+template <bool WSQ, int NB>
+__device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
+                              double* partial) {
   const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
-  const int chunk = blockIdx.x;
   if (J.a.bps == 2 && J.b.bps == 2) {
-    if (csa == 0 && csb == 0) ssim_wave<uint16_t, 0, uint16_t, 0, WSQ>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 1) ssim_wave<uint16_t, 1, uint16_t, 1, WSQ>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 0) ssim_wave<uint16_t, 1, uint16_t, 0, WSQ>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else ssim_wave<uint16_t, 0, uint16_t, 1, WSQ>(J, c, chunk, band, lane, c1n, c2n, partial);
+    if (csa == 0 && csb == 0) ssim_wave<uint16_t, 0, uint16_t, 0, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else if (csa == 1 && csb == 1) ssim_wave<uint16_t, 1, uint16_t, 1, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else if (csa == 1 && csb == 0) ssim_wave<uint16_t, 1, uint16_t, 0, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else ssim_wave<uint16_t, 0, uint16_t, 1, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
   } else if (!WSQ) {
     if (J.a.bps == 1 && J.b.bps == 1) {
-      if (csa == 0 && csb == 0) ssim_wave<uint8_t, 0, uint8_t, 0, false>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint8_t, 1, false>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0 && csb == 0) ssim_wave<uint8_t, 0, uint8_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      else ssim_wave<uint8_t, 1, uint8_t, 1, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
     } else if (J.a.bps == 1) {
-      if (csa == 0) ssim_wave<uint8_t, 0, uint16_t, 0, false>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint16_t, 0, false>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0) ssim_wave<uint8_t, 0, uint16_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      else ssim_wave<uint8_t, 1, uint16_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
     }
   }
 }
 
+template <int NB>
+__global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ jobs, int njobs,
+                                              double* __restrict__ partial) {
+  const int b = blockIdx.x;
+  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
+  const SsimJobDev J = jobs[ji];
+  const int lb = b - J.blk0;
+  const int per_c = J.nchunk * J.nbg;
+  const int c = lb / per_c, rem = lb - c * per_c;
+  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
+  const int lane = threadIdx.x & 63, band = bg * 4 + (threadIdx.x >> 6);
+  if (J.wsq) ssim_dispatch<true, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  else ssim_dispatch<false, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+}
+
 // One block per (job, component): FP64 frame totals of the window-row sums.
 // IV-SSIM ERP weighting follows libs/pmbb-ivqm/src/xIVSSIM.cpp:148-165.
-__global__ __launch_bounds__(256) void k_ssim_finalize(SsimJobTable T, const double* __restrict__ partial,
+__global__ __launch_bounds__(256) void k_ssim_finalize(const SsimJobDev* __restrict__ jobs,
+                                                       const double* __restrict__ partial,
                                                        const double* __restrict__ erp_w, int half_wnd, int stride,
                                                        double* __restrict__ out) {
   const int j = blockIdx.x / 3, c = blockIdx.x - 3 * (blockIdx.x / 3);
-  const SsimJob& J = T.job[j];
+  const SsimJobDev J = jobs[j];
   const int tid = threadIdx.x;
   double acc = 0.0, accw = 0.0;
   for (int wy = tid; wy < J.nuy; wy += 256) {
-    const double* p = partial + (size_t)J.part_off + ((size_t)c * J.nuy + wy) * J.nchunk;
+    const double* p = partial + J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk;
     double rs = 0.0;
     for (int k = 0; k < J.nchunk; k++) rs += p[k];
-    if (J.erp && erp_w) {
-      const double wt = erp_w[wy * stride + half_wnd];
+    if (J.erp && J.erp_off >= 0) {
+      const double wt = erp_w[J.erp_off + wy * stride + half_wnd];
       rs *= wt;
       accw += wt;
     }
@@ -437,21 +508,43 @@ __global__ __launch_bounds__(256) void k_ssim_finalize(SsimJobTable T, const dou
   if ((tid & 63) == 0) { s_a[tid >> 6] = acc; s_w[tid >> 6] = accw; }
   __syncthreads();
   if (tid == 0) {
-    out[((size_t)j * 3 + c) * 2 + 0] = (s_a[0] + s_a[1]) + (s_a[2] + s_a[3]);
-    out[((size_t)j * 3 + c) * 2 + 1] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+    out[(int64_t)J.out_idx * 6 + c * 2 + 0] = (s_a[0] + s_a[1]) + (s_a[2] + s_a[3]);
+    out[(int64_t)J.out_idx * 6 + c * 2 + 1] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
   }
 }
 
 // 2x2 rounded mean: libs/pmbb-core/src/xPixelOpsSTD.cpp:50-67.
 template <typename T>
-__global__ __launch_bounds__(256) void k_downsample(PicDev S, int dw, int dh, uint16_t* __restrict__ dst, int pitch) {
+__device__ __forceinline__ void down_px(const PicDev& S, int c, int x, int y, uint16_t* dst, int pitch, int dh) {
+  const int v = ldpix<T>(S, c, 2 * x, 2 * y) + ldpix<T>(S, c, 2 * x + 1, 2 * y) + ldpix<T>(S, c, 2 * x, 2 * y + 1) +
+                ldpix<T>(S, c, 2 * x + 1, 2 * y + 1) + 2;
+  dst[(size_t)c * pitch * dh + (size_t)y * pitch + x] = (uint16_t)(v >> 2);
+}
+
+__global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ jobs, int njobs) {
+  const int b = blockIdx.x;
+  const int ji = find_entry(jobs, njobs, b, [](const DownJob& j) { return j.blk0; });
+  const DownJob J = jobs[ji];
+  const int dw = J.w >> 1, dh = J.h >> 1;
+  const int nbx = (dw + kDownBlkW - 1) / kDownBlkW, nby = (dh + kDownBlkH - 1) / kDownBlkH;
+  int lb = b - J.blk0;
+  const int c = lb / (nbx * nby);
+  lb -= c * nbx * nby;
+  const int by = lb / nbx, bx = lb - by * nbx;
+  const int x = bx * kDownBlkW + (threadIdx.x & 63);
+  const int y = by * kDownBlkH + (threadIdx.x >> 6);
+  if (x >= dw || y >= dh) return;
+  if (J.src.bps == 2) down_px<uint16_t>(J.src, c, x, y, J.dst, J.pitch, dh);
+  else down_px<uint8_t>(J.src, c, x, y, J.dst, J.pitch, dh);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_downsample_one(PicDev S, int dw, int dh, uint16_t* __restrict__ dst, int pitch) {
   const int c = blockIdx.z;
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= dw || y >= dh) return;
-  const int v = ldpix<T>(S, c, 2 * x, 2 * y) + ldpix<T>(S, c, 2 * x + 1, 2 * y) + ldpix<T>(S, c, 2 * x, 2 * y + 1) +
-                ldpix<T>(S, c, 2 * x + 1, 2 * y + 1) + 2;
-  dst[(size_t)c * pitch * dh + (size_t)y * pitch + x] = (uint16_t)(v >> 2);
+  down_px<T>(S, c, x, y, dst, pitch, dh);
 }
 
 // 4:2:0 -> 4:4:4: libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49 (luma copied).
@@ -468,95 +561,77 @@ __global__ __launch_bounds__(256) void k_upsample420(PicDev S, int w, int h, uin
 
 // ------------------------------- launchers ---------------------------------
 
-hipError_t launch_stats(const PicDev& p0, const PicDev& p1, int w, int h, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* zero_buf, int n_zero, hipStream_t st) {
-  const int blocks = h + 2 * (h >> p0.cs);
-  if ((int64_t)blocks * 256 < n_zero) return hipErrorInvalidValue;
-  if (p0.bps == 2)
-    hipLaunchKernelGGL(k_stats<uint16_t>, dim3(blocks), dim3(256), 0, st, p0, p1, w, h, row_sd, row_ssd, zero_buf, n_zero);
-  else
-    hipLaunchKernelGGL(k_stats<uint8_t>, dim3(blocks), dim3(256), 0, st, p0, p1, w, h, row_sd, row_ssd, zero_buf, n_zero);
+hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
+                        uint32_t* rows, hipStream_t st) {
+  if (npairs <= 0 || nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(256), 0, st, pairs, npairs, row_sd, row_ssd, rows);
   return hipGetLastError();
 }
 
-hipError_t launch_gcd_finalize(const int64_t* row_sd, const uint64_t* row_ssd, int w, int h, int cs, int /*bit_depth*/,
-                               const int32_t maxdiff[3], int wrap32, int32_t* gcd_out, int64_t* sd_out,
-                               uint64_t* ssd_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_gcd_finalize, dim3(1), dim3(256), 0, st, row_sd, row_ssd, w, h, cs, maxdiff[0], maxdiff[1],
-                     maxdiff[2], wrap32, gcd_out, sd_out, ssd_out);
+hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
+                               int32_t* gcd, int64_t* sd, uint64_t* ssd, hipStream_t st) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gcd_finalize, dim3(npairs), dim3(256), 0, st, pairs, row_sd, row_ssd, gcd, sd, ssd);
   return hipGetLastError();
 }
 
-template <typename T, int R, int PY>
-static hipError_t search_t(const PicDev& p0, const PicDev& p1, int w, int h, int maxv, const int32_t* gcd,
-                           const int32_t cw[4], bool narrow, uint32_t* rows, uint16_t* scp, int scp_pitch,
-                           uint8_t* best, hipStream_t st) {
-  dim3 grid((w + 63) / 64, (h + 4 * PY - 1) / (4 * PY), 2);
-  if (narrow)
-    hipLaunchKernelGGL((k_search<T, R, PY, true>), grid, dim3(256), 0, st, p0, p1, w, h, maxv, gcd, cw[0], cw[1], cw[2],
-                       rows, scp, scp_pitch, best);
-  else
-    hipLaunchKernelGGL((k_search<T, R, PY, false>), grid, dim3(256), 0, st, p0, p1, w, h, maxv, gcd, cw[0], cw[1],
-                       cw[2], rows, scp, scp_pitch, best);
+template <int R, int PY>
+static hipError_t search_rp(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
+                            uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
+  hipLaunchKernelGGL((k_search<R, PY>), dim3(ntiles, 2), dim3(256), 0, st, pairs, npairs, cw[0], cw[1], cw[2], gcd,
+                     rows, scp, best);
   return hipGetLastError();
 }
 
-template <typename T, int PY>
-static hipError_t search_r(const PicDev& p0, const PicDev& p1, int w, int h, int maxv, const int32_t* gcd,
-                           const int32_t cw[4], int R, bool narrow, uint32_t* rows, uint16_t* scp, int scp_pitch,
-                           uint8_t* best, hipStream_t st) {
+template <int PY>
+static hipError_t search_p(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
+                           const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
   switch (R) {
-    case 1: return search_t<T, 1, PY>(p0, p1, w, h, maxv, gcd, cw, narrow, rows, scp, scp_pitch, best, st);
-    case 2: return search_t<T, 2, PY>(p0, p1, w, h, maxv, gcd, cw, narrow, rows, scp, scp_pitch, best, st);
-    case 3: return search_t<T, 3, PY>(p0, p1, w, h, maxv, gcd, cw, narrow, rows, scp, scp_pitch, best, st);
-    case 4: return search_t<T, 4, PY>(p0, p1, w, h, maxv, gcd, cw, narrow, rows, scp, scp_pitch, best, st);
+    case 1: return search_rp<1, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 2: return search_rp<2, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 3: return search_rp<3, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 4: return search_rp<4, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_search(const PicDev& p0, const PicDev& p1, int w, int h, int bit_depth, const int32_t* gcd,
-                         const int32_t cw[4], int R, bool narrow, uint32_t* rows, uint16_t* scp, int scp_pitch,
-                         uint8_t* best, int tile_py, hipStream_t st) {
-  const int maxv = (1 << bit_depth) - 1;
-  if (p0.bps == 2) {
-    if (tile_py == 4) return search_r<uint16_t, 4>(p0, p1, w, h, maxv, gcd, cw, R, narrow, rows, scp, scp_pitch, best, st);
-    return search_r<uint16_t, 8>(p0, p1, w, h, maxv, gcd, cw, R, narrow, rows, scp, scp_pitch, best, st);
+hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
+                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
+  if (npairs <= 0 || ntiles <= 0) return hipSuccess;
+  if (py == 4) return search_p<4>(pairs, npairs, ntiles, R, cw, gcd, rows, scp, best, st);
+  if (py == 8) return search_p<8>(pairs, npairs, ntiles, R, cw, gcd, rows, scp, best, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st) {
+  if (njobs <= 0 || nblk <= 0) return hipSuccess;
+  switch (nb) {
+    case 4: hipLaunchKernelGGL(k_ssim<4>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
+    case 8: hipLaunchKernelGGL(k_ssim<8>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
+    case 16: hipLaunchKernelGGL(k_ssim<16>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
+    default: return hipErrorInvalidValue;
   }
-  if (tile_py == 4) return search_r<uint8_t, 4>(p0, p1, w, h, maxv, gcd, cw, R, narrow, rows, scp, scp_pitch, best, st);
-  return search_r<uint8_t, 8>(p0, p1, w, h, maxv, gcd, cw, R, narrow, rows, scp, scp_pitch, best, st);
-}
-
-hipError_t launch_ssim(const SsimJobTable& jobs, int max_nchunk, int max_bands, float c1n, float c2n, bool wide_sq,
-                       double* partial, hipStream_t st) {
-  if (jobs.njobs <= 0) return hipSuccess;
-  dim3 grid(max_nchunk, (max_bands + 3) / 4, jobs.njobs * 3);
-  if (wide_sq) hipLaunchKernelGGL(k_ssim<true>, grid, dim3(256), 0, st, jobs, c1n, c2n, partial);
-  else hipLaunchKernelGGL(k_ssim<false>, grid, dim3(256), 0, st, jobs, c1n, c2n, partial);
   return hipGetLastError();
 }
 
-hipError_t launch_ssim_finalize(const SsimJobTable& jobs, const double* partial, const double* erp_w, int wnd,
-                                int stride, double* out, hipStream_t st) {
-  if (jobs.njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ssim_finalize, dim3(jobs.njobs * 3), dim3(256), 0, st, jobs, partial, erp_w, wnd >> 1, stride, out);
+hipError_t launch_ssim_finalize(const SsimJobDev* jobs, int njobs, const double* partial, const double* erp_w,
+                                int half_wnd, int stride, double* out, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ssim_finalize, dim3(njobs * 3), dim3(256), 0, st, jobs, partial, erp_w, half_wnd, stride, out);
   return hipGetLastError();
 }
 
-hipError_t launch_downsample(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st) {
+hipError_t launch_downsample(const DownJob* jobs, int njobs, int nblk, hipStream_t st) {
+  if (njobs <= 0 || nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_downsample, dim3(nblk), dim3(256), 0, st, jobs, njobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample_one(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st) {
   const int dw = w >> 1, dh = h >> 1;
   dim3 grid((dw + 63) / 64, (dh + 3) / 4, 3);
-  if (src.bps == 2) hipLaunchKernelGGL(k_downsample<uint16_t>, grid, dim3(256), 0, st, src, dw, dh, dst, dst_pitch);
-  else hipLaunchKernelGGL(k_downsample<uint8_t>, grid, dim3(256), 0, st, src, dw, dh, dst, dst_pitch);
-  return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, int n) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  if (src.bps == 2) hipLaunchKernelGGL(k_downsample_one<uint16_t>, grid, dim3(256), 0, st, src, dw, dh, dst, dst_pitch);
+  else hipLaunchKernelGGL(k_downsample_one<uint8_t>, grid, dim3(256), 0, st, src, dw, dh, dst, dst_pitch);
   return hipGetLastError();
 }
 
