@@ -84,6 +84,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m qmiv_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7. Loading
+    # torch first lets the library's libamdhip64.so.7 dependency resolve to that
+    # copy; loaded the other way round, torch finds no HIP device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.ivqm_strerror.restype = C.c_char_p
     L.ivqm_last_error.restype = C.c_char_p

@@ -170,6 +170,8 @@ class Engine {
 
  public:
   std::map<int, std::pair<std::vector<double>, double>> erp_host;  // height -> weights, correction
+  // Cosine-latitude row weights and correction of height h (computed once).
+  const std::pair<std::vector<double>, double>& erp_weights(int h);
  private:
   RingSlot ring_[kRing];
   BatchPlan last_;

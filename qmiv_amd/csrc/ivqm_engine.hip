@@ -141,11 +141,18 @@ bool Engine::narrow_ok(int maxv, int32_t gmax) const {
   return true;
 }
 
+const std::pair<std::vector<double>, double>& Engine::erp_weights(int h) {
+  auto it = erp_host.find(h);
+  if (it != erp_host.end()) return it->second;
+  auto& hw = erp_host[h];
+  host::ws_weights(h, cfg_.lat_range_deg, hw.first, hw.second);
+  return hw;
+}
+
 int Engine::erp_offset(int h, int64_t* off) {
   auto it = erp_off_.find(h);
   if (it != erp_off_.end()) { *off = it->second; return IVQM_OK; }
-  auto& hw = erp_host[h];
-  host::ws_weights(h, cfg_.lat_range_deg, hw.first, hw.second);
+  const auto& hw = erp_weights(h);
   if (erp_used_ + h > erp_cap_) {
     // grow: keep existing weights (copy old contents)
     const int64_t nc = grow(erp_used_ + h, std::max<int64_t>(erp_cap_, 4096));
@@ -617,7 +624,7 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
         for (int k = 0; k < 3; k++) r.wspsnr[k] = host::psnr_from_ssd((double)r.ssd[k], area, bd);
       } else {
         // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a 4:2:0 chroma plane row counts twice per luma row
-        const auto& hw = erp_host.at(H);
+        const auto& hw = erp_weights(H);
         const uint64_t* rs = S.r_row_ssd + pp.rowsd_off;
         for (int k = 0; k < 3; k++) {
           host::Kbns kb;
@@ -637,15 +644,7 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
     if (m & IVQM_M_IVPSNR) {
       const uint32_t* rows = S.r_rows + pp.rows_off;
       const double* ew = nullptr;
-      if (cfg_.erp) {
-        auto it = erp_host.find(H);
-        if (it == erp_host.end()) {
-          auto& hw = erp_host[H];
-          host::ws_weights(H, cfg_.lat_range_deg, hw.first, hw.second);
-          it = erp_host.find(H);
-        }
-        ew = it->second.first.data();
-      }
+      if (cfg_.erp) ew = erp_weights(H).first.data();
       r.ivpsnr_r2t = host::ivpsnr_quality(rows, W, H, bd, ew, cwa);
       r.ivpsnr_t2r = host::ivpsnr_quality(rows + (size_t)3 * H, W, H, bd, ew, cwa);
       r.ivpsnr = std::min(r.ivpsnr_r2t, r.ivpsnr_t2r);  // xIVPSNR.cpp:55 (xMin = std::min)

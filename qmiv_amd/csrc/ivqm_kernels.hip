@@ -413,9 +413,10 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         for (int j = 0; j < 4; j++) {
           A += a[j];
           B += b[j];
-          AA += (SQ)__umul24(a[j], a[j]);
-          BB += (SQ)__umul24(b[j], b[j]);
-          AB += (SQ)__umul24(a[j], b[j]);
+          // __umul24 returns int: widen through uint32_t (16-bit squares reach 2^32 - 2^17)
+          AA += (SQ)(uint32_t)__umul24(a[j], a[j]);
+          BB += (SQ)(uint32_t)__umul24(b[j], b[j]);
+          AB += (SQ)(uint32_t)__umul24(a[j], b[j]);
         }
       }
     }

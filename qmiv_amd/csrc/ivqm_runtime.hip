@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -165,6 +167,20 @@ int run_sync(ivqm_ctx* c, uint32_t m, const int32_t* gcd, ivqm_frame_result* out
   return IVQM_OK;
 }
 
+// C++ exceptions (allocation failures, container errors) must not cross the C ABI.
+template <typename F>
+int guarded(F&& f) noexcept {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return set_error(IVQM_ENOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return set_error(IVQM_EINVAL, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return set_error(IVQM_EINVAL, "internal error");
+  }
+}
+
 }  // namespace
 
 // ================================ C ABI ====================================
@@ -199,207 +215,257 @@ int ivqm_config_init(ivqm_config* cfg) {
 }
 
 int ivqm_create(const ivqm_config* cfg, ivqm_ctx** out) {
-  if (!cfg || !out) return set_error(IVQM_EINVAL, "NULL argument");
-  *out = nullptr;
-  RCCHK(validate_config(cfg, true));
-  std::unique_ptr<ivqm_ctx> c(new ivqm_ctx());
-  c->cfg = *cfg;
-  c->eng.reset(new Engine(*cfg));
-  RCCHK(c->eng->init());
-  *out = c.release();
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!cfg || !out) return set_error(IVQM_EINVAL, "NULL argument");
+    *out = nullptr;
+    RCCHK(validate_config(cfg, true));
+    std::unique_ptr<ivqm_ctx> c(new ivqm_ctx());
+    c->cfg = *cfg;
+    c->eng.reset(new Engine(*cfg));
+    RCCHK(c->eng->init());
+    *out = c.release();
+    return IVQM_OK;
+  });
 }
 
 int ivqm_destroy(ivqm_ctx* c) {
-  if (!c) return IVQM_OK;
-  if (c->eng) (void)c->eng->sync();
-  for (int s = 0; s < 2; s++)
-    if (c->own[s]) (void)hipFree(c->own[s]);
-  delete c;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c) return IVQM_OK;
+    if (c->eng) (void)c->eng->sync();
+    for (int s = 0; s < 2; s++)
+      if (c->own[s]) (void)hipFree(c->own[s]);
+    delete c;
+    return IVQM_OK;
+  });
 }
 
 int ivqm_set_stream(ivqm_ctx* c, void* s) {
-  if (!c) return set_error(IVQM_EINVAL, "ctx is NULL");
-  return c->eng->set_stream((hipStream_t)s);
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "ctx is NULL");
+    return c->eng->set_stream((hipStream_t)s);
+  });
 }
 
 int ivqm_upload(ivqm_ctx* c, int32_t slot, const ivqm_picture* p) {
-  if (!c || slot < 0 || slot > 1) return set_error(IVQM_EINVAL, "bad context or slot");
-  RCCHK(validate_picture(c->cfg.width, c->cfg.height, c->cfg.bit_depth, p));
-  RCCHK(copy_into_slot(c, slot, p, hipMemcpyHostToDevice));
-  return c->eng->sync();
+  return guarded([&]() -> int {
+    if (!c || slot < 0 || slot > 1) return set_error(IVQM_EINVAL, "bad context or slot");
+    RCCHK(validate_picture(c->cfg.width, c->cfg.height, c->cfg.bit_depth, p));
+    RCCHK(copy_into_slot(c, slot, p, hipMemcpyHostToDevice));
+    return c->eng->sync();
+  });
 }
 
 int ivqm_bind(ivqm_ctx* c, int32_t slot, const ivqm_picture* p) {
-  if (!c || slot < 0 || slot > 1) return set_error(IVQM_EINVAL, "bad context or slot");
-  RCCHK(validate_picture(c->cfg.width, c->cfg.height, c->cfg.bit_depth, p));
-  if (!aligned_for_kernels(p)) return copy_into_slot(c, slot, p, hipMemcpyDeviceToDevice);
-  c->pic[slot] = to_dev(p);
-  c->bound[slot] = true;
-  c->eng->set_scp_valid(false);
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || slot < 0 || slot > 1) return set_error(IVQM_EINVAL, "bad context or slot");
+    RCCHK(validate_picture(c->cfg.width, c->cfg.height, c->cfg.bit_depth, p));
+    if (!aligned_for_kernels(p)) return copy_into_slot(c, slot, p, hipMemcpyDeviceToDevice);
+    c->pic[slot] = to_dev(p);
+    c->bound[slot] = true;
+    c->eng->set_scp_valid(false);
+    return IVQM_OK;
+  });
 }
 
 int ivqm_gcd(ivqm_ctx* c, int32_t g[4]) {
-  if (!c || !g) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
-  std::memcpy(g, r.gcd_r2t, sizeof(r.gcd_r2t));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !g) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
+    std::memcpy(g, r.gcd_r2t, sizeof(r.gcd_r2t));
+    return IVQM_OK;
+  });
 }
 int ivqm_ssd(ivqm_ctx* c, uint64_t s[4]) {
-  if (!c || !s) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
-  for (int k = 0; k < 3; k++) s[k] = r.ssd[k];
-  s[3] = 0;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !s) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
+    for (int k = 0; k < 3; k++) s[k] = r.ssd[k];
+    s[3] = 0;
+    return IVQM_OK;
+  });
 }
 int ivqm_mse(ivqm_ctx* c, double v[4]) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
-  std::memcpy(v, r.mse, sizeof(r.mse));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_MSE, nullptr, &r));
+    std::memcpy(v, r.mse, sizeof(r.mse));
+    return IVQM_OK;
+  });
 }
 int ivqm_psnr(ivqm_ctx* c, double v[4]) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_PSNR, nullptr, &r));
-  std::memcpy(v, r.psnr, sizeof(r.psnr));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_PSNR, nullptr, &r));
+    std::memcpy(v, r.psnr, sizeof(r.psnr));
+    return IVQM_OK;
+  });
 }
 int ivqm_wspsnr(ivqm_ctx* c, double v[4]) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_WSPSNR, nullptr, &r));
-  std::memcpy(v, r.wspsnr, sizeof(r.wspsnr));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_WSPSNR, nullptr, &r));
+    std::memcpy(v, r.wspsnr, sizeof(r.wspsnr));
+    return IVQM_OK;
+  });
 }
 int ivqm_ivpsnr(ivqm_ctx* c, const int32_t* gcd, double* v, double* r2t, double* t2r) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_IVPSNR, gcd, &r));
-  *v = r.ivpsnr;
-  if (r2t) *r2t = r.ivpsnr_r2t;
-  if (t2r) *t2r = r.ivpsnr_t2r;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_IVPSNR, gcd, &r));
+    *v = r.ivpsnr;
+    if (r2t) *r2t = r.ivpsnr_r2t;
+    if (t2r) *t2r = r.ivpsnr_t2r;
+    return IVQM_OK;
+  });
 }
 int ivqm_gen_scp(ivqm_ctx* c, const int32_t* gcd) {
-  if (!c) return set_error(IVQM_EINVAL, "NULL argument");
-  return run_sync(c, kScpOnly, gcd, nullptr);
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL argument");
+    return run_sync(c, kScpOnly, gcd, nullptr);
+  });
 }
 int ivqm_ssim(ivqm_ctx* c, double v[4]) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_SSIM, nullptr, &r));
-  std::memcpy(v, r.ssim, sizeof(r.ssim));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_SSIM, nullptr, &r));
+    std::memcpy(v, r.ssim, sizeof(r.ssim));
+    return IVQM_OK;
+  });
 }
 int ivqm_msssim(ivqm_ctx* c, double v[4]) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_MSSSIM, nullptr, &r));
-  std::memcpy(v, r.msssim, sizeof(r.msssim));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_MSSSIM, nullptr, &r));
+    std::memcpy(v, r.msssim, sizeof(r.msssim));
+    return IVQM_OK;
+  });
 }
 int ivqm_ivssim(ivqm_ctx* c, double* v, double* r2t, double* t2r) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_IVSSIM | kReuseScp, nullptr, &r));
-  *v = r.ivssim;
-  if (r2t) *r2t = r.ivssim_r2t;
-  if (t2r) *t2r = r.ivssim_t2r;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_IVSSIM | kReuseScp, nullptr, &r));
+    *v = r.ivssim;
+    if (r2t) *r2t = r.ivssim_r2t;
+    if (t2r) *t2r = r.ivssim_t2r;
+    return IVQM_OK;
+  });
 }
 int ivqm_ivmsssim(ivqm_ctx* c, double* v, double* r2t, double* t2r) {
-  if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
-  ivqm_frame_result r;
-  RCCHK(run_sync(c, IVQM_M_IVMSSSIM | kReuseScp, nullptr, &r));
-  *v = r.ivmsssim;
-  if (r2t) *r2t = r.ivmsssim_r2t;
-  if (t2r) *t2r = r.ivmsssim_t2r;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !v) return set_error(IVQM_EINVAL, "NULL argument");
+    ivqm_frame_result r;
+    RCCHK(run_sync(c, IVQM_M_IVMSSSIM | kReuseScp, nullptr, &r));
+    *v = r.ivmsssim;
+    if (r2t) *r2t = r.ivmsssim_r2t;
+    if (t2r) *t2r = r.ivmsssim_t2r;
+    return IVQM_OK;
+  });
 }
 int ivqm_upload_scp(ivqm_ctx* c, int32_t idx, const ivqm_picture* p) {
-  if (!c || idx < 0 || idx > 1) return set_error(IVQM_EINVAL, "bad context or index");
-  const int W = c->cfg.width, H = c->cfg.height;
-  RCCHK(validate_picture(W, H, c->cfg.bit_depth, p));
-  if (p->chroma_420 || p->bytes_per_sample != 2) return set_error(IVQM_EINVAL, "SCP pictures are 4:4:4 uint16");
-  const int pitch = Engine::scp_pitch(W);
-  RCCHK(c->eng->ensure_scp(6 * (int64_t)pitch * H));
-  const size_t plane = (size_t)pitch * H;
-  for (int k = 0; k < 3; k++)
-    HIPCHK(hipMemcpy2DAsync(c->eng->d_scp() + ((size_t)idx * 3 + k) * plane, sizeof(uint16_t) * pitch, p->plane[k],
-                            sizeof(uint16_t) * p->stride[k], sizeof(uint16_t) * W, H, hipMemcpyHostToDevice,
-                            c->eng->stream()));
-  RCCHK(c->eng->sync());
-  c->eng->set_scp_valid(true);
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || idx < 0 || idx > 1) return set_error(IVQM_EINVAL, "bad context or index");
+    const int W = c->cfg.width, H = c->cfg.height;
+    RCCHK(validate_picture(W, H, c->cfg.bit_depth, p));
+    if (p->chroma_420 || p->bytes_per_sample != 2) return set_error(IVQM_EINVAL, "SCP pictures are 4:4:4 uint16");
+    const int pitch = Engine::scp_pitch(W);
+    RCCHK(c->eng->ensure_scp(6 * (int64_t)pitch * H));
+    const size_t plane = (size_t)pitch * H;
+    for (int k = 0; k < 3; k++)
+      HIPCHK(hipMemcpy2DAsync(c->eng->d_scp() + ((size_t)idx * 3 + k) * plane, sizeof(uint16_t) * pitch, p->plane[k],
+                              sizeof(uint16_t) * p->stride[k], sizeof(uint16_t) * W, H, hipMemcpyHostToDevice,
+                              c->eng->stream()));
+    RCCHK(c->eng->sync());
+    c->eng->set_scp_valid(true);
+    return IVQM_OK;
+  });
 }
 
 int ivqm_frame(ivqm_ctx* c, uint32_t m, ivqm_frame_result* out) {
-  if (!c || !out) return set_error(IVQM_EINVAL, "NULL argument");
-  return run_sync(c, m & 0xFFu, nullptr, out);
+  return guarded([&]() -> int {
+    if (!c || !out) return set_error(IVQM_EINVAL, "NULL argument");
+    return run_sync(c, m & 0xFFu, nullptr, out);
+  });
 }
 int ivqm_frame_enqueue(ivqm_ctx* c, uint32_t m, int32_t ring) {
-  if (!c) return set_error(IVQM_EINVAL, "NULL argument");
-  PairSpec s;
-  RCCHK(ctx_pair(c, s, nullptr));
-  c->eng->enable_best(c->best_enabled);
-  return c->eng->enqueue(&s, 1, m & 0xFFu, ring);
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL argument");
+    PairSpec s;
+    RCCHK(ctx_pair(c, s, nullptr));
+    c->eng->enable_best(c->best_enabled);
+    return c->eng->enqueue(&s, 1, m & 0xFFu, ring);
+  });
 }
 int ivqm_frame_collect(ivqm_ctx* c, int32_t ring, ivqm_frame_result* out) {
-  if (!c) return set_error(IVQM_EINVAL, "NULL argument");
-  return c->eng->collect(ring, out);
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL argument");
+    return c->eng->collect(ring, out);
+  });
 }
 
 int ivqm_get_rows(ivqm_ctx* c, int32_t dir, uint64_t* rows) {
-  if (!c || !rows || dir < 0 || dir > 1) return set_error(IVQM_EINVAL, "bad argument");
-  const BatchPlan& P = c->eng->last_plan();
-  if (P.n != 1 || !P.need_search) return set_error(IVQM_ESTATE, "no search ran on this context");
-  const int H = c->cfg.height;
-  std::vector<uint32_t> h((size_t)3 * H);
-  RCCHK(c->eng->sync());
-  HIPCHK(hipMemcpy(h.data(), c->eng->d_rows() + P.pp[0].rows_off + (size_t)dir * 3 * H, sizeof(uint32_t) * 3 * H,
-                   hipMemcpyDeviceToHost));
-  for (int y = 0; y < H; y++) {
-    for (int k = 0; k < 3; k++) rows[(size_t)y * 4 + k] = (uint64_t)(int64_t)(int32_t)h[(size_t)k * H + y];
-    rows[(size_t)y * 4 + 3] = 0;
-  }
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !rows || dir < 0 || dir > 1) return set_error(IVQM_EINVAL, "bad argument");
+    const BatchPlan& P = c->eng->last_plan();
+    if (P.n != 1 || !P.need_search) return set_error(IVQM_ESTATE, "no search ran on this context");
+    const int H = c->cfg.height;
+    std::vector<uint32_t> h((size_t)3 * H);
+    RCCHK(c->eng->sync());
+    HIPCHK(hipMemcpy(h.data(), c->eng->d_rows() + P.pp[0].rows_off + (size_t)dir * 3 * H, sizeof(uint32_t) * 3 * H,
+                     hipMemcpyDeviceToHost));
+    for (int y = 0; y < H; y++) {
+      for (int k = 0; k < 3; k++) rows[(size_t)y * 4 + k] = (uint64_t)(int64_t)(int32_t)h[(size_t)k * H + y];
+      rows[(size_t)y * 4 + 3] = 0;
+    }
+    return IVQM_OK;
+  });
 }
 int ivqm_get_scp(ivqm_ctx* c, int32_t idx, uint16_t* const planes[3]) {
-  if (!c || !planes || idx < 0 || idx > 1) return set_error(IVQM_EINVAL, "bad argument");
-  if (!c->eng->scp_valid()) return set_error(IVQM_ESTATE, "no shift-compensated pictures computed");
-  const int W = c->cfg.width, H = c->cfg.height, pitch = Engine::scp_pitch(W);
-  const size_t plane = (size_t)pitch * H;
-  RCCHK(c->eng->sync());
-  for (int k = 0; k < 3; k++)
-    HIPCHK(hipMemcpy2D(planes[k], sizeof(uint16_t) * W, c->eng->d_scp() + ((size_t)idx * 3 + k) * plane,
-                       sizeof(uint16_t) * pitch, sizeof(uint16_t) * W, H, hipMemcpyDeviceToHost));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !planes || idx < 0 || idx > 1) return set_error(IVQM_EINVAL, "bad argument");
+    if (!c->eng->scp_valid()) return set_error(IVQM_ESTATE, "no shift-compensated pictures computed");
+    const int W = c->cfg.width, H = c->cfg.height, pitch = Engine::scp_pitch(W);
+    const size_t plane = (size_t)pitch * H;
+    RCCHK(c->eng->sync());
+    for (int k = 0; k < 3; k++)
+      HIPCHK(hipMemcpy2D(planes[k], sizeof(uint16_t) * W, c->eng->d_scp() + ((size_t)idx * 3 + k) * plane,
+                         sizeof(uint16_t) * pitch, sizeof(uint16_t) * W, H, hipMemcpyDeviceToHost));
+    return IVQM_OK;
+  });
 }
 int ivqm_enable_best_map(ivqm_ctx* c, int32_t en) {
-  if (!c) return set_error(IVQM_EINVAL, "NULL argument");
-  c->best_enabled = en != 0;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL argument");
+    c->best_enabled = en != 0;
+    return IVQM_OK;
+  });
 }
 int ivqm_get_best_map(ivqm_ctx* c, int32_t dir, uint8_t* map) {
-  if (!c || !map || dir < 0 || dir > 1) return set_error(IVQM_EINVAL, "bad argument");
-  const BatchPlan& P = c->eng->last_plan();
-  if (!c->eng->d_best() || P.n != 1 || P.pairs.empty() || P.pairs[0].best_off < 0 || !P.need_search)
-    return set_error(IVQM_ESTATE, "best map not enabled for the last search");
-  const size_t n = (size_t)c->cfg.width * c->cfg.height;
-  RCCHK(c->eng->sync());
-  HIPCHK(hipMemcpy(map, c->eng->d_best() + (size_t)dir * n, n, hipMemcpyDeviceToHost));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!c || !map || dir < 0 || dir > 1) return set_error(IVQM_EINVAL, "bad argument");
+    const BatchPlan& P = c->eng->last_plan();
+    if (!c->eng->d_best() || P.n != 1 || P.pairs.empty() || P.pairs[0].best_off < 0 || !P.need_search)
+      return set_error(IVQM_ESTATE, "best map not enabled for the last search");
+    const size_t n = (size_t)c->cfg.width * c->cfg.height;
+    RCCHK(c->eng->sync());
+    HIPCHK(hipMemcpy(map, c->eng->d_best() + (size_t)dir * n, n, hipMemcpyDeviceToHost));
+    return IVQM_OK;
+  });
 }
 
 int ivqm_profile(ivqm_ctx* c, int32_t kernel, int32_t iters, double* avg_ms) {
-  if (!c || !avg_ms || iters <= 0) return set_error(IVQM_EINVAL, "bad argument");
-  return c->eng->profile(kernel, iters, avg_ms);
+  return guarded([&]() -> int {
+    if (!c || !avg_ms || iters <= 0) return set_error(IVQM_EINVAL, "bad argument");
+    return c->eng->profile(kernel, iters, avg_ms);
+  });
 }
 
 static int pic_from(const ivqm_picture* p, PicDev& d) {
@@ -410,55 +476,69 @@ static int pic_from(const ivqm_picture* p, PicDev& d) {
   return IVQM_OK;
 }
 int ivqm_upsample420(const ivqm_picture* src, int32_t w, int32_t h, uint16_t* dst, int32_t dst_stride, void* stream) {
-  PicDev d;
-  RCCHK(pic_from(src, d));
-  if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && ((w & 1) || (h & 1))))
-    return set_error(IVQM_EINVAL, "bad size");
-  HIPCHK(launch_upsample420(d, w, h, dst, dst_stride, (hipStream_t)stream));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    PicDev d;
+    RCCHK(pic_from(src, d));
+    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && ((w & 1) || (h & 1))))
+      return set_error(IVQM_EINVAL, "bad size");
+    HIPCHK(launch_upsample420(d, w, h, dst, dst_stride, (hipStream_t)stream));
+    return IVQM_OK;
+  });
 }
 int ivqm_downsample(const ivqm_picture* src, int32_t w, int32_t h, uint16_t* dst, void* stream) {
-  PicDev d;
-  RCCHK(pic_from(src, d));
-  if (!dst || w < 2 || h < 2) return set_error(IVQM_EINVAL, "bad size");
-  HIPCHK(launch_downsample_one(d, w, h, dst, w >> 1, (hipStream_t)stream));
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    PicDev d;
+    RCCHK(pic_from(src, d));
+    if (!dst || w < 2 || h < 2) return set_error(IVQM_EINVAL, "bad size");
+    HIPCHK(launch_downsample_one(d, w, h, dst, w >> 1, (hipStream_t)stream));
+    return IVQM_OK;
+  });
 }
 
 // ------------------------------- batches -----------------------------------
 
 int ivqm_batch_create(const ivqm_config* cfg, ivqm_batch** out) {
-  if (!cfg || !out) return set_error(IVQM_EINVAL, "NULL argument");
-  *out = nullptr;
-  RCCHK(validate_config(cfg, false));
-  std::unique_ptr<ivqm_batch> b(new ivqm_batch());
-  b->cfg = *cfg;
-  b->eng.reset(new Engine(*cfg));
-  RCCHK(b->eng->init());
-  *out = b.release();
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!cfg || !out) return set_error(IVQM_EINVAL, "NULL argument");
+    *out = nullptr;
+    RCCHK(validate_config(cfg, false));
+    std::unique_ptr<ivqm_batch> b(new ivqm_batch());
+    b->cfg = *cfg;
+    b->eng.reset(new Engine(*cfg));
+    RCCHK(b->eng->init());
+    *out = b.release();
+    return IVQM_OK;
+  });
 }
 int ivqm_batch_destroy(ivqm_batch* b) {
-  if (!b) return IVQM_OK;
-  if (b->eng) (void)b->eng->sync();
-  delete b;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!b) return IVQM_OK;
+    if (b->eng) (void)b->eng->sync();
+    delete b;
+    return IVQM_OK;
+  });
 }
 int ivqm_batch_set_stream(ivqm_batch* b, void* s) {
-  if (!b) return set_error(IVQM_EINVAL, "NULL argument");
-  return b->eng->set_stream((hipStream_t)s);
+  return guarded([&]() -> int {
+    if (!b) return set_error(IVQM_EINVAL, "NULL argument");
+    return b->eng->set_stream((hipStream_t)s);
+  });
 }
 int ivqm_batch_set_tuning(ivqm_batch* b, int32_t search_tile_rows, int32_t ssim_band_rows, int32_t use_graph) {
-  if (!b) return set_error(IVQM_EINVAL, "NULL argument");
-  b->eng->set_tuning(search_tile_rows, ssim_band_rows);
-  b->use_graph = use_graph != 0;
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!b) return set_error(IVQM_EINVAL, "NULL argument");
+    b->eng->set_tuning(search_tile_rows, ssim_band_rows);
+    b->use_graph = use_graph != 0;
+    return IVQM_OK;
+  });
 }
 int ivqm_batch_get_tuning(ivqm_batch* b, int32_t* search_tile_rows, int32_t* ssim_band_rows) {
-  if (!b) return set_error(IVQM_EINVAL, "NULL argument");
-  if (search_tile_rows) *search_tile_rows = b->eng->tuning_py();
-  if (ssim_band_rows) *ssim_band_rows = b->eng->tuning_nb();
-  return IVQM_OK;
+  return guarded([&]() -> int {
+    if (!b) return set_error(IVQM_EINVAL, "NULL argument");
+    if (search_tile_rows) *search_tile_rows = b->eng->tuning_py();
+    if (ssim_band_rows) *ssim_band_rows = b->eng->tuning_nb();
+    return IVQM_OK;
+  });
 }
 
 static int batch_specs(const ivqm_pair* pairs, int32_t n, std::vector<PairSpec>& specs) {
@@ -486,22 +566,30 @@ static int batch_specs(const ivqm_pair* pairs, int32_t n, std::vector<PairSpec>&
 }
 
 int ivqm_batch_enqueue(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, int32_t ring) {
-  if (!b) return set_error(IVQM_EINVAL, "NULL argument");
-  std::vector<PairSpec> specs;
-  RCCHK(batch_specs(pairs, n, specs));
-  return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
+  return guarded([&]() -> int {
+    if (!b) return set_error(IVQM_EINVAL, "NULL argument");
+    std::vector<PairSpec> specs;
+    RCCHK(batch_specs(pairs, n, specs));
+    return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
+  });
 }
 int ivqm_batch_collect(ivqm_batch* b, int32_t ring, ivqm_frame_result* results) {
-  if (!b || !results) return set_error(IVQM_EINVAL, "NULL argument");
-  return b->eng->collect(ring, results);
+  return guarded([&]() -> int {
+    if (!b || !results) return set_error(IVQM_EINVAL, "NULL argument");
+    return b->eng->collect(ring, results);
+  });
 }
 int ivqm_batch_run(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, ivqm_frame_result* results) {
-  RCCHK(ivqm_batch_enqueue(b, pairs, n, metrics, 0));
-  return ivqm_batch_collect(b, 0, results);
+  return guarded([&]() -> int {
+    RCCHK(ivqm_batch_enqueue(b, pairs, n, metrics, 0));
+    return ivqm_batch_collect(b, 0, results);
+  });
 }
 int ivqm_batch_profile(ivqm_batch* b, int32_t kernel, int32_t iters, double* avg_ms) {
-  if (!b || !avg_ms || iters <= 0) return set_error(IVQM_EINVAL, "bad argument");
-  return b->eng->profile(kernel, iters, avg_ms);
+  return guarded([&]() -> int {
+    if (!b || !avg_ms || iters <= 0) return set_error(IVQM_EINVAL, "bad argument");
+    return b->eng->profile(kernel, iters, avg_ms);
+  });
 }
 
 // Times every search tile height and SSIM band height on this batch and keeps

@@ -29,10 +29,10 @@ def test_headers_declare_the_abi():
 
 
 def test_library_exports_every_declared_symbol():
-    from qmiv_amd._lib import LIB_PATH
+    from qmiv_amd._lib import LIB_PATH, load
     if not os.path.exists(LIB_PATH):
         pytest.skip("libivqm_hip.so not built (run python -m qmiv_amd.build)")
-    L = C.CDLL(LIB_PATH)
+    L = load()
     for n in declared_symbols():
         assert hasattr(L, n), n
 
