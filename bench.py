@@ -8,16 +8,24 @@ timing. One step = one pass over all 17 pairs computing IV-PSNR and IV-SSIM
 shift-compensated pictures, both IV-SSIM directions, FP64 totals, result
 download and the host FP64 epilogue). value = 17 * 4096 * 2048 px-pairs / s / 1e6.
 
-Also reported: the roofline of the dominant kernel (the search: VALU-bound) and
-of the streaming SSIM kernel (HBM-bound), both timed with HIP events on the
-context's stream; and the reference library's own CPU path (oracle/_ref,
-compiled from the reference sources) timed on a bounded sample on the host.
+Two forms of the same step are timed:
+  batch  all 17 pairs as one batch: one launch per phase for the whole step,
+         the enqueued sequence replayed as a hipGraph, tile shapes from the
+         autotuner (the headline value);
+  plain  one pair per enqueue on a per-sequence context (ring of 8 in flight).
+
+Also reported: the roofline of the dominant kernel (the search: integer-VALU
+bound) and of the streaming kernels (SSIM, stats: HBM bound), timed with HIP
+events on the engine's stream; and the reference library's own CPU path
+(oracle/_ref, compiled from the reference sources) timed on a bounded sample
+of the same pairs on the host cores.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,8 +35,11 @@ sys.path.insert(0, ROOT)
 FRAMES = 17
 W, H, BD = 4096, 2048, 10
 PEAK_HBM_GBS = 8000.0
-PEAK_INT32_VALU_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, full-rate 32-bit integer VALU ops
+PEAK_INT32_VALU_TOPS = 39.3  # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, full-rate 32-bit integer VALU ops
 SEARCH_OPS_PER_PX = 2 * 25 * 12  # two directions x (2R+1)^2 candidates x 12 ops (R = 2)
+SSIM_BYTES_PER_PX = 3 + 3 + 6 + 6  # P0, P1 (4:2:0, 2-byte samples) and SCP0, SCP1 (4:4:4) per IV-SSIM pass
+STATS_BYTES_PER_PX = 6  # P0 and P1, 4:2:0, 2-byte samples
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_search_erp4096x2048_10b.md")
 
 
 def parse():
@@ -37,16 +48,30 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--metrics", default="ivpsnr+ivssim", choices=["ivpsnr+ivssim", "all"])
+    ap.add_argument("--mode", default="both", choices=["both", "batch", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="approximate CPU work of the baseline sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device copy (reference dispatch for HBM counter scaling)")
     return ap.parse_args()
+
+
+def measured_traffic():
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of k_search per 4096x2048 pair, from the
+    committed rocprofv3 --pmc summary, if present."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_pair:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(frame pairs shard across GPUs by pair; see DESIGN.md)"}))
+                                   "(frame pairs shard across GPUs by pair)"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -59,51 +84,107 @@ def main():
     t_gen = time.time()
     pairs_np = [synth.make_pair(wl, f) for f in range(FRAMES)]
     t_gen = time.time() - t_gen
-    # device-resident planes
     pairs = []
     for p0, p1 in pairs_np:
         pairs.append(([torch.from_numpy(p.view(np.int16)).to(dev) for p in p0],
                       [torch.from_numpy(p.view(np.int16)).to(dev) for p in p1]))
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     metrics = (Q.IVQM_M_IVPSNR | Q.IVQM_M_IVSSIM) if args.metrics == "ivpsnr+ivssim" else Q.IVQM_M_ALL
-    ctx = Q.IvqmContext(W, H, BD, erp=1)
-
-    def step():
-        res = []
-        for f, (a, b) in enumerate(pairs):
-            ring = f % 8
-            if f >= 8:
-                res.append(ctx.collect(ring))
-            ctx.bind(0, a, True)
-            ctx.bind(1, b, True)
-            ctx.enqueue(metrics, ring)
-        for f in range(max(0, FRAMES - 8), FRAMES):
-            res.append(ctx.collect(f % 8))
-        return res
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        last = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
     px = FRAMES * W * H
-    value = px / (ms_per_step / 1e3) / 1e6
 
-    # kernel rooflines (HIP events on the context stream), pair 0 bound
-    ctx.bind(0, pairs[0][0], True)
-    ctx.bind(1, pairs[0][1], True)
-    t_search = ctx.profile(1, 20)
-    t_ssim = ctx.profile(2, 20)
-    t_stats = ctx.profile(0, 20)
+    def timed(step, finish):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = step()
+        r = r + (finish() or [])
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, r
+
+    results = {}
+    prof = {}
+    # ---- plain: one pair per enqueue, ring of 8 in flight
+    if args.mode in ("both", "plain"):
+        ctx = Q.IvqmContext(W, H, BD, erp=1)
+
+        def plain_step():
+            res = []
+            for f, (a, b) in enumerate(pairs):
+                ring = f % 8
+                if f >= 8:
+                    res.append(ctx.collect(ring))
+                ctx.bind(0, a, True)
+                ctx.bind(1, b, True)
+                ctx.enqueue(metrics, ring)
+            for f in range(max(0, FRAMES - 8), FRAMES):
+                res.append(ctx.collect(f % 8))
+            return res
+
+        ms, res = timed(plain_step, lambda: [])
+        results["plain"] = (ms, res)
+        ctx.bind(0, pairs[0][0], True)
+        ctx.bind(1, pairs[0][1], True)
+        prof["plain"] = {k: ctx.profile(i, 20) for i, k in enumerate(("stats", "search", "ssim"))}
+        ctx.close()
+    # ---- batch: all pairs in one batch per step, hipGraph replay, autotuned tiles
+    if args.mode in ("both", "batch"):
+        bt = Q.IvqmBatch(erp=1)
+        items = [(a, b, W, H, BD, True) for a, b in pairs]
+        arr = Q.IvqmBatch.make_pairs(items)
+        tuned = bt.autotune(arr, metrics)
+        bt.set_tuning(tuned[0], tuned[1], use_graph=True)
+        state = {"k": 0, "pending": None}
+
+        def batch_step():
+            k = state["k"]
+            state["k"] = k + 1
+            bt.enqueue(arr, metrics, k % 2)
+            out = []
+            if state["pending"] is not None:
+                out = bt.collect(state["pending"], FRAMES)
+            state["pending"] = k % 2
+            return out
+
+        def finish():
+            if state["pending"] is None:
+                return []
+            r = bt.collect(state["pending"], FRAMES)
+            state["pending"] = None
+            return r
+        ms, res = timed(batch_step, finish)
+        results["batch"] = (ms, res)
+        launch = {k: bt.profile(i, 20) for i, k in enumerate(("stats", "search", "ssim"))}
+        prof["batch"] = {k: v / FRAMES for k, v in launch.items()}
+        prof["batch_launch_ms"] = launch
+        prof["batch_tuning"] = {"search_tile_rows": tuned[0], "ssim_band_rows": tuned[1]}
+        bt.close()
+
+    head = "batch" if "batch" in results else "plain"
+    ms_per_step, res = results[head]
+    value = px / (ms_per_step / 1e3) / 1e6
+    # results must agree between the two forms (IV-PSNR bit-exact, IV-SSIM same kernels)
+    agree = None
+    if "batch" in results and "plain" in results:
+        a = [(r.ivpsnr, r.ivssim) for r in results["plain"][1][-FRAMES:]]
+        b = [(r.ivpsnr, r.ivssim) for r in results["batch"][1][-FRAMES:]]
+        agree = a == b
+
+    p = prof[head]
+    t_search, t_ssim, t_stats = p["search"], p["ssim"], p["stats"]
     search_tops = W * H * SEARCH_OPS_PER_PX / (t_search * 1e-3) / 1e12
-    # IV-SSIM jobs read P0 and P1 (4:2:0, 3 B each per px) and both SCP pictures (6 B each)
-    ssim_bytes = W * H * (3 + 3 + 6 + 6)
-    ssim_gbs = ssim_bytes / (t_ssim * 1e-3) / 1e9
-    stats_gbs = W * H * 6 / (t_stats * 1e-3) / 1e9
+    ssim_gbs = W * H * SSIM_BYTES_PER_PX / (t_ssim * 1e-3) / 1e9
+    stats_gbs = W * H * STATS_BYTES_PER_PX / (t_stats * 1e-3) / 1e9
+    traffic = measured_traffic()
+    if traffic is not None:
+        traffic = round(traffic * (FRAMES if head == "batch" else 1))
 
     out = {
         "metric": "IV-PSNR+IV-SSIM Mpixel-pairs/s at 4096x2048 10b",
@@ -116,31 +197,44 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 (search, SSIM moments) + fp32 (SSIM ratios) + fp64 (totals)",
+        "dtype": "int32",
+        "precision": "search and SSIM window moments exact integer; SSIM ratios fp32; frame totals fp64",
         "data": "synthetic (seeded generator, qmiv_amd/csrc/synth.cpp)",
-        "config": {"workload": "ERP 4096x2048 10-bit 4:2:0, 17 frame pairs, IV-PSNR + IV-SSIM per pair "
-                               "(R=2, weights 4:1:1, SSIM 8x8/4), inputs resident in HBM",
+        "config": {"workload": "ERP 4096x2048 10-bit 4:2:0, 17 frame pairs per step, IV-PSNR + IV-SSIM per pair "
+                               "(R=2, weights 4:1:1, SSIM 8x8/4), inputs resident in HBM, "
+                               f"{head} form" + (" (one batch per step, hipGraph, autotuned)" if head == "batch"
+                                                 else " (one pair per enqueue)"),
                    "frames": FRAMES, "width": W, "height": H, "bit_depth": BD, "metrics": args.metrics,
                    "parallelism": "1 GPU"},
         "roofline": {"kernel": "k_search (both directions + SCP)", "bound": "valu",
                      "achieved": round(search_tops, 2), "peak": PEAK_INT32_VALU_TOPS, "unit": "TOP/s",
-                     "frac": round(search_tops / PEAK_INT32_VALU_TOPS, 4), "traffic": None,
-                     "ms_per_launch": round(t_search, 4),
+                     "frac": round(search_tops / PEAK_INT32_VALU_TOPS, 4),
+                     "traffic": traffic,
+                     "us_per_pair": round(t_search * 1e3, 2),
+                     "ms_per_launch": round(t_search * (FRAMES if head == "batch" else 1), 4),
+                     "pairs_per_launch": FRAMES if head == "batch" else 1,
                      "algorithmic_ops_per_px_pair": SEARCH_OPS_PER_PX},
-        "roofline_streaming": {"kernel": "k_ssim (IV-SSIM, both directions)", "bound": "hbm",
-                               "achieved": round(ssim_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                               "frac": round(ssim_gbs / PEAK_HBM_GBS, 4), "ms_per_launch": round(t_ssim, 4),
-                               "stats_kernel_gbs": round(stats_gbs, 1)},
-        "results_frame0": {"ivpsnr": last[0].ivpsnr, "ivssim": last[0].ivssim},
+        "roofline_streaming": {"k_ssim": {"bound": "hbm", "achieved": round(ssim_gbs, 1), "peak": PEAK_HBM_GBS,
+                                          "unit": "GB/s", "frac": round(ssim_gbs / PEAK_HBM_GBS, 4),
+                                          "us_per_pair": round(t_ssim * 1e3, 2),
+                                          "algorithmic_bytes_per_px_pair": SSIM_BYTES_PER_PX},
+                               "k_stats": {"bound": "hbm", "achieved": round(stats_gbs, 1), "peak": PEAK_HBM_GBS,
+                                           "unit": "GB/s", "frac": round(stats_gbs / PEAK_HBM_GBS, 4),
+                                           "us_per_pair": round(t_stats * 1e3, 2),
+                                           "algorithmic_bytes_per_px_pair": STATS_BYTES_PER_PX}},
+        "forms": {k: {"ms_per_step": round(v[0], 3), "value": round(px / (v[0] / 1e3) / 1e6, 1)}
+                  for k, v in results.items()},
+        "forms_agree": agree,
+        "tuning": prof.get("batch_tuning"),
+        "results_frame0": {"ivpsnr": res[-FRAMES].ivpsnr, "ivssim": res[-FRAMES].ivssim},
         "gen_seconds": round(t_gen, 1),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(pairs_np, args.cpu_frames)
-    ctx.close()
+        out["cpu_baseline"] = cpu_baseline(pairs_np, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(pairs_np, nframes):
+def cpu_baseline(pairs_np, seconds):
     """Reference library (oracle/_ref) on a bounded sample of the same workload."""
     try:
         from oracle.oracle import Reference, reference_available
@@ -150,11 +244,14 @@ def cpu_baseline(pairs_np, nframes):
         return {"value": None, "error": "oracle/_ref not built"}
     ref = Reference()
     threads = min(16, os.cpu_count() or 1)
-    sample = pairs_np[:nframes]
+    # probe two pairs, then size the sample to about `seconds` of work
+    t, _, _ = ref.bench_pairs(pairs_np[:2], W, H, BD, chroma420=True, erp=True, metrics=3, threads=threads)
+    n = max(2, min(8 * FRAMES, int(seconds / max(t / 2, 1e-3))))
+    sample = [pairs_np[i % FRAMES] for i in range(n)]
     t, iv, ss = ref.bench_pairs(sample, W, H, BD, chroma420=True, erp=True, metrics=3, threads=threads)
     return {"value": round(len(sample) * W * H / t / 1e6, 2), "unit": "Mpixel-pairs/s", "cores": threads,
             "kind": "reference",
-            "sample": f"{len(sample)} of the 17 frame pairs, IV-PSNR + IV-SSIM in the app's stage order "
+            "sample": f"{n} frame pairs (the 17 pairs cycled), IV-PSNR + IV-SSIM in the app's stage order "
                       f"(4:2:0 unpack, margins, rearrange, GCD, SCP, IV-PSNR, IV-SSIM), "
                       f"reference pmbb libs x86-64-{ref.level}, {threads} pool threads",
             "seconds": round(t, 2)}

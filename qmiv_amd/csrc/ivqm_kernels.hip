@@ -72,6 +72,33 @@ __device__ __forceinline__ int find_entry(const E* __restrict__ t, int n, int bl
 // ---------------------------------------------------------------------------
 // k_stats: one 256-thread block per plane row of some pair.
 // ---------------------------------------------------------------------------
+// Adds the differences r - a of one 16-byte vector of samples (8 x 16-bit or
+// 16 x 8-bit) to the row sums; a square fits uint32 (<= 65535^2).
+template <typename T>
+__device__ __forceinline__ void stats_vec(const uint4& va, const uint4& vr, int64_t& sd, uint64_t& ssd) {
+  const uint32_t wa[4] = {va.x, va.y, va.z, va.w}, wr[4] = {vr.x, vr.y, vr.z, vr.w};
+  int32_t s = 0;
+  uint64_t q = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (sizeof(T) == 2) {
+      const int d0 = (int)(wr[k] & 0xffffu) - (int)(wa[k] & 0xffffu);
+      const int d1 = (int)(wr[k] >> 16) - (int)(wa[k] >> 16);
+      s += d0 + d1;
+      q += (uint64_t)((uint32_t)d0 * (uint32_t)d0) + (uint64_t)((uint32_t)d1 * (uint32_t)d1);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int d = (int)((wr[k] >> (8 * b)) & 0xffu) - (int)((wa[k] >> (8 * b)) & 0xffu);
+        s += d;
+        q += (uint32_t)(d * d);
+      }
+    }
+  }
+  sd += s;
+  ssd += q;
+}
+
 template <typename T>
 __device__ void stats_row(const PairDev& P, int lb, int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd) {
   const int tid = threadIdx.x;
@@ -84,7 +111,24 @@ __device__ void stats_row(const PairDev& P, int lb, int64_t* __restrict__ row_sd
   const T* r = (const T*)P.p1.p[c] + (size_t)y * P.p1.stride[c];
   int64_t sd = 0;
   uint64_t ssd = 0;
-  for (int x = tid; x < wp; x += 256) {
+  // 16-byte loads where both rows are aligned (two in flight per thread), scalar tail
+  constexpr int V = 16 / sizeof(T);
+  int x_done = 0;
+  if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(r)) & 15) == 0) {
+    const int nv = wp / V;
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    const uint4* r4 = reinterpret_cast<const uint4*>(r);
+    for (int i = tid; i < nv; i += 512) {
+      const bool two = i + 256 < nv;
+      const uint4 va0 = a4[i], vr0 = r4[i];
+      uint4 va1 = make_uint4(0, 0, 0, 0), vr1 = va1;
+      if (two) { va1 = a4[i + 256]; vr1 = r4[i + 256]; }
+      stats_vec<T>(va0, vr0, sd, ssd);
+      if (two) stats_vec<T>(va1, vr1, sd, ssd);
+    }
+    x_done = nv * V;
+  }
+  for (int x = x_done + tid; x < wp; x += 256) {
     const int d = (int)r[x] - (int)a[x];
     sd += d;
     ssd += (uint64_t)((uint32_t)d * (uint32_t)d);
