@@ -6,7 +6,7 @@
 // entry by a binary search over the entries' first-block indices (uniform per
 // block, so it runs on the scalar unit).
 //
-//   k_stats          per plane row: sum(P1 - P0) and sum((P0 - P1)^2)
+//   k_stats          per plane row: sum(P1 - P0) and sum((P0 - P1)^2), one wave per row
 //                    (GCD input, PSNR / WS-PSNR row SSDs); zeroes the pair's
 //                    search row accumulators as a side job.
 //   k_gcd_finalize   per pair: frame totals + global colour difference (FP64
@@ -31,17 +31,33 @@
 
 #include <limits.h>
 
+#include <algorithm>
+
 #include <type_traits>
 
 namespace ivqm {
 
 namespace {
 
+// Plane c of a picture. Selects between loaded values instead of indexing
+// p[] / stride[] with a run-time c (or selecting between their addresses):
+// either keeps a private copy of the table entry in scratch memory.
+__device__ __forceinline__ const void* plane_ptr(const PicDev& P, int c) {
+  const void* p0 = P.p[0];
+  const void* p1 = P.p[1];
+  const void* p2 = P.p[2];
+  return c == 0 ? p0 : (c == 1 ? p1 : p2);
+}
+__device__ __forceinline__ int plane_stride(const PicDev& P, int c) {
+  const int s0 = P.stride[0], s1 = P.stride[1], s2 = P.stride[2];
+  return c == 0 ? s0 : (c == 1 ? s1 : s2);
+}
+
 template <typename T>
 __device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
   const int s = c ? P.cs : 0;
-  const T* p = (const T*)P.p[c];
-  return (int)p[(size_t)(y >> s) * (size_t)P.stride[c] + (x >> s)];
+  const T* p = (const T*)plane_ptr(P, c);
+  return (int)p[(size_t)(y >> s) * (size_t)plane_stride(P, c) + (x >> s)];
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -70,7 +86,8 @@ __device__ __forceinline__ int find_entry(const E* __restrict__ t, int n, int bl
 }
 
 // ---------------------------------------------------------------------------
-// k_stats: one 256-thread block per plane row of some pair.
+// k_stats: one wave per plane row (grid-stride over all rows of all pairs),
+// four 16-byte loads per picture in flight per lane; wave shuffles, no LDS.
 // ---------------------------------------------------------------------------
 // Adds the differences r - a of one 16-byte vector of samples (8 x 16-bit or
 // 16 x 8-bit) to the row sums; a square fits uint32 (<= 65535^2).
@@ -99,68 +116,71 @@ __device__ __forceinline__ void stats_vec(const uint4& va, const uint4& vr, int6
   ssd += q;
 }
 
+// Sums of one plane row (lb counts the rows of Y, then Cb, then Cr) by one wave.
 template <typename T>
-__device__ void stats_row(const PairDev& P, int lb, int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd) {
-  const int tid = threadIdx.x;
+__device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, int64_t* __restrict__ row_sd,
+                                          uint64_t* __restrict__ row_ssd) {
   const int H = P.h, W = P.w, Hc = H >> P.p0.cs, Wc = W >> P.p0.cs;
   int c, y, wp;
   if (lb < H) { c = 0; y = lb; wp = W; }
   else if (lb < H + Hc) { c = 1; y = lb - H; wp = Wc; }
   else { c = 2; y = lb - H - Hc; wp = Wc; }
-  const T* a = (const T*)P.p0.p[c] + (size_t)y * P.p0.stride[c];
-  const T* r = (const T*)P.p1.p[c] + (size_t)y * P.p1.stride[c];
+  const T* a = (const T*)plane_ptr(P.p0, c) + (size_t)y * plane_stride(P.p0, c);
+  const T* r = (const T*)plane_ptr(P.p1, c) + (size_t)y * plane_stride(P.p1, c);
   int64_t sd = 0;
   uint64_t ssd = 0;
-  // 16-byte loads where both rows are aligned (two in flight per thread), scalar tail
+  // 16-byte loads where both rows are aligned, scalar tail
   constexpr int V = 16 / sizeof(T);
   int x_done = 0;
   if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(r)) & 15) == 0) {
     const int nv = wp / V;
     const uint4* a4 = reinterpret_cast<const uint4*>(a);
     const uint4* r4 = reinterpret_cast<const uint4*>(r);
-    for (int i = tid; i < nv; i += 512) {
-      const bool two = i + 256 < nv;
-      const uint4 va0 = a4[i], vr0 = r4[i];
-      uint4 va1 = make_uint4(0, 0, 0, 0), vr1 = va1;
-      if (two) { va1 = a4[i + 256]; vr1 = r4[i + 256]; }
-      stats_vec<T>(va0, vr0, sd, ssd);
-      if (two) stats_vec<T>(va1, vr1, sd, ssd);
+    for (int i0 = 0; i0 < nv; i0 += 256) {
+      uint4 va[4], vr[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = i0 + 64 * k + lane;
+        va[k] = make_uint4(0, 0, 0, 0);
+        vr[k] = va[k];
+        if (i < nv) { va[k] = a4[i]; vr[k] = r4[i]; }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) stats_vec<T>(va[k], vr[k], sd, ssd);
     }
     x_done = nv * V;
   }
-  for (int x = x_done + tid; x < wp; x += 256) {
+  for (int x = x_done + lane; x < wp; x += 64) {
     const int d = (int)r[x] - (int)a[x];
     sd += d;
     ssd += (uint64_t)((uint32_t)d * (uint32_t)d);
   }
-  __shared__ int64_t s_sd[4];
-  __shared__ uint64_t s_ssd[4];
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     sd += __shfl_xor(sd, m, 64);
     ssd += __shfl_xor(ssd, m, 64);
   }
-  if ((tid & 63) == 0) { s_sd[tid >> 6] = sd; s_ssd[tid >> 6] = ssd; }
-  __syncthreads();
-  if (tid == 0) {
-    row_sd[P.rowsd_off + (int64_t)c * H + y] = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
-    row_ssd[P.rowsd_off + (int64_t)c * H + y] = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
+  if (lane == 0) {
+    row_sd[P.rowsd_off + (int64_t)c * H + y] = sd;
+    row_ssd[P.rowsd_off + (int64_t)c * H + y] = ssd;
   }
 }
 
-__global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs,
+// nrows = sum over pairs of H + 2 * Hc; a pair's rows start at stat_blk0.
+// Each luma row y also clears the six search row sums of y.
+__global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs, int nrows,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
                                                uint32_t* __restrict__ rows) {
-  const int b = blockIdx.x;
-  const int pi = find_entry(pairs, npairs, b, [](const PairDev& p) { return p.stat_blk0; });
-  const PairDev P = pairs[pi];
-  const int lb = b - P.stat_blk0;
-  {
-    const int z = lb * 256 + (int)threadIdx.x;
-    if (z < 6 * P.h) rows[P.rows_off + z] = 0u;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * 4;
+  for (int rw = blockIdx.x * 4 + (threadIdx.x >> 6); rw < nrows; rw += nw) {
+    const int pi = find_entry(pairs, npairs, rw, [](const PairDev& p) { return p.stat_blk0; });
+    const PairDev& P = pairs[pi];
+    const int lb = rw - P.stat_blk0;
+    if (lb < P.h && lane < 6) rows[P.rows_off + (int64_t)lane * P.h + lb] = 0u;
+    if (P.p0.bps == 2) stats_row<uint16_t>(P, lb, lane, row_sd, row_ssd);
+    else stats_row<uint8_t>(P, lb, lane, row_sd, row_ssd);
   }
-  if (P.p0.bps == 2) stats_row<uint16_t>(P, lb, row_sd, row_ssd);
-  else stats_row<uint8_t>(P, lb, row_sd, row_ssd);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,9 +457,9 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
   if (wy0 >= J.nuy) return;
   const int bx = chunk * 63 + lane;
   const bool col_ok = bx <= J.nux;
-  const TA* pa = (const TA*)J.a.p[c];
-  const TB* pb = (const TB*)J.b.p[c];
-  const int sa = J.a.stride[c], sb = J.b.stride[c];
+  const TA* pa = (const TA*)plane_ptr(J.a, c);
+  const TB* pb = (const TB*)plane_ptr(J.b, c);
+  const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
   uint32_t pA = 0, pB = 0;
   SQ pAA = 0, pBB = 0, pAB = 0;
   for (int k = 0; k <= NB; k++) {
@@ -609,7 +629,9 @@ __global__ __launch_bounds__(256) void k_upsample420(PicDev S, int w, int h, uin
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* rows, hipStream_t st) {
   if (npairs <= 0 || nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(256), 0, st, pairs, npairs, row_sd, row_ssd, rows);
+  // nblk = total plane rows; one wave per row, at most 8 blocks per CU resident
+  const int grid = std::min((nblk + 3) / 4, 2048);
+  hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows);
   return hipGetLastError();
 }
 
