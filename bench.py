@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "batch", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="approximate CPU work of the baseline sample")
+    ap.add_argument("--sections", default="plain_sync,ssim_family,mixed,served",
+                    help="extra sections: plain_sync, ssim_family, mixed (64 mixed pairs), served (Poisson loop)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="mean job arrivals per second")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference dispatch for HBM counter scaling)")
     return ap.parse_args()
@@ -130,6 +134,16 @@ def main():
 
         ms, res = timed(plain_step, lambda: [])
         results["plain"] = (ms, res)
+        if "plain_sync" in args.sections:
+            def plain_sync_step():
+                out = []
+                for a, b in pairs:
+                    ctx.bind(0, a, True)
+                    ctx.bind(1, b, True)
+                    out.append(ctx.frame(metrics))
+                return out
+            ms, res2 = timed(plain_sync_step, lambda: [])
+            results["plain_sync"] = (ms, res2)
         ctx.bind(0, pairs[0][0], True)
         ctx.bind(1, pairs[0][1], True)
         prof["plain"] = {k: ctx.profile(i, 20) for i, k in enumerate(("stats", "search", "ssim"))}
@@ -229,9 +243,177 @@ def main():
         "results_frame0": {"ivpsnr": res[-FRAMES].ivpsnr, "ivssim": res[-FRAMES].ivssim},
         "gen_seconds": round(t_gen, 1),
     }
+    sections = set(args.sections.split(",")) if args.sections else set()
+    if "ssim_family" in sections:
+        out["ssim_family"] = ssim_family(Q, torch, pairs, args)
+    del pairs
+    torch.cuda.empty_cache()
+    if "mixed" in sections:
+        out["mixed_batch"] = mixed_batch(Q, synth, torch, np, args)
+        torch.cuda.empty_cache()
+    if "served" in sections:
+        out["served"] = served(Q, synth, torch, np, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pairs_np, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def _time_steps(torch, step, steps, warmup, finish=None):
+    for _ in range(warmup):
+        step()
+    if finish:
+        finish()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    if finish:
+        finish()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def ssim_family(Q, torch, pairs, args):
+    """The 17 ERP pairs with SSIM, IV-SSIM, WS-PSNR and 5-scale MS-SSIM (batch form, hipGraph)."""
+    m = Q.IVQM_M_SSIM | Q.IVQM_M_IVSSIM | Q.IVQM_M_WSPSNR | Q.IVQM_M_MSSSIM
+    with Q.IvqmBatch(erp=1) as bt:
+        arr = Q.IvqmBatch.make_pairs([(a, b, W, H, BD, True) for a, b in pairs])
+        tuned = bt.autotune(arr, m)
+        bt.set_tuning(tuned[0], tuned[1], use_graph=True)
+        ms = _time_steps(torch, lambda: bt.run(arr, m), max(2, args.steps // 2), 1)
+    return {"metrics": "SSIM+IV-SSIM+WS-PSNR+MS-SSIM(5)", "pairs": FRAMES, "ms_per_step": round(ms, 3),
+            "value": round(FRAMES * W * H / (ms / 1e3) / 1e6, 1), "unit": "Mpixel-pairs/s"}
+
+
+def mixed_batch(Q, synth, torch, np, args):
+    """64 pairs of mixed sizes (16 each of 1280x720, 1920x1080, 2048x2048, 4096x2048), 8/10-bit
+    alternating, 4:2:0, all metrics: one pair per launch vs one batch (plain, hipGraph, autotuned)."""
+    items, px = [], 0
+    for i in range(64):
+        wl = synth.batch_workload(i)
+        u8 = wl.bit_depth == 8
+        p0, p1 = synth.make_pair(wl, i, as_u8=u8)
+        cv = (lambda a: torch.from_numpy(a).cuda()) if u8 else (lambda a: torch.from_numpy(a.view(np.int16)).cuda())
+        items.append(([cv(p) for p in p0], [cv(p) for p in p1], wl.width, wl.height, wl.bit_depth, True))
+        px += wl.width * wl.height
+    torch.cuda.synchronize()
+    m = Q.IVQM_M_ALL
+    steps, warm = max(2, args.steps // 3), 1
+    out = {"pairs": 64, "metrics": "all", "px_pairs_per_step": px}
+    with Q.IvqmBatch() as bt:
+        arr = Q.IvqmBatch.make_pairs(items)
+        singles = [Q.IvqmBatch.make_pairs([it]) for it in items]
+        state = {"k": 0}
+
+        def one_per_launch():
+            # one pair per enqueue, ring of 8 in flight
+            for i, a in enumerate(singles):
+                k = state["k"]
+                if k >= 8:
+                    bt.collect(k % 8, 1)
+                bt.enqueue(a, m, k % 8)
+                state["k"] = k + 1
+
+        def drain():
+            k = state["k"]
+            for j in range(max(0, k - 8), k):
+                bt.collect(j % 8, 1)
+            state["k"] = 0
+        bt.set_tuning(8, 8, use_graph=False)
+        out["one_pair_per_launch_ms"] = _time_steps(torch, one_per_launch, steps, warm, drain)
+        out["batched_ms"] = _time_steps(torch, lambda: bt.run(arr, m), steps, warm)
+        bt.set_tuning(8, 8, use_graph=True)
+        out["batched_graph_ms"] = _time_steps(torch, lambda: bt.run(arr, m), steps, warm)
+        tuned = bt.autotune(arr, m)
+        bt.set_tuning(tuned[0], tuned[1], use_graph=True)
+        out["batched_graph_autotuned_ms"] = _time_steps(torch, lambda: bt.run(arr, m), steps, warm)
+        out["tuning"] = {"search_tile_rows": tuned[0], "ssim_band_rows": tuned[1]}
+    for k in ("one_pair_per_launch", "batched", "batched_graph", "batched_graph_autotuned"):
+        ms = out[k + "_ms"]
+        out[k + "_ms"] = round(ms, 3)
+        out[k] = round(px / (ms / 1e3) / 1e6, 1)
+    out["unit"] = "Mpixel-pairs/s"
+    out["speedup_vs_one_pair_per_launch"] = round(out["batched_graph_autotuned"] / out["one_pair_per_launch"], 2)
+    return out
+
+
+def served(Q, synth, torch, np, args):
+    """Poisson arrivals of jobs (1-8 frame pairs, sizes and 8/10-bit depths uniform over the mixed set,
+    IV-PSNR or IV-PSNR+IV-SSIM 50/50) into the serving loop; reference sequences resident on the device,
+    test frames in pinned host memory, staged under a 64 GiB budget."""
+    rng = np.random.default_rng(2024)
+    srv = Q.IvqmServer(memory_budget=64 << 30, max_batch_pairs=16, max_inflight=4)
+    seqs = []
+    for (w, h) in synth.BATCH_SIZES:
+        for bd in (8, 10):
+            wl = synth.Workload(cfg=3, width=w, height=h, bit_depth=bd)
+            ref, tst = [], []
+            for f in range(8):
+                p0, p1 = synth.make_pair(wl, f, as_u8=(bd == 8))
+                ref.append(p1)
+                tst.append([torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).pin_memory()
+                            for p in p0])
+            sid = srv.add_sequence(ref, w, h, bd)
+            seqs.append((sid, w, h, tst))
+    kinds = [Q.IVQM_M_IVPSNR, Q.IVQM_M_IVPSNR | Q.IVQM_M_IVSSIM]
+    # warm-up: largest batches of both kinds (device buffers reach their working size)
+    tag = 1 << 40
+    for kind in kinds:
+        for sid, w, h, tst in seqs:
+            srv.submit(tag, sid, 0, tst[:8], kind)
+            tag += 1
+    srv.drain()
+    while srv.poll(64):
+        pass
+    jobs = {}
+    done = []
+    t0 = time.perf_counter()
+    t_next = t0
+    tag = 0
+    end = t0 + args.serve_seconds
+    while True:
+        now = time.perf_counter()
+        if now >= end:
+            break
+        if now >= t_next:
+            sid, w, h, tst = seqs[int(rng.integers(0, len(seqs)))]
+            nf = int(rng.integers(1, 9))
+            first = int(rng.integers(0, 8 - nf + 1))
+            kind = kinds[int(rng.integers(0, 2))]
+            srv.submit(tag, sid, first, tst[first:first + nf], kind)
+            jobs[tag] = (nf * w * h, kind)
+            tag += 1
+            t_next += rng.exponential(1.0 / args.serve_rate)
+            continue
+        done += srv.poll(16)
+        if t_next - time.perf_counter() > 0.0003:
+            time.sleep(0.0002)
+    srv.drain()
+    t_end = time.perf_counter()
+    while True:
+        r = srv.poll(64)
+        if not r:
+            break
+        done += r
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r.latency_us for r in done]) / 1e3
+    ok = all(r.status == 0 for r in done)
+    px = sum(jobs[r.tag][0] for r in done)
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 3) if len(a) else None
+    by_kind = {}
+    for kind, name in ((kinds[0], "ivpsnr"), (kinds[1], "ivpsnr+ivssim")):
+        a = np.array([r.latency_us for r in done if jobs[r.tag][1] == kind]) / 1e3
+        by_kind[name] = {"jobs": int(len(a)), "p50_ms": pct(a, 50), "p99_ms": pct(a, 99)}
+    return {"offered_jobs_per_s": args.serve_rate, "seconds": round(t_end - t0, 2), "jobs": len(done),
+            "all_ok": ok, "served_mpx_pairs_per_s": round(px / (t_end - t0) / 1e6, 1),
+            "offered_mpx_pairs_per_s": round(px / args.serve_seconds / 1e6, 1),
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "p999_ms": pct(lat, 99.9), "max_ms": pct(lat, 100),
+            "queue_p99_ms": pct(np.array([r.queue_us for r in done]) / 1e3, 99), "by_kind": by_kind,
+            "batches": st["batches"], "pairs": st["pairs_completed"], "max_batch_fill": st["max_batch_fill"],
+            "budget_peak_gib": round(st["budget_peak"] / 2**30, 3), "resident_gib": round(st["resident_bytes"] / 2**30, 3)}
 
 
 def cpu_baseline(pairs_np, seconds):

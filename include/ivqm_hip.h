@@ -38,6 +38,7 @@ extern "C" {
 #define IVQM_M_IVSSIM (1u << 5)
 #define IVQM_M_IVMSSSIM (1u << 6)
 #define IVQM_M_MSE (1u << 7)
+#define IVQM_M_ALL 0xFFu
 
 typedef struct ivqm_config {
   int32_t width, height;          /* luma size (even for 4:2:0) */
@@ -53,7 +54,10 @@ typedef struct ivqm_config {
   int32_t gcd_sd_wrap32;          /* reproduce the x86-64-v4 <=14-bit int32 SD accumulation (xDistortionAVX512.cpp:14-73) */
   int32_t search_tile_rows;       /* search tile height per wave (4 or 8); 0 = default (8) */
   int32_t ssim_wide_sums;         /* 64-bit squared sums in SSIM at any bit depth (always on above 14-bit) */
-  int32_t reserved[6];
+  int32_t search_mode_cap;        /* search arithmetic: 0 = fastest exact variant the ranges allow,
+                                     1 = reference-form int32 arithmetic only, 2 = no packed-int16 variant
+                                     (all variants give identical results) */
+  int32_t reserved[5];
 } ivqm_config;
 
 /* One picture: three planes of 8-bit (bytes_per_sample = 1) or 16-bit samples.

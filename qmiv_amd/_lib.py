@@ -31,7 +31,11 @@ EXPORTED = [
     "ivqm_batch_create", "ivqm_batch_destroy", "ivqm_batch_set_stream", "ivqm_batch_set_tuning",
     "ivqm_batch_get_tuning", "ivqm_batch_autotune", "ivqm_batch_run", "ivqm_batch_enqueue", "ivqm_batch_collect",
     "ivqm_batch_profile",
+    "ivqm_server_config_init", "ivqm_server_create", "ivqm_server_destroy", "ivqm_server_add_sequence",
+    "ivqm_server_remove_sequence", "ivqm_server_submit", "ivqm_server_poll", "ivqm_server_drain",
+    "ivqm_server_get_stats",
 ]
+MAX_JOB_FRAMES = 8
 
 
 class Config(C.Structure):
@@ -41,7 +45,7 @@ class Config(C.Structure):
         ("unnoticeable_coef", C.c_float * 4), ("erp", C.c_int32), ("lon_range_deg", C.c_int32),
         ("lat_range_deg", C.c_int32), ("ssim_window", C.c_int32), ("ssim_stride", C.c_int32),
         ("fake_vals_for_exact", C.c_int32), ("gcd_sd_wrap32", C.c_int32), ("search_tile_rows", C.c_int32),
-        ("ssim_wide_sums", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("ssim_wide_sums", C.c_int32), ("search_mode_cap", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -72,6 +76,38 @@ class FrameResult(C.Structure):
             v = getattr(self, name)
             out[name] = list(v) if hasattr(v, "__len__") else v
         return out
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("params", Config), ("memory_budget", C.c_uint64), ("max_batch_pairs", C.c_int32),
+                ("max_inflight", C.c_int32), ("reserved", C.c_int32 * 8)]
+
+
+class SequenceDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("bit_depth", C.c_int32), ("chroma_420", C.c_int32),
+                ("bytes_per_sample", C.c_int32), ("num_frames", C.c_int32), ("planes", C.POINTER(C.c_void_p)),
+                ("strides", C.POINTER(C.c_int32))]
+
+
+class Job(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("seq_id", C.c_int32), ("first_frame", C.c_int32), ("num_frames", C.c_int32),
+                ("metrics", C.c_uint32), ("planes_on_device", C.c_int32), ("reserved", C.c_int32),
+                ("planes", C.POINTER(C.c_void_p)), ("strides", C.POINTER(C.c_int32))]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("num_frames", C.c_int32), ("latency_us", C.c_double),
+                ("queue_us", C.c_double), ("frames", FrameResult * MAX_JOB_FRAMES)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_uint64), ("jobs_completed", C.c_uint64), ("jobs_failed", C.c_uint64),
+                ("pairs_completed", C.c_uint64), ("batches", C.c_uint64), ("budget_bytes", C.c_uint64),
+                ("budget_in_use", C.c_uint64), ("budget_peak", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("max_batch_fill", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -129,6 +165,15 @@ def load():
     L.ivqm_batch_enqueue.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.c_int32]
     L.ivqm_batch_collect.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FrameResult)]
     L.ivqm_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.ivqm_server_config_init.argtypes = [C.POINTER(ServerConfig)]
+    L.ivqm_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+    L.ivqm_server_destroy.argtypes = [C.c_void_p]
+    L.ivqm_server_add_sequence.argtypes = [C.c_void_p, C.POINTER(SequenceDesc), C.POINTER(C.c_int32)]
+    L.ivqm_server_remove_sequence.argtypes = [C.c_void_p, C.c_int32]
+    L.ivqm_server_submit.argtypes = [C.c_void_p, C.POINTER(Job)]
+    L.ivqm_server_poll.argtypes = [C.c_void_p, C.POINTER(JobResult), C.c_int32, C.POINTER(C.c_int32), C.c_int32]
+    L.ivqm_server_drain.argtypes = [C.c_void_p]
+    L.ivqm_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
     _lib = L
     return L
 

@@ -257,3 +257,103 @@ class IvqmBatch:
         ms = C.c_double()
         check(self.L.ivqm_batch_profile(self.h, kernel, iters, C.byref(ms)))
         return ms.value
+
+
+def _plane_ptrs(frames):
+    """frames: list of 3-plane pictures (numpy arrays, or torch tensors of one device) ->
+    (void* array, int32 stride array, on_device flag, objects to keep alive)."""
+    n = len(frames)
+    ptrs = (C.c_void_p * (3 * n))()
+    strides = (C.c_int32 * (3 * n))()
+    keep = []
+    on_device = False
+    for f, planes in enumerate(frames):
+        for k in range(3):
+            p = planes[k]
+            if hasattr(p, "data_ptr"):
+                on_device = bool(p.is_cuda)
+                ptrs[3 * f + k] = p.data_ptr()
+                strides[3 * f + k] = p.stride(0)
+            else:
+                a = np.ascontiguousarray(p)
+                ptrs[3 * f + k] = a.ctypes.data
+                strides[3 * f + k] = a.shape[1]
+                p = a
+            keep.append(p)
+    return ptrs, strides, on_device, keep
+
+
+class IvqmServer:
+    """Handle over one serving loop (include/ivqm_serve.h)."""
+
+    def __init__(self, memory_budget=0, max_batch_pairs=0, max_inflight=0, **kw):
+        self.L = _lib.load()
+        cfg = _lib.ServerConfig()
+        check(self.L.ivqm_server_config_init(C.byref(cfg)))
+        base = default_config(64, 64, 10, **kw)
+        cfg.params = base
+        if memory_budget:
+            cfg.memory_budget = int(memory_budget)
+        if max_batch_pairs:
+            cfg.max_batch_pairs = int(max_batch_pairs)
+        if max_inflight:
+            cfg.max_inflight = int(max_inflight)
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(self.L.ivqm_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._jobs = {}
+
+    def close(self):
+        if self.h:
+            check(self.L.ivqm_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_sequence(self, frames, width, height, bit_depth, chroma_420=True):
+        """frames: list of 3-plane pictures (host numpy or device torch); copied to the device once."""
+        ptrs, strides, _, keep = _plane_ptrs(frames)
+        p0 = keep[0]
+        bps = p0.element_size() if hasattr(p0, "element_size") else p0.dtype.itemsize
+        d = _lib.SequenceDesc(width, height, bit_depth, int(chroma_420), bps, len(frames), ptrs, strides)
+        sid = C.c_int32()
+        check(self.L.ivqm_server_add_sequence(self.h, C.byref(d), C.byref(sid)))
+        return sid.value
+
+    def remove_sequence(self, seq_id):
+        check(self.L.ivqm_server_remove_sequence(self.h, seq_id))
+
+    def submit(self, tag, seq_id, first_frame, frames, metrics):
+        """frames: test pictures (host numpy or device torch); kept referenced until polled."""
+        ptrs, strides, on_dev, keep = _plane_ptrs(frames)
+        j = _lib.Job(tag, seq_id, first_frame, len(frames), metrics, int(on_dev), 0, ptrs, strides)
+        check(self.L.ivqm_server_submit(self.h, C.byref(j)))
+        self._jobs[tag] = (ptrs, strides, keep)
+
+    def poll(self, max_results=64, wait_us=0):
+        out = (_lib.JobResult * max_results)()
+        n = C.c_int32()
+        check(self.L.ivqm_server_poll(self.h, out, max_results, C.byref(n), wait_us))
+        res = [out[i] for i in range(n.value)]
+        for r in res:
+            self._jobs.pop(r.tag, None)
+        return res
+
+    def drain(self):
+        check(self.L.ivqm_server_drain(self.h))
+
+    def stats(self):
+        st = _lib.ServerStats()
+        check(self.L.ivqm_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()

@@ -36,10 +36,12 @@ struct PairSpec {
   int w = 0, h = 0, bd = 8;
   bool has_gcd = false;
   int32_t gcd[4] = {0, 0, 0, 0};
+  uint32_t metrics = 0;  // this pair's IVQM_M_* set; 0 = the batch's set
 };
 
 // Per-pair bookkeeping the epilogue needs.
 struct PairPlan {
+  uint32_t m;  // metrics of this pair
   int w, h, bd, cs;
   int64_t rows_off, rowsd_off, scp_off;
   int ns;
@@ -105,6 +107,8 @@ class Engine {
   int enqueue(const PairSpec* pairs, int n, uint32_t metrics, int ring, bool use_graph = false);
   // Waits for ring slot `ring` and writes one result per pair.
   int collect(int ring, ivqm_frame_result* out);
+  // 1 when ring slot `ring` has finished on the device (or holds no batch), 0 if not, < 0 on error.
+  int query(int ring);
   int sync();
 
   void set_tuning(int py, int nb) {
@@ -136,7 +140,7 @@ class Engine {
   int materialize(const PicRef& r, PicDev& out) const;
   void drop_graphs();
   int erp_offset(int h, int64_t* off);
-  bool narrow_ok(int maxv, int32_t gcd_abs_max) const;
+  int search_mode(int maxv, int32_t gcd_abs_max) const;
 
   ivqm_config cfg_;
   hipStream_t stream_ = nullptr;

@@ -128,17 +128,21 @@ int Engine::sync() {
   return IVQM_OK;
 }
 
-bool Engine::narrow_ok(int maxv, int32_t gmax) const {
+int Engine::search_mode(int maxv, int32_t gmax) const {
+  if (cfg_.search_mode_cap == 1) return 0;  // exact variant only
   const int32_t* w = cfg_.cmp_weights_search;
   for (int k = 0; k < 3; k++)
-    if (w[k] < 0) return false;
+    if (w[k] < 0) return 0;
   const int64_t A = (int64_t)maxv + gmax;
   const int64_t S = 1 << key_bits(cfg_.search_range);
   const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
   const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
-  if (3 * S * wsum * A * A >= (1ll << 31)) return false;
-  if (2 * S * wmax * A >= (1ll << 23)) return false;
-  return true;
+  // key = S * (Q(r) - 2 sum w t r) + idx stays inside int32 and the 24-bit multiplier operands
+  if (3 * S * wsum * A * A >= (1ll << 31)) return 0;
+  if (2 * S * wmax * A >= (1ll << 23)) return 0;
+  // packed variant: S * r and 2 * w * t are int16 operands
+  if (cfg_.search_mode_cap != 2 && S * maxv <= 32767 && 2 * wmax * A <= 32767) return 2;
+  return 1;
 }
 
 const std::pair<std::vector<double>, double>& Engine::erp_weights(int h) {
@@ -177,7 +181,8 @@ int Engine::erp_offset(int h, int64_t* off) {
 
 int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   const bool reuse = (metrics & kReuseScp) != 0, scp_only = (metrics & kScpOnly) != 0;
-  const uint32_t m = metrics & 0xFFu;
+  uint32_t m = metrics & 0xFFu;
+  for (int i = 0; i < n; i++) m |= pairs[i].metrics & 0xFFu;  // the batch covers every pair's set
   const uint32_t ivs = IVQM_M_IVSSIM | IVQM_M_IVMSSSIM;
   P = BatchPlan();
   P.metrics = m;
@@ -204,6 +209,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     PairPlan& pp = P.pp[i];
     PairDev& d = P.pairs[i];
     std::memset(&d, 0, sizeof(d));
+    const uint32_t mp = s.metrics ? (s.metrics & 0xFFu) : m;
+    pp.m = mp;
     pp.w = s.w; pp.h = s.h; pp.bd = s.bd; pp.cs = s.p0.cs;
     d.p0 = s.p0;
     d.p1 = s.p1;
@@ -215,15 +222,16 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       d.md[k] = host::round_f32(cfg_.unnoticeable_coef[k] * (float)d.maxv);  // xGlobClrDiff.cpp:29
       gmax = std::max<int32_t>(gmax, s.has_gcd ? std::abs(s.gcd[k]) : std::abs(d.md[k]));
     }
-    d.narrow = narrow_ok(d.maxv, gmax) ? 1 : 0;
+    d.mode = search_mode(d.maxv, gmax);
     d.wrap32 = (cfg_.gcd_sd_wrap32 && s.bd <= 14) ? 1 : 0;
     d.gcd_given = s.has_gcd ? 1 : 0;
     for (int k = 0; k < 4; k++) d.gcd_in[k] = s.has_gcd ? s.gcd[k] : 0;
     d.res_idx = i;
     d.stat_blk0 = P.stat_blocks;
     P.stat_blocks += s.h + 2 * (s.h >> s.p0.cs);
-    d.tile0 = P.tiles;
-    P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
+    d.tile0 = P.tiles;  // pairs without a search own no tiles (the table lookup skips them)
+    if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only)))
+      P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
     d.rowsd_off = P.rowsd_total;
     pp.rowsd_off = d.rowsd_off;
     P.rowsd_total += 3 * (int64_t)s.h;
@@ -231,7 +239,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     pp.rows_off = d.rows_off;
     P.rows_total += 6 * (int64_t)s.h;
     d.scp_pitch = scp_pitch(s.w);
-    if (P.need_scp || (reuse && (m & ivs))) {
+    if ((P.need_scp && ((mp & ivs) || scp_only)) || (reuse && (mp & ivs))) {
       d.scp_off = P.scp_total;
       P.scp_total += 6 * (int64_t)d.scp_pitch * s.h;
     } else {
@@ -241,7 +249,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.best_off = (best_on_ && n == 1) ? 0 : -1;
     pp.ns = ms_num_scales(s.w, s.h);
     if (cfg_.erp) pp.erp_h = s.h;
-    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps);
+    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps, mp);
     P.signature += sigbuf;
   }
   // pyramid offsets: per pair, pictures 0..3 (P0, P1, SCP0, SCP1), levels 1..ns-1
@@ -249,9 +257,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   for (int i = 0; i < n; i++) {
     PairPlan& pp = P.pp[i];
     if (!ms_any || pp.ns <= 1) continue;
+    const uint32_t mp = pp.m;
+    if (!(mp & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM))) continue;
     for (int which = 0; which < 4; which++) {
-      if (which >= 2 && !(m & IVQM_M_IVMSSSIM)) continue;
-      if (which < 2 && !(m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM))) continue;
+      if (which >= 2 && !(mp & IVQM_M_IVMSSSIM)) continue;
       for (int l = 1; l < pp.ns; l++) {
         pyr[i][which][l] = P.pyr_total;
         P.pyr_total += 3 * (int64_t)scp_pitch(pp.w >> l) * (pp.h >> l);
@@ -283,8 +292,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   for (int i = 0; i < n; i++) {
     const PairPlan& pp = P.pp[i];
     if (!ms_any || pp.ns <= 1) continue;
+    const uint32_t mp = pp.m;
+    if (!(mp & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM))) continue;
     for (int which = 0; which < 4; which++) {
-      if (which >= 2 && !(m & IVQM_M_IVMSSSIM)) continue;
+      if (which >= 2 && !(mp & IVQM_M_IVMSSSIM)) continue;
       for (int l = 1; l < pp.ns; l++) {
         DownJob dj;
         std::memset(&dj, 0, sizeof(dj));
@@ -334,16 +345,17 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   };
   for (int i = 0; i < n; i++) {
     PairPlan& pp = P.pp[i];
-    if (m & IVQM_M_SSIM) RCCHK(add_job(i, &pp.ssim, ref_for(i, 0, 0), ref_for(i, 1, 0), pp.w, pp.h, 1, 0));
-    if (m & IVQM_M_IVSSIM) {
+    const uint32_t mp = pp.m;
+    if (mp & IVQM_M_SSIM) RCCHK(add_job(i, &pp.ssim, ref_for(i, 0, 0), ref_for(i, 1, 0), pp.w, pp.h, 1, 0));
+    if (mp & IVQM_M_IVSSIM) {
       RCCHK(add_job(i, &pp.iv_t2r, ref_for(i, 0, 0), ref_for(i, 3, 0), pp.w, pp.h, 1, cfg_.erp));
       RCCHK(add_job(i, &pp.iv_r2t, ref_for(i, 1, 0), ref_for(i, 2, 0), pp.w, pp.h, 1, cfg_.erp));
     }
-    if (ms_any) {
+    if (mp & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) {
       for (int l = 0; l < pp.ns; l++) {
         const int w = pp.w >> l, h = pp.h >> l, cl = (l == pp.ns - 1) ? 1 : 0;
-        if (m & IVQM_M_MSSSIM) RCCHK(add_job(i, &pp.ms[l], ref_for(i, 0, l), ref_for(i, 1, l), w, h, cl, 0));
-        if (m & IVQM_M_IVMSSSIM) {
+        if (mp & IVQM_M_MSSSIM) RCCHK(add_job(i, &pp.ms[l], ref_for(i, 0, l), ref_for(i, 1, l), w, h, cl, 0));
+        if (mp & IVQM_M_IVMSSSIM) {
           RCCHK(add_job(i, &pp.ivms_t2r[l], ref_for(i, 0, l), ref_for(i, 3, l), w, h, cl, 0));
           RCCHK(add_job(i, &pp.ivms_r2t[l], ref_for(i, 1, l), ref_for(i, 2, l), w, h, cl, 0));
         }
@@ -583,6 +595,16 @@ void ms_combine(const BatchPlan& P, const double* out, int ns, const int* idx, d
 }
 }  // namespace
 
+int Engine::query(int ring) {
+  if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+  RingSlot& S = ring_[ring];
+  if (!S.busy) return 1;
+  const hipError_t e = hipEventQuery(S.ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return hip_error(e, "hipEventQuery");
+}
+
 int Engine::collect(int ring, ivqm_frame_result* out) {
   if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
   RingSlot& S = ring_[ring];
@@ -590,10 +612,10 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
   HIPCHK(hipEventSynchronize(S.ev));
   S.busy = false;
   const BatchPlan& P = S.plan;
-  const uint32_t m = P.metrics;
   const int32_t* cwa = cfg_.cmp_weights_average;
   for (int i = 0; i < P.n; i++) {
     const PairPlan& pp = P.pp[i];
+    const uint32_t m = pp.m;
     const int W = pp.w, H = pp.h, bd = pp.bd;
     const int64_t area = (int64_t)W * H;
     ivqm_frame_result r;

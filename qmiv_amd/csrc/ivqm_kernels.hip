@@ -67,6 +67,19 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;
 }
+// Sum over the 64 lanes with DPP (no LDS traffic), uniform result: prefix
+// sums within each 16-lane row (row_shr 1, 2, 4, 8; lanes shifted in from
+// outside the row read 0), then row_bcast:15 / row_bcast:31 carry the row
+// totals up, and lane 63 holds the wave total.
+__device__ __forceinline__ uint32_t wave_sum_u32_dpp(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -257,8 +270,17 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
 // precomputed per reference pixel. Nothing wraps in this range, so
 // argmin E == argmin E'; the candidate's raster index is packed into the low
 // KB bits of key = 2^KB * E' + idx, which makes a plain integer min equal to
-// "first strict minimum in raster order".
+// "first strict minimum in raster order". Per candidate: three 24-bit
+// multiply-adds and a min.
+//
+// DOT variant (narrower ranges, host-checked: S * Max and 2 * w * (Max + |gcd|)
+// fit int16): the same key, with the reference samples stored pre-scaled by
+// S in LDS, {Y*S, Cb*S} packed as int16 pairs: one v_dot2 (Y, Cb), one 24-bit
+// multiply-add (Cr) and a min per candidate.
 // ---------------------------------------------------------------------------
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2 };
 template <int R> struct KeyBits {
   static constexpr int n = (2 * R + 1) * (2 * R + 1);
   static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
@@ -268,7 +290,7 @@ template <int R, int PY> struct SearchTile {
   static constexpr int TW = kSearchTileW, TH = 4 * PY, LW = TW + 2 * R, LH = TH + 2 * R;
 };
 
-template <typename T, int R, int PY, bool NARROW>
+template <typename T, int R, int PY, int MODE>
 __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*lds)[SearchTile<R, PY>::LW],
                             int wY, int wCb, int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
                             uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
@@ -288,8 +310,12 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     const int x = clampi(x0 + lx - R, 0, W - 1), y = clampi(y0 + ly - R, 0, H - 1);
     const int rY = ldpix<T>(Ref, 0, x, y), rCb = ldpix<T>(Ref, 1, x, y), rCr = ldpix<T>(Ref, 2, x, y);
     int q = 0;
-    if (NARROW) q = (wY * __mul24(rY, rY) + wCb * __mul24(rCb, rCb) + wCr * __mul24(rCr, rCr)) << KB;
-    lds[ly][lx] = make_int4(q, rY, rCb, rCr);
+    if (MODE != kSearchWide) q = (wY * __mul24(rY, rY) + wCb * __mul24(rCb, rCb) + wCr * __mul24(rCr, rCr)) << KB;
+    if (MODE == kSearchDot)
+      lds[ly][lx] = make_int4(q, (int)(((uint32_t)(rY << KB) & 0xffffu) | ((uint32_t)(rCb << KB) << 16)), rCr << KB,
+                              (int)((uint32_t)rY | ((uint32_t)rCb << 16)));
+    else
+      lds[ly][lx] = make_int4(q, rY, rCb, rCr);
   }
   __syncthreads();
 
@@ -308,7 +334,40 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     bi[i] = 0;
   }
 
-  if (NARROW) {
+  if (MODE == kSearchDot) {
+    short2_t mp[PY];
+    int m2[PY];
+#pragma unroll
+    for (int i = 0; i < PY; i++) {
+      mp[i] = short2_t{(short)(-2 * wY * t0[i]), (short)(-2 * wCb * t1[i])};
+      m2[i] = -2 * wCr * t2[i];
+    }
+#pragma unroll 1
+    for (int ry = 0; ry < PY + 2 * R; ry++) {
+      int3 v[D];
+#pragma unroll
+      for (int dx = 0; dx < D; dx++) {
+        const int4 e = lds[wv * PY + ry][tx + dx];
+        v[dx] = make_int3(e.x + dx, e.y, e.z);
+      }
+#pragma unroll
+      for (int i = 0; i < PY; i++) {
+        const int dy = ry - i;
+        if ((unsigned)dy >= (unsigned)D) continue;
+        int rm = INT_MAX;
+#pragma unroll
+        for (int dx = 0; dx < D; dx++) {
+          // multiply-add into a fresh register first: v_dot2c accumulates in place
+          int acc = __mul24(m2[i], v[dx].z) + v[dx].x;
+          acc = __builtin_amdgcn_sdot2(mp[i], __builtin_bit_cast(short2_t, v[dx].y), acc, false);
+          rm = min(rm, acc);
+        }
+        bk[i] = min(bk[i], rm + dy * D);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PY; i++) bi[i] = bk[i] & (S - 1);
+  } else if (MODE == kSearchNarrow) {
     int m0[PY], m1[PY], m2[PY];
 #pragma unroll
     for (int i = 0; i < PY; i++) {
@@ -374,14 +433,16 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     const int y = yb + i;
     const bool ok = (x < W) && (y < H);
     const int ddy = bi[i] / D, ddx = bi[i] - ddy * D;
-    const int4 r = lds[wv * PY + i + ddy][tx + ddx];
+    const int4 re = lds[wv * PY + i + ddy][tx + ddx];
+    int4 r = re;  // {_, Y, Cb, Cr}
+    if (MODE == kSearchDot) r = make_int4(0, re.w & 0xffff, (int)((uint32_t)re.w >> 16), re.z >> KB);
     const int e0 = t0[i] - r.y, e1 = t1[i] - r.z, e2 = t2[i] - r.w;
     uint32_t q0 = ok ? (uint32_t)__mul24(e0, e0) : 0u;
     uint32_t q1 = ok ? (uint32_t)__mul24(e1, e1) : 0u;
     uint32_t q2 = ok ? (uint32_t)__mul24(e2, e2) : 0u;
-    q0 = wave_sum_u32(q0);
-    q1 = wave_sum_u32(q1);
-    q2 = wave_sum_u32(q2);
+    q0 = wave_sum_u32_dpp(q0);
+    q1 = wave_sum_u32_dpp(q1);
+    q2 = wave_sum_u32_dpp(q2);
     if (tx == 0 && y < H) {
       atomicAdd(&prow[0 * (size_t)H + y], q0);
       atomicAdd(&prow[1 * (size_t)H + y], q1);
@@ -414,11 +475,19 @@ __global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pair
   const int ty = lt / ntx, tx = lt - ty * ntx;
   const int x0 = tx * TT::TW, y0 = ty * TT::TH;
   if (P.p0.bps == 2) {
-    if (P.narrow) search_tile<uint16_t, R, PY, true>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else search_tile<uint16_t, R, PY, false>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    if (P.mode == kSearchDot)
+      search_tile<uint16_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchNarrow)
+      search_tile<uint16_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else
+      search_tile<uint16_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   } else {
-    if (P.narrow) search_tile<uint8_t, R, PY, true>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else search_tile<uint8_t, R, PY, false>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    if (P.mode == kSearchDot)
+      search_tile<uint8_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchNarrow)
+      search_tile<uint8_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else
+      search_tile<uint8_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   }
 }
 

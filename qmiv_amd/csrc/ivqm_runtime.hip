@@ -63,6 +63,7 @@ int validate_config(const ivqm_config* cfg, bool need_size) {
   if (need_size && (cfg->width < 8 || cfg->height < 8)) return set_error(IVQM_EINVAL, "picture must be at least 8x8");
   if (need_size && (cfg->bit_depth < 8 || cfg->bit_depth > 16)) return set_error(IVQM_EINVAL, "bit_depth must be 8..16");
   if (cfg->search_range < 1 || cfg->search_range > 4) return set_error(IVQM_EINVAL, "search_range must be 1..4");
+  if (cfg->search_mode_cap < 0 || cfg->search_mode_cap > 2) return set_error(IVQM_EINVAL, "search_mode_cap must be 0..2");
   if (cfg->ssim_window != 8 || cfg->ssim_stride != 4)
     return set_error(IVQM_EINVAL, "only the default block-averaged SSIM (window 8, stride 4) is implemented");
   if (cfg->erp && (cfg->lat_range_deg <= 0 || cfg->lat_range_deg > 180)) return set_error(IVQM_EINVAL, "lat_range_deg");

@@ -114,6 +114,36 @@ def test_gpu_wide_path_large_weights(oracle):
         assert (repr(v), repr(a), repr(b)) == tuple(repr(x) for x in ov)
 
 
+@pytest.mark.parametrize("bd,R,u8", [(10, 2, False), (8, 2, True), (8, 4, True), (9, 3, False), (10, 1, False), (10, 3, False),
+                                      (12, 2, False)])
+def test_gpu_search_variants_identical(oracle, bd, R, u8):
+    """The packed-int16 dot, packed-key and reference-form searches give the same
+    rows, SCP pictures, winner maps and dB values; the default one also matches
+    the oracle."""
+    wl = synth.Workload(cfg=207, width=200, height=72, bit_depth=bd, global_shift=1, block_shift=R)
+    p0n, p1n = synth.make_pair(wl, 0, as_u8=u8)
+    out = []
+    for cap in (0, 1, 2):
+        with _ctx(wl, R=R, search_mode_cap=cap) as ctx:
+            ctx.upload_pair(p0n, p1n, True)
+            ctx.enable_best_map(True)
+            r = ctx.frame(Q.IVQM_M_IVPSNR | Q.IVQM_M_IVSSIM)
+            out.append((r.as_dict(), ctx.rows(0), ctx.rows(1), ctx.best_map(0), ctx.best_map(1),
+                        [ctx.scp(0)[c].copy() for c in range(3)], [ctx.scp(1)[c].copy() for c in range(3)]))
+    for o in out[1:]:
+        assert o[0] == out[0][0]
+        for a, b in zip(o[1:5], out[0][1:5]):
+            assert np.array_equal(a, b)
+        for c in range(3):
+            assert np.array_equal(o[5][c], out[0][5][c]) and np.array_equal(o[6][c], out[0][6][c])
+    p0 = oracle.to444([p.astype(np.uint16) for p in p0n], True)
+    p1 = oracle.to444([p.astype(np.uint16) for p in p1n], True)
+    g = out[0][0]["gcd_r2t"]
+    ov = oracle.ivpsnr(p0, p1, bd, g, R=R)
+    assert (repr(out[0][0]["ivpsnr"]), repr(out[0][0]["ivpsnr_r2t"]), repr(out[0][0]["ivpsnr_t2r"])) == \
+        tuple(repr(x) for x in ov)
+
+
 def test_gpu_per_metric_calls_match_fused_frame(oracle):
     wl = synth.Workload(cfg=203, width=256, height=128, global_shift=1)
     p0n, p1n = synth.make_pair(wl, 1)
