@@ -87,6 +87,7 @@ struct RingSlot {
   uint64_t* r_row_ssd = nullptr;
   uint32_t* r_rows = nullptr;
   double* r_ssim = nullptr;
+  int64_t c_pairs = 0, c_rows = 0, c_rowsd = 0, c_jobs = 0, c_down = 0;  // capacities of the buffers above
   hipEvent_t ev = nullptr;
   bool busy = false;
   BatchPlan plan;
@@ -134,7 +135,7 @@ class Engine {
  private:
   int plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P);
   int ensure_capacity(const BatchPlan& P);
-  int ensure_ring_capacity();
+  int ensure_slot_capacity(RingSlot& S, const BatchPlan& P);
   int issue(const BatchPlan& P, RingSlot& S);
   int fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_first[5]);
   int materialize(const PicRef& r, PicDev& out) const;
@@ -152,7 +153,6 @@ class Engine {
   // capacities (elements)
   int64_t cap_pairs_ = 0, cap_rows_ = 0, cap_rowsd_ = 0, cap_scp_ = 0, cap_pyr_ = 0, cap_partial_ = 0,
           cap_jobs_ = 0, cap_down_ = 0, cap_best_ = 0;
-  int64_t ring_pairs_ = 0, ring_rows_ = 0, ring_rowsd_ = 0, ring_jobs_ = 0, ring_down_ = 0;
 
   PairDev* d_pairs_ = nullptr;
   SsimJobDev* d_jobs_ = nullptr;

@@ -413,30 +413,50 @@ int Engine::ensure_capacity(const BatchPlan& P) {
   if (best_need > cap_best_) { cap_best_ = best_need; RCCHK(dmalloc(&d_best_, cap_best_)); }
   // device buffers moved: captured graphs are stale
   drop_graphs();
-  return ensure_ring_capacity();
+  return IVQM_OK;
 }
 
-int Engine::ensure_ring_capacity() {
-  if (ring_pairs_ >= cap_pairs_ && ring_rows_ >= cap_rows_ && ring_rowsd_ >= cap_rowsd_ && ring_jobs_ >= cap_jobs_ &&
-      ring_down_ >= cap_down_)
-    return IVQM_OK;
-  for (auto& S : ring_) {
-    if (S.busy) { HIPCHK(hipEventSynchronize(S.ev)); S.busy = false; }
-    RCCHK(hmalloc(&S.t_pairs, cap_pairs_));
-    RCCHK(hmalloc(&S.t_jobs, cap_jobs_));
-    RCCHK(hmalloc(&S.t_down, cap_down_));
-    RCCHK(hmalloc(&S.r_gcd, 4 * cap_pairs_));
-    RCCHK(hmalloc(&S.r_sd, 3 * cap_pairs_));
-    RCCHK(hmalloc(&S.r_ssd, 3 * cap_pairs_));
-    RCCHK(hmalloc(&S.r_row_ssd, cap_rowsd_));
-    RCCHK(hmalloc(&S.r_rows, cap_rows_));
-    RCCHK(hmalloc(&S.r_ssim, 6 * cap_jobs_));
+// Pinned tables and result buffers belong to one slot: growing them never
+// touches another slot, whose results may still wait to be collected.
+int Engine::ensure_slot_capacity(RingSlot& S, const BatchPlan& P) {
+  const int64_t nj = (int64_t)P.jobs.size();
+  int64_t nd = 0;
+  for (int l = 1; l < 5; l++) nd += (int64_t)P.down[l].size();
+  bool moved = false;
+  if (P.n > S.c_pairs) {
+    S.c_pairs = grow(P.n, S.c_pairs);
+    RCCHK(hmalloc(&S.t_pairs, S.c_pairs));
+    RCCHK(hmalloc(&S.r_gcd, 4 * S.c_pairs));
+    RCCHK(hmalloc(&S.r_sd, 3 * S.c_pairs));
+    RCCHK(hmalloc(&S.r_ssd, 3 * S.c_pairs));
+    moved = true;
   }
-  ring_pairs_ = cap_pairs_;
-  ring_rows_ = cap_rows_;
-  ring_rowsd_ = cap_rowsd_;
-  ring_jobs_ = cap_jobs_;
-  ring_down_ = cap_down_;
+  if (nj > S.c_jobs || !S.t_jobs) {
+    S.c_jobs = grow(std::max<int64_t>(nj, 1), S.c_jobs);
+    RCCHK(hmalloc(&S.t_jobs, S.c_jobs));
+    RCCHK(hmalloc(&S.r_ssim, 6 * S.c_jobs));
+    moved = true;
+  }
+  if (nd > S.c_down || !S.t_down) {
+    S.c_down = grow(std::max<int64_t>(nd, 1), S.c_down);
+    RCCHK(hmalloc(&S.t_down, S.c_down));
+    moved = true;
+  }
+  if (P.rowsd_total > S.c_rowsd || !S.r_row_ssd) {
+    S.c_rowsd = grow(std::max<int64_t>(P.rowsd_total, 1), S.c_rowsd);
+    RCCHK(hmalloc(&S.r_row_ssd, S.c_rowsd));
+    moved = true;
+  }
+  if (P.rows_total > S.c_rows || !S.r_rows) {
+    S.c_rows = grow(std::max<int64_t>(P.rows_total, 1), S.c_rows);
+    RCCHK(hmalloc(&S.r_rows, S.c_rows));
+    moved = true;
+  }
+  if (moved && S.graph) {  // its copy nodes refer to the old buffers
+    (void)hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+    S.graph_sig.clear();
+  }
   return IVQM_OK;
 }
 
@@ -535,6 +555,7 @@ int Engine::enqueue(const PairSpec* pairs, int n, uint32_t metrics, int ring, bo
   BatchPlan P;
   RCCHK(plan(pairs, n, metrics, P));
   RCCHK(ensure_capacity(P));
+  RCCHK(ensure_slot_capacity(S, P));
   // pointer identity enters the graph signature: a replay reuses the captured tables' device pointers
   // only through the staging copy, so the signature needs shapes only
   if (use_graph) {

@@ -93,3 +93,19 @@ def test_batch_matches_oracle_on_a_mixed_pair(oracle):
         assert (r.ivpsnr, r.ivpsnr_r2t, r.ivpsnr_t2r) == v
         s0, s1 = oracle.scp(p0, p1, wl.bit_depth, g)
         assert abs(r.ivssim - oracle.ivssim(p0, p1, s0, s1, wl.bit_depth)[0]) <= 1e-6
+
+
+def test_batch_ring_results_survive_buffer_growth():
+    """Batches waiting in ring slots keep their results while later, larger
+    batches grow the engine's buffers (one pair per enqueue, growing sizes)."""
+    items, host = _dev_pairs()
+    order = sorted(range(len(items)), key=lambda i: MIXED[i].width * MIXED[i].height)
+    want = {i: _single(*host[i], Q.IVQM_M_ALL) for i in order}
+    with IvqmBatch() as b:
+        for use_graph in (False, True):
+            b.set_tuning(0, 0, use_graph)
+            for ring, i in enumerate(order):
+                b.enqueue(b.make_pairs([items[i]]), Q.IVQM_M_ALL, ring)
+            for ring, i in enumerate(order):
+                (got,) = b.collect(ring, 1)
+                assert got.as_dict() == want[i], (use_graph, MIXED[i])
