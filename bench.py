@@ -51,8 +51,10 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "batch", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="approximate CPU work of the baseline sample")
-    ap.add_argument("--sections", default="plain_sync,ssim_family,mixed,served",
-                    help="extra sections: plain_sync, ssim_family, mixed (64 mixed pairs), served (Poisson loop)")
+    ap.add_argument("--sections", default="plain_sync,ssim_family,mixed,served,stress",
+                    help="extra sections: plain_sync, ssim_family, mixed (64 mixed pairs), served (Poisson loop), "
+                         "stress (8192x4096 16-bit R=4)")
+    ap.add_argument("--stress-frames", type=int, default=8)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="mean job arrivals per second")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -253,6 +255,9 @@ def main():
         torch.cuda.empty_cache()
     if "served" in sections:
         out["served"] = served(Q, synth, torch, np, args)
+        torch.cuda.empty_cache()
+    if "stress" in sections:
+        out["stress"] = stress(Q, synth, torch, np, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pairs_np, args.cpu_seconds)
     print(json.dumps(out), flush=True)
@@ -335,6 +340,37 @@ def mixed_batch(Q, synth, torch, np, args):
     out["unit"] = "Mpixel-pairs/s"
     out["speedup_vs_one_pair_per_launch"] = round(out["batched_graph_autotuned"] / out["one_pair_per_launch"], 2)
     return out
+
+
+def stress(Q, synth, torch, np, args):
+    """8192x4096 ERP, 16-bit 4:4:4 and 4:2:0, R=4, IV-PSNR + IV-SSIM + MS-SSIM, `stress_frames` pairs per
+    variant as one batch (hipGraph, autotuned)."""
+    m = Q.IVQM_M_IVPSNR | Q.IVQM_M_IVSSIM | Q.IVQM_M_MSSSIM
+    res = {"metrics": "IV-PSNR+IV-SSIM+MS-SSIM", "search_range": 4, "frames": args.stress_frames}
+    for wl in (synth.STRESS_8K_444, synth.STRESS_8K_420):
+        items = []
+        for f in range(args.stress_frames):
+            p0, p1 = synth.make_pair(wl, f)
+            items.append(([torch.from_numpy(p.view(np.int16)).cuda() for p in p0],
+                          [torch.from_numpy(p.view(np.int16)).cuda() for p in p1], wl.width, wl.height, 16,
+                          wl.chroma_420))
+        torch.cuda.synchronize()
+        px = args.stress_frames * wl.width * wl.height
+        with Q.IvqmBatch(search_range=4, erp=1) as bt:
+            arr = Q.IvqmBatch.make_pairs(items)
+            tuned = bt.autotune(arr, m)
+            bt.set_tuning(tuned[0], tuned[1], use_graph=True)
+            ms = _time_steps(torch, lambda: bt.run(arr, m), 2, 1)
+            t_search = bt.profile(1, 3)
+        name = "444" if not wl.chroma_420 else "420"
+        ops = wl.width * wl.height * args.stress_frames * 2 * 81 * 12
+        res[name] = {"ms_per_step": round(ms, 2), "value": round(px / (ms / 1e3) / 1e6, 1),
+                     "unit": "Mpixel-pairs/s", "search_ms_per_launch": round(t_search, 3),
+                     "search_algorithmic_tops": round(ops / (t_search * 1e-3) / 1e12, 2),
+                     "tuning": {"search_tile_rows": tuned[0], "ssim_band_rows": tuned[1]}}
+        del items, arr
+        torch.cuda.empty_cache()
+    return res
 
 
 def served(Q, synth, torch, np, args):

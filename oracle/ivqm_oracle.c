@@ -136,6 +136,8 @@ OR_EXPORT void or_cps(const uint16_t* const tst[3], const uint16_t* const ref[3]
                       int32_t bit_depth, const int32_t gcs[4], int32_t R, const int32_t cw[4], uint64_t* rows,
                       uint16_t* const scp[3], uint8_t* best) {
   const int32_t maxv = (1 << bit_depth) - 1;
+  /* rows are independent: threads over y change no result */
+#pragma omp parallel for schedule(dynamic, 16)
   for (int32_t y = 0; y < h; y++) {
     uint32_t row[4] = {0, 0, 0, 0};
     for (int32_t x = 0; x < w; x++) {
@@ -345,8 +347,11 @@ OR_EXPORT int32_t or_ms_num_scales(int32_t w, int32_t h) {
  * rowsums receives one value per window row (NumUnitY entries). */
 OR_EXPORT void or_ssim_rowsums(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t wnd,
                                int32_t stride, double c1, double c2, int32_t calc_l, double* rowsums) {
-  int32_t j = 0;
-  for (int32_t y = 0; y < h - wnd + 1; y += stride, j++) {
+  const int32_t ny = h < wnd ? 0 : (h - wnd) / stride + 1;
+  /* window rows are independent: threads over rows change no result */
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int32_t j = 0; j < ny; j++) {
+    const int32_t y = j * stride;
     or_kbns k = {0, 0};
     for (int32_t x = 0; x < w - wnd + 1; x += stride)
       or_kbns_acc(&k, or_ssim_window(tst + (size_t)y * w + x, ref + (size_t)y * w + x, w, w, wnd, c1, c2, calc_l));
