@@ -23,7 +23,8 @@ struct PicDev {
 struct PairDev {
   PicDev p0, p1;
   int32_t w, h, maxv;
-  int32_t mode;           // search variant: 0 exact (wide), 1 narrow key, 2 packed dot key (host-checked ranges)
+  int32_t mode;           // search variant: 0 reference form, 1 narrow key, 2 packed dot key, 3 reference form
+                          // with 24-bit multiplies (host-checked ranges)
   int32_t md[3];          // GCD clip limits round_f32(coef * Max)
   int32_t wrap32;         // x86-64-v4 int32 SD accumulation
   int32_t stat_blk0;      // first k_stats block of this pair

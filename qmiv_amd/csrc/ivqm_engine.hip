@@ -129,17 +129,21 @@ int Engine::sync() {
 }
 
 int Engine::search_mode(int maxv, int32_t gmax) const {
-  if (cfg_.search_mode_cap == 1) return 0;  // exact variant only
+  if (cfg_.search_mode_cap == 1) return 0;  // reference-form arithmetic only
   const int32_t* w = cfg_.cmp_weights_search;
-  for (int k = 0; k < 3; k++)
-    if (w[k] < 0) return 0;
   const int64_t A = (int64_t)maxv + gmax;
+  int64_t wabs = 0;
+  for (int k = 0; k < 3; k++) wabs = std::max<int64_t>(wabs, std::llabs((long long)w[k]));
+  // (d*w)*d by 24-bit multiplies: |d| <= A and |d * w| < 2^23
+  const int wide = (A * wabs < (1ll << 23)) ? 3 : 0;
+  for (int k = 0; k < 3; k++)
+    if (w[k] < 0) return wide;
   const int64_t S = 1 << key_bits(cfg_.search_range);
   const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
   const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
   // key = S * (Q(r) - 2 sum w t r) + idx stays inside int32 and the 24-bit multiplier operands
-  if (3 * S * wsum * A * A >= (1ll << 31)) return 0;
-  if (2 * S * wmax * A >= (1ll << 23)) return 0;
+  if (3 * S * wsum * A * A >= (1ll << 31)) return wide;
+  if (2 * S * wmax * A >= (1ll << 23)) return wide;
   // packed variant: S * r and 2 * w * t are int16 operands
   if (cfg_.search_mode_cap != 2 && S * maxv <= 32767 && 2 * wmax * A <= 32767) return 2;
   return 1;

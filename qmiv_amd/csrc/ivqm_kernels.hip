@@ -53,11 +53,16 @@ __device__ __forceinline__ int plane_stride(const PicDev& P, int c) {
   return c == 0 ? s0 : (c == 1 ? s1 : s2);
 }
 
+// Sample (x, y) of plane c on the luma grid. Plane offsets are 32-bit with
+// 24-bit operands (rows and strides stay below 2^24, planes below 2^32 samples).
+// Picture planes are device allocations: addressed as global memory they load
+// with a scalar base and a 32-bit vector offset (no 64-bit address math).
 template <typename T>
 __device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
+  typedef __attribute__((address_space(1))) const T GT;
   const int s = c ? P.cs : 0;
-  const T* p = (const T*)plane_ptr(P, c);
-  return (int)p[(size_t)(y >> s) * (size_t)plane_stride(P, c) + (x >> s)];
+  const GT* p = (const GT*)plane_ptr(P, c);
+  return (int)p[__umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> s)];
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -280,7 +285,11 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
 // ---------------------------------------------------------------------------
 typedef short short2_t __attribute__((ext_vector_type(2)));
 
-enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2 };
+//
+// WIDE24 variant (host-checked (Max + |gcd|) * |w| < 2^23): the reference-form
+// int32 arithmetic, with each weighted square (d*d)*w mod 2^32 computed as
+// (d*w)*d by two 24-bit multiplies -- the same value mod 2^32.
+enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3 };
 template <int R> struct KeyBits {
   static constexpr int n = (2 * R + 1) * (2 * R + 1);
   static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
@@ -305,12 +314,53 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
   const int g1 = dir ? -g4[1] : g4[1];
   const int g2 = dir ? -g4[2] : g4[2];
 
-  for (int i = threadIdx.x; i < LW * LH; i += 256) {
+  // Every global load of the tile (reference entries, then this lane's test
+  // samples) is issued before any is used: one memory round trip per block.
+  constexpr int NE = (LW * LH + 255) / 256;  // tile entries per thread
+  int ra[NE][3];
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    const int i = threadIdx.x + 256 * k;
+    ra[k][0] = ra[k][1] = ra[k][2] = 0;
+    if (i < LW * LH) {
+      const int ly = i / LW, lx = i - ly * LW;
+      const int x = clampi(x0 + lx - R, 0, W - 1), y = clampi(y0 + ly - R, 0, H - 1);
+      ra[k][0] = ldpix<T>(Ref, 0, x, y);
+      ra[k][1] = ldpix<T>(Ref, 1, x, y);
+      ra[k][2] = ldpix<T>(Ref, 2, x, y);
+    }
+  }
+
+  const int tx = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x = x0 + tx, xc = min(x, W - 1);
+  const int yb = y0 + wv * PY;
+
+  int s0[PY], s1[PY], s2[PY];  // raw test samples
+  int t0[PY], t1[PY], t2[PY], bk[PY], bi[PY];
+#pragma unroll
+  for (int i = 0; i < PY; i++) {
+    const int y = min(yb + i, H - 1);
+    s0[i] = ldpix<T>(Tst, 0, xc, y);
+    s1[i] = ldpix<T>(Tst, 1, xc, y);
+    s2[i] = ldpix<T>(Tst, 2, xc, y);
+  }
+
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    const int i = threadIdx.x + 256 * k;
+    if (i >= LW * LH) break;
     const int ly = i / LW, lx = i - ly * LW;
-    const int x = clampi(x0 + lx - R, 0, W - 1), y = clampi(y0 + ly - R, 0, H - 1);
-    const int rY = ldpix<T>(Ref, 0, x, y), rCb = ldpix<T>(Ref, 1, x, y), rCr = ldpix<T>(Ref, 2, x, y);
+    const int rY = ra[k][0], rCb = ra[k][1], rCr = ra[k][2];
     int q = 0;
-    if (MODE != kSearchWide) q = (wY * __mul24(rY, rY) + wCb * __mul24(rCb, rCb) + wCr * __mul24(rCr, rCr)) << KB;
+    // Packed variants only (host-checked): w < 2^16, w * r < 2^24 and Q fits int32, so
+    // 24-bit unsigned multiplies are exact (the masks tell the compiler the ranges).
+    if (MODE != kSearchWide && MODE != kSearchWide24) {
+      const uint32_t uY = (uint32_t)wY & 0xffffu, uCb = (uint32_t)wCb & 0xffffu, uCr = (uint32_t)wCr & 0xffffu;
+      q = (int)(__umul24(__umul24(uY, (uint32_t)rY), (uint32_t)rY) +
+                __umul24(__umul24(uCb, (uint32_t)rCb), (uint32_t)rCb) +
+                __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr))
+          << KB;
+    }
     if (MODE == kSearchDot)
       lds[ly][lx] = make_int4(q, (int)(((uint32_t)(rY << KB) & 0xffffu) | ((uint32_t)(rCb << KB) << 16)), rCr << KB,
                               (int)((uint32_t)rY | ((uint32_t)rCb << 16)));
@@ -319,28 +369,30 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
   }
   __syncthreads();
 
-  const int tx = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int x = x0 + tx, xc = min(x, W - 1);
-  const int yb = y0 + wv * PY;
-
-  int t0[PY], t1[PY], t2[PY], bk[PY], bi[PY];
 #pragma unroll
   for (int i = 0; i < PY; i++) {
-    const int y = min(yb + i, H - 1);
-    t0[i] = ldpix<T>(Tst, 0, xc, y) + g0;
-    t1[i] = ldpix<T>(Tst, 1, xc, y) + g1;
-    t2[i] = ldpix<T>(Tst, 2, xc, y) + g2;
+    t0[i] = s0[i] + g0;
+    t1[i] = s1[i] + g1;
+    t2[i] = s2[i] + g2;
     bk[i] = INT_MAX;
     bi[i] = 0;
   }
+  // Multipliers -k w t = -(k w s) - k w g of the packed variants: k w s by an exact
+  // 24-bit multiply (host-checked k w (Max + |g|) < 2^23), k w g uniform.
+  auto mult = [](int kw, int s, int g) -> int {
+    const uint32_t u = (uint32_t)kw & 0x7fffffu;
+    int p = (int)__umul24(u, (uint32_t)s);
+    asm volatile("" : "+v"(p));  // keeps the 24-bit product (the compiler would refold it into a 32-bit multiply)
+    return -p - kw * g;
+  };
 
   if (MODE == kSearchDot) {
     short2_t mp[PY];
     int m2[PY];
 #pragma unroll
     for (int i = 0; i < PY; i++) {
-      mp[i] = short2_t{(short)(-2 * wY * t0[i]), (short)(-2 * wCb * t1[i])};
-      m2[i] = -2 * wCr * t2[i];
+      mp[i] = short2_t{(short)mult(2 * wY, s0[i], g0), (short)mult(2 * wCb, s1[i], g1)};
+      m2[i] = mult(2 * wCr, s2[i], g2);
     }
 #pragma unroll 1
     for (int ry = 0; ry < PY + 2 * R; ry++) {
@@ -371,9 +423,9 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     int m0[PY], m1[PY], m2[PY];
 #pragma unroll
     for (int i = 0; i < PY; i++) {
-      m0[i] = -(2 * S) * wY * t0[i];
-      m1[i] = -(2 * S) * wCb * t1[i];
-      m2[i] = -(2 * S) * wCr * t2[i];
+      m0[i] = mult(2 * S * wY, s0[i], g0);
+      m1[i] = mult(2 * S * wCb, s1[i], g1);
+      m2[i] = mult(2 * S * wCr, s2[i], g2);
     }
 #pragma unroll 1
     for (int ry = 0; ry < PY + 2 * R; ry++) {
@@ -412,10 +464,19 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
         if ((unsigned)dy >= (unsigned)D) continue;
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
-          const uint32_t d0 = (uint32_t)__mul24(t0[i] - v[dx].y, t0[i] - v[dx].y);
-          const uint32_t d1 = (uint32_t)__mul24(t1[i] - v[dx].z, t1[i] - v[dx].z);
-          const uint32_t d2 = (uint32_t)__mul24(t2[i] - v[dx].w, t2[i] - v[dx].w);
-          const int e = (int)(d0 * (uint32_t)wY + d1 * (uint32_t)wCb + d2 * (uint32_t)wCr);
+          int e;
+          if (MODE == kSearchWide24) {
+            const int d0 = t0[i] - v[dx].y, d1 = t1[i] - v[dx].z, d2 = t2[i] - v[dx].w;
+            uint32_t acc = (uint32_t)__mul24(__mul24(d0, wY), d0);
+            acc += (uint32_t)__mul24(__mul24(d1, wCb), d1);
+            acc += (uint32_t)__mul24(__mul24(d2, wCr), d2);
+            e = (int)acc;
+          } else {
+            const uint32_t d0 = (uint32_t)__mul24(t0[i] - v[dx].y, t0[i] - v[dx].y);
+            const uint32_t d1 = (uint32_t)__mul24(t1[i] - v[dx].z, t1[i] - v[dx].z);
+            const uint32_t d2 = (uint32_t)__mul24(t2[i] - v[dx].w, t2[i] - v[dx].w);
+            e = (int)(d0 * (uint32_t)wY + d1 * (uint32_t)wCb + d2 * (uint32_t)wCr);
+          }
           if (e < bk[i]) { bk[i] = e; bi[i] = dy * D + dx; }
         }
       }
@@ -450,12 +511,12 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     }
     if (ok) {
       if (scp_pic) {
-        const size_t o = (size_t)y * Pr.scp_pitch + x;
+        const uint32_t o = __umul24((uint32_t)y, (uint32_t)Pr.scp_pitch) + (uint32_t)x;
         scp_pic[o] = (uint16_t)clampi(r.y - g0, 0, Pr.maxv);
         scp_pic[plane_sz + o] = (uint16_t)clampi(r.z - g1, 0, Pr.maxv);
         scp_pic[2 * plane_sz + o] = (uint16_t)clampi(r.w - g2, 0, Pr.maxv);
       }
-      if (bmap) bmap[(size_t)y * W + x] = (uint8_t)bi[i];
+      if (bmap) bmap[__umul24((uint32_t)y, (uint32_t)W) + (uint32_t)x] = (uint8_t)bi[i];
     }
   }
 }
@@ -479,6 +540,8 @@ __global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pair
       search_tile<uint16_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
       search_tile<uint16_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchWide24)
+      search_tile<uint16_t, R, PY, kSearchWide24>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
       search_tile<uint16_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   } else {
@@ -486,6 +549,8 @@ __global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pair
       search_tile<uint8_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
       search_tile<uint8_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchWide24)
+      search_tile<uint8_t, R, PY, kSearchWide24>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
       search_tile<uint8_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   }
