@@ -81,6 +81,10 @@ hipError_t launch_upsample420(const PicDev& src, int w, int h, uint16_t* dst, in
 
 constexpr int kSearchTileW = 64;
 inline int search_tile_h(int py) { return 4 * py; }
+// Rows per wave the search runs for search range R and a requested 4 or 8.
+inline int search_rows(int R, int py) { return R >= 3 ? 4 : py; }
 constexpr int kDownBlkW = 64, kDownBlkH = 4;
+// MI355X: 8 XCDs, each with its own L2; consecutive workgroups go to consecutive XCDs.
+constexpr int kXcds = 8;
 
 }  // namespace ivqm

@@ -116,7 +116,7 @@ class Engine {
     if (py == 4 || py == 8) py_ = py;
     if (nb == 4 || nb == 8 || nb == 16) nb_ = nb;
   }
-  int tuning_py() const { return py_; }
+  int tuning_py() const { return search_rows(cfg_.search_range, py_); }
   int tuning_nb() const { return nb_; }
   void enable_best(bool on) { best_on_ = on; }
   // Times `iters` launches of one phase of the last enqueued batch (HIP events).

@@ -191,7 +191,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   P = BatchPlan();
   P.metrics = m;
   P.n = n;
-  P.py = py_;
+  P.py = search_rows(cfg_.search_range, py_);
   P.nb = nb_;
   P.need_search = (m & IVQM_M_IVPSNR) || (!reuse && (m & ivs)) || scp_only;
   P.need_scp = (!reuse && (m & ivs)) || scp_only;

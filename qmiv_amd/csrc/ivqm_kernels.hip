@@ -55,6 +55,15 @@ __device__ __forceinline__ int plane_stride(const PicDev& P, int c) {
 
 // Sample (x, y) of plane c on the luma grid. Plane offsets are 32-bit with
 // 24-bit operands (rows and strides stay below 2^24, planes below 2^32 samples).
+// Row y (luma grid) of plane c as a global pointer; with a wave-uniform y the
+// pointer lives in scalar registers and a load adds only the lane's column.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(const PicDev& P, int c, int y) {
+  typedef __attribute__((address_space(1))) const T GT;
+  const int s = c ? P.cs : 0;
+  return (const GT*)plane_ptr(P, c) + __umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c));
+}
+
 // Picture planes are device allocations: addressed as global memory they load
 // with a scalar base and a 32-bit vector offset (no 64-bit address math).
 template <typename T>
@@ -65,7 +74,7 @@ __device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
   return (int)p[__umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> s)];
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // v_med3_i32
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
@@ -84,6 +93,27 @@ __device__ __forceinline__ uint32_t wave_sum_u32_dpp(uint32_t v) {
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+// Three wave sums with their DPP steps interleaved (independent instructions
+// between dependent DPP reads instead of wait states).
+__device__ __forceinline__ void wave_sum3_u32_dpp(uint32_t v[3]) {
+#define IVQM_DPP3(ctrl, rmask, bc)                                                                     \
+  {                                                                                                   \
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[0], ctrl, rmask, 0xf, bc);   \
+    const uint32_t a1 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[1], ctrl, rmask, 0xf, bc);   \
+    const uint32_t a2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[2], ctrl, rmask, 0xf, bc);   \
+    v[0] += a0;                                                                                       \
+    v[1] += a1;                                                                                       \
+    v[2] += a2;                                                                                       \
+  }
+  IVQM_DPP3(0x111, 0xf, true)
+  IVQM_DPP3(0x112, 0xf, true)
+  IVQM_DPP3(0x114, 0xf, true)
+  IVQM_DPP3(0x118, 0xf, true)
+  IVQM_DPP3(0x142, 0xa, false)
+  IVQM_DPP3(0x143, 0xc, false)
+#undef IVQM_DPP3
+  for (int k = 0; k < 3; k++) v[k] = (uint32_t)__builtin_amdgcn_readlane((int)v[k], 63);
 }
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
@@ -300,7 +330,7 @@ template <int R, int PY> struct SearchTile {
 };
 
 template <typename T, int R, int PY, int MODE>
-__device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*lds)[SearchTile<R, PY>::LW],
+__device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*lds)[SearchTile<R, PY>::LW],
                             int wY, int wCb, int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
                             uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
   constexpr int D = 2 * R + 1;
@@ -314,46 +344,91 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
   const int g1 = dir ? -g4[1] : g4[1];
   const int g2 = dir ? -g4[2] : g4[2];
 
-  // Every global load of the tile (reference entries, then this lane's test
-  // samples) is issued before any is used: one memory round trip per block.
-  constexpr int NE = (LW * LH + 255) / 256;  // tile entries per thread
-  int ra[NE][3];
+  // Every global load of the tile (reference rows, halo columns, then this
+  // lane's test samples) is issued before any is used: one memory round trip
+  // per block. Wave wv stages tile rows wv, wv + 4, ...: the row (and its plane
+  // offsets) is wave-uniform and each lane keeps one clamped column, so a load
+  // costs one vector add. The 2R halo columns are staged by threads < 2R * LH.
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  // Tile rows are staged in pairs of consecutive picture rows whose first row
+  // is even, (2m, 2m + 1): with 4:2:0 chroma both read the same chroma row,
+  // loaded once (clamped rows keep that property for even heights).
+  constexpr int PAR = R & 1;                  // tile row 0 is picture row y0 - R, odd for odd R
+  constexpr int NPT = (LH + PAR + 1) / 2;     // row pairs of the tile
+  constexpr int NPW = (NPT + 3) / 4;          // row pairs per wave
+  const int cx = clampi(x0 - R + lane, 0, W - 1);
+  int ra[NPW][2][3];
 #pragma unroll
-  for (int k = 0; k < NE; k++) {
-    const int i = threadIdx.x + 256 * k;
-    ra[k][0] = ra[k][1] = ra[k][2] = 0;
-    if (i < LW * LH) {
-      const int ly = i / LW, lx = i - ly * LW;
-      const int x = clampi(x0 + lx - R, 0, W - 1), y = clampi(y0 + ly - R, 0, H - 1);
-      ra[k][0] = ldpix<T>(Ref, 0, x, y);
-      ra[k][1] = ldpix<T>(Ref, 1, x, y);
-      ra[k][2] = ldpix<T>(Ref, 2, x, y);
+  for (int k = 0; k < NPW; k++) {
+    const int j = wv + 4 * k;
+    const int ly0 = 2 * j - PAR, ly1 = ly0 + 1;
+#pragma unroll
+    for (int h = 0; h < 2; h++) ra[k][h][0] = ra[k][h][1] = ra[k][h][2] = 0;
+    if (j < NPT) {
+      const int ya = clampi(y0 - R + ly0, 0, H - 1), yb2 = clampi(y0 - R + ly1, 0, H - 1);
+      if (ly0 >= 0) ra[k][0][0] = (int)plane_row<T>(Ref, 0, ya)[cx];
+      if (ly1 < LH) ra[k][1][0] = (int)plane_row<T>(Ref, 0, yb2)[cx];
+      if (Ref.cs) {
+        const int yc = ly0 >= 0 ? ya : yb2;
+        const int cb = (int)plane_row<T>(Ref, 1, yc)[cx >> 1], cr = (int)plane_row<T>(Ref, 2, yc)[cx >> 1];
+        ra[k][0][1] = ra[k][1][1] = cb;
+        ra[k][0][2] = ra[k][1][2] = cr;
+      } else {
+        if (ly0 >= 0) {
+          ra[k][0][1] = (int)plane_row<T>(Ref, 1, ya)[cx];
+          ra[k][0][2] = (int)plane_row<T>(Ref, 2, ya)[cx];
+        }
+        if (ly1 < LH) {
+          ra[k][1][1] = (int)plane_row<T>(Ref, 1, yb2)[cx];
+          ra[k][1][2] = (int)plane_row<T>(Ref, 2, yb2)[cx];
+        }
+      }
+    }
+  }
+  constexpr int NH = 2 * R * LH;          // halo entries (columns 64 .. 64 + 2R - 1)
+  constexpr int NHK = (NH + 255) / 256;   // halo entries per thread
+  int rh[NHK][3];
+#pragma unroll
+  for (int k = 0; k < NHK; k++) {
+    const int e = (int)threadIdx.x + 256 * k;
+    rh[k][0] = rh[k][1] = rh[k][2] = 0;
+    if (e < NH) {
+      const int hy = e / (2 * R), hx = e - hy * (2 * R);
+      const int x = clampi(x0 - R + 64 + hx, 0, W - 1), y = clampi(y0 - R + hy, 0, H - 1);
+      rh[k][0] = ldpix<T>(Ref, 0, x, y);
+      rh[k][1] = ldpix<T>(Ref, 1, x, y);
+      rh[k][2] = ldpix<T>(Ref, 2, x, y);
     }
   }
 
-  const int tx = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tx = lane;
   const int x = x0 + tx, xc = min(x, W - 1);
   const int yb = y0 + wv * PY;
 
   int s0[PY], s1[PY], s2[PY];  // raw test samples
   int t0[PY], t1[PY], t2[PY], bk[PY], bi[PY];
+  static_assert(PY % 2 == 0, "row pairs");
 #pragma unroll
-  for (int i = 0; i < PY; i++) {
-    const int y = min(yb + i, H - 1);
-    s0[i] = ldpix<T>(Tst, 0, xc, y);
-    s1[i] = ldpix<T>(Tst, 1, xc, y);
-    s2[i] = ldpix<T>(Tst, 2, xc, y);
+  for (int i = 0; i < PY; i += 2) {  // yb + i is even: a pair shares its 4:2:0 chroma row
+    const int ya = min(yb + i, H - 1), yb2 = min(yb + i + 1, H - 1);
+    s0[i] = (int)plane_row<T>(Tst, 0, ya)[xc];
+    s0[i + 1] = (int)plane_row<T>(Tst, 0, yb2)[xc];
+    if (Tst.cs) {
+      s1[i] = s1[i + 1] = (int)plane_row<T>(Tst, 1, ya)[xc >> 1];
+      s2[i] = s2[i + 1] = (int)plane_row<T>(Tst, 2, ya)[xc >> 1];
+    } else {
+      s1[i] = (int)plane_row<T>(Tst, 1, ya)[xc];
+      s2[i] = (int)plane_row<T>(Tst, 2, ya)[xc];
+      s1[i + 1] = (int)plane_row<T>(Tst, 1, yb2)[xc];
+      s2[i + 1] = (int)plane_row<T>(Tst, 2, yb2)[xc];
+    }
   }
 
-#pragma unroll
-  for (int k = 0; k < NE; k++) {
-    const int i = threadIdx.x + 256 * k;
-    if (i >= LW * LH) break;
-    const int ly = i / LW, lx = i - ly * LW;
-    const int rY = ra[k][0], rCb = ra[k][1], rCr = ra[k][2];
+  // Packed variants only (host-checked): w < 2^16, w * r < 2^24 and Q fits int32,
+  // so 24-bit unsigned multiplies are exact (the masks tell the compiler the ranges).
+  auto entry = [&](int rY, int rCb, int rCr) -> int4 {
     int q = 0;
-    // Packed variants only (host-checked): w < 2^16, w * r < 2^24 and Q fits int32, so
-    // 24-bit unsigned multiplies are exact (the masks tell the compiler the ranges).
     if (MODE != kSearchWide && MODE != kSearchWide24) {
       const uint32_t uY = (uint32_t)wY & 0xffffu, uCb = (uint32_t)wCb & 0xffffu, uCr = (uint32_t)wCr & 0xffffu;
       q = (int)(__umul24(__umul24(uY, (uint32_t)rY), (uint32_t)rY) +
@@ -362,10 +437,26 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
           << KB;
     }
     if (MODE == kSearchDot)
-      lds[ly][lx] = make_int4(q, (int)(((uint32_t)(rY << KB) & 0xffffu) | ((uint32_t)(rCb << KB) << 16)), rCr << KB,
-                              (int)((uint32_t)rY | ((uint32_t)rCb << 16)));
-    else
-      lds[ly][lx] = make_int4(q, rY, rCb, rCr);
+      return make_int4(q, (int)(((uint32_t)(rY << KB) & 0xffffu) | ((uint32_t)(rCb << KB) << 16)), rCr << KB,
+                       (int)((uint32_t)rY | ((uint32_t)rCb << 16)));
+    return make_int4(q, rY, rCb, rCr);
+  };
+#pragma unroll
+  for (int k = 0; k < NPW; k++) {
+    const int j = wv + 4 * k;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int ly = 2 * j - PAR + h;
+      if (j < NPT && ly >= 0 && ly < LH) lds[ly][lane] = entry(ra[k][h][0], ra[k][h][1], ra[k][h][2]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NHK; k++) {
+    const int e = (int)threadIdx.x + 256 * k;
+    if (e < NH) {
+      const int hy = e / (2 * R), hx = e - hy * (2 * R);
+      lds[hy][64 + hx] = entry(rh[k][0], rh[k][1], rh[k][2]);
+    }
   }
   __syncthreads();
 
@@ -486,9 +577,12 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
   // Epilogue: winner's squared differences, row sums, SCP, winner map.
   uint32_t* prow = rows + Pr.rows_off + (size_t)dir * 3 * H;
   const size_t plane_sz = (size_t)Pr.scp_pitch * H;
+  typedef __attribute__((address_space(1))) uint16_t GU16;
+  typedef __attribute__((address_space(1))) uint8_t GU8;
   // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0)
-  uint16_t* scp_pic = Pr.scp_off >= 0 ? scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz : nullptr;
-  uint8_t* bmap = Pr.best_off >= 0 ? best + Pr.best_off + (size_t)dir * W * H : nullptr;
+  GU16* scp_pic = Pr.scp_off >= 0 ? (GU16*)scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz : nullptr;
+  GU8* bmap = Pr.best_off >= 0 ? (GU8*)best + Pr.best_off + (size_t)dir * W * H : nullptr;
+  int rsum = 0;  // row sums of the wave, one (row, component) per lane
 #pragma unroll
   for (int i = 0; i < PY; i++) {
     const int y = yb + i;
@@ -498,37 +592,45 @@ __device__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*l
     int4 r = re;  // {_, Y, Cb, Cr}
     if (MODE == kSearchDot) r = make_int4(0, re.w & 0xffff, (int)((uint32_t)re.w >> 16), re.z >> KB);
     const int e0 = t0[i] - r.y, e1 = t1[i] - r.z, e2 = t2[i] - r.w;
-    uint32_t q0 = ok ? (uint32_t)__mul24(e0, e0) : 0u;
-    uint32_t q1 = ok ? (uint32_t)__mul24(e1, e1) : 0u;
-    uint32_t q2 = ok ? (uint32_t)__mul24(e2, e2) : 0u;
-    q0 = wave_sum_u32_dpp(q0);
-    q1 = wave_sum_u32_dpp(q1);
-    q2 = wave_sum_u32_dpp(q2);
-    if (tx == 0 && y < H) {
-      atomicAdd(&prow[0 * (size_t)H + y], q0);
-      atomicAdd(&prow[1 * (size_t)H + y], q1);
-      atomicAdd(&prow[2 * (size_t)H + y], q2);
-    }
+    uint32_t q[3] = {ok ? (uint32_t)__mul24(e0, e0) : 0u, ok ? (uint32_t)__mul24(e1, e1) : 0u,
+                     ok ? (uint32_t)__mul24(e2, e2) : 0u};
+    wave_sum3_u32_dpp(q);
+    // lane 3 i + c carries the sum of row i, component c
+#pragma unroll
+    for (int c = 0; c < 3; c++) rsum = tx == 3 * i + c ? (int)q[c] : rsum;
     if (ok) {
       if (scp_pic) {
-        const uint32_t o = __umul24((uint32_t)y, (uint32_t)Pr.scp_pitch) + (uint32_t)x;
-        scp_pic[o] = (uint16_t)clampi(r.y - g0, 0, Pr.maxv);
-        scp_pic[plane_sz + o] = (uint16_t)clampi(r.z - g1, 0, Pr.maxv);
-        scp_pic[2 * plane_sz + o] = (uint16_t)clampi(r.w - g2, 0, Pr.maxv);
+        GU16* srow = scp_pic + __umul24((uint32_t)y, (uint32_t)Pr.scp_pitch);  // wave-uniform row
+        srow[x] = (uint16_t)clampi(r.y - g0, 0, Pr.maxv);
+        srow[plane_sz + x] = (uint16_t)clampi(r.z - g1, 0, Pr.maxv);
+        srow[2 * plane_sz + x] = (uint16_t)clampi(r.w - g2, 0, Pr.maxv);
       }
       if (bmap) bmap[__umul24((uint32_t)y, (uint32_t)W) + (uint32_t)x] = (uint8_t)bi[i];
     }
   }
+  // one atomic instruction adds all 3 * PY row sums of the wave
+  {
+    const int i = tx / 3, c = tx - 3 * (tx / 3);
+    if (tx < 3 * PY && yb + i < H) atomicAdd(&rows[Pr.rows_off + ((size_t)dir * 3 + c) * H + yb + i], (uint32_t)rsum);
+  }
 }
 
 template <int R, int PY>
-__global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pairs, int npairs, int wY, int wCb,
+__global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pairs, int npairs, int ntiles, int wY,
+                                                int wCb,
                                                 int wCr, const int32_t* __restrict__ gcd,
                                                 uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
                                                 uint8_t* __restrict__ best) {
   using TT = SearchTile<R, PY>;
   __shared__ int4 lds[TT::LH][TT::LW];
-  const int tile = blockIdx.x, dir = blockIdx.y;
+  // XCD-aware order: workgroups are handed to the kXcds XCDs in turn, so XCD g
+  // runs work items g * per, g * per + 1, ...: each XCD walks a contiguous run of
+  // tiles, and both directions of a tile (which read the same two picture
+  // regions) run back to back on one XCD and share its L2.
+  const int nitems = 2 * ntiles, per = (nitems + kXcds - 1) / kXcds;
+  const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (item >= nitems) return;  // padding of the last XCD's run (uniform per block)
+  const int tile = item >> 1, dir = item & 1;
   const int pi = find_entry(pairs, npairs, tile, [](const PairDev& p) { return p.tile0; });
   const PairDev P = pairs[pi];
   const int lt = tile - P.tile0;
@@ -779,8 +881,9 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 template <int R, int PY>
 static hipError_t search_rp(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
                             uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
-  hipLaunchKernelGGL((k_search<R, PY>), dim3(ntiles, 2), dim3(256), 0, st, pairs, npairs, cw[0], cw[1], cw[2], gcd,
-                     rows, scp, best);
+  const int per = (2 * ntiles + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL((k_search<R, PY>), dim3(kXcds * per), dim3(256), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
+                     cw[2], gcd, rows, scp, best);
   return hipGetLastError();
 }
 
@@ -790,8 +893,9 @@ static hipError_t search_p(const PairDev* pairs, int npairs, int ntiles, int R, 
   switch (R) {
     case 1: return search_rp<1, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
     case 2: return search_rp<2, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 3: return search_rp<3, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 4: return search_rp<4, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    // R >= 3 runs 4-row waves only (the 8-row tile does not fit the register file)
+    case 3: return PY == 4 ? search_rp<3, 4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st) : hipErrorInvalidValue;
+    case 4: return PY == 4 ? search_rp<4, 4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st) : hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }
