@@ -330,30 +330,39 @@ template <int R, int PY> struct SearchTile {
 };
 
 template <typename T, int R, int PY, int MODE>
-__device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, int y0, int4 (*lds)[SearchTile<R, PY>::LW],
-                            int wY, int wCb, int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
-                            uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
+__device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
+                                            int4 (*lds2)[SearchTile<R, PY>::LH][SearchTile<R, PY>::LW], int wY,
+                                            int wCb, int wCr, const int32_t* __restrict__ gcd,
+                                            uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
+                                            uint8_t* __restrict__ best) {
   constexpr int D = 2 * R + 1;
   constexpr int LW = SearchTile<R, PY>::LW, LH = SearchTile<R, PY>::LH;
   constexpr int KB = KeyBits<R>::v, S = 1 << KB;
   const int W = Pr.w, H = Pr.h;
-  const PicDev& Tst = dir ? Pr.p1 : Pr.p0;  // dir 0: R2T (Tst = P0, Ref = P1, +gcd); dir 1: T2R (-gcd)
+  // Waves 0-3 run direction 0 (R2T: Tst = P0 + gcd, Ref = P1), waves 4-7
+  // direction 1 (T2R: Tst = P1 - gcd, Ref = P0). Tile lds2[d] holds direction
+  // d's reference region: P1 for d = 0, P0 for d = 1. Each direction reads its
+  // test samples from the other direction's tile, so each picture region is
+  // loaded from memory once per block.
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int dir = wv >> 2, wd = wv & 3;  // direction, wave within the direction
+  const int tl = (int)threadIdx.x & 255;  // thread within the direction
   const PicDev& Ref = dir ? Pr.p0 : Pr.p1;
+  int4 (*lds)[LW] = lds2[dir];
+  int4 (*tlds)[LW] = lds2[1 - dir];
   const int32_t* g4 = gcd + 4 * Pr.res_idx;
   const int g0 = dir ? -g4[0] : g4[0];
   const int g1 = dir ? -g4[1] : g4[1];
   const int g2 = dir ? -g4[2] : g4[2];
 
-  // Every global load of the tile (reference rows, halo columns, then this
-  // lane's test samples) is issued before any is used: one memory round trip
-  // per block. Wave wv stages tile rows wv, wv + 4, ...: the row (and its plane
-  // offsets) is wave-uniform and each lane keeps one clamped column, so a load
-  // costs one vector add. The 2R halo columns are staged by threads < 2R * LH.
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  // Tile rows are staged in pairs of consecutive picture rows whose first row
-  // is even, (2m, 2m + 1): with 4:2:0 chroma both read the same chroma row,
-  // loaded once (clamped rows keep that property for even heights).
+  // Every global load of the tile (reference rows, then halo columns) is
+  // issued before any is used: one memory round trip per block. Wave wd stages
+  // row pairs wd, wd + 4, ...: the row (and its plane offsets) is wave-uniform
+  // and each lane keeps one clamped column. Rows are staged in pairs of
+  // consecutive picture rows whose first row is even, (2m, 2m + 1): with 4:2:0
+  // chroma both read the same chroma row, loaded once (clamped rows keep that
+  // property for even heights). The 2R halo columns are staged by threads < 2R * LH.
   constexpr int PAR = R & 1;                  // tile row 0 is picture row y0 - R, odd for odd R
   constexpr int NPT = (LH + PAR + 1) / 2;     // row pairs of the tile
   constexpr int NPW = (NPT + 3) / 4;          // row pairs per wave
@@ -361,7 +370,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   int ra[NPW][2][3];
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
-    const int j = wv + 4 * k;
+    const int j = wd + 4 * k;
     const int ly0 = 2 * j - PAR, ly1 = ly0 + 1;
 #pragma unroll
     for (int h = 0; h < 2; h++) ra[k][h][0] = ra[k][h][1] = ra[k][h][2] = 0;
@@ -391,7 +400,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   int rh[NHK][3];
 #pragma unroll
   for (int k = 0; k < NHK; k++) {
-    const int e = (int)threadIdx.x + 256 * k;
+    const int e = tl + 256 * k;
     rh[k][0] = rh[k][1] = rh[k][2] = 0;
     if (e < NH) {
       const int hy = e / (2 * R), hx = e - hy * (2 * R);
@@ -403,27 +412,11 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   }
 
   const int tx = lane;
-  const int x = x0 + tx, xc = min(x, W - 1);
-  const int yb = y0 + wv * PY;
+  const int x = x0 + tx;
+  const int yb = y0 + wd * PY;
 
-  int s0[PY], s1[PY], s2[PY];  // raw test samples
   int t0[PY], t1[PY], t2[PY], bk[PY], bi[PY];
   static_assert(PY % 2 == 0, "row pairs");
-#pragma unroll
-  for (int i = 0; i < PY; i += 2) {  // yb + i is even: a pair shares its 4:2:0 chroma row
-    const int ya = min(yb + i, H - 1), yb2 = min(yb + i + 1, H - 1);
-    s0[i] = (int)plane_row<T>(Tst, 0, ya)[xc];
-    s0[i + 1] = (int)plane_row<T>(Tst, 0, yb2)[xc];
-    if (Tst.cs) {
-      s1[i] = s1[i + 1] = (int)plane_row<T>(Tst, 1, ya)[xc >> 1];
-      s2[i] = s2[i + 1] = (int)plane_row<T>(Tst, 2, ya)[xc >> 1];
-    } else {
-      s1[i] = (int)plane_row<T>(Tst, 1, ya)[xc];
-      s2[i] = (int)plane_row<T>(Tst, 2, ya)[xc];
-      s1[i + 1] = (int)plane_row<T>(Tst, 1, yb2)[xc];
-      s2[i + 1] = (int)plane_row<T>(Tst, 2, yb2)[xc];
-    }
-  }
 
   // Packed variants only (host-checked): w < 2^16, w * r < 2^24 and Q fits int32,
   // so 24-bit unsigned multiplies are exact (the masks tell the compiler the ranges).
@@ -443,7 +436,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   };
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
-    const int j = wv + 4 * k;
+    const int j = wd + 4 * k;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int ly = 2 * j - PAR + h;
@@ -452,7 +445,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   }
 #pragma unroll
   for (int k = 0; k < NHK; k++) {
-    const int e = (int)threadIdx.x + 256 * k;
+    const int e = tl + 256 * k;
     if (e < NH) {
       const int hy = e / (2 * R), hx = e - hy * (2 * R);
       lds[hy][64 + hx] = entry(rh[k][0], rh[k][1], rh[k][2]);
@@ -460,8 +453,21 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
   }
   __syncthreads();
 
+  // Test samples: the other direction's tile at (tx + R, row + R) holds the test
+  // picture at clamp(x), clamp(y) -- the same clamped sample a direct load reads.
+  int s0[PY], s1[PY], s2[PY];
 #pragma unroll
   for (int i = 0; i < PY; i++) {
+    const int4 e = tlds[wd * PY + i + R][tx + R];
+    if (MODE == kSearchDot) {
+      s0[i] = e.w & 0xffff;
+      s1[i] = (int)((uint32_t)e.w >> 16);
+      s2[i] = e.z >> KB;
+    } else {
+      s0[i] = e.y;
+      s1[i] = e.z;
+      s2[i] = e.w;
+    }
     t0[i] = s0[i] + g0;
     t1[i] = s1[i] + g1;
     t2[i] = s2[i] + g2;
@@ -490,7 +496,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
       int3 v[D];
 #pragma unroll
       for (int dx = 0; dx < D; dx++) {
-        const int4 e = lds[wv * PY + ry][tx + dx];
+        const int4 e = lds[wd * PY + ry][tx + dx];
         v[dx] = make_int3(e.x + dx, e.y, e.z);
       }
 #pragma unroll
@@ -523,7 +529,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
       int4 v[D];
 #pragma unroll
       for (int dx = 0; dx < D; dx++) {
-        v[dx] = lds[wv * PY + ry][tx + dx];
+        v[dx] = lds[wd * PY + ry][tx + dx];
         v[dx].x += dx;
       }
 #pragma unroll
@@ -548,7 +554,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
     for (int ry = 0; ry < PY + 2 * R; ry++) {
       int4 v[D];
 #pragma unroll
-      for (int dx = 0; dx < D; dx++) v[dx] = lds[wv * PY + ry][tx + dx];
+      for (int dx = 0; dx < D; dx++) v[dx] = lds[wd * PY + ry][tx + dx];
 #pragma unroll
       for (int i = 0; i < PY; i++) {
         const int dy = ry - i;
@@ -588,7 +594,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
     const int y = yb + i;
     const bool ok = (x < W) && (y < H);
     const int ddy = bi[i] / D, ddx = bi[i] - ddy * D;
-    const int4 re = lds[wv * PY + i + ddy][tx + ddx];
+    const int4 re = lds[wd * PY + i + ddy][tx + ddx];
     int4 r = re;  // {_, Y, Cb, Cr}
     if (MODE == kSearchDot) r = make_int4(0, re.w & 0xffff, (int)((uint32_t)re.w >> 16), re.z >> KB);
     const int e0 = t0[i] - r.y, e1 = t1[i] - r.z, e2 = t2[i] - r.w;
@@ -616,21 +622,18 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int dir, int x0, 
 }
 
 template <int R, int PY>
-__global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pairs, int npairs, int ntiles, int wY,
-                                                int wCb,
-                                                int wCr, const int32_t* __restrict__ gcd,
+__global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pairs, int npairs, int ntiles, int wY,
+                                                int wCb, int wCr, const int32_t* __restrict__ gcd,
                                                 uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
                                                 uint8_t* __restrict__ best) {
   using TT = SearchTile<R, PY>;
-  __shared__ int4 lds[TT::LH][TT::LW];
+  __shared__ int4 lds[2][TT::LH][TT::LW];
   // XCD-aware order: workgroups are handed to the kXcds XCDs in turn, so XCD g
-  // runs work items g * per, g * per + 1, ...: each XCD walks a contiguous run of
-  // tiles, and both directions of a tile (which read the same two picture
-  // regions) run back to back on one XCD and share its L2.
-  const int nitems = 2 * ntiles, per = (nitems + kXcds - 1) / kXcds;
-  const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-  if (item >= nitems) return;  // padding of the last XCD's run (uniform per block)
-  const int tile = item >> 1, dir = item & 1;
+  // runs tiles g * per, g * per + 1, ...: each XCD walks a contiguous run of
+  // tiles, whose halos overlap in its L2.
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int tile = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (tile >= ntiles) return;  // padding of the last XCD's run (uniform per block)
   const int pi = find_entry(pairs, npairs, tile, [](const PairDev& p) { return p.tile0; });
   const PairDev P = pairs[pi];
   const int lt = tile - P.tile0;
@@ -639,22 +642,22 @@ __global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pair
   const int x0 = tx * TT::TW, y0 = ty * TT::TH;
   if (P.p0.bps == 2) {
     if (P.mode == kSearchDot)
-      search_tile<uint16_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, PY, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
-      search_tile<uint16_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, PY, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
-      search_tile<uint16_t, R, PY, kSearchWide24>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, PY, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
-      search_tile<uint16_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, PY, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   } else {
     if (P.mode == kSearchDot)
-      search_tile<uint8_t, R, PY, kSearchDot>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, PY, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
-      search_tile<uint8_t, R, PY, kSearchNarrow>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, PY, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
-      search_tile<uint8_t, R, PY, kSearchWide24>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, PY, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
-      search_tile<uint8_t, R, PY, kSearchWide>(P, dir, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, PY, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   }
 }
 
@@ -667,20 +670,41 @@ __global__ __launch_bounds__(256) void k_search(const PairDev* __restrict__ pair
 // band. Window moments are exact integers; the ratio is FP32 on the exact
 // int64 numerators n*S_ab - S_a*S_b; window values are summed in FP64.
 // ---------------------------------------------------------------------------
+// Four samples of a picture row (luma grid x .. x + 3, x a multiple of 4) from
+// a wave-uniform row pointer: one vector load; 4:2:0 chroma reads 2 samples
+// (one load when the pair is aligned) and replicates them.
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
+typedef unsigned char u8x2 __attribute__((ext_vector_type(2)));
+
 template <typename T, int CS>
-__device__ __forceinline__ void load4(const T* __restrict__ p, int stride, int x, int y, uint32_t v[4]) {
+__device__ __forceinline__ void load4(const __attribute__((address_space(1))) T* row, int x, uint32_t v[4]) {
+  typedef __attribute__((address_space(1))) const T GT;
   if (CS == 0) {
-    const T* q = p + (size_t)y * stride + x;
+    GT* q = row + x;
     if (sizeof(T) == 2) {
-      const ushort4 u = *(const ushort4*)q;
+      const u16x4 u = *(const __attribute__((address_space(1))) u16x4*)q;
       v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
     } else {
-      const uchar4 u = *(const uchar4*)q;
+      const u8x4 u = *(const __attribute__((address_space(1))) u8x4*)q;
       v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
     }
   } else {
-    const T* q = p + (size_t)(y >> 1) * stride + (x >> 1);
-    const uint32_t a = q[0], b = q[1];
+    GT* q = row + (x >> 1);
+    uint32_t a, b;
+    if ((reinterpret_cast<uintptr_t>(q) & (2 * sizeof(T) - 1)) == 0) {  // uniform: x >> 1 is even
+      if (sizeof(T) == 2) {
+        const u16x2 u = *(const __attribute__((address_space(1))) u16x2*)q;
+        a = u.x; b = u.y;
+      } else {
+        const u8x2 u = *(const __attribute__((address_space(1))) u8x2*)q;
+        a = u.x; b = u.y;
+      }
+    } else {
+      a = q[0];
+      b = q[1];
+    }
     v[0] = a; v[1] = a; v[2] = b; v[3] = b;
   }
 }
@@ -693,8 +717,10 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
   if (wy0 >= J.nuy) return;
   const int bx = chunk * 63 + lane;
   const bool col_ok = bx <= J.nux;
-  const TA* pa = (const TA*)plane_ptr(J.a, c);
-  const TB* pb = (const TB*)plane_ptr(J.b, c);
+  typedef __attribute__((address_space(1))) const TA GTA;
+  typedef __attribute__((address_space(1))) const TB GTB;
+  GTA* pa = (GTA*)plane_ptr(J.a, c);
+  GTB* pb = (GTB*)plane_ptr(J.b, c);
   const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
   uint32_t pA = 0, pB = 0;
   SQ pAA = 0, pBB = 0, pAB = 0;
@@ -707,8 +733,9 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         uint32_t a[4], b[4];
-        load4<TA, CSA>(pa, sa, bx * 4, by * 4 + r, a);
-        load4<TB, CSB>(pb, sb, bx * 4, by * 4 + r, b);
+        const int y = by * 4 + r;  // wave-uniform row
+        load4<TA, CSA>(pa + (size_t)(y >> CSA) * sa, bx * 4, a);
+        load4<TB, CSB>(pb + (size_t)(y >> CSB) * sb, bx * 4, b);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           A += a[j];
@@ -881,8 +908,8 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 template <int R, int PY>
 static hipError_t search_rp(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
                             uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
-  const int per = (2 * ntiles + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_search<R, PY>), dim3(kXcds * per), dim3(256), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL((k_search<R, PY>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
                      cw[2], gcd, rows, scp, best);
   return hipGetLastError();
 }
