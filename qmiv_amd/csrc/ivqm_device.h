@@ -45,7 +45,8 @@ struct SsimJobDev {
   PicDev a, b;
   int32_t w, h;
   int32_t nux, nuy, nchunk, nbg;  // windows, 63-window chunks, band groups (4 bands of NB rows)
-  int32_t calc_l, erp, wsq;       // luminance term, ERP row weights, 64-bit squared sums
+  int32_t calc_l, erp, wsq;       // luminance term, ERP row weights, sum widths (0: 32-bit, 1: 64-bit
+                                  // windows, 2: 64-bit throughout)
   int32_t blk0;                   // first block of this job (all 3 components)
   int64_t part_off;               // [3][nuy][nchunk] doubles in the partial buffer
   int64_t erp_off;                // offset of this height's weights, or -1

@@ -328,7 +328,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     J.nbg = ((J.nuy + P.nb - 1) / P.nb + 3) / 4;
     J.calc_l = calc_l;
     J.erp = erp;
-    J.wsq = (P.pp[i].bd > 14 || cfg_.ssim_wide_sums) ? 1 : 0;
+    // moment sum widths: 32-bit up to 12-bit samples, 64-bit windows up to 14-bit, 64-bit throughout above
+    J.wsq = (P.pp[i].bd > 14 || cfg_.ssim_wide_sums) ? 2 : (P.pp[i].bd > 12 ? 1 : 0);
     J.erp_off = -1;
     if (erp) RCCHK(erp_offset(h, &J.erp_off));
     J.blk0 = P.ssim_blocks;

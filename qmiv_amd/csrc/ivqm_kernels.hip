@@ -115,6 +115,28 @@ __device__ __forceinline__ void wave_sum3_u32_dpp(uint32_t v[3]) {
 #undef IVQM_DPP3
   for (int k = 0; k < 3; k++) v[k] = (uint32_t)__builtin_amdgcn_readlane((int)v[k], 63);
 }
+// FP64 sum over the 64 lanes with DPP moves of both halves (fixed order: row
+// prefixes, then row broadcasts); the total is read from lane 63.
+__device__ __forceinline__ double wave_sum_f64_dpp(double v) {
+#define IVQM_DPPD(ctrl, rmask, bc)                                                                      \
+  {                                                                                                    \
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);                                                \
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, ctrl, rmask, 0xf, bc); \
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), ctrl, rmask, 0xf, bc); \
+    v += __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));                              \
+  }
+  IVQM_DPPD(0x111, 0xf, true)
+  IVQM_DPPD(0x112, 0xf, true)
+  IVQM_DPPD(0x114, 0xf, true)
+  IVQM_DPPD(0x118, 0xf, true)
+  IVQM_DPPD(0x142, 0xa, false)
+  IVQM_DPPD(0x143, 0xc, false)
+#undef IVQM_DPPD
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+  return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -709,10 +731,26 @@ __device__ __forceinline__ void load4(const __attribute__((address_space(1))) T*
   }
 }
 
-template <typename TA, int CSA, typename TB, int CSB, bool WSQ, int NB>
+// Value of lane + 1 (DPP wave shift; no LDS traffic). Lane 63 reads 0.
+__device__ __forceinline__ uint32_t next_lane32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint64_t next_lane64(uint64_t v) {
+  return (uint64_t)next_lane32((uint32_t)v) | ((uint64_t)next_lane32((uint32_t)(v >> 32)) << 32);
+}
+// int64 -> float for |n| < 2^51: high part times 2^20 plus the exact low 20 bits, one rounding
+// when |n| < 2^44 (every bit depth up to 14), at most 1.5 ulp above.
+__device__ __forceinline__ float i64_to_f32(int64_t n) {
+  return fmaf((float)(int32_t)(n >> 20), 1048576.0f, (float)(int32_t)(n & 0xFFFFF));
+}
+
+// Sum widths of the window moments (host picks by bit depth): SM 0 up to 12-bit, everything in
+// uint32 (64 * 4095^2 < 2^32); SM 1 up to 14-bit, sub-blocks uint32, windows uint64; SM 2 uint64.
+template <typename TA, int CSA, typename TB, int CSB, int SM, int NB>
 __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
                           double* __restrict__ partial) {
-  using SQ = typename std::conditional<WSQ, uint64_t, uint32_t>::type;
+  using SQ = typename std::conditional<SM == 2, uint64_t, uint32_t>::type;  // sub-block squared sums
+  using WQ = typename std::conditional<SM == 0, uint32_t, uint64_t>::type;  // window squared sums
   const int wy0 = band * NB;
   if (wy0 >= J.nuy) return;
   const int bx = chunk * 63 + lane;
@@ -724,6 +762,7 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
   const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
   uint32_t pA = 0, pB = 0;
   SQ pAA = 0, pBB = 0, pAB = 0;
+#pragma unroll 1
   for (int k = 0; k <= NB; k++) {
     const int by = wy0 + k;
     if (by > J.nuy) break;
@@ -749,47 +788,61 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
     }
     if (k > 0) {
       const int wy = by - 1;
+      // window = this lane's and the next lane's columns of the previous and this sub-block row
       const uint32_t vA = pA + A, vB = pB + B;
-      const uint64_t vAA = (uint64_t)pAA + AA, vBB = (uint64_t)pBB + BB, vAB = (uint64_t)pAB + AB;
-      const uint32_t nA = __shfl_down(vA, 1, 64), nB = __shfl_down(vB, 1, 64);
-      const uint64_t nAA = __shfl_down(vAA, 1, 64), nBB = __shfl_down(vBB, 1, 64), nAB = __shfl_down(vAB, 1, 64);
+      const WQ vAA = (WQ)pAA + AA, vBB = (WQ)pBB + BB, vAB = (WQ)pAB + AB;
+      uint32_t nA = next_lane32(vA), nB = next_lane32(vB);
+      WQ nAA, nBB, nAB;
+      if (SM == 0) {
+        nAA = (WQ)next_lane32((uint32_t)vAA);
+        nBB = (WQ)next_lane32((uint32_t)vBB);
+        nAB = (WQ)next_lane32((uint32_t)vAB);
+      } else {
+        nAA = (WQ)next_lane64((uint64_t)vAA);
+        nBB = (WQ)next_lane64((uint64_t)vBB);
+        nAB = (WQ)next_lane64((uint64_t)vAB);
+      }
       double s = 0.0;
       if (lane < 63 && bx < J.nux) {
-        const int64_t sA = (int64_t)(vA + nA), sB = (int64_t)(vB + nB);
-        const int64_t sAA = (int64_t)(vAA + nAA), sBB = (int64_t)(vBB + nBB), sAB = (int64_t)(vAB + nAB);
-        const int64_t cov = 64 * sAB - sA * sB;
-        const int64_t va = 64 * sAA - sA * sA;
-        const int64_t vb = 64 * sBB - sB * sB;
-        float cs = (2.0f * (float)cov + c2n) / ((float)va + (float)vb + c2n);
-        if (J.calc_l) {
-          const float l = (2.0f * (float)(sA * sB) + c1n) / ((float)(sA * sA) + (float)(sB * sB) + c1n);
-          cs = l * cs;
+        const uint32_t sA = vA + nA, sB = vB + nB;  // < 2^23
+        const int64_t sAA = (int64_t)(uint64_t)(vAA + nAA), sBB = (int64_t)(uint64_t)(vBB + nBB);
+        const int64_t sAB = (int64_t)(uint64_t)(vAB + nAB);
+        // exact numerators n S_ab - S_a S_b (the cancellation happens in integers)
+        const int64_t cov = 64 * sAB - (int64_t)((uint64_t)sA * sB);
+        const int64_t va = 64 * sAA - (int64_t)((uint64_t)sA * sA);
+        const int64_t vb = 64 * sBB - (int64_t)((uint64_t)sB * sB);
+        float num = 2.0f * i64_to_f32(cov) + c2n;
+        float den = i64_to_f32(va) + i64_to_f32(vb) + c2n;
+        if (J.calc_l) {  // luminance term: positive terms only, FP32 products suffice
+          const float fa = (float)sA, fb = (float)sB;
+          num *= 2.0f * fa * fb + c1n;
+          den *= fa * fa + fb * fb + c1n;
         }
-        s = (double)cs;
+        s = (double)(num * __builtin_amdgcn_rcpf(den));
       }
-      s = wave_sum_f64(s);
+      s = wave_sum_f64_dpp(s);
       if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
     }
     pA = A; pB = B; pAA = AA; pBB = BB; pAB = AB;
   }
 }
 
-template <bool WSQ, int NB>
+template <int SM, int NB>
 __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
                               double* partial) {
   const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
   if (J.a.bps == 2 && J.b.bps == 2) {
-    if (csa == 0 && csb == 0) ssim_wave<uint16_t, 0, uint16_t, 0, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 1) ssim_wave<uint16_t, 1, uint16_t, 1, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 0) ssim_wave<uint16_t, 1, uint16_t, 0, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else ssim_wave<uint16_t, 0, uint16_t, 1, WSQ, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-  } else if (!WSQ) {
+    if (csa == 0 && csb == 0) ssim_wave<uint16_t, 0, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else if (csa == 1 && csb == 1) ssim_wave<uint16_t, 1, uint16_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else if (csa == 1 && csb == 0) ssim_wave<uint16_t, 1, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    else ssim_wave<uint16_t, 0, uint16_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+  } else if (SM < 2) {
     if (J.a.bps == 1 && J.b.bps == 1) {
-      if (csa == 0 && csb == 0) ssim_wave<uint8_t, 0, uint8_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint8_t, 1, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0 && csb == 0) ssim_wave<uint8_t, 0, uint8_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      else ssim_wave<uint8_t, 1, uint8_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
     } else if (J.a.bps == 1) {
-      if (csa == 0) ssim_wave<uint8_t, 0, uint16_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint16_t, 0, false, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0) ssim_wave<uint8_t, 0, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      else ssim_wave<uint8_t, 1, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
     }
   }
 }
@@ -805,8 +858,9 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   const int c = lb / per_c, rem = lb - c * per_c;
   const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
   const int lane = threadIdx.x & 63, band = bg * 4 + (threadIdx.x >> 6);
-  if (J.wsq) ssim_dispatch<true, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
-  else ssim_dispatch<false, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  if (J.wsq == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
 }
 
 // One block per (job, component): FP64 frame totals of the window-row sums.
