@@ -81,9 +81,11 @@ hipError_t launch_downsample_one(const PicDev& src, int w, int h, uint16_t* dst,
 hipError_t launch_upsample420(const PicDev& src, int w, int h, uint16_t* dst, int dst_stride, hipStream_t st);
 
 constexpr int kSearchTileW = 64;
+// The search runs waves of 4 pixel rows (64 x 16 tiles) for every search range;
+// a requested tile height of 8 rows per wave is accepted and runs as 4.
+constexpr int kSearchRowsPerWave = 4;
 inline int search_tile_h(int py) { return 4 * py; }
-// Rows per wave the search runs for search range R and a requested 4 or 8.
-inline int search_rows(int R, int py) { return R >= 3 ? 4 : py; }
+inline int search_rows(int /*R*/, int /*py*/) { return kSearchRowsPerWave; }
 constexpr int kDownBlkW = 64, kDownBlkH = 4;
 // MI355X: 8 XCDs, each with its own L2; consecutive workgroups go to consecutive XCDs.
 constexpr int kXcds = 8;

@@ -141,6 +141,8 @@ int Engine::search_mode(int maxv, int32_t gmax) const {
   const int64_t S = 1 << key_bits(cfg_.search_range);
   const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
   const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
+  // the keyed variants' epilogue takes t - r as int16
+  if (A >= 32768) return wide;
   // key = S * (Q(r) - 2 sum w t r) + idx stays inside int32 and the 24-bit multiplier operands
   if (3 * S * wsum * A * A >= (1ll << 31)) return wide;
   if (2 * S * wmax * A >= (1ll << 23)) return wide;

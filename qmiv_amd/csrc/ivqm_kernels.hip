@@ -64,6 +64,15 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(
   return (const GT*)plane_ptr(P, c) + __umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c));
 }
 
+// Sample col of a row pointer, addressed by a 32-bit byte offset from the
+// (scalar) row pointer.
+template <typename T>
+__device__ __forceinline__ int ld_col(const __attribute__((address_space(1))) T* row, uint32_t col) {
+  typedef __attribute__((address_space(1))) const uint8_t GB;
+  typedef __attribute__((address_space(1))) const T GT;
+  return (int)*(const GT*)((const GB*)row + col * (uint32_t)sizeof(T));
+}
+
 // Picture planes are device allocations: addressed as global memory they load
 // with a scalar base and a 32-bit vector offset (no 64-bit address math).
 template <typename T>
@@ -315,80 +324,179 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
 // arithmetic that wraps; per-row int32-wrapping sums of the winner's
 // per-component squared differences; SCP = clip(r_best - gcs, 0, Max).
 //
-// Block = 4 waves; the block tile is 64 x (4*PY) pixels of one direction.
-// Each lane owns one column and PY consecutive rows. The reference tile
-// (64 + 2R) x (4*PY + 2R) is staged once in LDS as int4 {Q, Y, Cb, Cr}.
-// Walking the tile rows top to bottom visits every pixel's candidates in
-// dy-outer, dx-inner raster order, so "first strict minimum" holds for the
-// exact (wide) variant, while each loaded LDS row serves up to 2R+1 pixels.
+// Block = 8 waves on one 64 x 16 pixel tile: waves 0-3 run direction 0,
+// waves 4-7 direction 1. Each direction's (64 + 2R) x (16 + 2R) reference
+// region is staged once in LDS as 12-byte entries, and each direction reads
+// its test samples from the other direction's region. A wave covers 4 rows x
+// 64 columns; lane l owns the 4 adjacent pixels 4 (l % 16) .. + 3 of row
+// l / 16. For each candidate row a lane reads its 4 + 2R entries once and
+// scores its 4 pixels' 2R + 1 candidates from them (lanes 48 bytes apart: no
+// LDS bank conflicts). A pixel row's squared-difference sums are an in-lane
+// add plus a 16-lane DPP scan; SCP rows are written with 8-byte stores.
 //
 // NARROW variant (host checks the ranges): E = T + E', T = sum w t^2 constant
 // per pixel and E' = Q(r) - 2 sum_c w_c t_c r_c with Q(r) = sum_c w_c r_c^2
-// precomputed per reference pixel. Nothing wraps in this range, so
-// argmin E == argmin E'; the candidate's raster index is packed into the low
-// KB bits of key = 2^KB * E' + idx, which makes a plain integer min equal to
-// "first strict minimum in raster order". Per candidate: three 24-bit
-// multiply-adds and a min.
+// precomputed per reference entry. Nothing wraps in this range, so
+// argmin E == argmin E'. The key S * E' + idx (S = 2^KB > number of
+// candidates, idx the candidate's raster index) makes a plain integer min
+// equal to "first strict minimum in raster order". An entry at tile position
+// (ly, lx) stores S * Q + ly * D + lx (D = 2R + 1): the candidate (dy, dx) of
+// the pixel at tile position (py, px) then scores its key plus the constant
+// py * D + px, which leaves the order unchanged and is subtracted from the
+// pixel's minimum. Entry {S*Q + off, Y | Cb << 16, Cr}; per candidate three
+// 24-bit multiply-adds and a min.
 //
-// DOT variant (narrower ranges, host-checked: S * Max and 2 * w * (Max + |gcd|)
-// fit int16): the same key, with the reference samples stored pre-scaled by
-// S in LDS, {Y*S, Cb*S} packed as int16 pairs: one v_dot2 (Y, Cb), one 24-bit
-// multiply-add (Cr) and a min per candidate.
+// DOT variant (host-checked: S * Max and 2 * w * (Max + |gcd|) fit int16):
+// the same key, with {Y*S, Cb*S} stored as an int16 pair: entry
+// {S*Q + off, Y*S | Cb*S << 16, Cr}; per candidate one v_dot2 (Y, Cb), one
+// 24-bit multiply-add (Cr; its multiplier carries S) and a min.
+//
+// WIDE variant: the reference-form int32 arithmetic with an explicit
+// first-strict-minimum compare in raster order; entry {Y, Cb, Cr}.
+// WIDE24 variant (host-checked (Max + |gcd|) * |w| < 2^23): as WIDE, with each
+// weighted square (d*d)*w mod 2^32 computed as (d*w)*d by two 24-bit
+// multiplies -- the same value mod 2^32.
 // ---------------------------------------------------------------------------
 typedef short short2_t __attribute__((ext_vector_type(2)));
-
-//
-// WIDE24 variant (host-checked (Max + |gcd|) * |w| < 2^23): the reference-form
-// int32 arithmetic, with each weighted square (d*d)*w mod 2^32 computed as
-// (d*w)*d by two 24-bit multiplies -- the same value mod 2^32.
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3 };
 template <int R> struct KeyBits {
   static constexpr int n = (2 * R + 1) * (2 * R + 1);
   static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
 };
 
-template <int R, int PY> struct SearchTile {
-  static constexpr int TW = kSearchTileW, TH = 4 * PY, LW = TW + 2 * R, LH = TH + 2 * R;
+// LW is the region width in entries, LP the LDS row pitch: a multiple of 4
+// entries (48 bytes), so a lane's run of entries starting at a multiple of 4
+// is 16-byte aligned.
+template <int R> struct SearchTile {
+  static constexpr int TW = kSearchTileW, TH = 16, LW = TW + 2 * R, LH = TH + 2 * R, LP = (LW + 3) & ~3;
 };
+static_assert(SearchTile<1>::TH == 4 * kSearchRowsPerWave, "tile height");
 
-template <typename T, int R, int PY, int MODE>
+struct SEnt {
+  int x, y, z;
+};
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x2_t __attribute__((ext_vector_type(2)));
+
+// NE consecutive entries from a 16-byte aligned LDS address, read as 16-byte
+// vectors (lanes 48 bytes apart hit distinct banks) plus an 8-byte tail.
+template <int NE>
+__device__ __forceinline__ void lds_run(const SEnt* p, SEnt (&e)[NE]) {
+  constexpr int NW = 3 * NE;
+  static_assert(NW % 4 == 0 || NW % 4 == 2, "run length");
+  int w[NW];
+  const i32x4_t* q = reinterpret_cast<const i32x4_t*>(p);
+#pragma unroll
+  for (int i = 0; i < NW / 4; i++) {
+    const i32x4_t v = q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+  if (NW % 4) {
+    const i32x2_t v = *reinterpret_cast<const i32x2_t*>(q + NW / 4);
+    w[NW - 2] = v.x;
+    w[NW - 1] = v.y;
+  }
+#pragma unroll
+  for (int j = 0; j < NE; j++) e[j] = SEnt{w[3 * j], w[3 * j + 1], w[3 * j + 2]};
+}
+
+// The raw samples {Y, Cb, Cr} of an LDS entry.
+template <int MODE, int KB>
+__device__ __forceinline__ void sent_raw(const SEnt& e, int& Y, int& Cb, int& Cr) {
+  if (MODE == kSearchDot) {
+    Y = (int)(((uint32_t)e.y & 0xffffu) >> KB);
+    Cb = (int)((uint32_t)e.y >> (16 + KB));
+  } else if (MODE == kSearchNarrow) {
+    Y = (int)((uint32_t)e.y & 0xffffu);
+    Cb = (int)((uint32_t)e.y >> 16);
+  } else {
+    Y = e.x;
+    Cb = e.y;
+  }
+  Cr = e.z;
+}
+
+// Entries A .. A + N - 1 after a 16-byte aligned LDS address, read as the
+// 16-byte chunks that cover them (A * 12 need not be aligned).
+template <int A, int N>
+__device__ __forceinline__ void lds_entries(const SEnt* p, SEnt (&e)[N]) {
+  constexpr int C0 = 12 * A / 16, C1 = (12 * (A + N) + 15) / 16;
+  int w[4 * (C1 - C0)];
+  const i32x4_t* q = reinterpret_cast<const i32x4_t*>(p);
+#pragma unroll
+  for (int i = C0; i < C1; i++) {
+    const i32x4_t v = q[i];
+    w[4 * (i - C0)] = v.x;
+    w[4 * (i - C0) + 1] = v.y;
+    w[4 * (i - C0) + 2] = v.z;
+    w[4 * (i - C0) + 3] = v.w;
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const int o = 3 * (A + j) - 4 * C0;
+    e[j] = SEnt{w[o], w[o + 1], w[o + 2]};
+  }
+}
+
+// One step of an inclusive scan of three values within each 16-lane row.
+template <int CTRL>
+__device__ __forceinline__ void dpp3_add(uint32_t v[3]) {
+  const uint32_t a0 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[0], CTRL, 0xf, 0xf, true);
+  const uint32_t a1 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[1], CTRL, 0xf, 0xf, true);
+  const uint32_t a2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[2], CTRL, 0xf, 0xf, true);
+  v[0] += a0;
+  v[1] += a1;
+  v[2] += a2;
+}
+// row_shr 1, 2, 4, 8: lane 15 of every 16-lane row ends with the row's totals.
+__device__ __forceinline__ void row16_sum3_u32_dpp(uint32_t v[3]) {
+  dpp3_add<0x111>(v);
+  dpp3_add<0x112>(v);
+  dpp3_add<0x114>(v);
+  dpp3_add<0x118>(v);
+}
+
+template <typename T, int R, int MODE>
 __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
-                                            int4 (*lds2)[SearchTile<R, PY>::LH][SearchTile<R, PY>::LW], int wY,
-                                            int wCb, int wCr, const int32_t* __restrict__ gcd,
-                                            uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
-                                            uint8_t* __restrict__ best) {
-  constexpr int D = 2 * R + 1;
-  constexpr int LW = SearchTile<R, PY>::LW, LH = SearchTile<R, PY>::LH;
+                                            SEnt (*lds2)[SearchTile<R>::LH][SearchTile<R>::LP], int wY, int wCb,
+                                            int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
+                                            uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
+  constexpr int D = 2 * R + 1, NE = 4 + 2 * R;
+  constexpr int LH = SearchTile<R>::LH, LP = SearchTile<R>::LP;
   constexpr int KB = KeyBits<R>::v, S = 1 << KB;
+  constexpr bool KEYED = MODE == kSearchDot || MODE == kSearchNarrow;
   const int W = Pr.w, H = Pr.h;
   // Waves 0-3 run direction 0 (R2T: Tst = P0 + gcd, Ref = P1), waves 4-7
-  // direction 1 (T2R: Tst = P1 - gcd, Ref = P0). Tile lds2[d] holds direction
-  // d's reference region: P1 for d = 0, P0 for d = 1. Each direction reads its
-  // test samples from the other direction's tile, so each picture region is
-  // loaded from memory once per block.
+  // direction 1 (T2R: Tst = P1 - gcd, Ref = P0). Region lds2[d] holds
+  // direction d's reference: P1 for d = 0, P0 for d = 1.
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int dir = wv >> 2, wd = wv & 3;  // direction, wave within the direction
   const int tl = (int)threadIdx.x & 255;  // thread within the direction
   const PicDev& Ref = dir ? Pr.p0 : Pr.p1;
-  int4 (*lds)[LW] = lds2[dir];
-  int4 (*tlds)[LW] = lds2[1 - dir];
+  SEnt (*lds)[LP] = lds2[dir];
+  SEnt (*tlds)[LP] = lds2[1 - dir];
   const int32_t* g4 = gcd + 4 * Pr.res_idx;
   const int g0 = dir ? -g4[0] : g4[0];
   const int g1 = dir ? -g4[1] : g4[1];
   const int g2 = dir ? -g4[2] : g4[2];
 
-  // Every global load of the tile (reference rows, then halo columns) is
-  // issued before any is used: one memory round trip per block. Wave wd stages
-  // row pairs wd, wd + 4, ...: the row (and its plane offsets) is wave-uniform
-  // and each lane keeps one clamped column. Rows are staged in pairs of
+  // Every global load of the region (rows, then halo columns) is issued
+  // before any is used: one memory round trip per block. Wave wd stages row
+  // pairs wd, wd + 4, ...: the row (and its plane offsets) is wave-uniform and
+  // each lane keeps one clamped column. Rows are staged in pairs of
   // consecutive picture rows whose first row is even, (2m, 2m + 1): with 4:2:0
   // chroma both read the same chroma row, loaded once (clamped rows keep that
   // property for even heights). The 2R halo columns are staged by threads < 2R * LH.
-  constexpr int PAR = R & 1;                  // tile row 0 is picture row y0 - R, odd for odd R
-  constexpr int NPT = (LH + PAR + 1) / 2;     // row pairs of the tile
-  constexpr int NPW = (NPT + 3) / 4;          // row pairs per wave
-  const int cx = clampi(x0 - R + lane, 0, W - 1);
+  constexpr int PAR = R & 1;               // region row 0 is picture row y0 - R, odd for odd R
+  constexpr int NPT = (LH + PAR + 1) / 2;  // row pairs of the region
+  constexpr int NPW = (NPT + 3) / 4;       // row pairs per wave
+  const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);  // unsigned: 32-bit offsets from the row pointer
   int ra[NPW][2][3];
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
@@ -398,27 +506,27 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     for (int h = 0; h < 2; h++) ra[k][h][0] = ra[k][h][1] = ra[k][h][2] = 0;
     if (j < NPT) {
       const int ya = clampi(y0 - R + ly0, 0, H - 1), yb2 = clampi(y0 - R + ly1, 0, H - 1);
-      if (ly0 >= 0) ra[k][0][0] = (int)plane_row<T>(Ref, 0, ya)[cx];
-      if (ly1 < LH) ra[k][1][0] = (int)plane_row<T>(Ref, 0, yb2)[cx];
+      if (ly0 >= 0) ra[k][0][0] = ld_col<T>(plane_row<T>(Ref, 0, ya), cx);
+      if (ly1 < LH) ra[k][1][0] = ld_col<T>(plane_row<T>(Ref, 0, yb2), cx);
       if (Ref.cs) {
         const int yc = ly0 >= 0 ? ya : yb2;
-        const int cb = (int)plane_row<T>(Ref, 1, yc)[cx >> 1], cr = (int)plane_row<T>(Ref, 2, yc)[cx >> 1];
+        const int cb = ld_col<T>(plane_row<T>(Ref, 1, yc), cx >> 1), cr = ld_col<T>(plane_row<T>(Ref, 2, yc), cx >> 1);
         ra[k][0][1] = ra[k][1][1] = cb;
         ra[k][0][2] = ra[k][1][2] = cr;
       } else {
         if (ly0 >= 0) {
-          ra[k][0][1] = (int)plane_row<T>(Ref, 1, ya)[cx];
-          ra[k][0][2] = (int)plane_row<T>(Ref, 2, ya)[cx];
+          ra[k][0][1] = ld_col<T>(plane_row<T>(Ref, 1, ya), cx);
+          ra[k][0][2] = ld_col<T>(plane_row<T>(Ref, 2, ya), cx);
         }
         if (ly1 < LH) {
-          ra[k][1][1] = (int)plane_row<T>(Ref, 1, yb2)[cx];
-          ra[k][1][2] = (int)plane_row<T>(Ref, 2, yb2)[cx];
+          ra[k][1][1] = ld_col<T>(plane_row<T>(Ref, 1, yb2), cx);
+          ra[k][1][2] = ld_col<T>(plane_row<T>(Ref, 2, yb2), cx);
         }
       }
     }
   }
-  constexpr int NH = 2 * R * LH;          // halo entries (columns 64 .. 64 + 2R - 1)
-  constexpr int NHK = (NH + 255) / 256;   // halo entries per thread
+  constexpr int NH = 2 * R * LH;         // halo entries (columns 64 .. 64 + 2R - 1)
+  constexpr int NHK = (NH + 255) / 256;  // halo entries per thread
   int rh[NHK][3];
 #pragma unroll
   for (int k = 0; k < NHK; k++) {
@@ -433,28 +541,25 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     }
   }
 
-  const int tx = lane;
-  const int x = x0 + tx;
-  const int yb = y0 + wd * PY;
-
-  int t0[PY], t1[PY], t2[PY], bk[PY], bi[PY];
-  static_assert(PY % 2 == 0, "row pairs");
-
-  // Packed variants only (host-checked): w < 2^16, w * r < 2^24 and Q fits int32,
-  // so 24-bit unsigned multiplies are exact (the masks tell the compiler the ranges).
-  auto entry = [&](int rY, int rCb, int rCr) -> int4 {
-    int q = 0;
-    if (MODE != kSearchWide && MODE != kSearchWide24) {
-      const uint32_t uY = (uint32_t)wY & 0xffffu, uCb = (uint32_t)wCb & 0xffffu, uCr = (uint32_t)wCr & 0xffffu;
-      q = (int)(__umul24(__umul24(uY, (uint32_t)rY), (uint32_t)rY) +
-                __umul24(__umul24(uCb, (uint32_t)rCb), (uint32_t)rCb) +
-                __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr))
-          << KB;
+  // Keyed entries. Host-checked ranges: w * Max fits 16 bits (DOT), S * Q(r)
+  // fits int32 and w * r < 2^24, so the 16- and 24-bit multiplies are exact.
+  auto entry = [&](int rY, int rCb, int rCr, int off) -> SEnt {
+    if (MODE == kSearchDot) {
+      const u16x2_t pr = {(unsigned short)rY, (unsigned short)rCb};
+      const u16x2_t wp = {(unsigned short)wY, (unsigned short)wCb};
+      const uint32_t qc = __umul24(__umul24((uint32_t)wCr & 0xffffu, (uint32_t)rCr), (uint32_t)rCr);
+      const uint32_t q = __builtin_amdgcn_udot2(pr * wp, pr, qc, false);
+      const u16x2_t ps = pr << (unsigned short)KB;
+      return SEnt{(int)((q << KB) + (uint32_t)off), __builtin_bit_cast(int, ps), rCr};
     }
-    if (MODE == kSearchDot)
-      return make_int4(q, (int)(((uint32_t)(rY << KB) & 0xffffu) | ((uint32_t)(rCb << KB) << 16)), rCr << KB,
-                       (int)((uint32_t)rY | ((uint32_t)rCb << 16)));
-    return make_int4(q, rY, rCb, rCr);
+    if (MODE == kSearchNarrow) {
+      const uint32_t uY = (uint32_t)wY & 0xffffu, uCb = (uint32_t)wCb & 0xffffu, uCr = (uint32_t)wCr & 0xffffu;
+      const uint32_t q = __umul24(__umul24(uY, (uint32_t)rY), (uint32_t)rY) +
+                         __umul24(__umul24(uCb, (uint32_t)rCb), (uint32_t)rCb) +
+                         __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr);
+      return SEnt{(int)((q << KB) + (uint32_t)off), (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
+    }
+    return SEnt{rY, rCb, rCr};
   };
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
@@ -462,7 +567,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int ly = 2 * j - PAR + h;
-      if (j < NPT && ly >= 0 && ly < LH) lds[ly][lane] = entry(ra[k][h][0], ra[k][h][1], ra[k][h][2]);
+      if (j < NPT && ly >= 0 && ly < LH) lds[ly][lane] = entry(ra[k][h][0], ra[k][h][1], ra[k][h][2], ly * D + lane);
     }
   }
 #pragma unroll
@@ -470,186 +575,230 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     const int e = tl + 256 * k;
     if (e < NH) {
       const int hy = e / (2 * R), hx = e - hy * (2 * R);
-      lds[hy][64 + hx] = entry(rh[k][0], rh[k][1], rh[k][2]);
+      lds[hy][64 + hx] = entry(rh[k][0], rh[k][1], rh[k][2], hy * D + 64 + hx);
     }
   }
   __syncthreads();
 
-  // Test samples: the other direction's tile at (tx + R, row + R) holds the test
-  // picture at clamp(x), clamp(y) -- the same clamped sample a direct load reads.
-  int s0[PY], s1[PY], s2[PY];
-#pragma unroll
-  for (int i = 0; i < PY; i++) {
-    const int4 e = tlds[wd * PY + i + R][tx + R];
-    if (MODE == kSearchDot) {
-      s0[i] = e.w & 0xffff;
-      s1[i] = (int)((uint32_t)e.w >> 16);
-      s2[i] = e.z >> KB;
-    } else {
-      s0[i] = e.y;
-      s1[i] = e.z;
-      s2[i] = e.w;
-    }
-    t0[i] = s0[i] + g0;
-    t1[i] = s1[i] + g1;
-    t2[i] = s2[i] + g2;
-    bk[i] = INT_MAX;
-    bi[i] = 0;
-  }
-  // Multipliers -k w t = -(k w s) - k w g of the packed variants: k w s by an exact
-  // 24-bit multiply (host-checked k w (Max + |g|) < 2^23), k w g uniform.
+  // This lane's pixels: tile row iy, tile columns ix .. ix + 3.
+  const int iy = wd * 4 + (lane >> 4), ix = 4 * (lane & 15);
+  const int y = y0 + iy, x = x0 + ix;
+  // Test samples: the other direction's region at (iy + R, ix + k + R) holds
+  // the test picture at the clamped position -- the sample a direct load reads.
+  int t0[4], t1[4], t2[4];
+  int m0[4], m1[4], m2[4];  // NARROW: -2 S w t; DOT: m0 = {-2 wY t0, -2 wCb t1} as int16 pair, m2 = -2 S wCr t2
+  // Multipliers -k w t = -(k w s) - k w g: k w s by an exact 24-bit multiply
+  // (host-checked k w (Max + |g|) < 2^23), k w g uniform.
   auto mult = [](int kw, int s, int g) -> int {
     const uint32_t u = (uint32_t)kw & 0x7fffffu;
     int p = (int)__umul24(u, (uint32_t)s);
     asm volatile("" : "+v"(p));  // keeps the 24-bit product (the compiler would refold it into a 32-bit multiply)
     return -p - kw * g;
   };
+  SEnt te[4];
+  lds_entries<R, 4>(&tlds[iy + R][ix], te);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    int s0, s1, s2;
+    sent_raw<MODE, KB>(te[k], s0, s1, s2);
+    t0[k] = s0 + g0;
+    t1[k] = s1 + g1;
+    t2[k] = s2 + g2;
+    if (MODE == kSearchDot) {
+      const short2_t mp = {(short)mult(2 * wY, s0, g0), (short)mult(2 * wCb, s1, g1)};
+      m0[k] = __builtin_bit_cast(int, mp);
+      m1[k] = 0;
+      m2[k] = mult(2 * S * wCr, s2, g2);
+    } else if (MODE == kSearchNarrow) {
+      m0[k] = mult(2 * S * wY, s0, g0);
+      m1[k] = mult(2 * S * wCb, s1, g1);
+      m2[k] = mult(2 * S * wCr, s2, g2);
+    }
+  }
 
-  if (MODE == kSearchDot) {
-    short2_t mp[PY];
-    int m2[PY];
+  int bk[4], bi[4];
 #pragma unroll
-    for (int i = 0; i < PY; i++) {
-      mp[i] = short2_t{(short)mult(2 * wY, s0[i], g0), (short)mult(2 * wCb, s1[i], g1)};
-      m2[i] = mult(2 * wCr, s2[i], g2);
-    }
-#pragma unroll 1
-    for (int ry = 0; ry < PY + 2 * R; ry++) {
-      int3 v[D];
+  for (int k = 0; k < 4; k++) {
+    bk[k] = INT_MAX;
+    bi[k] = 0;
+  }
 #pragma unroll
-      for (int dx = 0; dx < D; dx++) {
-        const int4 e = lds[wd * PY + ry][tx + dx];
-        v[dx] = make_int3(e.x + dx, e.y, e.z);
-      }
+  for (int dy = 0; dy < D; dy++) {
+    SEnt e[NE];
+    lds_run<NE>(&lds[iy + dy][ix], e);
+    if (MODE == kSearchDot) {
 #pragma unroll
-      for (int i = 0; i < PY; i++) {
-        const int dy = ry - i;
-        if ((unsigned)dy >= (unsigned)D) continue;
-        int rm = INT_MAX;
+      for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
+          const SEnt& c = e[k + dx];
           // multiply-add into a fresh register first: v_dot2c accumulates in place
-          int acc = __mul24(m2[i], v[dx].z) + v[dx].x;
-          acc = __builtin_amdgcn_sdot2(mp[i], __builtin_bit_cast(short2_t, v[dx].y), acc, false);
-          rm = min(rm, acc);
+          int acc = __mul24(m2[k], c.z) + c.x;
+          acc = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, m0[k]), __builtin_bit_cast(short2_t, c.y), acc,
+                                       false);
+          bk[k] = min(bk[k], acc);
         }
-        bk[i] = min(bk[i], rm + dy * D);
-      }
-    }
+    } else if (MODE == kSearchNarrow) {
+      int eY[NE], eCb[NE];
 #pragma unroll
-    for (int i = 0; i < PY; i++) bi[i] = bk[i] & (S - 1);
-  } else if (MODE == kSearchNarrow) {
-    int m0[PY], m1[PY], m2[PY];
-#pragma unroll
-    for (int i = 0; i < PY; i++) {
-      m0[i] = mult(2 * S * wY, s0[i], g0);
-      m1[i] = mult(2 * S * wCb, s1[i], g1);
-      m2[i] = mult(2 * S * wCr, s2[i], g2);
-    }
-#pragma unroll 1
-    for (int ry = 0; ry < PY + 2 * R; ry++) {
-      int4 v[D];
-#pragma unroll
-      for (int dx = 0; dx < D; dx++) {
-        v[dx] = lds[wd * PY + ry][tx + dx];
-        v[dx].x += dx;
+      for (int j = 0; j < NE; j++) {
+        eY[j] = (int)((uint32_t)e[j].y & 0xffffu);
+        eCb[j] = (int)((uint32_t)e[j].y >> 16);
       }
 #pragma unroll
-      for (int i = 0; i < PY; i++) {
-        const int dy = ry - i;
-        if ((unsigned)dy >= (unsigned)D) continue;
-        int rm = INT_MAX;
+      for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
-          int acc = __mul24(m0[i], v[dx].y) + v[dx].x;
-          acc = __mul24(m1[i], v[dx].z) + acc;
-          acc = __mul24(m2[i], v[dx].w) + acc;
-          rm = min(rm, acc);
+          int acc = __mul24(m0[k], eY[k + dx]) + e[k + dx].x;
+          acc = __mul24(m1[k], eCb[k + dx]) + acc;
+          acc = __mul24(m2[k], e[k + dx].z) + acc;
+          bk[k] = min(bk[k], acc);
         }
-        bk[i] = min(bk[i], rm + dy * D);
-      }
-    }
+    } else {
 #pragma unroll
-    for (int i = 0; i < PY; i++) bi[i] = bk[i] & (S - 1);
-  } else {
-#pragma unroll 1
-    for (int ry = 0; ry < PY + 2 * R; ry++) {
-      int4 v[D];
-#pragma unroll
-      for (int dx = 0; dx < D; dx++) v[dx] = lds[wd * PY + ry][tx + dx];
-#pragma unroll
-      for (int i = 0; i < PY; i++) {
-        const int dy = ry - i;
-        if ((unsigned)dy >= (unsigned)D) continue;
+      for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
-          int e;
+          const SEnt& c = e[k + dx];
+          int v;
           if (MODE == kSearchWide24) {
-            const int d0 = t0[i] - v[dx].y, d1 = t1[i] - v[dx].z, d2 = t2[i] - v[dx].w;
+            const int d0 = t0[k] - c.x, d1 = t1[k] - c.y, d2 = t2[k] - c.z;
             uint32_t acc = (uint32_t)__mul24(__mul24(d0, wY), d0);
             acc += (uint32_t)__mul24(__mul24(d1, wCb), d1);
             acc += (uint32_t)__mul24(__mul24(d2, wCr), d2);
-            e = (int)acc;
+            v = (int)acc;
           } else {
-            const uint32_t d0 = (uint32_t)__mul24(t0[i] - v[dx].y, t0[i] - v[dx].y);
-            const uint32_t d1 = (uint32_t)__mul24(t1[i] - v[dx].z, t1[i] - v[dx].z);
-            const uint32_t d2 = (uint32_t)__mul24(t2[i] - v[dx].w, t2[i] - v[dx].w);
-            e = (int)(d0 * (uint32_t)wY + d1 * (uint32_t)wCb + d2 * (uint32_t)wCr);
+            const uint32_t d0 = (uint32_t)__mul24(t0[k] - c.x, t0[k] - c.x);
+            const uint32_t d1 = (uint32_t)__mul24(t1[k] - c.y, t1[k] - c.y);
+            const uint32_t d2 = (uint32_t)__mul24(t2[k] - c.z, t2[k] - c.z);
+            v = (int)(d0 * (uint32_t)wY + d1 * (uint32_t)wCb + d2 * (uint32_t)wCr);
           }
-          if (e < bk[i]) { bk[i] = e; bi[i] = dy * D + dx; }
+          if (v < bk[k]) {
+            bk[k] = v;
+            bi[k] = dy * D + dx;
+          }
         }
-      }
     }
   }
+  int idx[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) idx[k] = KEYED ? ((bk[k] - (iy * D + ix + k)) & (S - 1)) : bi[k];
 
-  // Epilogue: winner's squared differences, row sums, SCP, winner map.
-  uint32_t* prow = rows + Pr.rows_off + (size_t)dir * 3 * H;
-  const size_t plane_sz = (size_t)Pr.scp_pitch * H;
-  typedef __attribute__((address_space(1))) uint16_t GU16;
-  typedef __attribute__((address_space(1))) uint8_t GU8;
-  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0)
-  GU16* scp_pic = Pr.scp_off >= 0 ? (GU16*)scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz : nullptr;
-  GU8* bmap = Pr.best_off >= 0 ? (GU8*)best + Pr.best_off + (size_t)dir * W * H : nullptr;
-  int rsum = 0;  // row sums of the wave, one (row, component) per lane
+  // Epilogue: the winners' samples, squared differences, row sums, SCP, winner map.
+  // Pixels outside the picture (tiles on the right and bottom edges) add zero.
+  const bool edge = x0 + kSearchTileW > W || y0 + 16 > H;  // block-uniform
+  const bool row_ok = y < H;
+  uint32_t q[3] = {0u, 0u, 0u};
+  uint32_t rp[3][2];  // winner samples of pixels (0, 1) and (2, 3) as u16 pairs, per component
+  if (KEYED) {
+    // Keyed variants keep |t - r| <= Max + |gcd| < 2^15 (host-checked): the
+    // differences are int16 pairs and v_dot2 adds their squares.
+    uint32_t yc[4], cr[4];
 #pragma unroll
-  for (int i = 0; i < PY; i++) {
-    const int y = yb + i;
-    const bool ok = (x < W) && (y < H);
-    const int ddy = bi[i] / D, ddx = bi[i] - ddy * D;
-    const int4 re = lds[wd * PY + i + ddy][tx + ddx];
-    int4 r = re;  // {_, Y, Cb, Cr}
-    if (MODE == kSearchDot) r = make_int4(0, re.w & 0xffff, (int)((uint32_t)re.w >> 16), re.z >> KB);
-    const int e0 = t0[i] - r.y, e1 = t1[i] - r.z, e2 = t2[i] - r.w;
-    uint32_t q[3] = {ok ? (uint32_t)__mul24(e0, e0) : 0u, ok ? (uint32_t)__mul24(e1, e1) : 0u,
-                     ok ? (uint32_t)__mul24(e2, e2) : 0u};
-    wave_sum3_u32_dpp(q);
-    // lane 3 i + c carries the sum of row i, component c
+    for (int k = 0; k < 4; k++) {
+      const int ddy = (int)((uint32_t)idx[k] / (uint32_t)D), ddx = idx[k] - ddy * D;
+      const SEnt e = lds[iy + ddy][ix + k + ddx];
+      yc[k] = MODE == kSearchDot ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, e.y) >> (unsigned short)KB)
+                                 : (uint32_t)e.y;
+      cr[k] = (uint32_t)e.z;
+    }
 #pragma unroll
-    for (int c = 0; c < 3; c++) rsum = tx == 3 * i + c ? (int)q[c] : rsum;
-    if (ok) {
-      if (scp_pic) {
-        GU16* srow = scp_pic + __umul24((uint32_t)y, (uint32_t)Pr.scp_pitch);  // wave-uniform row
-        srow[x] = (uint16_t)clampi(r.y - g0, 0, Pr.maxv);
-        srow[plane_sz + x] = (uint16_t)clampi(r.z - g1, 0, Pr.maxv);
-        srow[2 * plane_sz + x] = (uint16_t)clampi(r.w - g2, 0, Pr.maxv);
+    for (int h = 0; h < 2; h++) {
+      rp[0][h] = (yc[2 * h] & 0xffffu) | (yc[2 * h + 1] << 16);
+      rp[1][h] = (yc[2 * h] >> 16) | (yc[2 * h + 1] & 0xffff0000u);
+      rp[2][h] = cr[2 * h] | (cr[2 * h + 1] << 16);
+    }
+    uint32_t msk[2] = {0xffffffffu, 0xffffffffu};
+    if (edge) {
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        msk[h] = (row_ok && x + 2 * h < W ? 0xffffu : 0u) | (row_ok && x + 2 * h + 1 < W ? 0xffff0000u : 0u);
+    }
+    const int* tc[3] = {t0, t1, t2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      int acc = 0;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const short2_t tp = {(short)tc[c][2 * h], (short)tc[c][2 * h + 1]};
+        short2_t d = tp - __builtin_bit_cast(short2_t, rp[c][h]);
+        if (edge) d = __builtin_bit_cast(short2_t, __builtin_bit_cast(uint32_t, d) & msk[h]);
+        acc = __builtin_amdgcn_sdot2(d, d, acc, false);
       }
-      if (bmap) bmap[__umul24((uint32_t)y, (uint32_t)W) + (uint32_t)x] = (uint8_t)bi[i];
+      q[c] = (uint32_t)acc;
+    }
+  } else {
+    int rr[3][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int ddy = (int)((uint32_t)idx[k] / (uint32_t)D), ddx = idx[k] - ddy * D;
+      sent_raw<MODE, KB>(lds[iy + ddy][ix + k + ddx], rr[0][k], rr[1][k], rr[2][k]);
+      int e0 = t0[k] - rr[0][k], e1 = t1[k] - rr[1][k], e2 = t2[k] - rr[2][k];
+      if (edge) {
+        const bool ok = row_ok && x + k < W;
+        e0 = ok ? e0 : 0;
+        e1 = ok ? e1 : 0;
+        e2 = ok ? e2 : 0;
+      }
+      q[0] += (uint32_t)__mul24(e0, e0);
+      q[1] += (uint32_t)__mul24(e1, e1);
+      q[2] += (uint32_t)__mul24(e2, e2);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) rp[c][h] = ((uint32_t)rr[c][2 * h] & 0xffffu) | ((uint32_t)rr[c][2 * h + 1] << 16);
+  }
+  row16_sum3_u32_dpp(q);
+  if ((lane & 15) == 15 && row_ok) {
+    uint32_t* rsp = rows + Pr.rows_off + (size_t)dir * 3 * H + y;
+    atomicAdd(rsp, q[0]);
+    atomicAdd(rsp + H, q[1]);
+    atomicAdd(rsp + 2 * H, q[2]);
+  }
+  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0). The SCP
+  // pitch is a multiple of the 64-column tile width, so a lane's 4 samples
+  // stay inside the row; columns >= W are padding.
+  if (Pr.scp_off >= 0 && row_ok) {
+    typedef __attribute__((address_space(1))) uint16_t GU16;
+    typedef __attribute__((address_space(1))) u32x2_t GU32x2;
+    const size_t plane_sz = (size_t)Pr.scp_pitch * H;
+    GU16* sp = (GU16*)scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz + (size_t)y * Pr.scp_pitch + x;
+    const int gc[3] = {g0, g1, g2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      // clip(r - g, 0, Max) for r in [0, Max] (g uniform): g >= 0 is a
+      // saturating u16 subtract, g < 0 a saturating add clipped to Max.
+      u16x2_t v[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const u16x2_t r = __builtin_bit_cast(u16x2_t, rp[c][h]);
+        if (gc[c] >= 0)
+          v[h] = __builtin_elementwise_sub_sat(r, (u16x2_t)(unsigned short)gc[c]);
+        else
+          v[h] = __builtin_elementwise_min(__builtin_elementwise_add_sat(r, (u16x2_t)(unsigned short)(-gc[c])),
+                                           (u16x2_t)(unsigned short)Pr.maxv);
+      }
+      const u32x2_t o = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
+      *(GU32x2*)(sp + c * plane_sz) = o;
     }
   }
-  // one atomic instruction adds all 3 * PY row sums of the wave
-  {
-    const int i = tx / 3, c = tx - 3 * (tx / 3);
-    if (tx < 3 * PY && yb + i < H) atomicAdd(&rows[Pr.rows_off + ((size_t)dir * 3 + c) * H + yb + i], (uint32_t)rsum);
+  if (Pr.best_off >= 0 && row_ok) {
+    typedef __attribute__((address_space(1))) uint8_t GU8;
+    GU8* bm = (GU8*)best + Pr.best_off + (size_t)dir * W * H + (size_t)y * W + x;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (x + k < W) bm[k] = (uint8_t)idx[k];
   }
 }
 
-template <int R, int PY>
+template <int R>
 __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pairs, int npairs, int ntiles, int wY,
                                                 int wCb, int wCr, const int32_t* __restrict__ gcd,
                                                 uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
                                                 uint8_t* __restrict__ best) {
-  using TT = SearchTile<R, PY>;
-  __shared__ int4 lds[2][TT::LH][TT::LW];
+  using TT = SearchTile<R>;
+  __shared__ __attribute__((aligned(16))) SEnt lds[2][TT::LH][TT::LP];
   // XCD-aware order: workgroups are handed to the kXcds XCDs in turn, so XCD g
   // runs tiles g * per, g * per + 1, ...: each XCD walks a contiguous run of
   // tiles, whose halos overlap in its L2.
@@ -664,22 +813,22 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
   const int x0 = tx * TT::TW, y0 = ty * TT::TH;
   if (P.p0.bps == 2) {
     if (P.mode == kSearchDot)
-      search_tile<uint16_t, R, PY, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
-      search_tile<uint16_t, R, PY, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
-      search_tile<uint16_t, R, PY, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
-      search_tile<uint16_t, R, PY, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint16_t, R, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   } else {
     if (P.mode == kSearchDot)
-      search_tile<uint8_t, R, PY, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
-      search_tile<uint8_t, R, PY, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
-      search_tile<uint8_t, R, PY, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
-      search_tile<uint8_t, R, PY, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+      search_tile<uint8_t, R, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
   }
 }
 
@@ -848,9 +997,14 @@ __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, i
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
                                               double* __restrict__ partial) {
-  const int b = blockIdx.x;
+  // XCD-aware order (as in k_search): XCD g runs blocks g * per, g * per + 1, ...,
+  // so neighbouring chunks and band groups, which share cache lines of their
+  // rows, run on one XCD close together and meet in its L2.
+  const int per = (nblk + kXcds - 1) / kXcds;
+  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (b >= nblk) return;  // padding of the last XCD's run (uniform per block)
   const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
   const SsimJobDev J = jobs[ji];
   const int lb = b - J.blk0;
@@ -959,42 +1113,35 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
   return hipGetLastError();
 }
 
-template <int R, int PY>
-static hipError_t search_rp(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
-                            uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
+template <int R>
+static hipError_t search_r(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
+                           uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
   const int per = (ntiles + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_search<R, PY>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
-                     cw[2], gcd, rows, scp, best);
+  hipLaunchKernelGGL((k_search<R>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1], cw[2],
+                     gcd, rows, scp, best);
   return hipGetLastError();
-}
-
-template <int PY>
-static hipError_t search_p(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                           const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
-  switch (R) {
-    case 1: return search_rp<1, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 2: return search_rp<2, PY>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    // R >= 3 runs 4-row waves only (the 8-row tile does not fit the register file)
-    case 3: return PY == 4 ? search_rp<3, 4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st) : hipErrorInvalidValue;
-    case 4: return PY == 4 ? search_rp<4, 4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st) : hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
-  }
 }
 
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
   if (npairs <= 0 || ntiles <= 0) return hipSuccess;
-  if (py == 4) return search_p<4>(pairs, npairs, ntiles, R, cw, gcd, rows, scp, best, st);
-  if (py == 8) return search_p<8>(pairs, npairs, ntiles, R, cw, gcd, rows, scp, best, st);
-  return hipErrorInvalidValue;
+  if (py != kSearchRowsPerWave) return hipErrorInvalidValue;
+  switch (R) {
+    case 1: return search_r<1>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 2: return search_r<2>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 3: return search_r<3>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 4: return search_r<4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st) {
   if (njobs <= 0 || nblk <= 0) return hipSuccess;
+  const int grid = kXcds * ((nblk + kXcds - 1) / kXcds);
   switch (nb) {
-    case 4: hipLaunchKernelGGL(k_ssim<4>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
-    case 8: hipLaunchKernelGGL(k_ssim<8>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
-    case 16: hipLaunchKernelGGL(k_ssim<16>, dim3(nblk), dim3(256), 0, st, jobs, njobs, partial); break;
+    case 4: hipLaunchKernelGGL(k_ssim<4>, dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 8: hipLaunchKernelGGL(k_ssim<8>, dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 16: hipLaunchKernelGGL(k_ssim<16>, dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
