@@ -245,6 +245,14 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     pp.rows_off = d.rows_off;
     P.rows_total += 6 * (int64_t)s.h;
     d.scp_pitch = scp_pitch(s.w);
+    // the kernels address a plane with 32-bit byte offsets (SCP, buffer resources)
+    for (int c = 0; c < 3; c++) {
+      const int64_t rows = c ? (s.h + s.p0.cs) >> s.p0.cs : s.h;
+      if ((int64_t)s.p0.stride[c] * rows * s.p0.bps >= (1ll << 32) ||
+          (int64_t)s.p1.stride[c] * rows * s.p1.bps >= (1ll << 32))
+        return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
+    }
+    if ((int64_t)d.scp_pitch * s.h * 2 >= (1ll << 32)) return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
     if ((P.need_scp && ((mp & ivs) || scp_only)) || (reuse && (mp & ivs))) {
       d.scp_off = P.scp_total;
       P.scp_total += 6 * (int64_t)d.scp_pitch * s.h;

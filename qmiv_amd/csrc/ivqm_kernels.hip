@@ -64,13 +64,20 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(
   return (const GT*)plane_ptr(P, c) + __umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c));
 }
 
-// Sample col of a row pointer, addressed by a 32-bit byte offset from the
-// (scalar) row pointer.
+// Raw buffer resource over plane c of a picture of height H (the exact plane
+// size in bytes; data-format word 32-bit). Loads through it take the row as a
+// scalar byte offset and the column as a 32-bit vector byte offset, so one
+// vector offset serves every row and no address arithmetic is per load.
 template <typename T>
-__device__ __forceinline__ int ld_col(const __attribute__((address_space(1))) T* row, uint32_t col) {
-  typedef __attribute__((address_space(1))) const uint8_t GB;
-  typedef __attribute__((address_space(1))) const T GT;
-  return (int)*(const GT*)((const GB*)row + col * (uint32_t)sizeof(T));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const PicDev& P, int c, int H) {
+  const int s = c ? P.cs : 0;
+  const uint32_t bytes = (uint32_t)((H + s) >> s) * (uint32_t)plane_stride(P, c) * (uint32_t)sizeof(T);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(plane_ptr(P, c)), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ int ld_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if (sizeof(T) == 2) return (int)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
 }
 
 // Picture planes are device allocations: addressed as global memory they load
@@ -496,31 +503,39 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   constexpr int PAR = R & 1;               // region row 0 is picture row y0 - R, odd for odd R
   constexpr int NPT = (LH + PAR + 1) / 2;  // row pairs of the region
   constexpr int NPW = (NPT + 3) / 4;       // row pairs per wave
-  const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);  // unsigned: 32-bit offsets from the row pointer
+  const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);
+  // byte offsets of the lane's column in luma / 4:2:0 chroma rows, and the
+  // planes' buffer resources; row byte offsets are scalar
+  const uint32_t ol = cx * (uint32_t)sizeof(T), oc = (cx >> 1) * (uint32_t)sizeof(T);
+  const __amdgpu_buffer_rsrc_t rs0 = plane_rsrc<T>(Ref, 0, H), rs1 = plane_rsrc<T>(Ref, 1, H),
+                               rs2 = plane_rsrc<T>(Ref, 2, H);
+  const uint32_t st0 = (uint32_t)plane_stride(Ref, 0) * (uint32_t)sizeof(T);
+  const uint32_t st1 = (uint32_t)plane_stride(Ref, 1) * (uint32_t)sizeof(T);
+  const uint32_t st2 = (uint32_t)plane_stride(Ref, 2) * (uint32_t)sizeof(T);
   int ra[NPW][2][3];
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
     const int j = wd + 4 * k;
     const int ly0 = 2 * j - PAR, ly1 = ly0 + 1;
-#pragma unroll
-    for (int h = 0; h < 2; h++) ra[k][h][0] = ra[k][h][1] = ra[k][h][2] = 0;
+    // ra[k][h] is read only under the conditions that load it
     if (j < NPT) {
       const int ya = clampi(y0 - R + ly0, 0, H - 1), yb2 = clampi(y0 - R + ly1, 0, H - 1);
-      if (ly0 >= 0) ra[k][0][0] = ld_col<T>(plane_row<T>(Ref, 0, ya), cx);
-      if (ly1 < LH) ra[k][1][0] = ld_col<T>(plane_row<T>(Ref, 0, yb2), cx);
+      const uint32_t ua = (uint32_t)ya, ub = (uint32_t)yb2;
+      if (ly0 >= 0) ra[k][0][0] = ld_buf<T>(rs0, ol, ua * st0);
+      if (ly1 < LH) ra[k][1][0] = ld_buf<T>(rs0, ol, ub * st0);
       if (Ref.cs) {
-        const int yc = ly0 >= 0 ? ya : yb2;
-        const int cb = ld_col<T>(plane_row<T>(Ref, 1, yc), cx >> 1), cr = ld_col<T>(plane_row<T>(Ref, 2, yc), cx >> 1);
+        const uint32_t yc = (ly0 >= 0 ? ua : ub) >> 1;
+        const int cb = ld_buf<T>(rs1, oc, yc * st1), cr = ld_buf<T>(rs2, oc, yc * st2);
         ra[k][0][1] = ra[k][1][1] = cb;
         ra[k][0][2] = ra[k][1][2] = cr;
       } else {
         if (ly0 >= 0) {
-          ra[k][0][1] = ld_col<T>(plane_row<T>(Ref, 1, ya), cx);
-          ra[k][0][2] = ld_col<T>(plane_row<T>(Ref, 2, ya), cx);
+          ra[k][0][1] = ld_buf<T>(rs1, ol, ua * st1);
+          ra[k][0][2] = ld_buf<T>(rs2, ol, ua * st2);
         }
         if (ly1 < LH) {
-          ra[k][1][1] = ld_col<T>(plane_row<T>(Ref, 1, yb2), cx);
-          ra[k][1][2] = ld_col<T>(plane_row<T>(Ref, 2, yb2), cx);
+          ra[k][1][1] = ld_buf<T>(rs1, ol, ub * st1);
+          ra[k][1][2] = ld_buf<T>(rs2, ol, ub * st2);
         }
       }
     }
@@ -585,34 +600,54 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   const int y = y0 + iy, x = x0 + ix;
   // Test samples: the other direction's region at (iy + R, ix + k + R) holds
   // the test picture at the clamped position -- the sample a direct load reads.
+  // WIDE: t = s + g per component and pixel. Keyed: the test values of
+  // pixels (0, 1) and (2, 3) as int16 pairs per component, tp[c][h], and the
+  // multipliers -2 S w t (NARROW, DOT Cr: exact 24-bit multiply-adds, |value|
+  // < 2^23 host-checked) and {-2 wY t0, -2 wCb t1} (DOT: one packed 16-bit
+  // multiply-add, exact mod 2^16 and the value fits int16, host-checked).
   int t0[4], t1[4], t2[4];
-  int m0[4], m1[4], m2[4];  // NARROW: -2 S w t; DOT: m0 = {-2 wY t0, -2 wCb t1} as int16 pair, m2 = -2 S wCr t2
-  // Multipliers -k w t = -(k w s) - k w g: k w s by an exact 24-bit multiply
-  // (host-checked k w (Max + |g|) < 2^23), k w g uniform.
-  auto mult = [](int kw, int s, int g) -> int {
-    const uint32_t u = (uint32_t)kw & 0x7fffffu;
-    int p = (int)__umul24(u, (uint32_t)s);
-    asm volatile("" : "+v"(p));  // keeps the 24-bit product (the compiler would refold it into a 32-bit multiply)
-    return -p - kw * g;
-  };
+  uint32_t tp[3][2];
+  int m0[4], m1[4], m2[4];
   SEnt te[4];
   lds_entries<R, 4>(&tlds[iy + R][ix], te);
+  if (KEYED) {
+    const int KY = -2 * S * wY, KCb = -2 * S * wCb, KCr = -2 * S * wCr;
+    uint32_t yc[4];  // raw {Y, Cb} per pixel
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    int s0, s1, s2;
-    sent_raw<MODE, KB>(te[k], s0, s1, s2);
-    t0[k] = s0 + g0;
-    t1[k] = s1 + g1;
-    t2[k] = s2 + g2;
-    if (MODE == kSearchDot) {
-      const short2_t mp = {(short)mult(2 * wY, s0, g0), (short)mult(2 * wCb, s1, g1)};
-      m0[k] = __builtin_bit_cast(int, mp);
-      m1[k] = 0;
-      m2[k] = mult(2 * S * wCr, s2, g2);
-    } else if (MODE == kSearchNarrow) {
-      m0[k] = mult(2 * S * wY, s0, g0);
-      m1[k] = mult(2 * S * wCb, s1, g1);
-      m2[k] = mult(2 * S * wCr, s2, g2);
+    for (int k = 0; k < 4; k++) {
+      const int cr = te[k].z;
+      m2[k] = __mul24(cr, KCr) + KCr * g2;
+      if (MODE == kSearchDot) {
+        const u16x2_t raw = __builtin_bit_cast(u16x2_t, te[k].y) >> (unsigned short)KB;
+        const u16x2_t kp = {(unsigned short)(-2 * wY), (unsigned short)(-2 * wCb)};
+        const u16x2_t kg = {(unsigned short)(-2 * wY * g0), (unsigned short)(-2 * wCb * g1)};
+        m0[k] = __builtin_bit_cast(int, raw * kp + kg);
+        m1[k] = 0;
+        yc[k] = __builtin_bit_cast(uint32_t, raw);
+      } else {
+        yc[k] = (uint32_t)te[k].y;
+        m0[k] = __mul24((int)(yc[k] & 0xffffu), KY) + KY * g0;
+        m1[k] = __mul24((int)(yc[k] >> 16), KCb) + KCb * g1;
+      }
+    }
+    const u16x2_t gy = (unsigned short)g0, gb = (unsigned short)g1, gr = (unsigned short)g2;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const u16x2_t sy = {(unsigned short)yc[2 * h], (unsigned short)yc[2 * h + 1]};
+      const u16x2_t sb = {(unsigned short)(yc[2 * h] >> 16), (unsigned short)(yc[2 * h + 1] >> 16)};
+      const u16x2_t sr = {(unsigned short)te[2 * h].z, (unsigned short)te[2 * h + 1].z};
+      tp[0][h] = __builtin_bit_cast(uint32_t, sy + gy);
+      tp[1][h] = __builtin_bit_cast(uint32_t, sb + gb);
+      tp[2][h] = __builtin_bit_cast(uint32_t, sr + gr);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      int s0, s1, s2;
+      sent_raw<MODE, KB>(te[k], s0, s1, s2);
+      t0[k] = s0 + g0;
+      t1[k] = s1 + g1;
+      t2[k] = s2 + g2;
     }
   }
 
@@ -685,6 +720,14 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   for (int k = 0; k < 4; k++) idx[k] = KEYED ? ((bk[k] - (iy * D + ix + k)) & (S - 1)) : bi[k];
 
   // Epilogue: the winners' samples, squared differences, row sums, SCP, winner map.
+  // The winner of pixel k, candidate idx = ddy * D + ddx, is the entry at byte
+  // offset idx * 12 + ddy * (LP - D) * 12 from the pixel's own tile position.
+  auto winner = [&](int k) -> SEnt {
+    const uint32_t u = (uint32_t)idx[k];
+    const uint32_t ddy = u / (uint32_t)D;
+    const uint32_t off = __umul24(u, 12u) + __umul24(ddy, (uint32_t)((LP - D) * 12));
+    return *reinterpret_cast<const SEnt*>(reinterpret_cast<const char*>(&lds[iy][ix + k]) + off);
+  };
   // Pixels outside the picture (tiles on the right and bottom edges) add zero.
   const bool edge = x0 + kSearchTileW > W || y0 + 16 > H;  // block-uniform
   const bool row_ok = y < H;
@@ -696,8 +739,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     uint32_t yc[4], cr[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int ddy = (int)((uint32_t)idx[k] / (uint32_t)D), ddx = idx[k] - ddy * D;
-      const SEnt e = lds[iy + ddy][ix + k + ddx];
+      const SEnt e = winner(k);
       yc[k] = MODE == kSearchDot ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, e.y) >> (unsigned short)KB)
                                  : (uint32_t)e.y;
       cr[k] = (uint32_t)e.z;
@@ -708,31 +750,30 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
       rp[1][h] = (yc[2 * h] >> 16) | (yc[2 * h + 1] & 0xffff0000u);
       rp[2][h] = cr[2 * h] | (cr[2 * h + 1] << 16);
     }
-    uint32_t msk[2] = {0xffffffffu, 0xffffffffu};
-    if (edge) {
+    short2_t d[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
 #pragma unroll
       for (int h = 0; h < 2; h++)
-        msk[h] = (row_ok && x + 2 * h < W ? 0xffffu : 0u) | (row_ok && x + 2 * h + 1 < W ? 0xffff0000u : 0u);
-    }
-    const int* tc[3] = {t0, t1, t2};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      int acc = 0;
+        d[c][h] = __builtin_bit_cast(short2_t, tp[c][h]) - __builtin_bit_cast(short2_t, rp[c][h]);
+    if (edge) [[unlikely]] {
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        const short2_t tp = {(short)tc[c][2 * h], (short)tc[c][2 * h + 1]};
-        short2_t d = tp - __builtin_bit_cast(short2_t, rp[c][h]);
-        if (edge) d = __builtin_bit_cast(short2_t, __builtin_bit_cast(uint32_t, d) & msk[h]);
-        acc = __builtin_amdgcn_sdot2(d, d, acc, false);
+        const uint32_t msk =
+            (row_ok && x + 2 * h < W ? 0xffffu : 0u) | (row_ok && x + 2 * h + 1 < W ? 0xffff0000u : 0u);
+#pragma unroll
+        for (int c = 0; c < 3; c++) d[c][h] = __builtin_bit_cast(short2_t, __builtin_bit_cast(uint32_t, d[c][h]) & msk);
       }
-      q[c] = (uint32_t)acc;
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      q[c] = (uint32_t)__builtin_amdgcn_sdot2(d[c][1], d[c][1], __builtin_amdgcn_sdot2(d[c][0], d[c][0], 0, false),
+                                              false);
   } else {
     int rr[3][4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int ddy = (int)((uint32_t)idx[k] / (uint32_t)D), ddx = idx[k] - ddy * D;
-      sent_raw<MODE, KB>(lds[iy + ddy][ix + k + ddx], rr[0][k], rr[1][k], rr[2][k]);
+      sent_raw<MODE, KB>(winner(k), rr[0][k], rr[1][k], rr[2][k]);
       int e0 = t0[k] - rr[0][k], e1 = t1[k] - rr[1][k], e2 = t2[k] - rr[2][k];
       if (edge) {
         const bool ok = row_ok && x + k < W;
@@ -759,28 +800,29 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0). The SCP
   // pitch is a multiple of the 64-column tile width, so a lane's 4 samples
   // stay inside the row; columns >= W are padding.
+  // SCP planes are below 4 GiB (host-checked): a uniform plane base plus a
+  // 32-bit byte offset per lane.
   if (Pr.scp_off >= 0 && row_ok) {
-    typedef __attribute__((address_space(1))) uint16_t GU16;
+    typedef __attribute__((address_space(1))) uint8_t GU8;
     typedef __attribute__((address_space(1))) u32x2_t GU32x2;
-    const size_t plane_sz = (size_t)Pr.scp_pitch * H;
-    GU16* sp = (GU16*)scp + Pr.scp_off + (size_t)(1 - dir) * 3 * plane_sz + (size_t)y * Pr.scp_pitch + x;
+    const size_t plane_bytes = (size_t)Pr.scp_pitch * H * 2;
+    GU8* pic = (GU8*)(scp + Pr.scp_off) + (size_t)(1 - dir) * 3 * plane_bytes;
+    const uint32_t eo = ((uint32_t)y * (uint32_t)Pr.scp_pitch + (uint32_t)x) * 2u;
     const int gc[3] = {g0, g1, g2};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      // clip(r - g, 0, Max) for r in [0, Max] (g uniform): g >= 0 is a
-      // saturating u16 subtract, g < 0 a saturating add clipped to Max.
+      // clip(r - g, 0, Max) for r in [0, Max]: saturating u16 subtract of
+      // max(g, 0), saturating add of max(-g, 0), min Max
+      const u16x2_t gp = (unsigned short)max(gc[c], 0), gn = (unsigned short)max(-gc[c], 0);
+      const u16x2_t mx = (unsigned short)Pr.maxv;
       u16x2_t v[2];
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const u16x2_t r = __builtin_bit_cast(u16x2_t, rp[c][h]);
-        if (gc[c] >= 0)
-          v[h] = __builtin_elementwise_sub_sat(r, (u16x2_t)(unsigned short)gc[c]);
-        else
-          v[h] = __builtin_elementwise_min(__builtin_elementwise_add_sat(r, (u16x2_t)(unsigned short)(-gc[c])),
-                                           (u16x2_t)(unsigned short)Pr.maxv);
-      }
+      for (int h = 0; h < 2; h++)
+        v[h] = __builtin_elementwise_min(
+            __builtin_elementwise_add_sat(__builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2_t, rp[c][h]), gp), gn),
+            mx);
       const u32x2_t o = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
-      *(GU32x2*)(sp + c * plane_sz) = o;
+      *(GU32x2*)(pic + c * plane_bytes + eo) = o;
     }
   }
   if (Pr.best_off >= 0 && row_ok) {
