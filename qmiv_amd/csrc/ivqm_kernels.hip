@@ -822,7 +822,8 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
             __builtin_elementwise_add_sat(__builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2_t, rp[c][h]), gp), gn),
             mx);
       const u32x2_t o = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
-      *(GU32x2*)(pic + c * plane_bytes + eo) = o;
+      // streaming store: the SCP is read once, by the SSIM pass, after the whole search
+      __builtin_nontemporal_store(o, (GU32x2*)(pic + c * plane_bytes + eo));
     }
   }
   if (Pr.best_off >= 0 && row_ok) {

@@ -604,13 +604,14 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
   int py_best = b->eng->tuning_py(), nb_best = b->eng->tuning_nb();
   const int pys[2] = {4, 8}, nbs[3] = {4, 8, 16};
   for (int pi = 0; pi < 2; pi++) {
+    if (pi > 0 && search_rows(0, pys[pi]) == search_rows(0, pys[0])) break;  // same tile shape
     b->eng->set_tuning(pys[pi], nb_best);
     RCCHK(b->eng->enqueue(specs.data(), n, metrics & 0xFFu, 0, false));
     RCCHK(b->eng->collect(0, nullptr));
     if (!b->eng->last_plan().need_search) break;
     double t = 0;
     RCCHK(b->eng->profile(1, 5, &t));
-    if (t < best_s) { best_s = t; py_best = pys[pi]; }
+    if (t < best_s) { best_s = t; py_best = search_rows(0, pys[pi]); }
   }
   for (int ni = 0; ni < 3; ni++) {
     b->eng->set_tuning(py_best, nbs[ni]);
