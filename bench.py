@@ -231,6 +231,7 @@ def main():
                      "achieved": round(search_tops, 2), "peak": PEAK_VALU_OPS_TOPS, "unit": "TOP/s",
                      "frac": round(search_tops / PEAK_VALU_OPS_TOPS, 4),
                      "peak_issue_tinstr_per_s": PEAK_VALU_ISSUE_TINSTR,
+                     "frac_of_one_op_per_lane_clock": round(search_tops / PEAK_VALU_ISSUE_TINSTR, 4),
                      "traffic": traffic,
                      "us_per_pair": round(t_search * 1e3, 2),
                      "ms_per_launch": round(t_search * (FRAMES if head == "batch" else 1), 4),
