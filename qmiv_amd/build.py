@@ -1,6 +1,7 @@
 """Builds the native libraries of qmiv_amd in-tree.
 
   qmiv_amd/libivqm_hip.so    gfx950 kernels + C ABI (hipcc)
+  qmiv_amd/bin/ivqm_qmiv     QMIV command-line front end over the batch ABI
   qmiv_amd/libivqm_synth.so  seeded synthetic frame generator (g++)
 
 Run ``python -m qmiv_amd.build`` (or ``__graft_entry__.build()``).
@@ -58,6 +59,17 @@ def build_dropin_check(lib: str) -> str:
     return out
 
 
+def build_app(lib: str) -> str:
+    """ivqm_qmiv: the QMIV command-line front end (raw YUV files -> per-frame and average metrics)."""
+    out_dir = os.path.join(HERE, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "ivqm_qmiv")
+    _run([HIPCC, "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "ivqm_app.cpp"), "-L" + os.path.dirname(lib),
+          "-livqm_hip", "-lpthread", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"])
 
@@ -66,6 +78,7 @@ def build(oracle: bool = True) -> None:
     build_synth()
     lib = build_hip()
     build_dropin_check(lib)
+    build_app(lib)
     if oracle:
         build_oracle()
 

@@ -1,0 +1,549 @@
+// ivqm_qmiv: the QMIV command-line front end over the batch ABI.
+//
+// Reads two raw planar YUV files, computes the requested metrics of every
+// frame pair on the GPU and prints the per-frame lines and the averages the
+// reference app prints:
+//   options            apps/QMIV/src/xAppQMIV.cpp:40-122 (help), :160-200, :217-340
+//   frame loop         apps/QMIV/src/xAppQMIV.cpp:756-832
+//   per-frame lines    apps/QMIV/src/xAppQMIV.cpp:950-1110
+//   averages, results  apps/QMIV/src/xAppQMIV.cpp:1118-1160, xMetricQMIV.cpp:44-160
+//
+// Pipeline: a reader thread fills a ring of pinned host slots with frame pairs
+// read from the files; each slot is copied to a device slot on a copy stream;
+// groups of up to 8 pairs run as one batch (ivqm_batch_enqueue) on the compute
+// stream once their copies are done, and a group's results are collected after
+// the next group is enqueued, so reading, copying and computing overlap.
+//
+// Not supported (the reference supports them): PNG/BMP input, 4:2:2, mask
+// mode, colour-space conversion, regular SSIM modes, SSIM windows other than
+// 8/4, config files. They are rejected with a message.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ivqm_batch.h"
+#include "ivqm_epilogue.h"
+
+namespace {
+
+enum Metric { kMSE, kPSNR, kWSPSNR, kIVPSNR, kSSIM, kMSSSIM, kIVSSIM, kIVMSSSIM, kNumMetrics };
+// apps/QMIV/src/xMetricQMIV.h:57-66: name, default, per-component, per-picture, normalized, unit
+struct MetricInfo {
+  const char* name;
+  bool is_default, per_cmp, per_pic, normalized;
+  const char* unit;
+  uint32_t bit;
+};
+const MetricInfo kInfo[kNumMetrics] = {
+    {"MSE", false, true, false, false, "  ", IVQM_M_MSE},
+    {"PSNR", true, true, false, false, "dB", IVQM_M_PSNR},
+    {"WSPSNR", false, true, false, false, "dB", IVQM_M_WSPSNR},
+    {"IVPSNR", true, false, true, false, "dB", IVQM_M_IVPSNR},
+    {"SSIM", false, true, false, true, "  ", IVQM_M_SSIM},
+    {"MSSSIM", false, true, false, true, "  ", IVQM_M_MSSSIM},
+    {"IVSSIM", true, false, true, true, "  ", IVQM_M_IVSSIM},
+    {"IVMSSSIM", false, false, true, true, "  ", IVQM_M_IVMSSSIM},
+};
+const char* kSuffixCmp = " Y:Cb:Cr  ";
+const char* kSuffixPicPerPic = "          ";
+const char* kSuffixPicPerCmp = "-YCbCr    ";
+
+struct Options {
+  std::string in0, in1, result_file;
+  int width = -1, height = -1, bit_depth = 8, chroma = 420;
+  int start0 = 0, start1 = 0, num_frames = -1;
+  bool calc[kNumMetrics] = {};
+  bool erp = false;
+  int lon = 360, lat = 180;
+  int search_range = 2;
+  int cws[4] = {4, 1, 1, 0}, cwa[4] = {4, 1, 1, 0};
+  float unc[4] = {0.01f, 0.01f, 0.01f, 0.0f};
+  int ss_stride = 4, ss_window = 8;
+  std::string ss_mode = "BlockAveraged";
+  std::string ipa = "STOP";
+  int verbose = 1;
+};
+
+std::string upper(std::string s) {
+  for (char& c : s) c = (char)toupper((unsigned char)c);
+  return s;
+}
+std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t\""), b = s.find_last_not_of(" \t\"");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+bool parse_int(const std::string& s, int& v) {
+  char* e = nullptr;
+  const long x = strtol(s.c_str(), &e, 10);
+  if (s.empty() || *e) return false;
+  v = (int)x;
+  return true;
+}
+template <typename T, typename F>
+bool parse_vec4(const std::string& s, T out[4], F conv) {
+  std::vector<std::string> parts;
+  size_t p = 0;
+  const std::string t = trim(s);
+  while (true) {
+    const size_t q = t.find(':', p);
+    parts.push_back(t.substr(p, q == std::string::npos ? std::string::npos : q - p));
+    if (q == std::string::npos) break;
+    p = q + 1;
+  }
+  if (parts.size() != 4) return false;
+  for (int k = 0; k < 4; k++)
+    if (!conv(parts[k], out[k])) return false;
+  return true;
+}
+
+void usage() {
+  printf(
+      "ivqm_qmiv: QMIV metrics of two raw YUV sequences on the GPU\n"
+      " -i0 InputFile0  -i1 InputFile1  -ps WxH | -pw W -ph H  -bd BitDepth (8..16)  -cf 420|444\n"
+      " -pf pix_fmt (yuv420p, yuv420p10le, yuv444p16le, ...)  -s0 / -s1 StartFrame  -nf NumberOfFrames\n"
+      " -r ResultFile  -ml \"PSNR, WSPSNR, IVPSNR, SSIM, MSSSIM, IVSSIM, IVMSSSIM, MSE\" | All\n"
+      " -erp  -lor LonRangeDeg  -lar LatRangeDeg  -sr SearchRange (1..4)  -cws / -cwa \"4:1:1:0\"\n"
+      " -unc \"0.01:0.01:0.01:0\"  -ssm BlockAveraged  -sss 4  -ssw 8  -ipa SKIP|WARN|STOP|CNCL  -v VerboseLevel\n");
+}
+
+// Returns false (after printing the problems) when the options are not usable.
+bool parse_options(int argc, char** argv, Options& o) {
+  std::string errs;
+  bool metric_list = false;
+  auto need = [&](int& i) -> std::string {
+    if (i + 1 >= argc) {
+      errs += std::string("!  missing value for ") + argv[i] + "\n";
+      return std::string();
+    }
+    return argv[++i];
+  };
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto int_opt = [&](int& v) {
+      const std::string s = need(i);
+      if (!parse_int(trim(s), v)) errs += "!  invalid value for " + a + ": " + s + "\n";
+    };
+    if (a == "-i0") o.in0 = need(i);
+    else if (a == "-i1") o.in1 = need(i);
+    else if (a == "-r") o.result_file = need(i);
+    else if (a == "-ps") {
+      const std::string s = need(i);
+      const size_t x = s.find('x');
+      if (x == std::string::npos || !parse_int(s.substr(0, x), o.width) || !parse_int(s.substr(x + 1), o.height))
+        errs += "!  invalid PictureSize " + s + "\n";
+    } else if (a == "-pw") int_opt(o.width);
+    else if (a == "-ph") int_opt(o.height);
+    else if (a == "-bd") int_opt(o.bit_depth);
+    else if (a == "-cf") int_opt(o.chroma);
+    else if (a == "-pf") {
+      // FFmpeg pix_fmt: yuv4CCp[BB][le]
+      const std::string s = need(i);
+      int cf = 0, bd = 8;
+      if (s.size() >= 7 && s.compare(0, 3, "yuv") == 0 && s[6] == 'p' && parse_int(s.substr(3, 3), cf)) {
+        const std::string rest = s.substr(7);
+        if (!rest.empty() && !(rest.size() > 2 && rest.compare(rest.size() - 2, 2, "le") == 0 &&
+                               parse_int(rest.substr(0, rest.size() - 2), bd)))
+          cf = -1;
+      } else {
+        cf = -1;
+      }
+      if (cf < 0) errs += "!  invalid or unsupported PictureFormat " + s + "\n";
+      else { o.chroma = cf; o.bit_depth = bd; }
+    } else if (a == "-s0") int_opt(o.start0);
+    else if (a == "-s1") int_opt(o.start1);
+    else if (a == "-nf") int_opt(o.num_frames);
+    else if (a == "-ml") {
+      metric_list = true;
+      const std::string s = need(i);
+      size_t p = 0;
+      while (p <= s.size()) {
+        const size_t q = s.find(',', p);
+        const std::string tok = upper(trim(s.substr(p, q == std::string::npos ? std::string::npos : q - p)));
+        if (tok == "ALL") {
+          for (bool& c : o.calc) c = true;
+        } else if (!tok.empty()) {
+          int m = 0;
+          while (m < kNumMetrics && tok != kInfo[m].name) m++;
+          if (m == kNumMetrics) errs += "!  MetricList contains not valid entry. Token \"" + tok + "\" in not a metric.\n";
+          else o.calc[m] = true;
+        }
+        if (q == std::string::npos) break;
+        p = q + 1;
+      }
+    } else if (a == "-erp") o.erp = true;
+    else if (a == "-lor") int_opt(o.lon);
+    else if (a == "-lar") int_opt(o.lat);
+    else if (a == "-sr") int_opt(o.search_range);
+    else if (a == "-cws" || a == "-cwa") {
+      const std::string s = need(i);
+      if (!parse_vec4(s, a == "-cws" ? o.cws : o.cwa, [](const std::string& t, int& v) { return parse_int(trim(t), v); }))
+        errs += "!  invalid component weights " + s + "\n";
+    } else if (a == "-unc") {
+      const std::string s = need(i);
+      if (!parse_vec4(s, o.unc, [](const std::string& t, float& v) {
+            char* e = nullptr;
+            v = strtof(trim(t).c_str(), &e);
+            return !trim(t).empty() && !*e;
+          }))
+        errs += "!  invalid UnnoticeableCoef " + s + "\n";
+    } else if (a == "-ssm") o.ss_mode = need(i);
+    else if (a == "-sss") int_opt(o.ss_stride);
+    else if (a == "-ssw") int_opt(o.ss_window);
+    else if (a == "-ipa") o.ipa = upper(need(i));
+    else if (a == "-v") int_opt(o.verbose);
+    else if (a == "-nth") (void)need(i);  // the GPU path has no worker-thread count
+    else if (a == "-ff" || a == "-im" || a == "-bdm" || a == "-cfm" || a == "-csi" || a == "-csm" || a == "-ssb" ||
+             a == "-c" || a == "-nma") {
+      const std::string v = need(i);
+      if (!(a == "-ff" && upper(v) == "RAW") && !(a == "-nma")) errs += "!  unsupported option " + a + " " + v + "\n";
+    } else errs += "!  unknown option " + a + "\n";
+  }
+  if (!metric_list)
+    for (int m = 0; m < kNumMetrics; m++) o.calc[m] = kInfo[m].is_default;
+  if (o.in0.empty()) errs += "!  InputFile0 is empty\n";
+  if (o.in1.empty()) errs += "!  InputFile1 is empty\n";
+  if (o.width <= 0 || o.height <= 0) errs += "!  Invalid or missing PictureSize\n";
+  if (o.bit_depth < 8 || o.bit_depth > 16) errs += "!  Invalid or unsuported BitDepth value\n";
+  if (o.chroma != 420 && o.chroma != 444) errs += "!  Invalid or unsuported ChromaFormat value (420 or 444)\n";
+  if (o.chroma == 420 && ((o.width | o.height) & 1)) errs += "!  4:2:0 needs even picture sizes\n";
+  if (o.search_range < 1 || o.search_range > 4) errs += "!  SearchRange must be 1..4\n";
+  if (upper(o.ss_mode) != "BLOCKAVERAGED") errs += "!  only StructSimMode=BlockAveraged is supported\n";
+  if (o.ss_stride != 4 || o.ss_window != 8) errs += "!  only StructSimWindow=8 / StructSimStride=4 are supported\n";
+  if (o.ipa != "SKIP" && o.ipa != "WARN" && o.ipa != "STOP" && o.ipa != "CNCL") errs += "!  invalid InvalidPelActn\n";
+  if (o.start0 < 0 || o.start1 < 0) errs += "!  invalid StartFrame\n";
+  if (!errs.empty()) {
+    fputs(errs.c_str(), stderr);
+    return false;
+  }
+  return true;
+}
+
+// ---- per-metric statistics: apps/QMIV/src/xMetricQMIV.cpp:44-160
+struct MetricStat {
+  std::vector<double> cmp[3], pic;
+};
+
+std::string fmt_per_cmp(int m, const double v[3]) {
+  char b[256];
+  const char* f = kInfo[m].normalized ? "%8.6f " : "%8.4f ";
+  int n = snprintf(b, sizeof b, "%8s%s ", kInfo[m].name, kSuffixCmp);
+  for (int c = 0; c < 3; c++) n += snprintf(b + n, sizeof b - n, f, v[c]);
+  return b;
+}
+std::string fmt_per_pic(int m, double v) {
+  char b[256];
+  int n = snprintf(b, sizeof b, "%8s%s ", kInfo[m].name, kInfo[m].per_pic ? kSuffixPicPerPic : kSuffixPicPerCmp);
+  n += snprintf(b + n, sizeof b - n, kInfo[m].normalized ? "%8.6f " : "%8.4f ", v);
+  snprintf(b + n, sizeof b - n, "                  ");
+  return b;
+}
+std::string fmt_avg(int m, const MetricStat& st, int nframes, const char* header) {
+  const bool hdr = header[0] != 0;
+  const char* single = kInfo[m].normalized ? "%10.8f %s  " : "%10.6f %s  ";
+  std::string out;
+  char b[512];
+  auto avg = [&](const std::vector<double>& v) {
+    ivqm::host::Kbns k;
+    for (double x : v) k.acc(x);
+    return k.result() / (double)nframes;
+  };
+  if (kInfo[m].per_cmp) {
+    int n = snprintf(b, sizeof b, hdr ? "%s%8s%s " : "%s%-8s%s ", header, kInfo[m].name, kSuffixCmp);
+    for (int c = 0; c < 3; c++) n += snprintf(b + n, sizeof b - n, single, avg(st.cmp[c]), kInfo[m].unit);
+    out += std::string(b) + "\n";
+  }
+  int n = snprintf(b, sizeof b, hdr ? "%s%8s%s " : "%s%-8s%s ", header, kInfo[m].name,
+                   kInfo[m].per_pic ? kSuffixPicPerPic : kSuffixPicPerCmp);
+  n += snprintf(b + n, sizeof b - n, single, avg(st.pic), kInfo[m].unit);
+  snprintf(b + n, sizeof b - n, "                  ");
+  return out + b;
+}
+
+#define HIPX(x)                                                                            \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "ERROR --> HIP %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return 2;                                                                            \
+    }                                                                                      \
+  } while (0)
+#define IVQMX(x)                                                                          \
+  do {                                                                                    \
+    int s_ = (x);                                                                         \
+    if (s_ != IVQM_OK) {                                                                  \
+      fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(s_), ivqm_last_error());       \
+      return 2;                                                                           \
+    }                                                                                     \
+  } while (0)
+
+constexpr int kGroup = 8;            // frame pairs per batch
+constexpr int kSlots = 2 * kGroup;   // ring of pinned host / device slots
+
+struct Ring {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<int> state;  // per slot: frame index loaded into it, or -1 (free)
+  std::vector<int> error;  // per slot: 0, or 1 = read error, 2 = invalid sample value
+  bool stop = false;
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Options o;
+  for (int i = 1; i < argc; i++)
+    if (!strcmp(argv[i], "-h") || !strcmp(argv[i], "--help")) {
+      usage();
+      return 0;
+    }
+  if (!parse_options(argc, argv, o)) {
+    usage();
+    return 1;
+  }
+  const int W = o.width, H = o.height, cs = o.chroma == 420 ? 1 : 0, bps = o.bit_depth > 8 ? 2 : 1;
+  const size_t luma = (size_t)W * H, chroma = (size_t)(W >> cs) * (H >> cs);
+  const size_t frame_bytes = (luma + 2 * chroma) * bps;
+
+  FILE* f0 = fopen(o.in0.c_str(), "rb");
+  FILE* f1 = fopen(o.in1.c_str(), "rb");
+  if (!f0 || !f1) {
+    fprintf(stderr, "ERROR --> InputFile%d does not exist (%s)\n", f0 ? 1 : 0, (f0 ? o.in1 : o.in0).c_str());
+    return 1;
+  }
+  auto frames_in = [&](FILE* f) {
+    fseeko(f, 0, SEEK_END);
+    const long long n = (long long)(ftello(f) / (off_t)frame_bytes);
+    return n;
+  };
+  const long long n0 = frames_in(f0) - o.start0, n1 = frames_in(f1) - o.start1;
+  long long nf = std::min(n0, n1);
+  if (o.num_frames >= 0) nf = std::min<long long>(nf, o.num_frames);
+  if (nf <= 0) {
+    fprintf(stderr, "ERROR --> no frames to process (%lld and %lld frames after the start frames)\n", n0, n1);
+    return 1;
+  }
+  const int N = (int)nf;
+
+  uint32_t metrics = 0;
+  for (int m = 0; m < kNumMetrics; m++)
+    if (o.calc[m]) metrics |= kInfo[m].bit;
+
+  ivqm_config cfg;
+  IVQMX(ivqm_config_init(&cfg));
+  cfg.width = W;
+  cfg.height = H;
+  cfg.bit_depth = o.bit_depth;
+  cfg.search_range = o.search_range;
+  for (int k = 0; k < 4; k++) {
+    cfg.cmp_weights_search[k] = o.cws[k];
+    cfg.cmp_weights_average[k] = o.cwa[k];
+    cfg.unnoticeable_coef[k] = o.unc[k];
+  }
+  cfg.erp = o.erp;
+  cfg.lon_range_deg = o.lon;
+  cfg.lat_range_deg = o.lat;
+  cfg.fake_vals_for_exact = 1;  // apps/QMIV/src/xAppQMIV.cpp:974-981, :1002-1009
+  ivqm_batch* batch = nullptr;
+  IVQMX(ivqm_batch_create(&cfg, &batch));
+
+  hipStream_t cstream, xstream;
+  HIPX(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+  HIPX(hipStreamCreateWithFlags(&xstream, hipStreamNonBlocking));
+  IVQMX(ivqm_batch_set_stream(batch, (void*)cstream));
+  IVQMX(ivqm_batch_set_tuning(batch, 0, 0, 1));
+  std::vector<uint8_t*> host(kSlots), dev(kSlots);
+  std::vector<hipEvent_t> copied(kSlots);
+  for (int s = 0; s < kSlots; s++) {
+    HIPX(hipHostMalloc((void**)&host[s], 2 * frame_bytes, hipHostMallocDefault));
+    HIPX(hipMalloc((void**)&dev[s], 2 * frame_bytes));
+    HIPX(hipEventCreateWithFlags(&copied[s], hipEventDisableTiming));
+  }
+
+  // ---- reader: frame f goes to slot f % kSlots once that slot is free
+  Ring ring;
+  ring.state.assign(kSlots, -1);
+  ring.error.assign(kSlots, 0);
+  const uint32_t maxv = (1u << o.bit_depth) - 1;
+  const bool check = o.ipa != "SKIP";
+  std::thread reader([&]() {
+    for (int f = 0; f < N; f++) {
+      const int s = f % kSlots;
+      {
+        std::unique_lock<std::mutex> lk(ring.mu);
+        ring.cv.wait(lk, [&] { return ring.stop || ring.state[s] == -1; });
+        if (ring.stop) return;
+      }
+      int err = 0;
+      uint8_t* dst = host[s];
+      FILE* fs[2] = {f0, f1};
+      const int st[2] = {o.start0, o.start1};
+      for (int i = 0; i < 2 && !err; i++) {
+        if (fseeko(fs[i], (off_t)(st[i] + f) * (off_t)frame_bytes, SEEK_SET) != 0 ||
+            fread(dst + i * frame_bytes, 1, frame_bytes, fs[i]) != frame_bytes)
+          err = 1;
+      }
+      // libs/pmbb-core/src/xSeq.cpp: samples above the bit depth's maximum are invalid
+      if (!err && check && bps == 2) {
+        uint16_t* p = (uint16_t*)dst;
+        for (size_t k = 0; k < 2 * frame_bytes / 2; k++)
+          if (p[k] > maxv) {
+            if (o.ipa == "CNCL") p[k] = (uint16_t)maxv;
+            else err = 2;
+            if (o.ipa == "WARN") err = 0;
+          }
+      }
+      std::lock_guard<std::mutex> lk(ring.mu);
+      ring.state[s] = f;
+      ring.error[s] = err;
+      ring.cv.notify_all();
+    }
+  });
+  auto fail = [&](int rc) {
+    {
+      std::lock_guard<std::mutex> lk(ring.mu);
+      ring.stop = true;
+      ring.cv.notify_all();
+    }
+    reader.join();
+    return rc;
+  };
+
+  std::vector<ivqm_frame_result> res(N);
+  auto picture = [&](uint8_t* base) {
+    ivqm_picture p;
+    p.plane[0] = base;
+    p.plane[1] = base + luma * bps;
+    p.plane[2] = base + (luma + chroma) * bps;
+    p.stride[0] = W;
+    p.stride[1] = p.stride[2] = W >> cs;
+    p.chroma_420 = cs;
+    p.bytes_per_sample = bps;
+    return p;
+  };
+  const int ngroups = (N + kGroup - 1) / kGroup;
+  auto release = [&](int g) {  // the group's slots may be refilled
+    std::lock_guard<std::mutex> lk(ring.mu);
+    for (int f = g * kGroup; f < std::min(N, (g + 1) * kGroup); f++) ring.state[f % kSlots] = -1;
+    ring.cv.notify_all();
+  };
+  for (int g = 0; g <= ngroups; g++) {
+    if (g < ngroups) {
+      const int fb = g * kGroup, fe = std::min(N, fb + kGroup);
+      std::vector<ivqm_pair> pairs;
+      for (int f = fb; f < fe; f++) {
+        const int s = f % kSlots;
+        int err;
+        {
+          std::unique_lock<std::mutex> lk(ring.mu);
+          ring.cv.wait(lk, [&] { return ring.state[s] == f; });
+          err = ring.error[s];
+        }
+        if (err == 1) {
+          fprintf(stderr, "Frame %08d ERROR --> InputFile read error\n", f);
+          return fail(1);
+        }
+        if (err == 2) {
+          fprintf(stderr, "Frame %08d ERROR --> sample value above %u (InvalidPelActn=STOP)\n", f, maxv);
+          return fail(1);
+        }
+        if (hipMemcpyAsync(dev[s], host[s], 2 * frame_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess ||
+            hipEventRecord(copied[s], xstream) != hipSuccess)
+          return fail(2);
+        ivqm_pair pr;
+        pr.p0 = picture(dev[s]);
+        pr.p1 = picture(dev[s] + frame_bytes);
+        pr.width = W;
+        pr.height = H;
+        pr.bit_depth = o.bit_depth;
+        pairs.push_back(pr);
+      }
+      if (hipStreamWaitEvent(cstream, copied[(fe - 1) % kSlots], 0) != hipSuccess) return fail(2);
+      const int rc = ivqm_batch_enqueue(batch, pairs.data(), (int)pairs.size(), metrics, g % 2);
+      if (rc != IVQM_OK) {
+        fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(rc), ivqm_last_error());
+        return fail(2);
+      }
+    }
+    if (g > 0) {
+      const int pg = g - 1;
+      const int rc = ivqm_batch_collect(batch, pg % 2, res.data() + pg * kGroup);
+      if (rc != IVQM_OK) {
+        fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(rc), ivqm_last_error());
+        return fail(2);
+      }
+      release(pg);
+    }
+  }
+  reader.join();
+
+  // ---- per-frame lines (apps/QMIV/src/xAppQMIV.cpp:950-1110) and statistics
+  MetricStat st[kNumMetrics];
+  const int32_t* cw = cfg.cmp_weights_average;
+  for (int f = 0; f < N; f++) {
+    const ivqm_frame_result& r = res[f];
+    const bool print = o.verbose >= 2;
+    std::string exact;
+    if (r.exact[0]) exact += " ExactY";
+    if (r.exact[1]) exact += " ExactU";
+    if (r.exact[2]) exact += " ExactV";
+    auto per_cmp = [&](int m, const double* v, double pic, bool with_exact, bool always) {
+      for (int c = 0; c < 3; c++) st[m].cmp[c].push_back(v[c]);
+      st[m].pic.push_back(pic);
+      if (print || always) {
+        printf("Frame %08d %s%s\n", f, fmt_per_cmp(m, v).c_str(), with_exact ? exact.c_str() : "");
+        printf("Frame %08d %s\n", f, fmt_per_pic(m, pic).c_str());
+      }
+    };
+    auto per_pic = [&](int m, double v) {
+      st[m].pic.push_back(v);
+      if (print) printf("Frame %08d %s\n", f, fmt_per_pic(m, v).c_str());
+    };
+    if (o.calc[kMSE]) per_cmp(kMSE, r.mse, ivqm::host::pic_value(r.mse, cw), true, false);
+    if (o.calc[kPSNR]) per_cmp(kPSNR, r.psnr, r.psnr_pic, true, false);
+    if (o.calc[kWSPSNR]) per_cmp(kWSPSNR, r.wspsnr, r.wspsnr_pic, true, false);
+    if (o.calc[kIVPSNR]) per_pic(kIVPSNR, r.ivpsnr);
+    if (o.calc[kSSIM]) per_cmp(kSSIM, r.ssim, r.ssim_pic, false, false);
+    if (o.calc[kMSSSIM]) per_cmp(kMSSSIM, r.msssim, r.msssim_pic, false, true);  // printed at every level
+    if (o.calc[kIVSSIM]) per_pic(kIVSSIM, r.ivssim);
+    if (o.calc[kIVMSSSIM]) per_pic(kIVMSSSIM, r.ivmsssim);
+  }
+
+  // ---- averages (apps/QMIV/src/xAppQMIV.cpp:1131-1160)
+  for (int m = 0; m < kNumMetrics; m++)
+    if (o.calc[m]) printf("%s\n", fmt_avg(m, st[m], N, "Average      ").c_str());
+  if (!o.result_file.empty()) {
+    FILE* rf = fopen(o.result_file.c_str(), "w");
+    if (!rf) {
+      fprintf(stderr, "ERROR --> cannot write ResultFile %s\n", o.result_file.c_str());
+      return 1;
+    }
+    char tb[64];
+    const time_t now = time(nullptr);
+    strftime(tb, sizeof tb, "%Y-%m-%d  %H:%M:%S", localtime(&now));
+    fprintf(rf, "FILE0  \"%s\"\nFILE1  \"%s\"\nTIME   %s\n", o.in0.c_str(), o.in1.c_str(), tb);
+    for (int m = 0; m < kNumMetrics; m++)
+      if (o.calc[m]) fprintf(rf, "%s\n", fmt_avg(m, st[m], N, "").c_str());
+    fclose(rf);
+  }
+  for (int s = 0; s < kSlots; s++) {
+    (void)hipHostFree(host[s]);
+    (void)hipFree(dev[s]);
+    (void)hipEventDestroy(copied[s]);
+  }
+  ivqm_batch_destroy(batch);
+  (void)hipStreamDestroy(cstream);
+  (void)hipStreamDestroy(xstream);
+  fclose(f0);
+  fclose(f1);
+  return 0;
+}

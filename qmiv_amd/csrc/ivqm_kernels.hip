@@ -894,9 +894,10 @@ typedef unsigned char u8x2 __attribute__((ext_vector_type(2)));
 
 template <typename T, int CS>
 __device__ __forceinline__ void load4(const __attribute__((address_space(1))) T* row, int x, uint32_t v[4]) {
-  typedef __attribute__((address_space(1))) const T GT;
+  // 32-bit byte offsets from the (scalar) row pointer
+  typedef __attribute__((address_space(1))) const uint8_t GB;
   if (CS == 0) {
-    GT* q = row + x;
+    GB* q = (GB*)row + (uint32_t)x * (uint32_t)sizeof(T);
     if (sizeof(T) == 2) {
       const u16x4 u = *(const __attribute__((address_space(1))) u16x4*)q;
       v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
@@ -905,7 +906,7 @@ __device__ __forceinline__ void load4(const __attribute__((address_space(1))) T*
       v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
     }
   } else {
-    GT* q = row + (x >> 1);
+    GB* q = (GB*)row + ((uint32_t)x >> 1) * (uint32_t)sizeof(T);
     uint32_t a, b;
     if ((reinterpret_cast<uintptr_t>(q) & (2 * sizeof(T) - 1)) == 0) {  // uniform: x >> 1 is even
       if (sizeof(T) == 2) {
@@ -916,8 +917,8 @@ __device__ __forceinline__ void load4(const __attribute__((address_space(1))) T*
         a = u.x; b = u.y;
       }
     } else {
-      a = q[0];
-      b = q[1];
+      a = ((const __attribute__((address_space(1))) T*)q)[0];
+      b = ((const __attribute__((address_space(1))) T*)q)[1];
     }
     v[0] = a; v[1] = a; v[2] = b; v[3] = b;
   }
@@ -1054,7 +1055,8 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   const int per_c = J.nchunk * J.nbg;
   const int c = lb / per_c, rem = lb - c * per_c;
   const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
-  const int lane = threadIdx.x & 63, band = bg * 4 + (threadIdx.x >> 6);
+  // band is wave-uniform: row pointers stay scalar and every load adds only the lane's column
+  const int lane = threadIdx.x & 63, band = bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (J.wsq == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);

@@ -1,0 +1,53 @@
+"""Option handling of the QMIV command-line front end (qmiv_amd/bin/ivqm_qmiv):
+rejected configurations exit with status 1 and the reference's style of
+message before any GPU call (apps/QMIV/src/xAppQMIV.cpp:217-370)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "qmiv_amd", "bin", "ivqm_qmiv")
+
+
+def _run(*args):
+    if not os.path.exists(EXE):
+        pytest.skip("front end not built (python -m qmiv_amd.build)")
+    return subprocess.run([EXE, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_help():
+    r = _run("-h")
+    assert r.returncode == 0 and "-i0" in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-i0", "a", "-ps", "64x64"], "InputFile1 is empty"),
+    (["-i0", "a", "-i1", "b"], "PictureSize"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cf", "422"], "ChromaFormat"),
+    (["-i0", "a", "-i1", "b", "-ps", "65x64", "-cf", "420"], "even picture sizes"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-bd", "17"], "BitDepth"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ml", "PSNR, FOO"], 'Token "FOO"'),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "RegularGaussianFlt"], "BlockAveraged"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssw", "16"], "StructSimWindow"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-sr", "5"], "SearchRange"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv"], "unsupported option -im"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cws", "4:1:1"], "component weights"),
+    (["-i0", "a", "-i1", "b", "-pf", "rgb24", "-ps", "64x64"], "PictureFormat"),
+])
+def test_rejected_options(args, msg):
+    r = _run(*args)
+    assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+def test_missing_input_file(tmp_path):
+    r = _run("-i0", str(tmp_path / "none0.yuv"), "-i1", str(tmp_path / "none1.yuv"), "-ps", "64x64")
+    assert r.returncode == 1 and "does not exist" in r.stderr
+
+
+def test_pix_fmt_sets_depth_and_chroma(tmp_path):
+    # yuv444p12le: 12-bit 4:4:4; an empty file has no frames, reported after the options are accepted
+    a = tmp_path / "a.yuv"
+    a.write_bytes(b"")
+    r = _run("-i0", str(a), "-i1", str(a), "-pf", "yuv444p12le", "-ps", "64x64")
+    assert r.returncode == 1 and "no frames to process" in r.stderr

@@ -1,0 +1,128 @@
+"""The QMIV command-line front end (qmiv_amd/bin/ivqm_qmiv) on raw YUV files,
+against the oracle: every per-frame line and every average line the reference
+app prints (apps/QMIV/src/xAppQMIV.cpp:950-1160, xMetricQMIV.cpp:91-160).
+PSNR / WS-PSNR / IV-PSNR / MSE lines must be identical text (bit-exact values
+through the same printf formats); SSIM-family numbers agree within 1e-6 (plus
+the printed rounding)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from qmiv_amd import synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "qmiv_amd", "bin", "ivqm_qmiv")
+
+NAMES = ["MSE", "PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM", "IVMSSSIM"]
+PER_CMP = {"MSE", "PSNR", "WSPSNR", "SSIM", "MSSSIM"}
+NORMALIZED = {"SSIM", "MSSSIM", "IVSSIM", "IVMSSSIM"}
+UNIT = {"PSNR": "dB", "WSPSNR": "dB", "IVPSNR": "dB"}
+
+
+def _frame_lines(f, name, cmp_vals, pic, exact):
+    """The reference's per-frame lines (xMetricQMIV.cpp:102-124) for one metric."""
+    single = "%8.6f " if name in NORMALIZED else "%8.4f "
+    out = []
+    if name in PER_CMP:
+        line = "Frame %08d " % f + "%8s%s " % (name, " Y:Cb:Cr  ") + "".join(single % v for v in cmp_vals)
+        if name in ("MSE", "PSNR", "WSPSNR"):
+            line += "".join(" Exact" + "YUV"[c] for c in range(3) if exact[c])
+        out.append(line)
+        out.append("Frame %08d " % f + "%8s%s " % (name, "-YCbCr    ") + single % pic + " " * 18)
+    else:
+        out.append("Frame %08d " % f + "%8s%s " % (name, " " * 10) + single % pic + " " * 18)
+    return out
+
+
+def _avg_lines(name, cmp_avgs, pic_avg, header):
+    """xMetricStat::formatAvgMetric (xMetricQMIV.cpp:126-149)."""
+    nf = "%s%8s%s " if header else "%s%-8s%s "
+    single = "%10.8f %s  " if name in NORMALIZED else "%10.6f %s  "
+    unit = UNIT.get(name, "  ")
+    out = []
+    if name in PER_CMP:
+        out.append(nf % (header, name, " Y:Cb:Cr  ") + "".join(single % (v, unit) for v in cmp_avgs))
+    out.append(nf % (header, name, "-YCbCr    " if name in PER_CMP else " " * 10) + single % (pic_avg, unit) + " " * 18)
+    return out
+
+
+def _numbers(line):
+    return [float(x) for x in re.findall(r"-?\d+\.\d+", line)]
+
+
+def test_qmiv_front_end_matches_oracle(tmp_path, oracle):
+    if not os.path.exists(EXE):
+        pytest.fail("front end not built (python -m qmiv_amd.build)")
+    wl = synth.Workload(cfg=410, width=200, height=136, global_shift=1)
+    n, bd, area = 11, 10, wl.width * wl.height
+    pairs = [synth.make_pair(wl, i) for i in range(n)]
+    # pair 3: identical Cb planes (an exact component: fake PSNR / WS-PSNR, "ExactU")
+    pairs[3] = (pairs[3][0], [pairs[3][1][0], pairs[3][0][1].copy(), pairs[3][1][2]])
+    junk = synth.make_pair(wl, 99)[0]
+    f0, f1, rf = tmp_path / "a.yuv", tmp_path / "b.yuv", tmp_path / "r.txt"
+    raw = lambda pic: b"".join(np.ascontiguousarray(p, np.uint16).tobytes() for p in pic)
+    f0.write_bytes(raw(junk) + b"".join(raw(p[0]) for p in pairs) + raw(junk))   # 13 frames, start at 1
+    f1.write_bytes(b"".join(raw(p[1]) for p in pairs))                          # 11 frames
+    out = subprocess.run([EXE, "-i0", str(f0), "-i1", str(f1), "-ps", f"{wl.width}x{wl.height}", "-bd", str(bd),
+                          "-cf", "420", "-s0", "1", "-ml", "All", "-erp", "-v", "2", "-r", str(rf)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    got = out.stdout.splitlines()
+
+    fake = oracle.psnr_from_ssd(1, area, bd)
+    per = {m: ([[], [], []], []) for m in NAMES}
+    expect = []
+    for f, (p0n, p1n) in enumerate(pairs):
+        p0, p1 = oracle.to444(p0n, True), oracle.to444(p1n, True)
+        exact = [bool(np.array_equal(p0[c], p1[c])) for c in range(3)]
+        g = oracle.gcd(p0, p1, bd)
+        s0, s1 = oracle.scp(p0, p1, bd, g)
+        vals = {
+            "MSE": [oracle.ssd(p1[c], p0[c]) / area for c in range(3)],
+            "PSNR": [fake if exact[c] else v for c, v in enumerate(oracle.psnr(p0, p1, bd))],
+            "WSPSNR": [fake if exact[c] else v for c, v in enumerate(oracle.wspsnr(p0, p1, bd, erp=True))],
+            "IVPSNR": oracle.ivpsnr(p0, p1, bd, g, erp=True)[0],
+            "SSIM": oracle.ssim(p0, p1, bd),
+            "MSSSIM": oracle.msssim(p0, p1, bd),
+            "IVSSIM": oracle.ivssim(p0, p1, s0, s1, bd, erp=True)[0],
+            "IVMSSSIM": oracle.ivmsssim(p0, p1, s0, s1, bd),
+        }
+        for m in NAMES:
+            v = vals[m]
+            if m in PER_CMP:
+                pic = oracle.pic_value(v)
+                for c in range(3):
+                    per[m][0][c].append(v[c])
+                expect += _frame_lines(f, m, v, pic, exact)
+            else:
+                pic = v
+                expect += _frame_lines(f, m, None, pic, exact)
+            per[m][1].append(pic)
+    for m in NAMES:
+        cmp_avg = [oracle.kbns(per[m][0][c]) / n for c in range(3)] if m in PER_CMP else None
+        expect += _avg_lines(m, cmp_avg, oracle.kbns(per[m][1]) / n, "Average      ")
+
+    assert len(got) == len(expect), "\n".join(got[-20:])
+    worst = 0.0
+    for g_line, e_line in zip(got, expect):
+        name = e_line.split()[2] if e_line.startswith("Frame") else e_line.split()[1]
+        if name.rstrip("-YCbCr") in NORMALIZED or name.split("-")[0] in NORMALIZED:
+            assert re.sub(r"-?\d+\.\d+", "#", g_line) == re.sub(r"-?\d+\.\d+", "#", e_line), (g_line, e_line)
+            d = np.max(np.abs(np.array(_numbers(g_line)) - np.array(_numbers(e_line))))
+            worst = max(worst, d)
+            assert d <= 1.5e-6, (g_line, e_line)
+        else:
+            assert g_line == e_line
+    print(f"front end: {len(got)} lines, SSIM-family largest printed difference {worst:.1e}")
+
+    # the result file holds the same averages without the line header
+    res = rf.read_text().splitlines()
+    assert res[0] == f'FILE0  "{f0}"' and res[1] == f'FILE1  "{f1}"' and res[2].startswith("TIME   ")
+    avgs = [l for l in got if l.startswith("Average")]
+    assert len(res) - 3 == len(avgs)
+    for r_line, a_line in zip(res[3:], avgs):
+        assert _numbers(r_line) == _numbers(a_line)
