@@ -8,7 +8,7 @@
 //   per-frame lines    apps/QMIV/src/xAppQMIV.cpp:950-1110
 //   averages, results  apps/QMIV/src/xAppQMIV.cpp:1118-1160, xMetricQMIV.cpp:44-160
 //
-// Pipeline: a reader thread fills a ring of pinned host slots with frame pairs
+// Pipeline: reader threads fill a ring of pinned host slots with frame pairs
 // read from the files; each slot is copied to a device slot on a copy stream;
 // groups of up to 8 pairs run as one batch (ivqm_batch_enqueue) on the compute
 // stream once their copies are done, and a group's results are collected after
@@ -25,6 +25,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -375,46 +376,58 @@ int main(int argc, char** argv) {
   ring.error.assign(kSlots, 0);
   const uint32_t maxv = (1u << o.bit_depth) - 1;
   const bool check = o.ipa != "SKIP";
-  std::thread reader([&]() {
-    for (int f = 0; f < N; f++) {
+  // kReaders threads read frames f = t, t + kReaders, ... (file reads and the
+  // sample check run in parallel; the slot order is kept by frame index)
+  const int kReaders = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+  auto read_frames = [&](int t) {
+    FILE* fs[2] = {fopen(o.in0.c_str(), "rb"), fopen(o.in1.c_str(), "rb")};
+    const int st[2] = {o.start0, o.start1};
+    for (int f = t; f < N; f += kReaders) {
       const int s = f % kSlots;
       {
         std::unique_lock<std::mutex> lk(ring.mu);
         ring.cv.wait(lk, [&] { return ring.stop || ring.state[s] == -1; });
-        if (ring.stop) return;
+        if (ring.stop) break;
       }
       int err = 0;
       uint8_t* dst = host[s];
-      FILE* fs[2] = {f0, f1};
-      const int st[2] = {o.start0, o.start1};
       for (int i = 0; i < 2 && !err; i++) {
-        if (fseeko(fs[i], (off_t)(st[i] + f) * (off_t)frame_bytes, SEEK_SET) != 0 ||
+        if (!fs[i] || fseeko(fs[i], (off_t)(st[i] + f) * (off_t)frame_bytes, SEEK_SET) != 0 ||
             fread(dst + i * frame_bytes, 1, frame_bytes, fs[i]) != frame_bytes)
           err = 1;
       }
-      // libs/pmbb-core/src/xSeq.cpp: samples above the bit depth's maximum are invalid
+      // samples above the bit depth's maximum are invalid (InvalidPelActn,
+      // apps/QMIV/src/xAppQMIV.cpp:770-775): a vectorised max first, the
+      // per-sample action only when one is found
       if (!err && check && bps == 2) {
         uint16_t* p = (uint16_t*)dst;
-        for (size_t k = 0; k < 2 * frame_bytes / 2; k++)
-          if (p[k] > maxv) {
-            if (o.ipa == "CNCL") p[k] = (uint16_t)maxv;
-            else err = 2;
-            if (o.ipa == "WARN") err = 0;
-          }
+        const size_t ns = 2 * frame_bytes / 2;
+        uint16_t mx = 0;
+        for (size_t k = 0; k < ns; k++) mx = std::max(mx, p[k]);
+        if (mx > maxv) {
+          if (o.ipa == "CNCL")
+            for (size_t k = 0; k < ns; k++) p[k] = (uint16_t)std::min<uint32_t>(p[k], maxv);
+          else if (o.ipa == "STOP")
+            err = 2;
+        }
       }
       std::lock_guard<std::mutex> lk(ring.mu);
       ring.state[s] = f;
       ring.error[s] = err;
       ring.cv.notify_all();
     }
-  });
+    for (FILE* f : fs)
+      if (f) fclose(f);
+  };
+  std::vector<std::thread> readers;
+  for (int t = 0; t < kReaders; t++) readers.emplace_back(read_frames, t);
   auto fail = [&](int rc) {
     {
       std::lock_guard<std::mutex> lk(ring.mu);
       ring.stop = true;
       ring.cv.notify_all();
     }
-    reader.join();
+    for (std::thread& r : readers) r.join();
     return rc;
   };
 
@@ -431,6 +444,7 @@ int main(int argc, char** argv) {
     return p;
   };
   const int ngroups = (N + kGroup - 1) / kGroup;
+  const auto t_begin = std::chrono::steady_clock::now();
   auto release = [&](int g) {  // the group's slots may be refilled
     std::lock_guard<std::mutex> lk(ring.mu);
     for (int f = g * kGroup; f < std::min(N, (g + 1) * kGroup); f++) ring.state[f % kSlots] = -1;
@@ -484,7 +498,8 @@ int main(int argc, char** argv) {
       release(pg);
     }
   }
-  reader.join();
+  for (std::thread& r : readers) r.join();
+  const double loop_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
 
   // ---- per-frame lines (apps/QMIV/src/xAppQMIV.cpp:950-1110) and statistics
   MetricStat st[kNumMetrics];
@@ -521,6 +536,11 @@ int main(int argc, char** argv) {
   // ---- averages (apps/QMIV/src/xAppQMIV.cpp:1131-1160)
   for (int m = 0; m < kNumMetrics; m++)
     if (o.calc[m]) printf("%s\n", fmt_avg(m, st[m], N, "Average      ").c_str());
+  if (o.verbose >= 3) {
+    // wall time of the frame loop: file reads, host-to-device copies and every metric, overlapped
+    printf("\nAvgTime         TOTAL %9.2f ms   (%d frame pairs, %.1f Mpixel-pairs/s)\n", loop_ms / N, N,
+           (double)W * H * N / (loop_ms * 1e3));
+  }
   if (!o.result_file.empty()) {
     FILE* rf = fopen(o.result_file.c_str(), "w");
     if (!rf) {
