@@ -298,10 +298,160 @@ OR_EXPORT void or_wspsnr(const uint16_t* const p0[3], const uint16_t* const p1[3
 }
 
 /* ---------------------------------------------------------------------------
- * Block-averaged SSIM window: libs/pmbb-ivqm/src/xStructSimSTD.cpp:420-461,
- * with exact int64 moments (the FP64/int64 oracle; the reference's own STD
- * kernel squares in int32, which differs only above 15-bit samples).
+ * Structural similarity.
+ *
+ * Window modes (libs/pmbb-ivqm/src/xSSIM.h:24-31):
+ *   0 RegularGaussianFlt  11x11 centred, FP64 sums of float32 Gaussian weights
+ *                         (xStructSimSTD.cpp:19-88)
+ *   1 RegularGaussianInt  11x11 centred, int64 sums of Q18 weights
+ *                         (xStructSimSTD.cpp:89-154)
+ *   2 RegularAveraged     11x11 centred, plain int64 sums (xStructSimSTD.cpp:155-193)
+ *   3 BlockGaussianInt    wnd x wnd (8 or 16) from the top-left, Q18 weights
+ *                         (xStructSimSTD.cpp:353-419)
+ *   4 BlockAveraged       wnd x wnd from the top-left, plain sums
+ *                         (xStructSimSTD.cpp:420-461)
+ * Integer moments are exact int64 here; the reference's STD kernels square in
+ * int32 for the averaged and block-Gaussian modes, which differs only above
+ * 15-bit samples (its SIMD kernels are limited to 15 bits by 16-bit madd).
+ *
+ * Border extension (libs/pmbb-core/src/xCommonDefCORE.h:322-331): regular
+ * windows with an extension mode other than None run over every sample and
+ * read the picture extended as xMarginOps.h:40-215 extends it (Constant uses
+ * 65535, the value xPicP::extend passes: libs/pmbb-core/src/xPic.cpp:94-101).
  * ------------------------------------------------------------------------- */
+enum { OR_RGLR_FLT = 0, OR_RGLR_INT = 1, OR_RGLR_AVG = 2, OR_BLCK_INT = 3, OR_BLCK_AVG = 4 };
+enum { OR_EXT_NONE = 0, OR_EXT_NEAREST = 1, OR_EXT_REFLECT = 2, OR_EXT_MIRROR = 3, OR_EXT_CONSTANT = 4, OR_EXT_ZERO = 5 };
+
+typedef struct {
+  int32_t mode, ext, wnd, stride;
+} or_ssim_cfg;
+
+/* Window weights.
+ * Float (xStructSimConsts.h:51-66, sigma c_Sigma = 1.5 at :35): the 11x11
+ * Gaussian normalised to sum 1 in FP64, each weight rounded to float32.
+ * Q18 integer tables (c_FltrIntShift = 18, :95-98): round(2^18 * normalised
+ * Gaussian), plus one at the listed (row, column) positions --
+ *   11x11 sigma 1.5 (:100-113), 8x8 sigma 1.5 (:134-144), 16x16 sigma 2.4
+ *   (:146-164); 8x8 and 16x16 are centred between samples. */
+static double or_w_flt11[121], or_w_int11[121], or_w_int8[64], or_w_int16[256];
+static int or_w_ready = 0;
+
+static void or_gauss(int32_t n, double sigma, double* g) {
+  const double c = 0.5 * (double)(n - 1);
+  double s = 0.0;
+  for (int32_t i = 0; i < n; i++)
+    for (int32_t j = 0; j < n; j++) {
+      const double v = exp(-((i - c) * (i - c) + (j - c) * (j - c)) / (2.0 * sigma * sigma));
+      g[i * n + j] = v;
+      s += v;
+    }
+  for (int32_t k = 0; k < n * n; k++) g[k] /= s;
+}
+static void or_q18(int32_t n, double sigma, const int32_t* plus1, int32_t nplus, double* out) {
+  or_gauss(n, sigma, out);
+  for (int32_t k = 0; k < n * n; k++) out[k] = floor(out[k] * 262144.0 + 0.5);
+  for (int32_t k = 0; k < nplus; k++) out[plus1[2 * k] * n + plus1[2 * k + 1]] += 1.0;
+}
+static void or_weights_init(void) {
+  static const int32_t p11[] = {3, 4, 3, 6, 4, 3, 4, 7, 5, 4, 5, 6, 6, 3, 6, 7, 7, 4, 7, 6};
+  static const int32_t p8[] = {2, 2, 2, 5, 5, 2, 5, 5};
+  static const int32_t p16[] = {0, 3, 0, 12, 3, 0, 3, 15, 12, 0, 12, 15, 15, 3, 15, 12};
+  if (or_w_ready) return;
+  or_gauss(11, 1.5, or_w_flt11);
+  for (int32_t k = 0; k < 121; k++) or_w_flt11[k] = (double)(float)or_w_flt11[k];
+  or_q18(11, 1.5, p11, 10, or_w_int11);
+  or_q18(8, 1.5, p8, 4, or_w_int8);
+  or_q18(16, 2.4, p16, 8, or_w_int16);
+  or_w_ready = 1;
+}
+/* which: 0 = float 11x11, 1 = Q18 11x11, 2 = Q18 8x8, 3 = Q18 16x16 */
+OR_EXPORT void or_ssim_weights(int32_t which, double* out) {
+  or_weights_init();
+  const double* src = which == 0 ? or_w_flt11 : which == 1 ? or_w_int11 : which == 2 ? or_w_int8 : or_w_int16;
+  const int32_t n = which == 2 ? 64 : which == 3 ? 256 : 121;
+  memcpy(out, src, (size_t)n * sizeof(double));
+}
+
+/* Sample (x, y) of a w x h plane extended by `ext` (one fold for Reflect and
+ * Mirror: overhangs up to the plane size). */
+static inline int32_t or_ext_sample(const uint16_t* p, int32_t w, int32_t h, int32_t x, int32_t y, int32_t ext) {
+  if (x < 0 || x >= w || y < 0 || y >= h) {
+    switch (ext) {
+      case OR_EXT_NEAREST:  /* ( a a a | a b c d | d d d ) */
+        x = or_clampi(x, 0, w - 1);
+        y = or_clampi(y, 0, h - 1);
+        break;
+      case OR_EXT_REFLECT:  /* ( c b a | a b c d | d c b ) */
+        x = x < 0 ? -x - 1 : (x >= w ? 2 * w - 1 - x : x);
+        y = y < 0 ? -y - 1 : (y >= h ? 2 * h - 1 - y : y);
+        break;
+      case OR_EXT_MIRROR:   /* ( d c b | a b c d | c b a ) */
+        x = x < 0 ? -x : (x >= w ? 2 * w - 2 - x : x);
+        y = y < 0 ? -y : (y >= h ? 2 * h - 2 - y : y);
+        break;
+      case OR_EXT_CONSTANT:
+        return 65535;
+      default:
+        return 0;
+    }
+  }
+  return p[(size_t)y * w + x];
+}
+
+static double or_ssim_ratio(double ar, double at, double vr, double vt, double cv, double c1, double c2,
+                            int32_t calc_l) {
+  const double cs = (2 * cv + c2) / (vr + vt + c2);
+  if (calc_l) {
+    const double l = (2 * ar * at + c1) / (ar * ar + at * at + c1);
+    return l * cs;
+  }
+  return cs;
+}
+
+/* One window at (x, y): the top-left sample for block modes, the centre for
+ * regular modes. */
+static double or_ssim_win(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t x, int32_t y,
+                          const or_ssim_cfg* p, double c1, double c2, int32_t calc_l) {
+  const int32_t regular = p->mode <= OR_RGLR_AVG;
+  const int32_t n = regular ? 11 : p->wnd;
+  const int32_t x0 = regular ? x - 5 : x, y0 = regular ? y - 5 : y;
+  if (p->mode == OR_RGLR_FLT) {
+    double sr = 0, st = 0, srr = 0, stt = 0, srt = 0;
+    for (int32_t dy = 0; dy < n; dy++)
+      for (int32_t dx = 0; dx < n; dx++) {
+        const double r = or_ext_sample(ref, w, h, x0 + dx, y0 + dy, p->ext);
+        const double t = or_ext_sample(tst, w, h, x0 + dx, y0 + dy, p->ext);
+        const double c = or_w_flt11[dy * 11 + dx];
+        sr += r * c;
+        st += t * c;
+        srr += (r * r) * c;
+        stt += (t * t) * c;
+        srt += (r * t) * c;
+      }
+    return or_ssim_ratio(sr, st, srr - sr * sr, stt - st * st, srt - sr * st, c1, c2, calc_l);
+  }
+  const double* wt = p->mode == OR_RGLR_INT ? or_w_int11
+                     : p->mode == OR_BLCK_INT ? (n == 8 ? or_w_int8 : or_w_int16) : NULL;
+  int64_t sr = 0, st = 0, srr = 0, stt = 0, srt = 0;
+  for (int32_t dy = 0; dy < n; dy++)
+    for (int32_t dx = 0; dx < n; dx++) {
+      const int64_t r = or_ext_sample(ref, w, h, x0 + dx, y0 + dy, p->ext);
+      const int64_t t = or_ext_sample(tst, w, h, x0 + dx, y0 + dy, p->ext);
+      const int64_t c = wt ? (int64_t)wt[dy * n + dx] : 1;
+      sr += r * c;
+      st += t * c;
+      srr += r * r * c;
+      stt += t * t * c;
+      srt += r * t * c;
+    }
+  const double k = wt ? (double)1.0 / (double)262144 : (p->mode == OR_RGLR_AVG ? (double)1.0 / (double)121
+                                                                                : (double)1.0 / (double)(n * n));
+  const double ar = (double)sr * k, at = (double)st * k;
+  return or_ssim_ratio(ar, at, (double)srr * k - ar * ar, (double)stt * k - at * at, (double)srt * k - ar * at, c1,
+                       c2, calc_l);
+}
+
+/* Block-averaged window with explicit strides (kept for the tests' brute-force checks). */
 OR_EXPORT double or_ssim_window(const uint16_t* tst, const uint16_t* ref, int32_t stride_t, int32_t stride_r,
                                 int32_t wnd, double c1, double c2, int32_t calc_l) {
   int64_t sr = 0, st = 0, srr = 0, stt = 0, srt = 0;
@@ -314,17 +464,9 @@ OR_EXPORT double or_ssim_window(const uint16_t* tst, const uint16_t* ref, int32_
     tst += stride_t;
   }
   const double inv = (double)1.0 / (double)(wnd * wnd);
-  double avg_r = (double)sr * inv;
-  double avg_t = (double)st * inv;
-  double var_r = (double)srr * inv - avg_r * avg_r;
-  double var_t = (double)stt * inv - avg_t * avg_t;
-  double cov = (double)srt * inv - avg_r * avg_t;
-  double cs = (2 * cov + c2) / (var_r + var_t + c2);
-  if (calc_l) {
-    double l = (2 * avg_r * avg_t + c1) / (avg_r * avg_r + avg_t * avg_t + c1);
-    return l * cs;
-  }
-  return cs;
+  const double ar = (double)sr * inv, at = (double)st * inv;
+  return or_ssim_ratio(ar, at, (double)srr * inv - ar * ar, (double)stt * inv - at * at, (double)srt * inv - ar * at,
+                       c1, c2, calc_l);
 }
 
 /* Number of windows along one axis: libs/pmbb-ivqm/src/xSSIM.cpp:381-404. */
@@ -342,39 +484,70 @@ OR_EXPORT int32_t or_ms_num_scales(int32_t w, int32_t h) {
   return 5;
 }
 
+/* Window grid: xSSIM::xInitLoopRanges, libs/pmbb-ivqm/src/xSSIM.cpp:161-196.
+ * Regular: centres from FilterRange (5) to size - 5, or over the whole plane
+ * with a border extension; block: top-left corners 0 .. size - wnd. */
+OR_EXPORT void or_ssim_grid(int32_t w, int32_t h, int32_t mode, int32_t ext, int32_t wnd, int32_t stride,
+                            int32_t out[4]) {
+  if (mode <= OR_RGLR_AVG) {
+    const int32_t b = ext != OR_EXT_NONE ? 0 : 5;
+    out[0] = b;
+    out[1] = (w - 2 * b + stride - 1) / stride;
+    out[2] = b;
+    out[3] = (h - 2 * b + stride - 1) / stride;
+  } else {
+    out[0] = 0;
+    out[1] = or_ssim_num_blocks(w, wnd, stride);
+    out[2] = 0;
+    out[3] = or_ssim_num_blocks(h, wnd, stride);
+  }
+}
+
 /* Window-row sums of one plane: row loop of xSSIM::xCalcRowSSIM,
  * libs/pmbb-ivqm/src/xSSIM.cpp:238-254 (KBNS over x), task loop :219-231.
  * rowsums receives one value per window row (NumUnitY entries). */
-OR_EXPORT void or_ssim_rowsums(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t wnd,
-                               int32_t stride, double c1, double c2, int32_t calc_l, double* rowsums) {
-  const int32_t ny = h < wnd ? 0 : (h - wnd) / stride + 1;
+OR_EXPORT void or_ssim_rowsums_ex(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t mode,
+                                  int32_t ext, int32_t wnd, int32_t stride, double c1, double c2, int32_t calc_l,
+                                  double* rowsums) {
+  const or_ssim_cfg p = {mode, ext, wnd, stride};
+  int32_t g[4];
+  or_weights_init();
+  or_ssim_grid(w, h, mode, ext, wnd, stride, g);
   /* window rows are independent: threads over rows change no result */
 #pragma omp parallel for schedule(dynamic, 8)
-  for (int32_t j = 0; j < ny; j++) {
-    const int32_t y = j * stride;
+  for (int32_t j = 0; j < g[3]; j++) {
+    const int32_t y = g[2] + j * stride;
     or_kbns k = {0, 0};
-    for (int32_t x = 0; x < w - wnd + 1; x += stride)
-      or_kbns_acc(&k, or_ssim_window(tst + (size_t)y * w + x, ref + (size_t)y * w + x, w, w, wnd, c1, c2, calc_l));
+    for (int32_t i = 0; i < g[1]; i++) or_kbns_acc(&k, or_ssim_win(tst, ref, w, h, g[0] + i * stride, y, &p, c1, c2, calc_l));
     rowsums[j] = or_kbns_result(&k);
   }
+}
+OR_EXPORT void or_ssim_rowsums(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t wnd,
+                               int32_t stride, double c1, double c2, int32_t calc_l, double* rowsums) {
+  or_ssim_rowsums_ex(tst, ref, w, h, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, c1, c2, calc_l, rowsums);
 }
 
 /* SSIM of one plane: xSSIM::xCalcCmpSSIM, libs/pmbb-ivqm/src/xSSIM.cpp:213-237
  * (KBNS over rows, / NumUnitY*NumUnitX). With erp_w, the IV-SSIM weighting of
- * xIVSSIM::xCalcCmpSSIM, libs/pmbb-ivqm/src/xIVSSIM.cpp:148-165: row j is
- * scaled by w[y_j + wnd/2], the picture sum is divided by
+ * xIVSSIM::xCalcCmpSSIM, libs/pmbb-ivqm/src/xIVSSIM.cpp:148-165: the row of
+ * windows at y is scaled by w[y] (regular, y = centre) or w[y + wnd/2]
+ * (block, y = top), the picture sum is divided by
  * NumActive * (KBNS(weights) / NumUnitY). */
-OR_EXPORT double or_ssim_plane(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t wnd,
-                               int32_t stride, double c1, double c2, int32_t calc_l, const double* erp_w) {
-  const int32_t nx = or_ssim_num_blocks(w, wnd, stride), ny = or_ssim_num_blocks(h, wnd, stride);
+OR_EXPORT double or_ssim_plane_ex(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t mode,
+                                  int32_t ext, int32_t wnd, int32_t stride, double c1, double c2, int32_t calc_l,
+                                  const double* erp_w) {
+  int32_t g[4];
+  or_ssim_grid(w, h, mode, ext, wnd, stride, g);
+  const int32_t nx = g[1], ny = g[3];
   if (nx <= 0 || ny <= 0) return NAN;
   double* rs = (double*)malloc((size_t)ny * sizeof(double));
-  or_ssim_rowsums(tst, ref, w, h, wnd, stride, c1, c2, calc_l, rs);
+  or_ssim_rowsums_ex(tst, ref, w, h, mode, ext, wnd, stride, c1, c2, calc_l, rs);
   double res;
   if (erp_w) {
     or_kbns wk = {0, 0};
     for (int32_t j = 0; j < ny; j++) {
-      const double wt = erp_w[j * stride + (wnd >> 1)];
+      const int32_t y = g[2] + j * stride;
+      const double wt = erp_w[mode <= OR_RGLR_AVG ? y : y + (wnd >> 1)];
       rs[j] = rs[j] * wt;
       or_kbns_acc(&wk, wt);
     }
@@ -386,6 +559,10 @@ OR_EXPORT double or_ssim_plane(const uint16_t* tst, const uint16_t* ref, int32_t
   free(rs);
   return res;
 }
+OR_EXPORT double or_ssim_plane(const uint16_t* tst, const uint16_t* ref, int32_t w, int32_t h, int32_t wnd,
+                               int32_t stride, double c1, double c2, int32_t calc_l, const double* erp_w) {
+  return or_ssim_plane_ex(tst, ref, w, h, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, c1, c2, calc_l, erp_w);
+}
 
 static void or_ssim_consts(int32_t bit_depth, double* c1, double* c2) {
   const double maxv = (double)((1 << bit_depth) - 1);
@@ -394,11 +571,15 @@ static void or_ssim_consts(int32_t bit_depth, double* c1, double* c2) {
 }
 
 /* Per-component SSIM of a pair (xSSIM::calcPicSSIM, xSSIM.h:83). */
-OR_EXPORT void or_ssim(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
-                       int32_t bit_depth, int32_t wnd, int32_t stride, double out[3]) {
+OR_EXPORT void or_ssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
+                          int32_t bit_depth, int32_t mode, int32_t ext, int32_t wnd, int32_t stride, double out[3]) {
   double c1, c2;
   or_ssim_consts(bit_depth, &c1, &c2);
-  for (int c = 0; c < 3; c++) out[c] = or_ssim_plane(p0[c], p1[c], w, h, wnd, stride, c1, c2, 1, NULL);
+  for (int c = 0; c < 3; c++) out[c] = or_ssim_plane_ex(p0[c], p1[c], w, h, mode, ext, wnd, stride, c1, c2, 1, NULL);
+}
+OR_EXPORT void or_ssim(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
+                       int32_t bit_depth, int32_t wnd, int32_t stride, double out[3]) {
+  or_ssim_ex(p0, p1, w, h, bit_depth, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, out);
 }
 
 /* Multi-scale exponents: libs/pmbb-ivqm/src/xStructSimConsts.h:42-49. */
@@ -408,10 +589,12 @@ static const double or_ms_exp[5][5] = {
     {0.0448, 0.2856, 0.3001, 0.2363, 0.1333}};
 
 /* MS-SSIM per component: libs/pmbb-ivqm/src/xSSIM.cpp:115-141. Scale 0 on the
- * full picture, scale i on the 2x2 rounded-mean pyramid level i; luminance only
- * at the last scale; product of ReLU(score_i)^exp_i. No ERP weighting. */
-OR_EXPORT void or_msssim(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
-                         int32_t bit_depth, int32_t wnd, int32_t stride, double out[3]) {
+ * full picture, scale i on the 2x2 rounded-mean pyramid level i (extended by
+ * the same border mode, :128); luminance only at the last scale; product of
+ * ReLU(score_i)^exp_i. No ERP weighting. */
+OR_EXPORT void or_msssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
+                            int32_t bit_depth, int32_t mode, int32_t ext, int32_t wnd, int32_t stride,
+                            double out[3]) {
   double c1, c2;
   or_ssim_consts(bit_depth, &c1, &c2);
   const int32_t ns = or_ms_num_scales(w, h);
@@ -431,7 +614,7 @@ OR_EXPORT void or_msssim(const uint16_t* const p0[3], const uint16_t* const p1[3
         la = na; lb = nb; a = na; b = nb;
         cw >>= 1; ch >>= 1;
       }
-      sc[i] = or_ssim_plane(a, b, cw, ch, wnd, stride, c1, c2, i == ns - 1, NULL);
+      sc[i] = or_ssim_plane_ex(a, b, cw, ch, mode, ext, wnd, stride, c1, c2, i == ns - 1, NULL);
     }
     free(la); free(lb);
     double v = 1.0;
@@ -439,37 +622,54 @@ OR_EXPORT void or_msssim(const uint16_t* const p0[3], const uint16_t* const p1[3
     out[c] = v;
   }
 }
+OR_EXPORT void or_msssim(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h,
+                         int32_t bit_depth, int32_t wnd, int32_t stride, double out[3]) {
+  or_msssim_ex(p0, p1, w, h, bit_depth, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, out);
+}
 
 /* IV-SSIM: libs/pmbb-ivqm/src/xIVSSIM.cpp:31-61 as called from
  * xAppQMIV.cpp:1074-1078: T2R = SSIM(P0, SCP1), R2T = SSIM(P1, SCP0), each the
  * component-weighted mean, result = min. ERP weights apply when erp_w != NULL. */
-OR_EXPORT double or_ivssim(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* const scp0[3],
-                           const uint16_t* const scp1[3], int32_t w, int32_t h, int32_t bit_depth, int32_t wnd,
-                           int32_t stride, const int32_t cw_avg[4], const double* erp_w, double* r2t_out,
-                           double* t2r_out) {
+OR_EXPORT double or_ivssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3],
+                              const uint16_t* const scp0[3], const uint16_t* const scp1[3], int32_t w, int32_t h,
+                              int32_t bit_depth, int32_t mode, int32_t ext, int32_t wnd, int32_t stride,
+                              const int32_t cw_avg[4], const double* erp_w, double* r2t_out, double* t2r_out) {
   double c1, c2;
   or_ssim_consts(bit_depth, &c1, &c2);
   double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
   for (int c = 0; c < 3; c++) {
-    a[c] = or_ssim_plane(p0[c], scp1[c], w, h, wnd, stride, c1, c2, 1, erp_w);
-    b[c] = or_ssim_plane(p1[c], scp0[c], w, h, wnd, stride, c1, c2, 1, erp_w);
+    a[c] = or_ssim_plane_ex(p0[c], scp1[c], w, h, mode, ext, wnd, stride, c1, c2, 1, erp_w);
+    b[c] = or_ssim_plane_ex(p1[c], scp0[c], w, h, mode, ext, wnd, stride, c1, c2, 1, erp_w);
   }
   const double t2r = or_weighted_mean4(a, cw_avg), r2t = or_weighted_mean4(b, cw_avg);
   if (r2t_out) *r2t_out = r2t;
   if (t2r_out) *t2r_out = t2r;
   return r2t < t2r ? r2t : t2r;
 }
+OR_EXPORT double or_ivssim(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* const scp0[3],
+                           const uint16_t* const scp1[3], int32_t w, int32_t h, int32_t bit_depth, int32_t wnd,
+                           int32_t stride, const int32_t cw_avg[4], const double* erp_w, double* r2t_out,
+                           double* t2r_out) {
+  return or_ivssim_ex(p0, p1, scp0, scp1, w, h, bit_depth, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, cw_avg, erp_w,
+                      r2t_out, t2r_out);
+}
 
 /* IV-MS-SSIM: libs/pmbb-ivqm/src/xIVSSIM.cpp:62-92 (MS-SSIM of (P0, SCP1) and
  * (P1, SCP0), weighted means, min; no ERP weighting). */
+OR_EXPORT double or_ivmsssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3],
+                                const uint16_t* const scp0[3], const uint16_t* const scp1[3], int32_t w, int32_t h,
+                                int32_t bit_depth, int32_t mode, int32_t ext, int32_t wnd, int32_t stride,
+                                const int32_t cw_avg[4]) {
+  double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+  or_msssim_ex(p0, scp1, w, h, bit_depth, mode, ext, wnd, stride, a);
+  or_msssim_ex(p1, scp0, w, h, bit_depth, mode, ext, wnd, stride, b);
+  const double t2r = or_weighted_mean4(a, cw_avg), r2t = or_weighted_mean4(b, cw_avg);
+  return r2t < t2r ? r2t : t2r;
+}
 OR_EXPORT double or_ivmsssim(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* const scp0[3],
                              const uint16_t* const scp1[3], int32_t w, int32_t h, int32_t bit_depth, int32_t wnd,
                              int32_t stride, const int32_t cw_avg[4]) {
-  double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
-  or_msssim(p0, scp1, w, h, bit_depth, wnd, stride, a);
-  or_msssim(p1, scp0, w, h, bit_depth, wnd, stride, b);
-  const double t2r = or_weighted_mean4(a, cw_avg), r2t = or_weighted_mean4(b, cw_avg);
-  return r2t < t2r ? r2t : t2r;
+  return or_ivmsssim_ex(p0, p1, scp0, scp1, w, h, bit_depth, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, cw_avg);
 }
 
 /* Component-weighted mean exported for the picture value of per-component

@@ -88,6 +88,9 @@ class Oracle:
         L.or_pic_value.restype = C.c_double
         L.or_ssim_num_blocks.restype = C.c_int32
         L.or_ms_num_scales.restype = C.c_int32
+        L.or_ssim_plane_ex.restype = C.c_double
+        L.or_ivssim_ex.restype = C.c_double
+        L.or_ivmsssim_ex.restype = C.c_double
 
     # -- pixel ops ---------------------------------------------------------
     def upsample(self, src):
@@ -257,6 +260,55 @@ class Oracle:
     def pic_value(self, v3, cw=(4, 1, 1, 0)):
         return self.L.or_pic_value((C.c_double * 3)(*v3), _i32x4(cw))
 
+    # -- structural-similarity modes (mode: xSSIM::eMode 0..4, ext: eMrgExt 0..5) --
+    def ssim_weights(self, which):
+        n = {0: 121, 1: 121, 2: 64, 3: 256}[which]
+        out = np.zeros(n, np.float64)
+        self.L.or_ssim_weights(which, out.ctypes.data_as(_f64p))
+        k = int(round(n ** 0.5))
+        return out.reshape(k, k)
+
+    def ssim_grid(self, w, h, mode, ext=0, wnd=8, stride=4):
+        out = (C.c_int32 * 4)()
+        self.L.or_ssim_grid(w, h, mode, ext, wnd, stride, out)
+        return tuple(out[i] for i in range(4))  # first x, count x, first y, count y
+
+    def ssim_ex(self, p0, p1, bd, mode, ext=0, wnd=8, stride=4, ms=False):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        h, w = a[0].shape
+        out = (C.c_double * 3)()
+        fn = self.L.or_msssim_ex if ms else self.L.or_ssim_ex
+        fn(pa, pb, w, h, bd, mode, ext, wnd, stride, out)
+        return [out[i] for i in range(3)]
+
+    def ssim_plane_ex(self, t, r, bd, mode, ext=0, wnd=8, stride=4, calc_l=True, erp_w=None):
+        t = np.ascontiguousarray(t, np.uint16)
+        r = np.ascontiguousarray(r, np.uint16)
+        h, w = t.shape
+        maxv = float((1 << bd) - 1)
+        c1 = (0.01 * maxv) * (0.01 * maxv)
+        c2 = (0.03 * maxv) * (0.03 * maxv)
+        ew = np.ascontiguousarray(erp_w, np.float64) if erp_w is not None else None
+        return self.L.or_ssim_plane_ex(t.ctypes.data_as(_u16p), r.ctypes.data_as(_u16p), w, h, mode, ext, wnd,
+                                       stride, C.c_double(c1), C.c_double(c2), int(calc_l),
+                                       ew.ctypes.data_as(_f64p) if ew is not None else None)
+
+    def ivssim_ex(self, p0, p1, scp0, scp1, bd, mode, ext=0, wnd=8, stride=4, cwa=(4, 1, 1, 0), erp=False, lat=180,
+                  ms=False):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        s0, ps0 = _planes(scp0)
+        s1, ps1 = _planes(scp1)
+        h, w = a[0].shape
+        if ms:
+            return self.L.or_ivmsssim_ex(pa, pb, ps0, ps1, w, h, bd, mode, ext, wnd, stride, _i32x4(cwa)), None, None
+        ew = self.ws_weights(h, lat)[0] if erp else None
+        r2t, t2r = C.c_double(0), C.c_double(0)
+        v = self.L.or_ivssim_ex(pa, pb, ps0, ps1, w, h, bd, mode, ext, wnd, stride, _i32x4(cwa),
+                                ew.ctypes.data_as(_f64p) if ew is not None else None, C.byref(r2t), C.byref(t2r))
+        return v, r2t.value, t2r.value
+
     def num_blocks(self, n, wnd=8, stride=4):
         return self.L.or_ssim_num_blocks(n, wnd, stride)
 
@@ -382,6 +434,27 @@ class Reference:
         v = C.c_double(0)
         self.L.ref_ivssim(pa, pb, ps0, ps1, w, h, bd, wnd, stride, _i32x4(cwa), int(erp), lat, int(ms), C.byref(v))
         return v.value
+
+    def ssim_ex(self, p0, p1, bd, mode, ext=0, wnd=8, stride=4, margin=8, ms=False):
+        """Any mode / border extension on pictures with a margin of `margin` samples."""
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        h, w = a[0].shape
+        out = (C.c_double * 4)()
+        self.L.ref_ssim_ex(pa, pb, w, h, bd, mode, ext, wnd, stride, margin, int(ms), out)
+        return [out[i] for i in range(3)]
+
+    def ivssim_ex(self, p0, p1, scp0, scp1, bd, mode, ext=0, wnd=8, stride=4, margin=8, cwa=(4, 1, 1, 0),
+                  erp=False, lat=180, ms=False):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        s0, ps0 = _planes(scp0)
+        s1, ps1 = _planes(scp1)
+        h, w = a[0].shape
+        v, r2t, t2r = C.c_double(0), C.c_double(0), C.c_double(0)
+        self.L.ref_ivssim_ex(pa, pb, ps0, ps1, w, h, bd, mode, ext, wnd, stride, margin, _i32x4(cwa), int(erp), lat,
+                             int(ms), C.byref(v), C.byref(r2t), C.byref(t2r))
+        return v.value, r2t.value, t2r.value
 
     def bench_pairs(self, pairs, w, h, bd, chroma420, erp, metrics, threads):
         """Times the app-order frame loop over in-memory pairs; returns (seconds, ivpsnr[], ivssim[])."""

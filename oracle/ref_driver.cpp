@@ -239,6 +239,64 @@ int ref_ivssim(const uint16_t* const p0[3], const uint16_t* const p1[3], const u
   return 0;
 }
 
+// Any structural-similarity mode (xSSIM.h:24-31) and border extension
+// (xCommonDefCORE.h:322-331), on pictures with a margin of `margin` samples.
+// With ext != None the pictures are extended by that mode before the call, as
+// xAppQMIV::addStructSimMargs does (xAppQMIV.cpp:918-925); the library extends
+// its MS-SSIM sub-pictures itself (xSSIM.cpp:128).
+namespace {
+void load_ext(xPicP& pic, const uint16_t* const planes[3], int32 w, int32 h, int32 bd, int32 margin, int32 ext) {
+  load_pic(pic, planes, w, h, bd, margin);
+  if (ext != 0) pic.extend((eMrgExt)ext);
+}
+}  // namespace
+
+int ref_ssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3], int32_t w, int32_t h, int32_t bd,
+                int32_t mode, int32_t ext, int32_t wnd, int32_t stride, int32_t margin, int32_t ms, double out[4]) {
+  xPicP a, b;
+  load_ext(a, p0, w, h, bd, margin, ext);
+  load_ext(b, p1, w, h, bd, margin, ext);
+  Inline in;
+  const int32_t cwa[4] = {4, 1, 1, 0};
+  xIVSSIM p;
+  p.create(int32V2(w, h), bd, margin, ms != 0);
+  p.setCmpWeightsAverage(v4(cwa));
+  p.setStructSimParams((xSSIM::eMode)mode, (eMrgExt)ext, wnd, stride);
+  p.bindThrdPoolIntf(&in.tpi);
+  p.initRowBuffers(h);
+  flt64V4 r = ms ? p.calcPicMSSSIM(&a, &b) : p.calcPicSSIM(&a, &b);
+  for (int c = 0; c < 4; c++) out[c] = r[c];
+  p.destroy();
+  return 0;
+}
+
+int ref_ivssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* const scp0[3],
+                  const uint16_t* const scp1[3], int32_t w, int32_t h, int32_t bd, int32_t mode, int32_t ext,
+                  int32_t wnd, int32_t stride, int32_t margin, const int32_t cwa[4], int32_t erp, int32_t lat,
+                  int32_t ms, double* out, double* r2t, double* t2r) {
+  xPicP a, b, s0, s1;
+  load_ext(a, p0, w, h, bd, margin, ext);
+  load_ext(b, p1, w, h, bd, margin, ext);
+  load_ext(s0, scp0, w, h, bd, margin, ext);
+  load_ext(s1, scp1, w, h, bd, margin, ext);
+  Inline in;
+  xIVSSIM p;
+  p.create(int32V2(w, h), bd, margin, ms != 0);
+  p.setCmpWeightsAverage(v4(cwa));
+  p.setStructSimParams((xSSIM::eMode)mode, (eMrgExt)ext, wnd, stride);
+  p.bindThrdPoolIntf(&in.tpi);
+  p.initRowBuffers(h);
+  if (erp) p.initWS(true, w, h, bd, 360, lat);
+  double lr = 0, lt = 0;
+  p.setDebugCallbackQAP([&](flt64 x, flt64 y) { lr = x; lt = y; });
+  *out = ms ? p.calcPicIVMSSSIM(&a, &b, &s0, &s1) : p.calcPicIVSSIM(&a, &b, &s0, &s1);
+  if (r2t) *r2t = lr;
+  if (t2r) *t2r = lt;
+  p.destroy();
+  return 0;
+}
+
+
 // ---------------------------------------------------------------------------
 // CPU baseline: replays the per-frame stage order of xAppQMIV::processAllFrames
 // (xAppQMIV.cpp:761-827) on frames already in memory, with the reference

@@ -102,6 +102,52 @@ def make_case(name, wl, frame, R, erp, ref: Reference, orc: Oracle):
     return case
 
 
+# Structural-similarity modes (xSSIM::eMode: 0 RegularGaussianFlt, 1 RegularGaussianInt,
+# 2 RegularAveraged, 3 BlockGaussianInt, 4 BlockAveraged) and border extensions (eMrgExt:
+# 0 None, 1 Nearest, 2 Reflect, 3 Mirror, 4 Constant, 5 Zero), from the x86-64-v3 build:
+# on x86-64-v4 plain SSIM in BlockGaussianInt 8/4 and 8/8 runs the multi-block averaging
+# kernels (xSSIM.cpp:217-226 selects them by window and stride only). Pictures carry a
+# margin of 8 so that the 11x11 windows of the extension modes stay inside it.
+MODE_WORKLOADS = {
+    "w112h80_10b_444": synth.Workload(cfg=130, width=112, height=80, chroma_420=False),
+    "w128h96_8b_420": synth.Workload(cfg=131, width=128, height=96, bit_depth=8, global_shift=1),
+}
+MODE_CASES = (
+    [(0, e, 11, s) for e in (0, 1, 2, 3, 4, 5) for s in (1, 3)]
+    + [(1, e, 11, s) for e in (0, 1, 2, 3, 4, 5) for s in (1, 4)]
+    # RegularAveraged with Constant extension is left out: the reference squares the
+    # Constant value 65535 in int32 (xStructSimSTD.cpp:172-174), which wraps
+    + [(2, e, 11, s) for e in (0, 1, 2, 3, 5) for s in (1, 11)]
+    + [(3, 0, 8, s) for s in (1, 4, 8)] + [(3, 0, 16, s) for s in (3, 16)]
+    + [(4, 0, 8, s) for s in (1, 2, 8)] + [(4, 0, 16, s) for s in (4, 8)] + [(4, 0, 32, 8)]
+)
+
+
+def make_mode_cases(orc: Oracle):
+    ref = Reference("v3")
+    out = []
+    for wname, wl in MODE_WORKLOADS.items():
+        p0n, p1n = synth.make_pair(wl, 0)
+        p0 = orc.to444(p0n, wl.chroma_420)
+        p1 = orc.to444(p1n, wl.chroma_420)
+        bd = wl.bit_depth
+        g = ref.gcd(p0, p1, bd)
+        s0, s1 = ref.scp(p0, p1, bd, g)
+        for mode, ext, wnd, stride in MODE_CASES:
+            if wname.startswith("w128") and stride == 1 and mode != 2:
+                continue  # keep the fixture small: stride 1 on the first picture only (plus averaged)
+            print("mode case", wname, mode, ext, wnd, stride, flush=True)
+            case = {"workload": wname, "mode": mode, "ext": ext, "wnd": wnd, "stride": stride,
+                    "ssim": [f(x) for x in ref.ssim_ex(p0, p1, bd, mode, ext, wnd, stride)]}
+            iv = ref.ivssim_ex(p0, p1, s0, s1, bd, mode, ext, wnd, stride, erp=True)
+            case["ivssim_erp"] = [f(x) for x in iv]
+            if stride > 1:
+                case["msssim"] = [f(x) for x in ref.ssim_ex(p0, p1, bd, mode, ext, wnd, stride, ms=True)]
+                case["ivmsssim"] = f(ref.ivssim_ex(p0, p1, s0, s1, bd, mode, ext, wnd, stride, ms=True)[0])
+            out.append(case)
+    return out
+
+
 def main():
     ref = Reference()
     orc = Oracle()
@@ -113,6 +159,9 @@ def main():
         "generator": "qmiv_amd/csrc/synth.cpp",
         "reference_build": f"oracle/Makefile (x86-64-{ref.level}) over /root/reference/libs",
         "cases": cases,
+        "ssim_mode_workloads": {k: dataclasses.asdict(v) for k, v in MODE_WORKLOADS.items()},
+        "ssim_mode_reference_build": "oracle/Makefile (x86-64-v3), pictures with a margin of 8",
+        "ssim_mode_cases": make_mode_cases(orc),
     }
     with open(OUT, "w") as fh:
         json.dump(doc, fh, indent=1)

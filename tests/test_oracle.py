@@ -330,3 +330,134 @@ def test_search_tie_break_is_first_in_raster_order(oracle):
     tst2 = [np.full((h, w), 510, np.uint16) for _ in range(3)]
     _, _, best2 = oracle.cps(tst2, ref2, 10, [0, 0, 0, 0], R=2, want_best=True)
     assert best2[6, 6] == 1 * 5 + 2
+
+
+# ---------------------------------------------------------------------------
+# structural-similarity modes and border extensions
+# ---------------------------------------------------------------------------
+def _mode_inputs(golden, wname, oracle):
+    w = dict(golden["ssim_mode_workloads"][wname])
+    w["offsets"] = tuple(w["offsets"])
+    wl = synth.Workload(**w)
+    p0n, p1n = synth.make_pair(wl, 0)
+    p0, p1 = oracle.to444(p0n, wl.chroma_420), oracle.to444(p1n, wl.chroma_420)
+    g = oracle.gcd(p0, p1, wl.bit_depth)
+    s0, s1 = oracle.scp(p0, p1, wl.bit_depth, g)
+    return wl, p0, p1, s0, s1
+
+
+def _mode_ids(golden):
+    return [f'{c["workload"]}-m{c["mode"]}-e{c["ext"]}-w{c["wnd"]}-s{c["stride"]}' for c in golden]
+
+
+def test_ssim_modes_match_reference_golden(golden, oracle):
+    """Every mode (RegularGaussianFlt/Int, RegularAveraged, BlockGaussianInt, BlockAveraged),
+    border extension, window and stride against the reference library (x86-64-v3 build):
+    SSIM, IV-SSIM with ERP weights (both directions), MS-SSIM and IV-MS-SSIM. Integer
+    moments make the window values exact, so FP64 agreement is to summation order only."""
+    cache = {}
+    worst = 0.0
+    for c in golden["ssim_mode_cases"]:
+        if c["workload"] not in cache:
+            cache[c["workload"]] = _mode_inputs(golden, c["workload"], oracle)
+        wl, p0, p1, s0, s1 = cache[c["workload"]]
+        bd, args = wl.bit_depth, (c["mode"], c["ext"], c["wnd"], c["stride"])
+        tol = 2e-12
+        got = oracle.ssim_ex(p0, p1, bd, *args)
+        for g, e in zip(got, c["ssim"]):
+            assert abs(g - float(e)) <= tol, (c, got)
+            worst = max(worst, abs(g - float(e)))
+        got = oracle.ivssim_ex(p0, p1, s0, s1, bd, *args, erp=True)
+        for g, e in zip(got, c["ivssim_erp"]):
+            assert abs(g - float(e)) <= tol, (c, got)
+        if "msssim" in c:
+            got = oracle.ssim_ex(p0, p1, bd, *args, ms=True)
+            for g, e in zip(got, c["msssim"]):
+                assert abs(g - float(e)) <= tol, (c, got)
+            got = oracle.ivssim_ex(p0, p1, s0, s1, bd, *args, ms=True)[0]
+            assert abs(got - float(c["ivmsssim"])) <= tol, (c, got)
+    print(f"{len(golden['ssim_mode_cases'])} mode cases, largest SSIM difference {worst:.1e}")
+
+
+# numpy.pad modes equal to each border extension (xCommonDefCORE.h:322-331 names them)
+_PAD = {1: dict(mode="edge"), 2: dict(mode="symmetric"), 3: dict(mode="reflect"),
+        4: dict(mode="constant", constant_values=65535), 5: dict(mode="constant", constant_values=0)}
+
+
+@pytest.mark.parametrize("ext", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_border_extension_equals_numpy_pad(oracle, mode, ext):
+    """A regular window grid over the extended plane is the no-extension grid over the plane
+    padded by 5 samples with numpy.pad: every window sees the same samples in the same
+    order, so the results are identical."""
+    rng = np.random.default_rng(40 + ext)
+    t = rng.integers(0, 1024, (37, 45)).astype(np.uint16)
+    r = np.clip(t.astype(np.int32) + rng.integers(-40, 41, t.shape), 0, 1023).astype(np.uint16)
+    for stride in (1, 4):
+        a = oracle.ssim_plane_ex(t, r, 10, mode, ext, 11, stride)
+        b = oracle.ssim_plane_ex(np.pad(t, 5, **_PAD[ext]), np.pad(r, 5, **_PAD[ext]), 10, mode, 0, 11, stride)
+        assert a == b
+
+
+@pytest.mark.parametrize("n", [10, 11, 12, 37, 64, 1080, 2048])
+@pytest.mark.parametrize("stride", [1, 3, 4, 11])
+def test_regular_grid_brute_force(oracle, n, stride):
+    """Regular window centres: [5, n - 5) or, with an extension, [0, n) (xSSIM.cpp:161-181)."""
+    got = oracle.ssim_grid(n, n, 1, 0, 11, stride)
+    assert got[0] == 5 and got[1] == len(range(5, n - 5, stride))
+    got = oracle.ssim_grid(n, n, 1, 1, 11, stride)
+    assert got[0] == 0 and got[1] == len(range(0, n, stride))
+
+
+def test_ssim_weights_closed_form(oracle):
+    """Window weights: the float table is the normalised Gaussian (sums to 1 up to float32
+    rounding, peak at the centre); the Q18 tables sum to 2^18 within a few units and are
+    within 1.5 units of 2^18 times the same Gaussian (rounding plus the unit added at a few positions)."""
+    f = oracle.ssim_weights(0)
+    assert abs(f.sum() - 1.0) < 1e-6 and f.argmax() == 60 and np.array_equal(f, f.T)
+    for which, n, sigma in ((1, 11, 1.5), (2, 8, 1.5), (3, 16, 2.4)):
+        q = oracle.ssim_weights(which)
+        x = np.arange(n) - (n - 1) / 2
+        g = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2 * sigma * sigma))
+        g = g / g.sum() * 262144
+        assert np.all(np.abs(q - g) < 1.5) and abs(q.sum() - 262144) <= 4
+        assert np.all(q == np.round(q))
+
+
+def test_ssim_mode_planted_bugs(golden, oracle):
+    """A transposed Q18 table, the block weights applied at the window centre, or the ERP
+    weight taken at the window top instead of its centre all move the result off the
+    reference's."""
+    c = next(x for x in golden["ssim_mode_cases"] if x["mode"] == 1 and x["ext"] == 0 and x["stride"] == 4)
+    wl, p0, p1, s0, s1 = _mode_inputs(golden, c["workload"], oracle)
+    good = oracle.ssim_ex(p0, p1, wl.bit_depth, 1, 0, 11, 4)
+    assert abs(good[0] - float(c["ssim"][0])) < 1e-12
+    # transposed pictures stand in for a transposed table (the picture grid is symmetric in x/y)
+    bad = oracle.ssim_ex([p.T.copy() for p in p0], [p.T.copy() for p in p1], wl.bit_depth, 1, 0, 11, 4)
+    assert abs(bad[0] - float(c["ssim"][0])) > 1e-9
+    # averaged weights instead of the Gaussian
+    bad = oracle.ssim_ex(p0, p1, wl.bit_depth, 2, 0, 11, 4)
+    assert abs(bad[0] - float(c["ssim"][0])) > 1e-9
+    # ERP weight of a regular window row taken 5 rows below its centre (the block-mode
+    # convention applied to regular windows): the T2R direction no longer matches
+    import ctypes as C
+    f64p = C.POINTER(C.c_double)
+    bd = wl.bit_depth
+    M = float((1 << bd) - 1)
+    c1, c2 = (0.01 * M) ** 2, (0.03 * M) ** 2
+    h, w = p0[0].shape
+    x0, nx, y0, ny = oracle.ssim_grid(w, h, 1, 0, 11, 4)
+    wts = oracle.ws_weights(h)[0]
+    vals = []
+    for shift in (0, 5):
+        per = []
+        for k in range(3):
+            rs = np.zeros(ny)
+            oracle.L.or_ssim_rowsums_ex(np.ascontiguousarray(p0[k]).ctypes.data_as(C.POINTER(C.c_uint16)),
+                                        np.ascontiguousarray(s1[k]).ctypes.data_as(C.POINTER(C.c_uint16)),
+                                        w, h, 1, 0, 11, 4, C.c_double(c1), C.c_double(c2), 1, rs.ctypes.data_as(f64p))
+            wy = np.array([wts[y0 + j * 4 + shift] for j in range(ny)])
+            per.append(oracle.kbns(rs * wy) / (ny * nx * (oracle.kbns(wy) / ny)))
+        vals.append((4 * per[0] + per[1] + per[2]) / 6)
+    assert abs(vals[0] - float(c["ivssim_erp"][2])) < 1e-9
+    assert abs(vals[1] - float(c["ivssim_erp"][2])) > 1e-9
