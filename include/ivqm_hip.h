@@ -49,7 +49,9 @@ typedef struct ivqm_config {
   float unnoticeable_coef[4];     /* xGlobClrDiff.h:20 (default 0.01,0.01,0.01,0) */
   int32_t erp;                    /* equirectangular weighting (xWeightedSpherically.h:21) */
   int32_t lon_range_deg, lat_range_deg; /* default 360, 180 */
-  int32_t ssim_window, ssim_stride;     /* block-averaged SSIM, default 8 and 4 (xSSIM.h:35-37) */
+  int32_t ssim_window, ssim_stride;     /* SSIM window and stride, default 8 and 4 (xSSIM.h:35-37); regular
+                                           modes always use 11 x 11 windows (xSSIM.h:95) and ignore
+                                           ssim_window; stride 1..window */
   int32_t fake_vals_for_exact;    /* PSNR of an exact component = PSNR(SSD=1) (xPSNR.cpp:147, xAppQMIV.cpp:974-981) */
   int32_t gcd_sd_wrap32;          /* reproduce the x86-64-v4 <=14-bit int32 SD accumulation (xDistortionAVX512.cpp:14-73) */
   int32_t search_tile_rows;       /* accepted: 0, 4 or 8; the search runs 4-row waves (64 x 16 tiles) */
@@ -57,7 +59,12 @@ typedef struct ivqm_config {
   int32_t search_mode_cap;        /* search arithmetic: 0 = fastest exact variant the ranges allow,
                                      1 = reference-form int32 arithmetic only, 2 = no packed-int16 variant
                                      (all variants give identical results) */
-  int32_t reserved[5];
+  int32_t ssim_mode;              /* xSSIM::eMode (xSSIM.h:24-31): 0 RegularGaussianFlt, 1 RegularGaussianInt,
+                                     2 RegularAveraged, 3 BlockGaussianInt (window 8 or 16),
+                                     4 BlockAveraged (default; window 1..64) */
+  int32_t ssim_border;            /* eMrgExt (xCommonDefCORE.h:322-331), regular modes only: 0 None (default),
+                                     1 Nearest, 2 Reflect, 3 Mirror, 4 Constant (65535), 5 Zero */
+  int32_t reserved[3];
 } ivqm_config;
 
 /* One picture: three planes of 8-bit (bytes_per_sample = 1) or 16-bit samples.

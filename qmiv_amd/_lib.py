@@ -45,7 +45,8 @@ class Config(C.Structure):
         ("unnoticeable_coef", C.c_float * 4), ("erp", C.c_int32), ("lon_range_deg", C.c_int32),
         ("lat_range_deg", C.c_int32), ("ssim_window", C.c_int32), ("ssim_stride", C.c_int32),
         ("fake_vals_for_exact", C.c_int32), ("gcd_sd_wrap32", C.c_int32), ("search_tile_rows", C.c_int32),
-        ("ssim_wide_sums", C.c_int32), ("search_mode_cap", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("ssim_wide_sums", C.c_int32), ("search_mode_cap", C.c_int32), ("ssim_mode", C.c_int32),
+        ("ssim_border", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

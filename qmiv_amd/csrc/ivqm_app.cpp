@@ -69,8 +69,9 @@ struct Options {
   int search_range = 2;
   int cws[4] = {4, 1, 1, 0}, cwa[4] = {4, 1, 1, 0};
   float unc[4] = {0.01f, 0.01f, 0.01f, 0.0f};
-  int ss_stride = 4, ss_window = 8;
-  std::string ss_mode = "BlockAveraged";
+  int ss_stride = 4, ss_window = -1;  // window default: 11 for regular modes, else 8 (xAppQMIV.cpp:330)
+  std::string ss_mode = "BlockAveraged", ss_border = "None";
+  int ss_mode_id = 4, ss_border_id = 0;
   std::string ipa = "STOP";
   int verbose = 1;
 };
@@ -114,7 +115,9 @@ void usage() {
       " -pf pix_fmt (yuv420p, yuv420p10le, yuv444p16le, ...)  -s0 / -s1 StartFrame  -nf NumberOfFrames\n"
       " -r ResultFile  -ml \"PSNR, WSPSNR, IVPSNR, SSIM, MSSSIM, IVSSIM, IVMSSSIM, MSE\" | All\n"
       " -erp  -lor LonRangeDeg  -lar LatRangeDeg  -sr SearchRange (1..4)  -cws / -cwa \"4:1:1:0\"\n"
-      " -unc \"0.01:0.01:0.01:0\"  -ssm BlockAveraged  -sss 4  -ssw 8  -ipa SKIP|WARN|STOP|CNCL  -v VerboseLevel\n");
+      " -unc \"0.01:0.01:0.01:0\"  -ipa SKIP|WARN|STOP|CNCL  -v VerboseLevel\n"
+      " -ssm RegularGaussianFlt|RegularGaussianInt|RegularAveraged|BlockGaussianInt|BlockAveraged\n"
+      " -ssb None|Nearest|Reflect|Mirror|Constant|Zero (regular modes)  -sss Stride  -ssw Window (block modes)\n");
 }
 
 // Returns false (after printing the problems) when the options are not usable.
@@ -198,13 +201,14 @@ bool parse_options(int argc, char** argv, Options& o) {
           }))
         errs += "!  invalid UnnoticeableCoef " + s + "\n";
     } else if (a == "-ssm") o.ss_mode = need(i);
+    else if (a == "-ssb") o.ss_border = need(i);
     else if (a == "-sss") int_opt(o.ss_stride);
     else if (a == "-ssw") int_opt(o.ss_window);
     else if (a == "-ipa") o.ipa = upper(need(i));
     else if (a == "-v") int_opt(o.verbose);
     else if (a == "-nth") (void)need(i);  // the GPU path has no worker-thread count
-    else if (a == "-ff" || a == "-im" || a == "-bdm" || a == "-cfm" || a == "-csi" || a == "-csm" || a == "-ssb" ||
-             a == "-c" || a == "-nma") {
+    else if (a == "-ff" || a == "-im" || a == "-bdm" || a == "-cfm" || a == "-csi" || a == "-csm" || a == "-c" ||
+             a == "-nma") {
       const std::string v = need(i);
       if (!(a == "-ff" && upper(v) == "RAW") && !(a == "-nma")) errs += "!  unsupported option " + a + " " + v + "\n";
     } else errs += "!  unknown option " + a + "\n";
@@ -218,8 +222,25 @@ bool parse_options(int argc, char** argv, Options& o) {
   if (o.chroma != 420 && o.chroma != 444) errs += "!  Invalid or unsuported ChromaFormat value (420 or 444)\n";
   if (o.chroma == 420 && ((o.width | o.height) & 1)) errs += "!  4:2:0 needs even picture sizes\n";
   if (o.search_range < 1 || o.search_range > 4) errs += "!  SearchRange must be 1..4\n";
-  if (upper(o.ss_mode) != "BLOCKAVERAGED") errs += "!  only StructSimMode=BlockAveraged is supported\n";
-  if (o.ss_stride != 4 || o.ss_window != 8) errs += "!  only StructSimWindow=8 / StructSimStride=4 are supported\n";
+  // xSSIM::xStrToMode (xSSIM.cpp:17-26, case-insensitive), xStr2MrgExt (xMiscUtilsCORE.cpp:152-164)
+  const char* kModes[] = {"REGULARGAUSSIANFLT", "REGULARGAUSSIANINT", "REGULARAVERAGED", "BLOCKGAUSSIANINT",
+                          "BLOCKAVERAGED"};
+  const char* kBorders[] = {"None", "Nearest", "Reflect", "Mirror", "Constant", "Zero"};
+  o.ss_mode_id = -1;
+  for (int k = 0; k < 5; k++)
+    if (upper(trim(o.ss_mode)) == kModes[k]) o.ss_mode_id = k;
+  o.ss_border_id = -1;
+  for (int k = 0; k < 6; k++)
+    if (trim(o.ss_border) == kBorders[k]) o.ss_border_id = k;
+  if (o.ss_mode_id < 0) errs += "!  StructSimMode value is not valid\n";
+  if (o.ss_border_id < 0) errs += "!  StructSimBrdExt value is not valid\n";
+  const bool regular = o.ss_mode_id >= 0 && o.ss_mode_id <= 2;
+  if (o.ss_window < 0) o.ss_window = regular ? 11 : 8;
+  if (o.ss_stride < 1 || o.ss_stride > o.ss_window) errs += "! StructSimStride must be in range 1-StructSimWindow\n";
+  if (regular && o.ss_window != 11) errs += "! In regular struct sim mode only StructSimWindow==11 is allowed\n";
+  if (o.ss_mode_id == 3 && o.ss_window != 8 && o.ss_window != 16)
+    errs += "!  BlockGaussianInt supports StructSimWindow 8 and 16\n";
+  if (o.ss_mode_id == 4 && (o.ss_window < 1 || o.ss_window > 64)) errs += "!  StructSimWindow must be 1..64\n";
   if (o.ipa != "SKIP" && o.ipa != "WARN" && o.ipa != "STOP" && o.ipa != "CNCL") errs += "!  invalid InvalidPelActn\n";
   if (o.start0 < 0 || o.start1 < 0) errs += "!  invalid StartFrame\n";
   if (!errs.empty()) {
@@ -354,6 +375,10 @@ int main(int argc, char** argv) {
   cfg.lon_range_deg = o.lon;
   cfg.lat_range_deg = o.lat;
   cfg.fake_vals_for_exact = 1;  // apps/QMIV/src/xAppQMIV.cpp:974-981, :1002-1009
+  cfg.ssim_mode = o.ss_mode_id;
+  cfg.ssim_border = o.ss_border_id;
+  cfg.ssim_window = o.ss_window;
+  cfg.ssim_stride = o.ss_stride;
   ivqm_batch* batch = nullptr;
   IVQMX(ivqm_batch_create(&cfg, &batch));
 

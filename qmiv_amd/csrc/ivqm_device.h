@@ -52,7 +52,17 @@ struct SsimJobDev {
   int64_t erp_off;                // offset of this height's weights, or -1
   int32_t out_idx;                // [3][2] doubles at out[6 * out_idx]
   float c1n, c2n;                 // C1 * n^2, C2 * n^2 of this job's bit depth (xSSIM.cpp:50-52)
+  // Window grid and weights of k_ssim_gen (every mode other than block-averaged 8 / 4):
+  int32_t mode, ext;              // window mode (xSSIM::eMode 0..4), border extension (eMrgExt 0..5)
+  int32_t wnd, stride;            // window side (11 for regular modes) and stride
+  int32_t gx0, gy0;               // first window position: regular = centre (5, or 0 with an extension),
+                                  // block = top-left (0)
+  int32_t erp_base;               // ERP weight row of window row j = erp_base + j * stride
   int32_t pad;
+  double c1, c2;                  // xSSIM.cpp:50-52
+  double kscale;                  // sums -> moments: 1 (float weights), 2^-18 (Q18), 1/area (averaged)
+  const double* wt;               // wnd x wnd weights (row pitch wnd) followed by their transpose, or null
+                                  // for averaged modes
 };
 
 // One downsampling job of a pyramid level.
@@ -73,8 +83,13 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st);
+// Any window mode / size / stride / border extension: one wave per 64 window columns and
+// kSsimGenRows window rows, 4 waves per block.
+constexpr int kSsimGenRows = 8;
+hipError_t launch_ssim_gen(const SsimJobDev* jobs, int njobs, int nblk, int mode, int wnd, int ext, int stride,
+                           double* partial, hipStream_t st);
 hipError_t launch_ssim_finalize(const SsimJobDev* jobs, int njobs, const double* partial, const double* erp_w,
-                                int half_wnd, int stride, double* out, hipStream_t st);
+                                double* out, hipStream_t st);
 hipError_t launch_downsample(const DownJob* jobs, int njobs, int nblk, hipStream_t st);
 // Standalone pixel operations on one picture.
 hipError_t launch_downsample_one(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st);

@@ -71,6 +71,7 @@ struct BatchPlan {
   std::vector<DownJob> down[5];      // src / dst filled from down_src / down_dst when issued
   std::vector<PicRef> down_src[5], down_dst[5];
   int stat_blocks = 0, tiles = 0, ssim_blocks = 0, down_blocks[5] = {0, 0, 0, 0, 0};
+  bool ssim_gen = false;  // SSIM jobs run k_ssim_gen (any configuration but block-averaged 8 / 4)
   int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
   int py = 8, nb = 8;
   std::string signature;
@@ -94,6 +95,12 @@ struct RingSlot {
   hipGraphExec_t graph = nullptr;
   std::string graph_sig;
 };
+
+// Checks the SSIM mode / window / stride / border of a configuration
+// (the rules of xAppQMIV.cpp:325-332 and xStructSimSTD.cpp:353-419).
+int validate_ssim_config(const ivqm_config& cfg);
+// Window weights of a Gaussian SSIM mode, row-major wnd x wnd (empty for averaged modes).
+std::vector<double> ssim_window_weights(int mode, int wnd);
 
 class Engine {
  public:
@@ -169,6 +176,7 @@ class Engine {
   double* d_partial_ = nullptr;
   double* d_ssim_out_ = nullptr;
   double* d_erp_ = nullptr;
+  double* d_wts_ = nullptr;  // SSIM window weights of the configured mode (Gaussian modes)
   int64_t erp_used_ = 0, erp_cap_ = 0;
   std::map<int, int64_t> erp_off_;
 

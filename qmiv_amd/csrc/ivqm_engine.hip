@@ -52,7 +52,61 @@ template <typename T> int hmalloc(T** p, int64_t n) {
 }
 int64_t grow(int64_t need, int64_t have) { return need <= have ? have : std::max<int64_t>(need, have + have / 2); }
 
+// Normalised n x n Gaussian (centre (n - 1) / 2, sum 1) in FP64.
+std::vector<double> gaussian(int n, double sigma) {
+  std::vector<double> g((size_t)n * n);
+  const double c = 0.5 * (n - 1);
+  double sum = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      const double v = std::exp(-((i - c) * (i - c) + (j - c) * (j - c)) / (2.0 * sigma * sigma));
+      g[(size_t)i * n + j] = v;
+      sum += v;
+    }
+  for (double& v : g) v /= sum;
+  return g;
+}
+
 }  // namespace
+
+// libs/pmbb-ivqm/src/xStructSimConsts.h: the float weights (:51-66) are the
+// 11 x 11 Gaussian of sigma 1.5 (c_Sigma, :35) rounded to float32; the Q18
+// tables (c_FltrIntShift = 18, :95-98) are round(2^18 x Gaussian) with one
+// added at a few positions -- 11 x 11 sigma 1.5 (:100-113), 8 x 8 sigma 1.5
+// (:134-144), 16 x 16 sigma 2.4 (:146-164).
+std::vector<double> ssim_window_weights(int mode, int wnd) {
+  if (mode == 0) {
+    std::vector<double> g = gaussian(11, 1.5);
+    for (double& v : g) v = (double)(float)v;
+    return g;
+  }
+  struct Q18 {
+    int n;
+    double sigma;
+    std::vector<int> plus_one;  // (row, column) pairs
+  };
+  static const Q18 k11 = {11, 1.5, {3, 4, 3, 6, 4, 3, 4, 7, 5, 4, 5, 6, 6, 3, 6, 7, 7, 4, 7, 6}};
+  static const Q18 k8 = {8, 1.5, {2, 2, 2, 5, 5, 2, 5, 5}};
+  static const Q18 k16 = {16, 2.4, {0, 3, 0, 12, 3, 0, 3, 15, 12, 0, 12, 15, 15, 3, 15, 12}};
+  const Q18* q = mode == 1 ? &k11 : (mode == 3 && wnd == 8) ? &k8 : (mode == 3 && wnd == 16) ? &k16 : nullptr;
+  if (!q) return {};
+  std::vector<double> g = gaussian(q->n, q->sigma);
+  for (double& v : g) v = std::floor(v * 262144.0 + 0.5);
+  for (size_t k = 0; k + 1 < q->plus_one.size(); k += 2) g[(size_t)q->plus_one[k] * q->n + q->plus_one[k + 1]] += 1.0;
+  return g;
+}
+
+int validate_ssim_config(const ivqm_config& cfg) {
+  const int mode = cfg.ssim_mode, wnd = cfg.ssim_window, s = cfg.ssim_stride;
+  if (mode < 0 || mode > 4) return set_error(IVQM_EINVAL, "ssim_mode must be 0..4 (xSSIM::eMode)");
+  if (cfg.ssim_border < 0 || cfg.ssim_border > 5) return set_error(IVQM_EINVAL, "ssim_border must be 0..5 (eMrgExt)");
+  const int ew = mode <= 2 ? 11 : wnd;  // regular windows are 11 x 11 whatever ssim_window says (xSSIM.h:95)
+  if (mode == 3 && wnd != 8 && wnd != 16)
+    return set_error(IVQM_EINVAL, "BlockGaussianInt supports windows 8 and 16 (xStructSimSTD.cpp:353-419)");
+  if (mode == 4 && (wnd < 1 || wnd > 64)) return set_error(IVQM_EINVAL, "BlockAveraged window must be 1..64");
+  if (s < 1 || s > ew) return set_error(IVQM_EINVAL, "ssim_stride must be in range 1..window (xAppQMIV.cpp:331)");
+  return IVQM_OK;
+}
 
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -89,7 +143,7 @@ Engine::~Engine() {
       if (h) (void)hipHostFree(h);
   }
   void* ds[] = {d_pairs_, d_jobs_, d_down_, d_row_sd_, d_row_ssd_, d_rows_, d_gcd_, d_sd_, d_ssd_,
-                d_scp_,   d_best_, d_pyr_,  d_partial_, d_ssim_out_, d_erp_};
+                d_scp_,   d_best_, d_pyr_,  d_partial_, d_ssim_out_, d_erp_, d_wts_};
   for (void* d : ds)
     if (d) (void)hipFree(d);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -102,6 +156,15 @@ int Engine::init() {
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   own_stream_ = true;
   for (auto& S : ring_) HIPCHK(hipEventCreateWithFlags(&S.ev, hipEventDisableTiming));
+  std::vector<double> wts = ssim_window_weights(cfg_.ssim_mode, cfg_.ssim_window);
+  if (!wts.empty()) {  // row-major table, then its transpose (column-outer kernels)
+    const int n = (int)std::lround(std::sqrt((double)wts.size()));
+    wts.resize(2 * (size_t)n * n);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) wts[(size_t)n * n + (size_t)j * n + i] = wts[(size_t)i * n + j];
+    RCCHK(dmalloc(&d_wts_, (int64_t)wts.size()));
+    HIPCHK(hipMemcpy(d_wts_, wts.data(), sizeof(double) * wts.size(), hipMemcpyHostToDevice));
+  }
   return IVQM_OK;
 }
 
@@ -210,6 +273,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
                 P.py, P.nb, n);
   P.signature = sigbuf;
   const bool ms_any = (m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) != 0;
+  const bool regular = cfg_.ssim_mode <= 2;
+  P.ssim_gen = !(cfg_.ssim_mode == 4 && wnd == 8 && str == 4);
   for (int i = 0; i < n; i++) {
     const PairSpec& s = pairs[i];
     PairPlan& pp = P.pp[i];
@@ -331,11 +396,32 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     std::memset(&J, 0, sizeof(J));
     J.w = w;
     J.h = h;
-    J.nux = num_blocks(w, wnd, str);
-    J.nuy = num_blocks(h, wnd, str);
-    if (J.nux <= 0 || J.nuy <= 0) { *slot = -1; return IVQM_OK; }
-    J.nchunk = (J.nux + 62) / 63;
-    J.nbg = ((J.nuy + P.nb - 1) / P.nb + 3) / 4;
+    // window grid: xSSIM::xInitLoopRanges, libs/pmbb-ivqm/src/xSSIM.cpp:161-196
+    J.mode = cfg_.ssim_mode;
+    J.stride = str;
+    if (regular) {
+      const int b = cfg_.ssim_border ? 0 : 5;  // centres over the whole plane with an extension
+      J.ext = cfg_.ssim_border;
+      J.wnd = 11;
+      J.gx0 = J.gy0 = b;
+      J.nux = (w - 2 * b + str - 1) / str;
+      J.nuy = (h - 2 * b + str - 1) / str;
+      J.erp_base = b;  // ERP weight at the window's centre row (xIVSSIM.cpp:152)
+    } else {
+      J.wnd = wnd;
+      J.nux = num_blocks(w, wnd, str);
+      J.nuy = num_blocks(h, wnd, str);
+      J.erp_base = wnd >> 1;  // ... at its top row + half the window
+    }
+    if (J.nux <= 0 || J.nuy <= 0)
+      return set_error(IVQM_EINVAL, "picture (or MS-SSIM scale) smaller than the SSIM window");
+    if (P.ssim_gen) {
+      J.nchunk = (J.nux + 63) / 64;
+      J.nbg = (J.nuy + 4 * kSsimGenRows - 1) / (4 * kSsimGenRows);
+    } else {
+      J.nchunk = (J.nux + 62) / 63;
+      J.nbg = ((J.nuy + P.nb - 1) / P.nb + 3) / 4;
+    }
     J.calc_l = calc_l;
     J.erp = erp;
     // moment sum widths: 32-bit up to 12-bit samples, 64-bit windows up to 14-bit, 64-bit throughout above
@@ -352,6 +438,13 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     const double n2 = (double)(wnd * wnd) * (double)(wnd * wnd);
     J.c1n = (float)(c1 * n2);
     J.c2n = (float)(c2 * n2);
+    J.c1 = c1;
+    J.c2 = c2;
+    // sums -> moments: xStructSimSTD.cpp:38-42 (float weights), :104-108 (Q18), :176-180 (1 / 121),
+    // :446-450 (1 / window area)
+    J.kscale = J.mode == 0 ? 1.0 : (J.mode == 1 || J.mode == 3) ? (double)1.0 / (double)262144
+               : J.mode == 2 ? (double)1.0 / (double)121 : (double)1.0 / (double)(wnd * wnd);
+    J.wt = d_wts_;
     *slot = J.out_idx;
     P.jobs.push_back(J);
     P.job_a.push_back(a);
@@ -542,9 +635,12 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
     if (!P.down[l].empty())
       HIPCHK(launch_downsample(d_down_ + down_first[l], (int)P.down[l].size(), P.down_blocks[l], st));
   if (nj) {
-    HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, st));
-    HIPCHK(launch_ssim_finalize(d_jobs_, nj, d_partial_, d_erp_, cfg_.ssim_window >> 1, cfg_.ssim_stride,
-                                d_ssim_out_, st));
+    if (P.ssim_gen)
+      HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
+                             d_partial_, st));
+    else
+      HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, st));
+    HIPCHK(launch_ssim_finalize(d_jobs_, nj, d_partial_, d_erp_, d_ssim_out_, st));
   }
   if (P.need_stats) {
     HIPCHK(hipMemcpyAsync(S.r_gcd, d_gcd_, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st));
@@ -757,7 +853,13 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
         HIPCHK(launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
                              d_scp_, d_best_, stream_));
         break;
-      case 2: HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, stream_)); break;
+      case 2:
+        if (P.ssim_gen)
+          HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
+                                 d_partial_, stream_));
+        else
+          HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, stream_));
+        break;
       default: return set_error(IVQM_EINVAL, "unknown kernel id");
     }
   }

@@ -64,8 +64,7 @@ int validate_config(const ivqm_config* cfg, bool need_size) {
   if (need_size && (cfg->bit_depth < 8 || cfg->bit_depth > 16)) return set_error(IVQM_EINVAL, "bit_depth must be 8..16");
   if (cfg->search_range < 1 || cfg->search_range > 4) return set_error(IVQM_EINVAL, "search_range must be 1..4");
   if (cfg->search_mode_cap < 0 || cfg->search_mode_cap > 2) return set_error(IVQM_EINVAL, "search_mode_cap must be 0..2");
-  if (cfg->ssim_window != 8 || cfg->ssim_stride != 4)
-    return set_error(IVQM_EINVAL, "only the default block-averaged SSIM (window 8, stride 4) is implemented");
+  RCCHK(validate_ssim_config(*cfg));
   if (cfg->erp && (cfg->lat_range_deg <= 0 || cfg->lat_range_deg > 180)) return set_error(IVQM_EINVAL, "lat_range_deg");
   if (cfg->cmp_weights_average[0] + cfg->cmp_weights_average[1] + cfg->cmp_weights_average[2] +
           cfg->cmp_weights_average[3] == 0)
@@ -212,6 +211,8 @@ int ivqm_config_init(ivqm_config* cfg) {
   cfg->lat_range_deg = 180;
   cfg->ssim_window = 8;
   cfg->ssim_stride = 4;
+  cfg->ssim_mode = 4;  // BlockAveraged (xSSIM.h:35)
+  cfg->ssim_border = 0;
   return IVQM_OK;
 }
 

@@ -474,8 +474,7 @@ int ivqm_server_create(const ivqm_server_config* cfg_in, ivqm_server** out) {
     p.width = p.height = 64;  // per-pair sizes come from the sequences
     p.bit_depth = 10;
     if (p.search_range < 1 || p.search_range > 4) return set_error(IVQM_EINVAL, "search_range must be 1..4");
-    if (p.ssim_window != 8 || p.ssim_stride != 4)
-      return set_error(IVQM_EINVAL, "only the default block-averaged SSIM (window 8, stride 4) is implemented");
+    RCCHK(validate_ssim_config(p));
     if (p.search_mode_cap < 0 || p.search_mode_cap > 2) return set_error(IVQM_EINVAL, "search_mode_cap must be 0..2");
     std::unique_ptr<ivqm_server> s(new ivqm_server());
     s->cfg = cfg;

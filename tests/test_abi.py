@@ -50,7 +50,41 @@ def test_config_defaults_match_reference():
     assert list(cfg.cmp_weights_average) == [4, 1, 1, 0]
     assert [round(x, 6) for x in cfg.unnoticeable_coef] == [0.01, 0.01, 0.01, 0.0]
     assert (cfg.ssim_window, cfg.ssim_stride) == (8, 4)
+    assert (cfg.ssim_mode, cfg.ssim_border) == (4, 0)  # BlockAveraged, no extension (xSSIM.h:35, xAppQMIV.cpp:327)
     assert (cfg.lon_range_deg, cfg.lat_range_deg) == (360, 180)
+
+
+@pytest.mark.parametrize("field,value,ok", [
+    ("ssim_mode", 5, False), ("ssim_mode", -1, False), ("ssim_border", 6, False),
+    ("ssim_stride", 0, False), ("ssim_stride", 9, False),
+    (("ssim_mode", "ssim_window"), (3, 32), False), (("ssim_mode", "ssim_window"), (3, 16), True),
+    (("ssim_mode", "ssim_stride"), (0, 11), True), (("ssim_mode", "ssim_stride"), (1, 12), False),
+    (("ssim_mode", "ssim_window", "ssim_stride"), (2, 8, 11), True),  # regular modes ignore ssim_window
+    (("ssim_window", "ssim_stride"), (65, 4), False), (("ssim_window", "ssim_stride"), (32, 32), True),
+])
+def test_ssim_config_validation(field, value, ok):
+    """SSIM mode / window / stride / border rules (xAppQMIV.cpp:325-332, xStructSimSTD.cpp:353-419):
+    rejected settings return IVQM_EINVAL before any device work; accepted ones get past
+    validation (to the HIP error of a GPU-less host, or a context on a GPU host)."""
+    from qmiv_amd._lib import LIB_PATH, Config, load
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libivqm_hip.so not built")
+    L = load()
+    cfg = Config()
+    L.ivqm_config_init(C.byref(cfg))
+    cfg.width, cfg.height = 64, 64
+    fields = field if isinstance(field, tuple) else (field,)
+    values = value if isinstance(value, tuple) else (value,)
+    for f, v in zip(fields, values):
+        setattr(cfg, f, v)
+    h = C.c_void_p()
+    rc = L.ivqm_create(C.byref(cfg), C.byref(h))
+    if ok:
+        assert rc != -22, L.ivqm_last_error()
+        if rc == 0:
+            L.ivqm_destroy(h)
+    else:
+        assert rc == -22
 
 
 def test_invalid_arguments_return_status_not_crash():

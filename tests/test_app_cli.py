@@ -28,8 +28,13 @@ def test_help():
     (["-i0", "a", "-i1", "b", "-ps", "65x64", "-cf", "420"], "even picture sizes"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-bd", "17"], "BitDepth"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ml", "PSNR, FOO"], 'Token "FOO"'),
-    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "RegularGaussianFlt"], "BlockAveraged"),
-    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssw", "16"], "StructSimWindow"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "GaussianFlt"], "StructSimMode value is not valid"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "RegularGaussianFlt", "-ssw", "8"],
+     "only StructSimWindow==11"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssb", "Wrap"], "StructSimBrdExt value is not valid"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-sss", "9"], "StructSimStride must be in range"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "RegularAveraged", "-sss", "12"], "StructSimStride"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "BlockGaussianInt", "-ssw", "32"], "BlockGaussianInt"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-sr", "5"], "SearchRange"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv"], "unsupported option -im"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cws", "4:1:1"], "component weights"),
@@ -38,6 +43,20 @@ def test_help():
 def test_rejected_options(args, msg):
     r = _run(*args)
     assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+@pytest.mark.parametrize("args", [
+    ["-ssm", "regulargaussianint", "-ssb", "Mirror", "-sss", "1"],
+    ["-ssm", "BlockGaussianInt", "-ssw", "16", "-sss", "16"],
+    ["-ssm", "BlockAveraged", "-ssw", "32", "-sss", "8"],
+    ["-ssm", "RegularGaussianFlt", "-ssb", " Constant "],
+])
+def test_accepted_ssim_options(tmp_path, args):
+    # accepted SSIM settings get past option checking (an empty file then has no frames)
+    a = tmp_path / "a.yuv"
+    a.write_bytes(b"")
+    r = _run("-i0", str(a), "-i1", str(a), "-ps", "64x64", *args)
+    assert r.returncode == 1 and "no frames to process" in r.stderr, r.stderr
 
 
 def test_missing_input_file(tmp_path):
