@@ -41,6 +41,7 @@ PEAK_HBM_GBS = 8000.0
 # is issue-bound; the 39.3 T instruction rate is reported beside it.
 PEAK_VALU_OPS_TOPS = 157.3
 PEAK_VALU_ISSUE_TINSTR = 39.3
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
 SEARCH_OPS_PER_PX = 2 * 25 * 12  # two directions x (2R+1)^2 candidates x 12 ops (R = 2)
 SSIM_BYTES_PER_PX = 3 + 3 + 6 + 6  # P0, P1 (4:2:0, 2-byte samples) and SCP0, SCP1 (4:4:4) per IV-SSIM pass
 STATS_BYTES_PER_PX = 6  # P0 and P1, 4:2:0, 2-byte samples
@@ -56,8 +57,9 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "batch", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="approximate CPU work of the baseline sample")
-    ap.add_argument("--sections", default="plain_sync,ssim_family,mixed,served,stress,frontend",
-                    help="extra sections: plain_sync, ssim_family, mixed (64 mixed pairs), served (Poisson loop), "
+    ap.add_argument("--sections", default="plain_sync,ssim_family,ssim_modes,mixed,served,stress,frontend",
+                    help="extra sections: plain_sync, ssim_family, ssim_modes (regular / block SSIM modes), "
+                         "mixed (64 mixed pairs), served (Poisson loop), "
                          "stress (8192x4096 16-bit R=4), frontend (raw files through qmiv_amd/bin/ivqm_qmiv)")
     ap.add_argument("--stress-frames", type=int, default=8)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -255,6 +257,8 @@ def main():
     sections = set(args.sections.split(",")) if args.sections else set()
     if "ssim_family" in sections:
         out["ssim_family"] = ssim_family(Q, torch, pairs, args)
+    if "ssim_modes" in sections:
+        out["ssim_modes"] = ssim_modes(Q, torch, pairs, args)
     del pairs
     torch.cuda.empty_cache()
     if "mixed" in sections:
@@ -297,6 +301,45 @@ def ssim_family(Q, torch, pairs, args):
         ms = _time_steps(torch, lambda: bt.run(arr, m), max(2, args.steps // 2), 1)
     return {"metrics": "SSIM+IV-SSIM+WS-PSNR+MS-SSIM(5)", "pairs": FRAMES, "ms_per_step": round(ms, 3),
             "value": round(FRAMES * W * H / (ms / 1e3) / 1e6, 1), "unit": "Mpixel-pairs/s"}
+
+
+# SSIM window modes (xSSIM::eMode) beyond the default block-averaged 8 / 4: (name, mode, border, window, stride)
+SSIM_MODES = [("RegularGaussianFlt s1", 0, 0, 11, 1), ("RegularGaussianInt s1", 1, 0, 11, 1),
+              ("RegularAveraged s1", 2, 0, 11, 1), ("RegularGaussianFlt s1 Nearest", 0, 1, 11, 1),
+              ("RegularGaussianInt s4", 1, 0, 11, 4), ("BlockGaussianInt 8/4", 3, 0, 8, 4),
+              ("BlockAveraged 16/4", 4, 0, 16, 4)]
+
+
+def ssim_modes(Q, torch, pairs, args):
+    """SSIM + IV-SSIM of 4 ERP pairs in each SSIM mode (batch form, hipGraph), and the SSIM
+    kernel's own time (HIP events around 10 launches on the batch's stream). Algorithmic
+    work of the weighted modes: 121 taps x 5 FP64 FMAs per window, 3 jobs x 3 components of
+    NumUnitX x NumUnitY windows per pair."""
+    n = 4
+    m = Q.IVQM_M_SSIM | Q.IVQM_M_IVSSIM
+    out = []
+    for name, mode, ext, wnd, stride in SSIM_MODES:
+        with Q.IvqmBatch(erp=1, ssim_mode=mode, ssim_border=ext, ssim_window=wnd, ssim_stride=stride) as bt:
+            arr = Q.IvqmBatch.make_pairs([(a, b, W, H, BD, True) for a, b in pairs[:n]])
+            bt.set_tuning(4, 8, use_graph=True)
+            ms = _time_steps(torch, lambda: bt.run(arr, m), 3, 1)
+            k_ms = bt.profile(2, 10)
+        if mode <= 2:
+            b = 0 if ext else 5
+            nux, nuy = (W - 2 * b + stride - 1) // stride, (H - 2 * b + stride - 1) // stride
+        else:
+            nux, nuy = (W - wnd) // stride + 1, (H - wnd) // stride + 1
+        taps = 121 if mode <= 2 else wnd * wnd
+        windows = 9 * n * nux * nuy
+        rec = {"mode": name, "ms_per_step": round(ms, 3),
+               "value": round(n * W * H / (ms / 1e3) / 1e6, 1), "unit": "Mpixel-pairs/s",
+               "ssim_kernel_ms": round(k_ms, 3), "windows": windows}
+        if mode in (0, 1, 3):
+            tf = windows * taps * 10 / (k_ms / 1e3) / 1e12
+            rec["roofline"] = {"bound": "fp64 valu", "achieved": round(tf, 2), "peak": PEAK_FP64_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 3)}
+        out.append(rec)
+    return out
 
 
 def mixed_batch(Q, synth, torch, np, args):

@@ -18,10 +18,11 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_engine.hip", "ivqm_runtime.hip", "ivqm_serve.hip"]
+HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_ssim_modes.hip", "ivqm_engine.hip", "ivqm_runtime.hip", "ivqm_serve.hip"]
+OBJ_DIR = os.path.join(HERE, "_build")
 
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 VALU forms stay off (hipcc -O3 would otherwise emit v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -34,10 +35,21 @@ def _run(cmd, **kw):
 
 
 def build_hip(verbose_isa: bool = False) -> str:
+    """Compiles every HIP source to an object (in parallel), then links the shared library."""
     out = os.path.join(HERE, "libivqm_hip.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    procs, objs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-I" + os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
+        objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-Wl,-soname,libivqm_hip.so", "-I" + os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libivqm_hip.so", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

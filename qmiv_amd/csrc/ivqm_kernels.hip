@@ -17,11 +17,7 @@
 //   k_ssim           block-averaged 8x8 / stride-4 SSIM: exact integer 4x4
 //                    sub-block moments, 2x2 combination, FP32 ratios, FP64
 //                    per-window-row partial sums.
-//   k_ssim_gen       every other SSIM configuration: regular 11x11 windows
-//                    (Gaussian float / Q18 / averaged, with border
-//                    extensions), block Gaussian Q18, block averaged of any
-//                    window and stride; FP64 moments (exact for integer
-//                    weights), FP64 ratios.
+//   (k_ssim_gen*     every other SSIM configuration: ivqm_ssim_modes.hip)
 //   k_ssim_finalize  FP64 frame totals per job and component (ERP-weighted
 //                    for IV-SSIM).
 //   k_downsample     2x2 rounded-mean pyramid level (MS-SSIM).
@@ -33,6 +29,7 @@
 // The reference's Nearest margin (xMarginOps.h:40-73) is reproduced by
 // clamping coordinates.
 #include "ivqm_device.h"
+#include "ivqm_device_util.h"
 
 #include <limits.h>
 
@@ -44,79 +41,6 @@ namespace ivqm {
 
 namespace {
 
-// Plane c of a picture. Selects between loaded values instead of indexing
-// p[] / stride[] with a run-time c (or selecting between their addresses):
-// either keeps a private copy of the table entry in scratch memory.
-__device__ __forceinline__ const void* plane_ptr(const PicDev& P, int c) {
-  const void* p0 = P.p[0];
-  const void* p1 = P.p[1];
-  const void* p2 = P.p[2];
-  return c == 0 ? p0 : (c == 1 ? p1 : p2);
-}
-__device__ __forceinline__ int plane_stride(const PicDev& P, int c) {
-  const int s0 = P.stride[0], s1 = P.stride[1], s2 = P.stride[2];
-  return c == 0 ? s0 : (c == 1 ? s1 : s2);
-}
-
-// Sample (x, y) of plane c on the luma grid. Plane offsets are 32-bit with
-// 24-bit operands (rows and strides stay below 2^24, planes below 2^32 samples).
-// Row y (luma grid) of plane c as a global pointer; with a wave-uniform y the
-// pointer lives in scalar registers and a load adds only the lane's column.
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(const PicDev& P, int c, int y) {
-  typedef __attribute__((address_space(1))) const T GT;
-  const int s = c ? P.cs : 0;
-  return (const GT*)plane_ptr(P, c) + __umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c));
-}
-
-// Raw buffer resource over plane c of a picture of height H (the exact plane
-// size in bytes; data-format word 32-bit). Loads through it take the row as a
-// scalar byte offset and the column as a 32-bit vector byte offset, so one
-// vector offset serves every row and no address arithmetic is per load.
-template <typename T>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const PicDev& P, int c, int H) {
-  const int s = c ? P.cs : 0;
-  const uint32_t bytes = (uint32_t)((H + s) >> s) * (uint32_t)plane_stride(P, c) * (uint32_t)sizeof(T);
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(plane_ptr(P, c)), (short)0, (int)bytes, 0x00020000);
-}
-template <typename T>
-__device__ __forceinline__ int ld_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  if (sizeof(T) == 2) return (int)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
-  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
-}
-
-// Picture planes are device allocations: addressed as global memory they load
-// with a scalar base and a 32-bit vector offset (no 64-bit address math).
-template <typename T>
-__device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
-  typedef __attribute__((address_space(1))) const T GT;
-  const int s = c ? P.cs : 0;
-  const GT* p = (const GT*)plane_ptr(P, c);
-  return (int)p[__umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> s)];
-}
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // v_med3_i32
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-// Sum over the 64 lanes with DPP (no LDS traffic), uniform result: prefix
-// sums within each 16-lane row (row_shr 1, 2, 4, 8; lanes shifted in from
-// outside the row read 0), then row_bcast:15 / row_bcast:31 carry the row
-// totals up, and lane 63 holds the wave total.
-__device__ __forceinline__ uint32_t wave_sum_u32_dpp(uint32_t v) {
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-// Three wave sums with their DPP steps interleaved (independent instructions
-// between dependent DPP reads instead of wait states).
 __device__ __forceinline__ void wave_sum3_u32_dpp(uint32_t v[3]) {
 #define IVQM_DPP3(ctrl, rmask, bc)                                                                     \
   {                                                                                                   \
@@ -136,44 +60,10 @@ __device__ __forceinline__ void wave_sum3_u32_dpp(uint32_t v[3]) {
 #undef IVQM_DPP3
   for (int k = 0; k < 3; k++) v[k] = (uint32_t)__builtin_amdgcn_readlane((int)v[k], 63);
 }
-// FP64 sum over the 64 lanes with DPP moves of both halves (fixed order: row
-// prefixes, then row broadcasts); the total is read from lane 63.
-__device__ __forceinline__ double wave_sum_f64_dpp(double v) {
-#define IVQM_DPPD(ctrl, rmask, bc)                                                                      \
-  {                                                                                                    \
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);                                                \
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, ctrl, rmask, 0xf, bc); \
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), ctrl, rmask, 0xf, bc); \
-    v += __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));                              \
-  }
-  IVQM_DPPD(0x111, 0xf, true)
-  IVQM_DPPD(0x112, 0xf, true)
-  IVQM_DPPD(0x114, 0xf, true)
-  IVQM_DPPD(0x118, 0xf, true)
-  IVQM_DPPD(0x142, 0xa, false)
-  IVQM_DPPD(0x143, 0xc, false)
-#undef IVQM_DPPD
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
-  return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
-}
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;
-}
-
-// Index of the last table entry whose first-block index is <= blk.
-template <typename E, typename F>
-__device__ __forceinline__ int find_entry(const E* __restrict__ t, int n, int blk, F first) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (first(t[mid]) <= blk) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -1067,262 +957,6 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
 }
 
-// ---------------------------------------------------------------------------
-// k_ssim_gen: every SSIM window mode, size, stride and border extension.
-// libs/pmbb-ivqm/src/xStructSimSTD.cpp:19-193 (regular 11x11: Gaussian float
-// weights with FP64 sums, Q18 integer weights, plain sums), :353-419 (block
-// Gaussian Q18), :420-461 (block averaged); window grid xSSIM.cpp:161-196;
-// border extension xMarginOps.h:40-215 (xCommonDefCORE.h:322-331).
-//
-// Lane = one window column (64 per wave), wave = kSsimGenRows consecutive
-// window rows; each sample row the wave's windows cover is read once and fed
-// to every window containing it. Moments accumulate in FP64: with integer
-// weights every product and partial sum is an integer below 2^53, so they are
-// the reference's int64 sums exactly; with float weights the sums run in the
-// reference's order (rows, then columns). The ratio is FP64 as written in the
-// reference.
-// ---------------------------------------------------------------------------
-// Index of the extended plane's sample u (0 <= u < n inside): Nearest clamps,
-// Reflect repeats the edge sample (c b a | a b c), Mirror does not (d c b | a b c d);
-// -1 for the Constant / Zero border, whose samples hold a fixed value.
-__device__ __forceinline__ int ext_index(int u, int n, int ext) {
-  if ((unsigned)u < (unsigned)n) return u;
-  if (ext == 1) return u < 0 ? 0 : n - 1;
-  if (ext == 2) return u < 0 ? -u - 1 : 2 * n - 1 - u;
-  if (ext == 3) return u < 0 ? -u : 2 * n - 2 - u;
-  return -1;
-}
-
-template <typename TA, typename TB, int NW, bool WEIGHTED, bool EXT>
-__device__ void ssim_gen_wave(const SsimJobDev& J, int c, int chunk, int wr0, int lane, double* __restrict__ partial) {
-  constexpr int KY = kSsimGenRows;
-  const int n = NW ? NW : J.wnd;
-  const int s = J.stride;
-  const int nk = min(KY, J.nuy - wr0);
-  const int wx = chunk * 64 + lane;
-  const bool active = wx < J.nux;
-  const int wxc = active ? wx : J.nux - 1;  // idle lanes repeat the last window (in bounds)
-  const int off = J.mode <= 2 ? 5 : 0;      // regular windows are centred
-  const int xs = J.gx0 + wxc * s - off;     // first sample column of this lane's windows
-  const int ys = J.gy0 + wr0 * s - off;     // first sample row of the wave's windows
-  const int rows = (nk - 1) * s + n;
-  typedef __attribute__((address_space(1))) const TA GTA;
-  typedef __attribute__((address_space(1))) const TB GTB;
-  GTA* pa = (GTA*)plane_ptr(J.a, c);
-  GTB* pb = (GTB*)plane_ptr(J.b, c);
-  const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
-  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
-  const double kfix = J.ext == 4 ? 65535.0 : 0.0;  // Constant border: xPic.cpp:94-101 passes uint16 max
-  double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
-#pragma unroll
-  for (int k = 0; k < KY; k++) mA[k] = mB[k] = mAA[k] = mBB[k] = mAB[k] = 0.0;
-
-#pragma unroll 1
-  for (int r = 0; r < rows; r++) {
-    // windows k fed by this row: 0 <= r - k s < n (wave-uniform)
-    const int khi = min(nk - 1, r / s);
-    const int klo = r < n ? 0 : (r - n) / s + 1;
-    int y = ys + r;
-    bool yfix = false;
-    if (EXT) {
-      y = ext_index(y, J.h, J.ext);
-      yfix = y < 0;
-    }
-    GTA* ra = pa + (size_t)(uint32_t)((yfix ? 0 : y) >> csa) * (uint32_t)sa;
-    GTB* rb = pb + (size_t)(uint32_t)((yfix ? 0 : y) >> csb) * (uint32_t)sb;
-#pragma unroll 4
-    for (int dx = 0; dx < n; dx++) {
-      double a, b;
-      if (EXT) {
-        const int x = ext_index(xs + dx, J.w, J.ext);
-        if (yfix || x < 0) {
-          a = kfix;
-          b = kfix;
-        } else {
-          a = (double)ra[x >> csa];
-          b = (double)rb[x >> csb];
-        }
-      } else {
-        const int x = xs + dx;
-        a = (double)ra[x >> csa];
-        b = (double)rb[x >> csb];
-      }
-      const double aa = a * a, bb = b * b, ab = a * b;  // exact: integers below 2^32
-#pragma unroll
-      for (int k = 0; k < KY; k++) {
-        if (k < klo || k > khi) continue;
-        if (WEIGHTED) {
-          const double w = J.wt[(r - k * s) * n + dx];
-          mA[k] = fma(a, w, mA[k]);
-          mB[k] = fma(b, w, mB[k]);
-          mAA[k] = fma(aa, w, mAA[k]);
-          mBB[k] = fma(bb, w, mBB[k]);
-          mAB[k] = fma(ab, w, mAB[k]);
-        } else {
-          mA[k] += a;
-          mB[k] += b;
-          mAA[k] += aa;
-          mBB[k] += bb;
-          mAB[k] += ab;
-        }
-      }
-    }
-  }
-  const double K = J.kscale;
-#pragma unroll
-  for (int k = 0; k < KY; k++) {
-    if (k >= nk) break;
-    double v = 0.0;
-    if (active) {
-      // xStructSimSTD.cpp:104-108 / :176-180 / :399-414: means, variances, covariance, L * CS
-      const double ar = mA[k] * K, at = mB[k] * K;
-      const double vr = mAA[k] * K - ar * ar, vt = mBB[k] * K - at * at, cv = mAB[k] * K - ar * at;
-      v = (2 * cv + J.c2) / (vr + vt + J.c2);
-      if (J.calc_l) v = ((2 * ar * at + J.c1) / (ar * ar + at * at + J.c1)) * v;
-    }
-    v = wave_sum_f64_dpp(v);
-    if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wr0 + k) * J.nchunk + chunk] = v;
-  }
-}
-
-// Stride-1 regular windows (the original SSIM grid): the wave's 8 windows per
-// column span 18 sample rows. The column offset dx is the outer loop, with the
-// 11 weights of that column (one per window row) loaded once as wave-uniform
-// values; the 18 rows are unrolled, so which windows a row feeds and with which
-// weight is fixed at compile time and the FMA stream has no branches.
-template <typename TA, typename TB, bool WEIGHTED, bool EXT>
-__device__ void ssim_gen_s1_wave(const SsimJobDev& J, int c, int chunk, int wr0, int lane,
-                                 double* __restrict__ partial) {
-  constexpr int KY = kSsimGenRows, N = 11, ROWS = KY + N - 1;
-  const int nk = min(KY, J.nuy - wr0);
-  const int wx = chunk * 64 + lane;
-  const bool active = wx < J.nux;
-  const int wxc = active ? wx : J.nux - 1;
-  const int xs = J.gx0 + wxc - 5, ys = J.gy0 + wr0 - 5;
-  const int ylast = ys + nk - 1 + N - 1;  // last row a live window reads; rows of idle windows are clamped to it
-  typedef __attribute__((address_space(1))) const TA GTA;
-  typedef __attribute__((address_space(1))) const TB GTB;
-  GTA* pa = (GTA*)plane_ptr(J.a, c);
-  GTB* pb = (GTB*)plane_ptr(J.b, c);
-  const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
-  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
-  const double kfix = J.ext == 4 ? 65535.0 : 0.0;
-  const double* __restrict__ wt_t = J.wt + N * N;  // transposed weights: [dx][dy]
-  double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
-#pragma unroll
-  for (int k = 0; k < KY; k++) mA[k] = mB[k] = mAA[k] = mBB[k] = mAB[k] = 0.0;
-
-#pragma unroll 1
-  for (int dx = 0; dx < N; dx++) {
-    double w[N];
-#pragma unroll
-    for (int dy = 0; dy < N; dy++) w[dy] = WEIGHTED ? wt_t[dx * N + dy] : 1.0;
-    int x = xs + dx;
-    if (EXT) x = ext_index(x, J.w, J.ext);
-    const bool xfix = EXT && x < 0;
-    const uint32_t xa = (uint32_t)(xfix ? 0 : x) >> csa, xb = (uint32_t)(xfix ? 0 : x) >> csb;
-#pragma unroll
-    for (int rho = 0; rho < ROWS; rho++) {
-      int y = min(ys + rho, ylast);
-      bool fix = xfix;
-      if (EXT) {
-        y = ext_index(y, J.h, J.ext);
-        fix = fix || y < 0;
-        y = y < 0 ? 0 : y;
-      }
-      double a = (double)pa[(uint32_t)(y >> csa) * (uint32_t)sa + xa];
-      double b = (double)pb[(uint32_t)(y >> csb) * (uint32_t)sb + xb];
-      if (EXT && fix) {
-        a = kfix;
-        b = kfix;
-      }
-      const double aa = a * a, bb = b * b, ab = a * b;
-#pragma unroll
-      for (int k = 0; k < KY; k++) {
-        const int dy = rho - k;  // compile-time
-        if (dy < 0 || dy >= N) continue;
-        if (WEIGHTED) {
-          mA[k] = fma(a, w[dy], mA[k]);
-          mB[k] = fma(b, w[dy], mB[k]);
-          mAA[k] = fma(aa, w[dy], mAA[k]);
-          mBB[k] = fma(bb, w[dy], mBB[k]);
-          mAB[k] = fma(ab, w[dy], mAB[k]);
-        } else {
-          mA[k] += a;
-          mB[k] += b;
-          mAA[k] += aa;
-          mBB[k] += bb;
-          mAB[k] += ab;
-        }
-      }
-    }
-  }
-  const double K = J.kscale;
-#pragma unroll
-  for (int k = 0; k < KY; k++) {
-    if (k >= nk) break;
-    double v = 0.0;
-    if (active) {
-      const double ar = mA[k] * K, at = mB[k] * K;
-      const double vr = mAA[k] * K - ar * ar, vt = mBB[k] * K - at * at, cv = mAB[k] * K - ar * at;
-      v = (2 * cv + J.c2) / (vr + vt + J.c2);
-      if (J.calc_l) v = ((2 * ar * at + J.c1) / (ar * ar + at * at + J.c1)) * v;
-    }
-    v = wave_sum_f64_dpp(v);
-    if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wr0 + k) * J.nchunk + chunk] = v;
-  }
-}
-
-template <bool WEIGHTED, bool EXT>
-__global__ __launch_bounds__(256) void k_ssim_gen_s1(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
-                                                     double* __restrict__ partial) {
-  const int per = (nblk + kXcds - 1) / kXcds;
-  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-  if (b >= nblk) return;
-  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
-  const SsimJobDev J = jobs[ji];
-  const int lb = b - J.blk0;
-  const int per_c = J.nchunk * J.nbg;
-  const int c = lb / per_c, rem = lb - c * per_c;
-  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
-  const int lane = threadIdx.x & 63;
-  const int wr0 = (bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * kSsimGenRows;
-  if (wr0 >= J.nuy) return;
-  if (J.a.bps == 2 && J.b.bps == 2)
-    ssim_gen_s1_wave<uint16_t, uint16_t, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else if (J.a.bps == 1 && J.b.bps == 1)
-    ssim_gen_s1_wave<uint8_t, uint8_t, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else if (J.a.bps == 1)
-    ssim_gen_s1_wave<uint8_t, uint16_t, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else
-    ssim_gen_s1_wave<uint16_t, uint8_t, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-}
-
-template <int NW, bool WEIGHTED, bool EXT>
-__global__ __launch_bounds__(256) void k_ssim_gen(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
-                                                  double* __restrict__ partial) {
-  const int per = (nblk + kXcds - 1) / kXcds;  // XCD-aware order, as in k_ssim
-  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-  if (b >= nblk) return;
-  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
-  const SsimJobDev J = jobs[ji];
-  const int lb = b - J.blk0;
-  const int per_c = J.nchunk * J.nbg;
-  const int c = lb / per_c, rem = lb - c * per_c;
-  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
-  const int lane = threadIdx.x & 63;
-  const int wr0 = (bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * kSsimGenRows;
-  if (wr0 >= J.nuy) return;  // whole wave
-  if (J.a.bps == 2 && J.b.bps == 2)
-    ssim_gen_wave<uint16_t, uint16_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else if (J.a.bps == 1 && J.b.bps == 1)
-    ssim_gen_wave<uint8_t, uint8_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else if (J.a.bps == 1)
-    ssim_gen_wave<uint8_t, uint16_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-  else
-    ssim_gen_wave<uint16_t, uint8_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
-}
-
 // One block per (job, component): FP64 frame totals of the window-row sums.
 // IV-SSIM ERP weighting follows libs/pmbb-ivqm/src/xIVSSIM.cpp:148-165: the
 // weight of window row j is taken at its centre row (regular) or at its top
@@ -1450,33 +1084,6 @@ hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, doub
     case 8: hipLaunchKernelGGL(k_ssim<8>, dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
     case 16: hipLaunchKernelGGL(k_ssim<16>, dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
     default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_ssim_gen(const SsimJobDev* jobs, int njobs, int nblk, int mode, int wnd, int ext, int stride,
-                           double* partial, hipStream_t st) {
-  if (njobs <= 0 || nblk <= 0) return hipSuccess;
-  const dim3 g(kXcds * ((nblk + kXcds - 1) / kXcds)), t(256);
-  const bool regular = mode <= 2, weighted = mode == 0 || mode == 1 || mode == 3, ext_on = regular && ext != 0;
-  if (regular && stride == 1) {
-    if (weighted && ext_on) hipLaunchKernelGGL((k_ssim_gen_s1<true, true>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (weighted) hipLaunchKernelGGL((k_ssim_gen_s1<true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (ext_on) hipLaunchKernelGGL((k_ssim_gen_s1<false, true>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else hipLaunchKernelGGL((k_ssim_gen_s1<false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-  } else if (regular) {
-    if (weighted && ext_on) hipLaunchKernelGGL((k_ssim_gen<11, true, true>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (weighted) hipLaunchKernelGGL((k_ssim_gen<11, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (ext_on) hipLaunchKernelGGL((k_ssim_gen<11, false, true>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else hipLaunchKernelGGL((k_ssim_gen<11, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-  } else if (mode == 3) {
-    if (wnd == 8) hipLaunchKernelGGL((k_ssim_gen<8, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (wnd == 16) hipLaunchKernelGGL((k_ssim_gen<16, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else return hipErrorInvalidValue;
-  } else {
-    if (wnd == 8) hipLaunchKernelGGL((k_ssim_gen<8, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else if (wnd == 16) hipLaunchKernelGGL((k_ssim_gen<16, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
-    else hipLaunchKernelGGL((k_ssim_gen<0, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
   }
   return hipGetLastError();
 }

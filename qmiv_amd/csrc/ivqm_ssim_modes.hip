@@ -1,0 +1,334 @@
+// gfx950 kernels of every structural-similarity configuration other than the
+// default block-averaged 8 x 8 / stride 4 (which k_ssim in ivqm_kernels.hip
+// runs): regular 11 x 11 windows (Gaussian float, Gaussian Q18, averaged) with
+// the border extensions, block Gaussian Q18, block averaged of any window and
+// stride. Window sums feed the same per-window-row partial buffer and
+// k_ssim_finalize as k_ssim.
+#include "ivqm_device.h"
+#include "ivqm_device_util.h"
+
+namespace ivqm {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// k_ssim_gen: every SSIM window mode, size, stride and border extension.
+// libs/pmbb-ivqm/src/xStructSimSTD.cpp:19-193 (regular 11x11: Gaussian float
+// weights with FP64 sums, Q18 integer weights, plain sums), :353-419 (block
+// Gaussian Q18), :420-461 (block averaged); window grid xSSIM.cpp:161-196;
+// border extension xMarginOps.h:40-215 (xCommonDefCORE.h:322-331).
+//
+// Lane = one window column (64 per wave), wave = kSsimGenRows consecutive
+// window rows; each sample row the wave's windows cover is read once and fed
+// to every window containing it. Moments accumulate in FP64: with integer
+// weights every product and partial sum is an integer below 2^53, so they are
+// the reference's int64 sums exactly; with float weights the sums run in the
+// reference's order (rows, then columns). The ratio is FP64 as written in the
+// reference.
+// ---------------------------------------------------------------------------
+// Index of the extended plane's sample u (0 <= u < n inside): Nearest clamps,
+// Reflect repeats the edge sample (c b a | a b c), Mirror does not (d c b | a b c d);
+// -1 for the Constant / Zero border, whose samples hold a fixed value.
+__device__ __forceinline__ int ext_index(int u, int n, int ext) {
+  if ((unsigned)u < (unsigned)n) return u;
+  if (ext == 1) return u < 0 ? 0 : n - 1;
+  if (ext == 2) return u < 0 ? -u - 1 : 2 * n - 1 - u;
+  if (ext == 3) return u < 0 ? -u : 2 * n - 2 - u;
+  return -1;
+}
+
+template <typename TA, typename TB, int NW, bool WEIGHTED, bool EXT>
+__device__ void ssim_gen_wave(const SsimJobDev& J, int c, int chunk, int wr0, int lane, double* __restrict__ partial) {
+  constexpr int KY = kSsimGenRows;
+  const int n = NW ? NW : J.wnd;
+  const int s = J.stride;
+  const int nk = min(KY, J.nuy - wr0);
+  const int wx = chunk * 64 + lane;
+  const bool active = wx < J.nux;
+  const int wxc = active ? wx : J.nux - 1;  // idle lanes repeat the last window (in bounds)
+  const int off = J.mode <= 2 ? 5 : 0;      // regular windows are centred
+  const int xs = J.gx0 + wxc * s - off;     // first sample column of this lane's windows
+  const int ys = J.gy0 + wr0 * s - off;     // first sample row of the wave's windows
+  const int rows = (nk - 1) * s + n;
+  typedef __attribute__((address_space(1))) const TA GTA;
+  typedef __attribute__((address_space(1))) const TB GTB;
+  GTA* pa = (GTA*)plane_ptr(J.a, c);
+  GTB* pb = (GTB*)plane_ptr(J.b, c);
+  const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
+  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
+  const double kfix = J.ext == 4 ? 65535.0 : 0.0;  // Constant border: xPic.cpp:94-101 passes uint16 max
+  double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
+#pragma unroll
+  for (int k = 0; k < KY; k++) mA[k] = mB[k] = mAA[k] = mBB[k] = mAB[k] = 0.0;
+
+#pragma unroll 1
+  for (int r = 0; r < rows; r++) {
+    // windows k fed by this row: 0 <= r - k s < n (wave-uniform)
+    const int khi = min(nk - 1, r / s);
+    const int klo = r < n ? 0 : (r - n) / s + 1;
+    int y = ys + r;
+    bool yfix = false;
+    if (EXT) {
+      y = ext_index(y, J.h, J.ext);
+      yfix = y < 0;
+    }
+    GTA* ra = pa + (size_t)(uint32_t)((yfix ? 0 : y) >> csa) * (uint32_t)sa;
+    GTB* rb = pb + (size_t)(uint32_t)((yfix ? 0 : y) >> csb) * (uint32_t)sb;
+#pragma unroll 4
+    for (int dx = 0; dx < n; dx++) {
+      double a, b;
+      if (EXT) {
+        const int x = ext_index(xs + dx, J.w, J.ext);
+        if (yfix || x < 0) {
+          a = kfix;
+          b = kfix;
+        } else {
+          a = (double)ra[x >> csa];
+          b = (double)rb[x >> csb];
+        }
+      } else {
+        const int x = xs + dx;
+        a = (double)ra[x >> csa];
+        b = (double)rb[x >> csb];
+      }
+      const double aa = a * a, bb = b * b, ab = a * b;  // exact: integers below 2^32
+#pragma unroll
+      for (int k = 0; k < KY; k++) {
+        if (k < klo || k > khi) continue;
+        if (WEIGHTED) {
+          const double w = J.wt[(r - k * s) * n + dx];
+          mA[k] = fma(a, w, mA[k]);
+          mB[k] = fma(b, w, mB[k]);
+          mAA[k] = fma(aa, w, mAA[k]);
+          mBB[k] = fma(bb, w, mBB[k]);
+          mAB[k] = fma(ab, w, mAB[k]);
+        } else {
+          mA[k] += a;
+          mB[k] += b;
+          mAA[k] += aa;
+          mBB[k] += bb;
+          mAB[k] += ab;
+        }
+      }
+    }
+  }
+  const double K = J.kscale;
+#pragma unroll
+  for (int k = 0; k < KY; k++) {
+    if (k >= nk) break;
+    double v = 0.0;
+    if (active) {
+      // xStructSimSTD.cpp:104-108 / :176-180 / :399-414: means, variances, covariance, L * CS
+      const double ar = mA[k] * K, at = mB[k] * K;
+      const double vr = mAA[k] * K - ar * ar, vt = mBB[k] * K - at * at, cv = mAB[k] * K - ar * at;
+      v = (2 * cv + J.c2) / (vr + vt + J.c2);
+      if (J.calc_l) v = ((2 * ar * at + J.c1) / (ar * ar + at * at + J.c1)) * v;
+    }
+    v = wave_sum_f64_dpp(v);
+    if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wr0 + k) * J.nchunk + chunk] = v;
+  }
+}
+
+// Static-row kernels (compile-time stride S and window N): the wave's 8
+// windows per column span (8 - 1) S + N sample rows. The column offset dx is
+// the outer loop, with the N weights of that column (one per window row)
+// loaded once as wave-uniform values; the rows are unrolled, so which windows
+// a row feeds and with which weight is fixed at compile time and the FMA
+// stream has no branches. SSIM is symmetric in its two pictures, so a pair of
+// 16-bit and 8-bit planes runs as 8-bit / 16-bit.
+template <typename TA, typename TB, int S, int N, bool WEIGHTED, bool EXT>
+__device__ void ssim_static_wave(const SsimJobDev& J, const PicDev& A, const PicDev& B, int c, int chunk, int wr0,
+                                 int lane, double* __restrict__ partial) {
+  constexpr int KY = kSsimGenRows, ROWS = (KY - 1) * S + N;
+  const int nk = min(KY, J.nuy - wr0);
+  const int wx = chunk * 64 + lane;
+  const bool active = wx < J.nux;
+  const int wxc = active ? wx : J.nux - 1;
+  const int off = J.mode <= 2 ? 5 : 0;  // regular windows are centred
+  const int xs = J.gx0 + wxc * S - off, ys = J.gy0 + wr0 * S - off;
+  const int ylast = ys + (nk - 1) * S + N - 1;  // last row a live window reads; later rows are clamped to it
+  typedef __attribute__((address_space(1))) const TA GTA;
+  typedef __attribute__((address_space(1))) const TB GTB;
+  GTA* pa = (GTA*)plane_ptr(A, c);
+  GTB* pb = (GTB*)plane_ptr(B, c);
+  const int sa = plane_stride(A, c), sb = plane_stride(B, c);
+  const int csa = c ? A.cs : 0, csb = c ? B.cs : 0;
+  const double kfix = J.ext == 4 ? 65535.0 : 0.0;
+  const double* __restrict__ wt_t = J.wt + N * N;  // transposed weights: [dx][dy]
+  double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
+#pragma unroll
+  for (int k = 0; k < KY; k++) mA[k] = mB[k] = mAA[k] = mBB[k] = mAB[k] = 0.0;
+
+#pragma unroll 1
+  for (int dx = 0; dx < N; dx++) {
+    double w[N];
+#pragma unroll
+    for (int dy = 0; dy < N; dy++) w[dy] = WEIGHTED ? wt_t[dx * N + dy] : 1.0;
+    int x = xs + dx;
+    if (EXT) x = ext_index(x, J.w, J.ext);
+    const bool xfix = EXT && x < 0;
+    const uint32_t xa = (uint32_t)(xfix ? 0 : x) >> csa, xb = (uint32_t)(xfix ? 0 : x) >> csb;
+#pragma unroll
+    for (int rho = 0; rho < ROWS; rho++) {
+      int y = min(ys + rho, ylast);
+      bool fix = xfix;
+      if (EXT) {
+        y = ext_index(y, J.h, J.ext);
+        fix = fix || y < 0;
+        y = y < 0 ? 0 : y;
+      }
+      double a = (double)pa[(uint32_t)(y >> csa) * (uint32_t)sa + xa];
+      double b = (double)pb[(uint32_t)(y >> csb) * (uint32_t)sb + xb];
+      if (EXT && fix) {
+        a = kfix;
+        b = kfix;
+      }
+      const double aa = a * a, bb = b * b, ab = a * b;
+#pragma unroll
+      for (int k = 0; k < KY; k++) {
+        const int dy = rho - k * S;  // compile-time
+        if (dy < 0 || dy >= N) continue;
+        if (WEIGHTED) {
+          mA[k] = fma(a, w[dy], mA[k]);
+          mB[k] = fma(b, w[dy], mB[k]);
+          mAA[k] = fma(aa, w[dy], mAA[k]);
+          mBB[k] = fma(bb, w[dy], mBB[k]);
+          mAB[k] = fma(ab, w[dy], mAB[k]);
+        } else {
+          mA[k] += a;
+          mB[k] += b;
+          mAA[k] += aa;
+          mBB[k] += bb;
+          mAB[k] += ab;
+        }
+      }
+    }
+  }
+  const double K = J.kscale;
+#pragma unroll
+  for (int k = 0; k < KY; k++) {
+    if (k >= nk) break;
+    double v = 0.0;
+    if (active) {
+      const double ar = mA[k] * K, at = mB[k] * K;
+      const double vr = mAA[k] * K - ar * ar, vt = mBB[k] * K - at * at, cv = mAB[k] * K - ar * at;
+      v = (2 * cv + J.c2) / (vr + vt + J.c2);
+      if (J.calc_l) v = ((2 * ar * at + J.c1) / (ar * ar + at * at + J.c1)) * v;
+    }
+    v = wave_sum_f64_dpp(v);
+    if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wr0 + k) * J.nchunk + chunk] = v;
+  }
+}
+
+template <int S, int N, bool WEIGHTED, bool EXT>
+__global__ __launch_bounds__(256) void k_ssim_static(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
+                                                     double* __restrict__ partial) {
+  const int per = (nblk + kXcds - 1) / kXcds;
+  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (b >= nblk) return;
+  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
+  const SsimJobDev J = jobs[ji];
+  const int lb = b - J.blk0;
+  const int per_c = J.nchunk * J.nbg;
+  const int c = lb / per_c, rem = lb - c * per_c;
+  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
+  const int lane = threadIdx.x & 63;
+  const int wr0 = (bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * kSsimGenRows;
+  if (wr0 >= J.nuy) return;
+  if (J.a.bps == 2 && J.b.bps == 2)
+    ssim_static_wave<uint16_t, uint16_t, S, N, WEIGHTED, EXT>(J, J.a, J.b, c, chunk, wr0, lane, partial);
+  else if (J.a.bps == 1 && J.b.bps == 1)
+    ssim_static_wave<uint8_t, uint8_t, S, N, WEIGHTED, EXT>(J, J.a, J.b, c, chunk, wr0, lane, partial);
+  else if (J.a.bps == 1)
+    ssim_static_wave<uint8_t, uint16_t, S, N, WEIGHTED, EXT>(J, J.a, J.b, c, chunk, wr0, lane, partial);
+  else
+    ssim_static_wave<uint8_t, uint16_t, S, N, WEIGHTED, EXT>(J, J.b, J.a, c, chunk, wr0, lane, partial);
+}
+
+template <int NW, bool WEIGHTED, bool EXT>
+__global__ __launch_bounds__(256) void k_ssim_gen(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
+                                                  double* __restrict__ partial) {
+  const int per = (nblk + kXcds - 1) / kXcds;  // XCD-aware order, as in k_ssim
+  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (b >= nblk) return;
+  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
+  const SsimJobDev J = jobs[ji];
+  const int lb = b - J.blk0;
+  const int per_c = J.nchunk * J.nbg;
+  const int c = lb / per_c, rem = lb - c * per_c;
+  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
+  const int lane = threadIdx.x & 63;
+  const int wr0 = (bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * kSsimGenRows;
+  if (wr0 >= J.nuy) return;  // whole wave
+  if (J.a.bps == 2 && J.b.bps == 2)
+    ssim_gen_wave<uint16_t, uint16_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
+  else if (J.a.bps == 1 && J.b.bps == 1)
+    ssim_gen_wave<uint8_t, uint8_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
+  else if (J.a.bps == 1)
+    ssim_gen_wave<uint8_t, uint16_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
+  else
+    ssim_gen_wave<uint16_t, uint8_t, NW, WEIGHTED, EXT>(J, c, chunk, wr0, lane, partial);
+}
+
+}  // namespace
+
+namespace {
+template <int S, int N, bool W, bool E>
+void launch_static(dim3 g, dim3 t, hipStream_t st, const SsimJobDev* jobs, int njobs, int nblk, double* partial) {
+  hipLaunchKernelGGL((k_ssim_static<S, N, W, E>), g, t, 0, st, jobs, njobs, nblk, partial);
+}
+template <int S, int N, bool W>
+bool launch_static_ext(bool ext, dim3 g, dim3 t, hipStream_t st, const SsimJobDev* jobs, int njobs, int nblk,
+                       double* partial) {
+  if (ext) launch_static<S, N, W, true>(g, t, st, jobs, njobs, nblk, partial);
+  else launch_static<S, N, W, false>(g, t, st, jobs, njobs, nblk, partial);
+  return true;
+}
+// Static-row kernel for (stride, window) when one is instantiated; false otherwise.
+bool launch_static_any(int s, int n, bool w, bool ext, dim3 g, dim3 t, hipStream_t st, const SsimJobDev* jobs,
+                       int njobs, int nblk, double* partial) {
+#define IVQM_REG(S)                                                                                   \
+  if (n == 11 && s == S)                                                                              \
+    return w ? launch_static_ext<S, 11, true>(ext, g, t, st, jobs, njobs, nblk, partial)              \
+             : launch_static_ext<S, 11, false>(ext, g, t, st, jobs, njobs, nblk, partial);
+#define IVQM_BLK(S, N)                                                                                \
+  if (n == N && s == S) {                                                                             \
+    if (w) launch_static<S, N, true, false>(g, t, st, jobs, njobs, nblk, partial);                    \
+    else launch_static<S, N, false, false>(g, t, st, jobs, njobs, nblk, partial);                     \
+    return true;                                                                                      \
+  }
+  IVQM_REG(1) IVQM_REG(2) IVQM_REG(3) IVQM_REG(4)
+  IVQM_BLK(1, 8) IVQM_BLK(2, 8) IVQM_BLK(4, 8) IVQM_BLK(8, 8)
+  IVQM_BLK(1, 16) IVQM_BLK(2, 16) IVQM_BLK(4, 16) IVQM_BLK(8, 16) IVQM_BLK(16, 16)
+#undef IVQM_REG
+#undef IVQM_BLK
+  return false;
+}
+}  // namespace
+
+hipError_t launch_ssim_gen(const SsimJobDev* jobs, int njobs, int nblk, int mode, int wnd, int ext, int stride,
+                           double* partial, hipStream_t st) {
+  if (njobs <= 0 || nblk <= 0) return hipSuccess;
+  const dim3 g(kXcds * ((nblk + kXcds - 1) / kXcds)), t(256);
+  const bool regular = mode <= 2, weighted = mode == 0 || mode == 1 || mode == 3, ext_on = regular && ext != 0;
+  const int n = regular ? 11 : wnd;
+  // static rows pay at stride 1 and 2, and for weighted windows (measured at 4096 x 2048: averaged 16 / 4
+  // runs faster in the general kernel, whose lanes share fewer cache lines per load at larger strides)
+  if ((stride <= 2 || weighted) && launch_static_any(stride, n, weighted, ext_on, g, t, st, jobs, njobs, nblk, partial))
+    return hipGetLastError();
+  if (regular) {
+    if (weighted && ext_on) hipLaunchKernelGGL((k_ssim_gen<11, true, true>), g, t, 0, st, jobs, njobs, nblk, partial);
+    else if (weighted) hipLaunchKernelGGL((k_ssim_gen<11, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
+    else if (ext_on) hipLaunchKernelGGL((k_ssim_gen<11, false, true>), g, t, 0, st, jobs, njobs, nblk, partial);
+    else hipLaunchKernelGGL((k_ssim_gen<11, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
+  } else if (mode == 3) {
+    if (wnd == 8) hipLaunchKernelGGL((k_ssim_gen<8, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
+    else if (wnd == 16) hipLaunchKernelGGL((k_ssim_gen<16, true, false>), g, t, 0, st, jobs, njobs, nblk, partial);
+    else return hipErrorInvalidValue;
+  } else {
+    hipLaunchKernelGGL((k_ssim_gen<0, false, false>), g, t, 0, st, jobs, njobs, nblk, partial);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ivqm
