@@ -199,9 +199,11 @@ class xIVSSIM : public Processor {
   double LastR2T = 0, LastT2R = 0;
   void create(std::array<int32_t, 2>, int32_t BitDepth, int32_t, bool) { m_Cfg.bit_depth = BitDepth; }
   void destroy() { xReset(); }
-  // Only BlockAveraged (mode 4, xSSIM.h:30) with window 8, stride 4 is implemented.
-  void setStructSimParams(int Mode, int, int32_t Window, int32_t Stride) {
-    if (Mode != 4) throw Error(IVQM_EINVAL, "only the BlockAveraged SSIM mode is implemented");
+  // Mode = xSSIM::eMode (xSSIM.h:24-31), MarginMode = eMrgExt (xCommonDefCORE.h:322-331); the
+  // reference's enum values pass through (the checks run when the context is created).
+  template <class ModeT, class ExtT> void setStructSimParams(ModeT Mode, ExtT MarginMode, int32_t Window, int32_t Stride) {
+    m_Cfg.ssim_mode = (int32_t)Mode;
+    m_Cfg.ssim_border = (int32_t)MarginMode;
     m_Cfg.ssim_window = Window;
     m_Cfg.ssim_stride = Stride;
     xReset();
