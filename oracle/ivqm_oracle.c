@@ -672,6 +672,171 @@ OR_EXPORT double or_ivmsssim(const uint16_t* const p0[3], const uint16_t* const 
   return or_ivmsssim_ex(p0, p1, scp0, scp1, w, h, bit_depth, OR_BLCK_AVG, OR_EXT_NONE, wnd, stride, cw_avg);
 }
 
+/* ---------------------------------------------------------------------------
+ * Mask mode (apps/QMIV/src/xAppQMIV.cpp:902-906, :930, :953, :971, :999,
+ * :1026-1029). The mask is a luma-resolution weight plane M (0..MaxM,
+ * BitDepthM 8..14); chroma of 4:2:0 pictures is compared at 4:4:4, so every
+ * sum below runs over the 4:4:4 planes with M at the same position.
+ * ------------------------------------------------------------------------- */
+/* xPixelOpsSTD::CountNonZero (libs/pmbb-core/src/xPixelOpsSTD.cpp). */
+OR_EXPORT int64_t or_count_nonzero(const uint16_t* m, int64_t n) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) k += m[i] != 0;
+  return k;
+}
+/* xDistortionSTD::CalcWeightedSD (libs/pmbb-core/src/xDistortionSTD.cpp:45-56):
+ * sum of ((int32)T - (int32)R) * (int32)M, the product in int32. */
+OR_EXPORT int64_t or_weighted_sd(const uint16_t* tst, const uint16_t* ref, const uint16_t* msk, int64_t n) {
+  int64_t sd = 0;
+  for (int64_t i = 0; i < n; i++) sd += (int32_t)(((int32_t)tst[i] - (int32_t)ref[i]) * (int32_t)msk[i]);
+  return sd;
+}
+/* xDistortionSTD::CalcWeightedSSD (:57-68): sum of (uint64)(int32 square) * M;
+ * the square is int32 (wraps for |d| > 46340) and sign-extends to uint64. */
+static uint64_t or_wssd_term(int32_t d, uint16_t m) {
+  const int32_t sq = (int32_t)((uint32_t)d * (uint32_t)d);
+  return (uint64_t)(int64_t)sq * (uint64_t)m;
+}
+OR_EXPORT uint64_t or_weighted_ssd(const uint16_t* tst, const uint16_t* ref, const uint16_t* msk, int64_t n) {
+  uint64_t s = 0;
+  for (int64_t i = 0; i < n; i++) s += or_wssd_term((int32_t)tst[i] - (int32_t)ref[i], msk[i]);
+  return s;
+}
+
+/* xGlobClrDiff::CalcGlobalColorDiffM (libs/pmbb-ivqm/src/xGlobClrDiff.cpp:60-91)
+ * as xGlobClrDiffProc calls it (xGlobClrDiff.h:42 swaps the pictures):
+ * SD_c = sum (P1 - P0) * M, avg = SD / (NumNonMasked * MaxM), round half
+ * away, clip to the FP32 MaxDiff. */
+OR_EXPORT void or_gcd_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w,
+                             int32_t h, int32_t bit_depth, int32_t bit_depth_m, const float coef[4], int32_t gcd[4]) {
+  const int64_t n = (int64_t)w * h;
+  const int64_t nnm = or_count_nonzero(msk, n);
+  const int32_t maxv = (1 << bit_depth) - 1, maxm = (1 << bit_depth_m) - 1;
+  for (int c = 0; c < 4; c++) {
+    const int32_t maxdiff = or_round_f32(coef[c] * (float)maxv);
+    const int64_t sd = c < 3 ? or_weighted_sd(p1[c], p0[c], msk, n) : 0;
+    const double avg = (double)sd / (double)(nnm * (int64_t)maxm);
+    int32_t g = or_round_f64(avg);
+    gcd[c] = or_clampi(g, -maxdiff, maxdiff);
+  }
+}
+
+/* Masked PSNR family as the app computes it:
+ *   MSE_c  = SSDM_c / (NNM * MaxM)                        (xPSNR.cpp:153-163)
+ *   PSNR_c = 10 log10(NNM * MaxP^2 * MaxM / SSDM_c), DBL_MAX for SSDM 0
+ *            (xPSNR.cpp:164-179) where calcPicPSNRM passes the MASK bit depth
+ *            as the picture's (xPSNR.cpp:60: MaxP = MaxM), and so does
+ *            WS-PSNR without ERP (xWSPSNR.cpp:61-64 calls calcPicPSNRM);
+ *   WS-PSNR with ERP: rows of weighted SSD, KBNS of row * w[y], times the
+ *            correction, then the same formula with the picture's own bit
+ *            depth (xWSPSNR.cpp:108-136).
+ * SSDM_c = sum (P0 - P1)^2 * M. psnr_bd is the bit depth used for MaxP. */
+OR_EXPORT double or_psnr_masked(double ssdm, int64_t nnm, int32_t psnr_bd, int32_t bit_depth_m) {
+  const int64_t maxp = (1 << psnr_bd) - 1, maxm = (1 << bit_depth_m) - 1;
+  const int64_t mx = nnm * (maxp * maxp) * maxm;
+  return ssdm > 0 ? 10.0 * log10((double)mx / ssdm) : DBL_MAX;
+}
+OR_EXPORT void or_psnr_family_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk,
+                                     int32_t w, int32_t h, int32_t bit_depth, int32_t bit_depth_m, const double* erp_w,
+                                     double correction, double mse[3], double psnr[3], double wspsnr[3]) {
+  const int64_t n = (int64_t)w * h;
+  const int64_t nnm = or_count_nonzero(msk, n);
+  const int64_t maxm = (1 << bit_depth_m) - 1;
+  for (int c = 0; c < 3; c++) {
+    const uint64_t ssdm = or_weighted_ssd(p0[c], p1[c], msk, n);
+    mse[c] = (double)ssdm / (double)(nnm * maxm);
+    psnr[c] = or_psnr_masked((double)ssdm, nnm, bit_depth_m, bit_depth_m);
+    if (erp_w) {
+      or_kbns k = {0, 0};
+      for (int32_t y = 0; y < h; y++) {
+        const size_t o = (size_t)y * w;
+        or_kbns_acc(&k, (double)or_weighted_ssd(p1[c] + o, p0[c] + o, msk + o, w) * erp_w[y]);
+      }
+      wspsnr[c] = or_psnr_masked(or_kbns_result(&k) * correction, nnm, bit_depth, bit_depth_m);
+    } else {
+      wspsnr[c] = psnr[c];
+    }
+  }
+}
+
+/* Masked corresponding-pixel search of one direction:
+ * xCorrespPixelShiftSTD::CalcDistAsymmetricRowM / FindBestPixelWithinBlockM
+ * (libs/pmbb-ivqm/src/xCorrespPixelShiftSTD.cpp:151-215). Test pixels with
+ * M = 0 are skipped; candidates with M = 0 are skipped (M outside the
+ * picture is the Nearest-extended mask, xAppQMIV.cpp:899); the winner is the
+ * first strict minimum of the int32 (wrapping) weighted error; the row sum is
+ * uint64 of (uint64)(int32 square) * M per component. */
+OR_EXPORT void or_cps_masked(const uint16_t* const tst[3], const uint16_t* const ref[3], const uint16_t* msk,
+                             int32_t w, int32_t h, const int32_t gcs[4], int32_t R, const int32_t cw[4],
+                             uint64_t* rows) {
+  for (int32_t y = 0; y < h; y++) {
+    uint64_t row[4] = {0, 0, 0, 0};
+    for (int32_t x = 0; x < w; x++) {
+      const uint16_t m = msk[(size_t)y * w + x];
+      if (m == 0) continue;
+      int32_t t[3];
+      for (int c = 0; c < 3; c++) t[c] = (int32_t)tst[c][(size_t)y * w + x] + gcs[c];
+      int32_t best_err = INT32_MAX, bx = -1, by = -1;
+      for (int32_t dy = -R; dy <= R; dy++) {
+        const int32_t ry = or_clampi(y + dy, 0, h - 1);
+        for (int32_t dx = -R; dx <= R; dx++) {
+          const int32_t rx = or_clampi(x + dx, 0, w - 1);
+          if (msk[(size_t)ry * w + rx] == 0) continue;
+          uint32_t e = 0;
+          for (int c = 0; c < 3; c++) {
+            const uint32_t d = (uint32_t)(t[c] - (int32_t)ref[c][(size_t)ry * w + rx]);
+            e += d * d * (uint32_t)cw[c];
+          }
+          if ((int32_t)e < best_err) { best_err = (int32_t)e; bx = rx; by = ry; }
+        }
+      }
+      for (int c = 0; c < 3; c++) row[c] += or_wssd_term(t[c] - (int32_t)ref[c][(size_t)by * w + bx], m);
+    }
+    for (int c = 0; c < 4; c++) rows[(size_t)y * 4 + c] = row[c];
+  }
+}
+
+/* IV-PSNR with a mask: xIVPSNRM::calcPicIVPSNRM(TstI, RefI, Msk, NNM, GCD_R2T)
+ * (libs/pmbb-ivqm/src/xIVPSNR.cpp:152-208). Its labels are swapped against
+ * the unmasked calcPicIVPSNR (:45-55): here R2T is P1 - GCD searched in P0
+ * and T2R is P0 + GCD searched in P1 (:161-162). The component error is the
+ * uint64 row total (or, ERP, the 4-lane KBNS of (double)row * w[y], no
+ * correction); PSNR_c = 10 log10(NNM * MaxP^2 * MaxM / max(err, ->1 if 0)). */
+OR_EXPORT double or_ivpsnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk,
+                                  int32_t w, int32_t h, int32_t bit_depth, int32_t bit_depth_m,
+                                  const int32_t gcd_r2t[4], int32_t R, const int32_t cw_search[4],
+                                  const int32_t cw_avg[4], const double* erp_w, double* r2t_out, double* t2r_out) {
+  const int64_t nnm = or_count_nonzero(msk, (int64_t)w * h);
+  int32_t ng[4];
+  for (int c = 0; c < 4; c++) ng[c] = -gcd_r2t[c];
+  uint64_t* rows = (uint64_t*)malloc((size_t)h * 4 * sizeof(uint64_t));
+  double q[2];
+  for (int dir = 0; dir < 2; dir++) {
+    /* dir 0 = R2T: Tst = P0 (+GCD) against Ref = P1; dir 1 = T2R: P1 (-GCD) against P0 */
+    if (dir == 0) or_cps_masked(p0, p1, msk, w, h, gcd_r2t, R, cw_search, rows);
+    else or_cps_masked(p1, p0, msk, w, h, ng, R, cw_search, rows);
+    double err[4] = {0, 0, 0, 0};
+    if (erp_w) {
+      or_kbns k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+      for (int32_t y = 0; y < h; y++)
+        for (int c = 0; c < 4; c++) or_kbns_acc(&k[c], (double)rows[(size_t)y * 4 + c] * erp_w[y]);
+      for (int c = 0; c < 4; c++) err[c] = or_kbns_result(&k[c]);
+    } else {
+      uint64_t tot[4] = {0, 0, 0, 0};
+      for (int32_t y = 0; y < h; y++)
+        for (int c = 0; c < 4; c++) tot[c] += rows[(size_t)y * 4 + c];
+      for (int c = 0; c < 4; c++) err[c] = (double)tot[c];
+    }
+    double v[4] = {0, 0, 0, 0};
+    for (int c = 0; c < 3; c++) v[c] = or_psnr_masked(err[c] > 0 ? err[c] : 1.0, nnm, bit_depth, bit_depth_m);
+    q[dir] = or_weighted_mean4(v, cw_avg);
+  }
+  free(rows);
+  if (r2t_out) *r2t_out = q[1];
+  if (t2r_out) *t2r_out = q[0];
+  return q[1] < q[0] ? q[1] : q[0];
+}
+
 /* Component-weighted mean exported for the picture value of per-component
  * metrics (xMetricQMIV.cpp:91-97 uses v0*w0 + v1*w1 + v2*w2). */
 OR_EXPORT double or_pic_value(const double v[3], const int32_t cw[4]) {

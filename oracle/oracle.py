@@ -89,6 +89,9 @@ class Oracle:
         L.or_ssim_num_blocks.restype = C.c_int32
         L.or_ms_num_scales.restype = C.c_int32
         L.or_ssim_plane_ex.restype = C.c_double
+        L.or_ivpsnr_masked.restype = C.c_double
+        L.or_psnr_masked.restype = C.c_double
+        L.or_count_nonzero.restype = C.c_int64
         L.or_ivssim_ex.restype = C.c_double
         L.or_ivmsssim_ex.restype = C.c_double
 
@@ -259,6 +262,55 @@ class Oracle:
 
     def pic_value(self, v3, cw=(4, 1, 1, 0)):
         return self.L.or_pic_value((C.c_double * 3)(*v3), _i32x4(cw))
+
+    # -- mask mode (the mask: one uint16 plane at luma resolution, bit depth bdm) --
+    def count_nonzero(self, m):
+        m = np.ascontiguousarray(m, np.uint16)
+        return self.L.or_count_nonzero(m.ctypes.data_as(_u16p), C.c_int64(m.size))
+
+    def gcd_masked(self, p0, p1, msk, bd, bdm, coef=(0.01, 0.01, 0.01, 0.0)):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        out = (C.c_int32 * 4)()
+        self.L.or_gcd_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, _f32x4(coef), out)
+        return [out[i] for i in range(4)]
+
+    def psnr_family_masked(self, p0, p1, msk, bd, bdm, erp=False, lat=180):
+        """(mse[3], psnr[3], wspsnr[3]) as the app computes them with a mask."""
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        ew, corr = self.ws_weights(h, lat) if erp else (None, 1.0)
+        mse, psnr, ws = (C.c_double * 3)(), (C.c_double * 3)(), (C.c_double * 3)()
+        self.L.or_psnr_family_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm,
+                                     ew.ctypes.data_as(_f64p) if ew is not None else None, C.c_double(corr),
+                                     mse, psnr, ws)
+        return list(mse), list(psnr), list(ws)
+
+    def cps_masked(self, tst, ref, msk, gcs, R=2, cw=(4, 1, 1, 0)):
+        a, pa = _planes(tst)
+        b, pb = _planes(ref)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        rows = np.zeros((h, 4), np.uint64)
+        self.L.or_cps_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, _i32x4(gcs), R, _i32x4(cw),
+                             rows.ctypes.data_as(_u64p))
+        return rows
+
+    def ivpsnr_masked(self, p0, p1, msk, bd, bdm, gcd, R=2, cws=(4, 1, 1, 0), cwa=(4, 1, 1, 0), erp=False, lat=180):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        ew = self.ws_weights(h, lat)[0] if erp else None
+        r2t, t2r = C.c_double(0), C.c_double(0)
+        v = self.L.or_ivpsnr_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, _i32x4(gcd), R, _i32x4(cws),
+                                    _i32x4(cwa), ew.ctypes.data_as(_f64p) if ew is not None else None,
+                                    C.byref(r2t), C.byref(t2r))
+        return v, r2t.value, t2r.value
 
     # -- structural-similarity modes (mode: xSSIM::eMode 0..4, ext: eMrgExt 0..5) --
     def ssim_weights(self, which):
@@ -454,6 +506,34 @@ class Reference:
         v, r2t, t2r = C.c_double(0), C.c_double(0), C.c_double(0)
         self.L.ref_ivssim_ex(pa, pb, ps0, ps1, w, h, bd, mode, ext, wnd, stride, margin, _i32x4(cwa), int(erp), lat,
                              int(ms), C.byref(v), C.byref(r2t), C.byref(t2r))
+        return v.value, r2t.value, t2r.value
+
+    def gcd_masked(self, p0, p1, msk, bd, bdm, R=2, coef=(0.01, 0.01, 0.01, 0.0)):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        out = (C.c_int32 * 4)()
+        self.L.ref_gcd_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, R, _f32x4(coef), out)
+        return [out[i] for i in range(4)]
+
+    def psnr_family_masked(self, p0, p1, msk, bd, bdm, erp=False, lat=180):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        mse, psnr, ws = (C.c_double * 4)(), (C.c_double * 4)(), (C.c_double * 4)()
+        self.L.ref_psnr_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, int(erp), lat, mse, psnr, ws)
+        return list(mse)[:3], list(psnr)[:3], list(ws)[:3]
+
+    def ivpsnr_masked(self, p0, p1, msk, bd, bdm, gcd, R=2, cws=(4, 1, 1, 0), cwa=(4, 1, 1, 0), erp=False, lat=180):
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        v, r2t, t2r = C.c_double(0), C.c_double(0), C.c_double(0)
+        self.L.ref_ivpsnr_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, _i32x4(gcd), R, _i32x4(cws),
+                                 _i32x4(cwa), int(erp), lat, C.byref(v), C.byref(r2t), C.byref(t2r))
         return v.value, r2t.value, t2r.value
 
     def bench_pairs(self, pairs, w, h, bd, chroma420, erp, metrics, threads):

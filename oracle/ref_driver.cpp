@@ -297,6 +297,77 @@ int ref_ivssim_ex(const uint16_t* const p0[3], const uint16_t* const p1[3], cons
 }
 
 
+// Mask mode, as the app runs it (xAppQMIV.cpp:899-906 extends every input
+// picture, the mask included, with Nearest margins and counts the non-zero
+// mask samples; :930, :953, :971, :999, :1026-1029 call the masked
+// processors). The mask is one luma-resolution plane of bit depth bdm.
+namespace {
+struct Masked {
+  xPicP a, b, m;
+  int32 nnm = 0;
+  Masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32 w, int32 h, int32 bd,
+         int32 bdm, int32 margin) {
+    load_pic(a, p0, w, h, bd, margin);
+    load_pic(b, p1, w, h, bd, margin);
+    const uint16_t* mp[3] = {msk, msk, msk};
+    load_pic(m, mp, w, h, bdm, margin);
+    nnm = xPixelOps::CountNonZero(m.getAddr(eCmp::LM), m.getStride(), m.getWidth(), m.getHeight());
+  }
+};
+}  // namespace
+
+int ref_gcd_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w, int32_t h,
+                   int32_t bd, int32_t bdm, int32_t R, const float coef[4], int32_t out[4]) {
+  Masked M(p0, p1, msk, w, h, bd, bdm, margin_for(R));
+  Inline in;
+  xGlobClrDiffProc g;
+  g.setUnntcbCoef(flt32V4(coef[0], coef[1], coef[2], coef[3]));
+  g.bindThrdPoolIntf(&in.tpi);
+  int32V4 r = g.CalcGlobalColorDiffM(&M.a, &M.b, &M.m, M.nnm);
+  for (int c = 0; c < 4; c++) out[c] = r[c];
+  return M.nnm;
+}
+
+int ref_psnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w,
+                    int32_t h, int32_t bd, int32_t bdm, int32_t erp, int32_t lat, double mse[4], double psnr[4],
+                    double wspsnr[4]) {
+  Masked M(p0, p1, msk, w, h, bd, bdm, 4);
+  Inline in;
+  xIVPSNRM p;
+  p.bindThrdPoolIntf(&in.tpi);
+  p.initRowBuffers(h);
+  if (erp) p.initWS(true, w, h, bd, 360, lat);
+  flt64V4 a = p.calcPicMSEM(&M.a, &M.b, &M.m, M.nnm);
+  flt64V4 b = p.calcPicPSNRM(&M.a, &M.b, &M.m, M.nnm);
+  flt64V4 c = p.calcPicWSPSNRM(&M.a, &M.b, &M.m, M.nnm);
+  for (int k = 0; k < 4; k++) { mse[k] = a[k]; psnr[k] = b[k]; wspsnr[k] = c[k]; }
+  return M.nnm;
+}
+
+int ref_ivpsnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w,
+                      int32_t h, int32_t bd, int32_t bdm, const int32_t gcd[4], int32_t R, const int32_t cws[4],
+                      const int32_t cwa[4], int32_t erp, int32_t lat, double* out, double* r2t, double* t2r) {
+  const int32 mg = margin_for(R);
+  Masked M(p0, p1, msk, w, h, bd, bdm, mg);
+  xPicI ai(int32V2(w, h), bd, mg), bi(int32V2(w, h), bd, mg);
+  ai.rearrangeFromPlanar(&M.a);
+  bi.rearrangeFromPlanar(&M.b);
+  Inline in;
+  xIVPSNRM p;
+  p.setSearchRange(R);
+  p.setCmpWeightsSearch(v4(cws));
+  p.setCmpWeightsAverage(v4(cwa));
+  p.bindThrdPoolIntf(&in.tpi);
+  p.initRowBuffers(h);
+  if (erp) p.initWS(true, w, h, bd, 360, lat);
+  double lr = 0, lt = 0;
+  p.setDebugCallbackQAP([&](flt64 x, flt64 y) { lr = x; lt = y; });
+  *out = p.calcPicIVPSNRM(&ai, &bi, &M.m, M.nnm, v4(gcd));
+  if (r2t) *r2t = lr;
+  if (t2r) *t2r = lt;
+  return M.nnm;
+}
+
 // ---------------------------------------------------------------------------
 // CPU baseline: replays the per-frame stage order of xAppQMIV::processAllFrames
 // (xAppQMIV.cpp:761-827) on frames already in memory, with the reference

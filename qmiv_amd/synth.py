@@ -99,3 +99,22 @@ def make_pair(wl: Workload, frame: int, as_u8: bool = False):
     if rc != 0:
         raise ValueError(f"ivqm_synth_pair failed ({rc}) for {wl}")
     return p0, p1
+
+
+def make_mask(width: int, height: int, bit_depth_m: int, seed: int = 0, zero_per_mille: int = 350) -> np.ndarray:
+    """Deterministic weight mask (one uint16 plane, 0 .. 2^bit_depth_m - 1) for mask-mode tests and
+    benches: hashed weights, about zero_per_mille / 1000 zeros, a fully masked band of rows and a
+    band of maximum weights."""
+    y, x = np.mgrid[0:height, 0:width].astype(np.uint64)
+    h = x * np.uint64(2654435761) + y * np.uint64(40503) + np.uint64(seed * 97531 + 1)
+    h ^= h >> np.uint64(13)
+    h *= np.uint64(0x5BD1E995)
+    h ^= h >> np.uint64(15)
+    maxm = (1 << bit_depth_m) - 1
+    m = (h % np.uint64(maxm + 1)).astype(np.uint16)
+    m[((h >> np.uint64(24)) % np.uint64(1000)).astype(np.int64) < zero_per_mille] = 0
+    if height > 24:
+        m[height // 3:height // 3 + 6, :] = 0
+    if width > 24:
+        m[:, width // 5:width // 5 + 4] = maxm
+    return m

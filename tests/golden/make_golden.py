@@ -148,6 +148,41 @@ def make_mode_cases(orc: Oracle):
     return out
 
 
+# Mask mode (apps/QMIV/src/xAppQMIV.cpp:902-906, :930-1029): GCD, MSE / PSNR / WS-PSNR and
+# IV-PSNR with a weight mask (qmiv_amd/synth.py make_mask), bit-exact doubles from the reference.
+MASK_CASES = [
+    ("m96x72_10b_420_m10", synth.Workload(cfg=140, width=96, height=72, global_shift=2), 10, 2),
+    ("m80x64_8b_420_m8_r3", synth.Workload(cfg=141, width=80, height=64, bit_depth=8), 8, 3),
+    ("m72x64_16b_444_m14_wrap", synth.Workload(cfg=142, width=72, height=64, bit_depth=16, chroma_420=False,
+                                               sigma_r=30000, sigma_t=30000, block_shift=2), 14, 2),
+    ("m96x64_10b_420_m12_r4", synth.Workload(cfg=143, width=96, height=64), 12, 4),
+    ("m128x96_12b_444_m8_r1", synth.Workload(cfg=144, width=128, height=96, bit_depth=12, chroma_420=False), 8, 1),
+]
+
+
+def make_mask_cases(ref: Reference, orc: Oracle):
+    out = []
+    for name, wl, bdm, R in MASK_CASES:
+        print("mask case", name, flush=True)
+        p0n, p1n = synth.make_pair(wl, 0)
+        p0, p1 = orc.to444(p0n, wl.chroma_420), orc.to444(p1n, wl.chroma_420)
+        bd = wl.bit_depth
+        m = synth.make_mask(wl.width, wl.height, bdm, seed=wl.cfg)
+        g = ref.gcd_masked(p0, p1, m, bd, bdm, R=R)
+        case = {"name": name, "workload": dataclasses.asdict(wl), "bit_depth_m": bdm, "R": R,
+                "mask_sha1": sha1(m), "nnm": int(np.count_nonzero(m)), "gcd": g}
+        for erp in (False, True):
+            mse, psnr, ws = ref.psnr_family_masked(p0, p1, m, bd, bdm, erp=erp)
+            iv = ref.ivpsnr_masked(p0, p1, m, bd, bdm, g, R=R, erp=erp)
+            sfx = "_erp" if erp else ""
+            case["mse" + sfx] = [f(x) for x in mse]
+            case["psnr" + sfx] = [f(x) for x in psnr]
+            case["wspsnr" + sfx] = [f(x) for x in ws]
+            case["ivpsnr" + sfx] = [f(x) for x in iv]
+        out.append(case)
+    return out
+
+
 def main():
     ref = Reference()
     orc = Oracle()
@@ -162,6 +197,7 @@ def main():
         "ssim_mode_workloads": {k: dataclasses.asdict(v) for k, v in MODE_WORKLOADS.items()},
         "ssim_mode_reference_build": "oracle/Makefile (x86-64-v3), pictures with a margin of 8",
         "ssim_mode_cases": make_mode_cases(orc),
+        "mask_cases": make_mask_cases(ref, orc),
     }
     with open(OUT, "w") as fh:
         json.dump(doc, fh, indent=1)

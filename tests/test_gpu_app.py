@@ -126,3 +126,35 @@ def test_qmiv_front_end_matches_oracle(tmp_path, oracle):
     assert len(res) - 3 == len(avgs)
     for r_line, a_line in zip(res[3:], avgs):
         assert _numbers(r_line) == _numbers(a_line)
+
+
+def test_qmiv_front_end_regular_ssim_mode(tmp_path, oracle):
+    """-ssm / -ssb / -sss reach the library: RegularGaussianInt, Mirror extension, stride 2,
+    per-frame SSIM / MS-SSIM / IV-SSIM lines against the oracle in that mode."""
+    if not os.path.exists(EXE):
+        pytest.fail("front end not built (python -m qmiv_amd.build)")
+    wl = synth.Workload(cfg=411, width=160, height=128)
+    n, bd = 3, 10
+    pairs = [synth.make_pair(wl, i) for i in range(n)]
+    f0, f1 = tmp_path / "a.yuv", tmp_path / "b.yuv"
+    raw = lambda pic: b"".join(np.ascontiguousarray(p, np.uint16).tobytes() for p in pic)
+    f0.write_bytes(b"".join(raw(p[0]) for p in pairs))
+    f1.write_bytes(b"".join(raw(p[1]) for p in pairs))
+    out = subprocess.run([EXE, "-i0", str(f0), "-i1", str(f1), "-ps", f"{wl.width}x{wl.height}", "-bd", str(bd),
+                          "-cf", "420", "-ml", "SSIM, MSSSIM, IVSSIM", "-ssm", "RegularGaussianInt", "-ssb", "Mirror",
+                          "-sss", "2", "-v", "2"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    args = (1, 3, 11, 2)
+    for f, (p0n, p1n) in enumerate(pairs):
+        p0, p1 = oracle.to444(p0n, True), oracle.to444(p1n, True)
+        g = oracle.gcd(p0, p1, bd)
+        s0, s1 = oracle.scp(p0, p1, bd, g)
+        want = {"SSIM": oracle.ssim_ex(p0, p1, bd, *args), "MSSSIM": oracle.ssim_ex(p0, p1, bd, *args, ms=True),
+                "IVSSIM": [oracle.ivssim_ex(p0, p1, s0, s1, bd, *args)[0]]}
+        for name, vals in want.items():
+            tag = " Y:Cb:Cr  " if name != "IVSSIM" else " " * 10
+            line = next(l for l in lines if l.startswith("Frame %08d " % f + "%8s%s " % (name, tag)))
+            got = _numbers(line)
+            assert len(got) == len(vals)
+            assert np.max(np.abs(np.array(got) - np.array(vals))) <= 1.5e-6, (line, vals)

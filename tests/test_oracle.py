@@ -461,3 +461,89 @@ def test_ssim_mode_planted_bugs(golden, oracle):
         vals.append((4 * per[0] + per[1] + per[2]) / 6)
     assert abs(vals[0] - float(c["ivssim_erp"][2])) < 1e-9
     assert abs(vals[1] - float(c["ivssim_erp"][2])) > 1e-9
+
+
+# ---------------------------------------------------------------------------
+# mask mode
+# ---------------------------------------------------------------------------
+def _mask_inputs(case, oracle):
+    w = dict(case["workload"])
+    w["offsets"] = tuple(w["offsets"])
+    wl = synth.Workload(**w)
+    p0n, p1n = synth.make_pair(wl, 0)
+    m = synth.make_mask(wl.width, wl.height, case["bit_depth_m"], seed=wl.cfg)
+    return wl, oracle.to444(p0n, wl.chroma_420), oracle.to444(p1n, wl.chroma_420), m
+
+
+def test_mask_cases_match_reference_golden(golden, oracle):
+    """GCD, MSE / PSNR / WS-PSNR (with and without ERP) and IV-PSNR (value, R2T, T2R) with a weight
+    mask: identical doubles to the reference library (xAppQMIV.cpp:930-1029 call order)."""
+    for c in golden["mask_cases"]:
+        wl, p0, p1, m = _mask_inputs(c, oracle)
+        bd, bdm = wl.bit_depth, c["bit_depth_m"]
+        assert sha1(m) == c["mask_sha1"] and oracle.count_nonzero(m) == c["nnm"]
+        g = oracle.gcd_masked(p0, p1, m, bd, bdm)
+        assert g == c["gcd"], (c["name"], g)
+        for erp in (False, True):
+            sfx = "_erp" if erp else ""
+            mse, psnr, ws = oracle.psnr_family_masked(p0, p1, m, bd, bdm, erp=erp)
+            assert [repr(float(x)) for x in mse] == c["mse" + sfx], c["name"]
+            assert [repr(float(x)) for x in psnr] == c["psnr" + sfx], c["name"]
+            assert [repr(float(x)) for x in ws] == c["wspsnr" + sfx], c["name"]
+            iv = oracle.ivpsnr_masked(p0, p1, m, bd, bdm, g, R=c["R"], erp=erp)
+            assert [repr(float(x)) for x in iv] == c["ivpsnr" + sfx], (c["name"], erp, iv)
+
+
+def test_mask_of_ones_is_the_unmasked_search(oracle):
+    """An all-ones mask at bit depth 1 (MaxM = 1) leaves the search unchanged: the masked rows equal
+    the unmasked row sums where those do not wrap int32 (xCorrespPixelShiftSTD.cpp:151-215 vs the
+    unmasked search)."""
+    wl = synth.Workload(cfg=150, width=64, height=48, chroma_420=False)
+    p0n, p1n = synth.make_pair(wl, 0)
+    p0, p1 = oracle.to444(p0n, False), oracle.to444(p1n, False)
+    g = oracle.gcd(p0, p1, wl.bit_depth)
+    ones = np.ones((wl.height, wl.width), np.uint16)
+    masked = oracle.cps_masked(p0, p1, ones, g, R=2)
+    plain, _, _ = oracle.cps(p0, p1, wl.bit_depth, g, R=2)
+    assert np.array_equal(masked[:, :3].astype(np.uint64), plain[:, :3].astype(np.uint64))
+
+
+def test_masked_search_brute_force(oracle):
+    """numpy brute force of the masked search on a tiny picture: masked test pixels contribute
+    nothing, masked candidates are never chosen, the weight multiplies the winner's squares."""
+    rng = np.random.default_rng(9)
+    h, w, R = 10, 12, 2
+    tst = [rng.integers(0, 1024, (h, w)).astype(np.uint16) for _ in range(3)]
+    ref = [rng.integers(0, 1024, (h, w)).astype(np.uint16) for _ in range(3)]
+    m = rng.integers(0, 4, (h, w)).astype(np.uint16)
+    g, cw = [3, -2, 1, 0], (4, 1, 1, 0)
+    want = np.zeros((h, 4), np.uint64)
+    for y in range(h):
+        for x in range(w):
+            if m[y, x] == 0:
+                continue
+            t = [int(tst[c][y, x]) + g[c] for c in range(3)]
+            best, bxy = None, None
+            for dy in range(-R, R + 1):
+                for dx in range(-R, R + 1):
+                    ry, rx = min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)
+                    if m[ry, rx] == 0:
+                        continue
+                    e = sum(cw[c] * (t[c] - int(ref[c][ry, rx])) ** 2 for c in range(3))
+                    if best is None or e < best:
+                        best, bxy = e, (ry, rx)
+            for c in range(3):
+                want[y, c] += np.uint64((t[c] - int(ref[c][bxy])) ** 2 * int(m[y, x]))
+    assert np.array_equal(oracle.cps_masked(tst, ref, m, g, R=R), want)
+
+
+def test_masked_gcd_closed_form(oracle):
+    """Constant planes: SD = d * sum(M), so GCD = round(d * sum(M) / (NNM * MaxM)) clipped
+    (xGlobClrDiff.cpp:60-91)."""
+    h, w, bdm = 16, 16, 8
+    m = synth.make_mask(w, h, bdm, seed=3)
+    p0 = [np.full((h, w), 500, np.uint16) for _ in range(3)]
+    p1 = [np.full((h, w), 503, np.uint16) for _ in range(3)]
+    want = 3 * int(m.astype(np.int64).sum()) / (np.count_nonzero(m) * 255)
+    g = oracle.gcd_masked(p0, p1, m, 10, bdm)
+    assert g[0] == int(want + 0.5) and g[3] == 0
