@@ -42,6 +42,11 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
 int ivqm_batch_run(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, ivqm_frame_result* results);
 /* Asynchronous over a ring of 8 result slots. */
 int ivqm_batch_enqueue(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, int32_t ring);
+/* As ivqm_batch_enqueue, with one device-resident mask per pair (ivqm_hip.h,
+ * mask mode); masks[i].plane == NULL leaves pair i unmasked. The app's mask
+ * mode (apps/QMIV/src/xAppQMIV.cpp:930-1029) for whole batches. */
+int ivqm_batch_enqueue_masked(ivqm_batch* b, const ivqm_pair* pairs, const ivqm_mask* masks, int32_t n,
+                              uint32_t metrics, int32_t ring);
 int ivqm_batch_collect(ivqm_batch* b, int32_t ring, ivqm_frame_result* results);
 
 #ifdef __cplusplus

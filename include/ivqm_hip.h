@@ -142,6 +142,31 @@ int ivqm_ivssim(ivqm_ctx* ctx, double* ivssim, double* r2t, double* t2r);
 int ivqm_upload_scp(ivqm_ctx* ctx, int32_t idx, const ivqm_picture* host_pic);
 int ivqm_ivmsssim(ivqm_ctx* ctx, double* ivmsssim, double* r2t, double* t2r);
 
+/* ---- mask mode --------------------------------------------------------------
+ * A weight mask, as the QMIV app's InputFileM (apps/QMIV/src/xAppQMIV.cpp:69-71,
+ * :284-288): one plane at luma resolution, samples 0 .. 2^bit_depth - 1
+ * (bit_depth 8..14 as BitDepthM; 1 byte per sample needs bit_depth 8); the
+ * stride is in samples. */
+typedef struct ivqm_mask {
+  const void* plane;
+  int32_t stride;
+  int32_t bytes_per_sample;
+  int32_t bit_depth;
+  int32_t reserved;
+} ivqm_mask;
+/* Binds a device-resident mask to the context (NULL unbinds). With a mask
+ * bound, ivqm_gcd / ivqm_ssd / ivqm_mse / ivqm_psnr / ivqm_wspsnr / ivqm_ivpsnr
+ * and ivqm_frame run the masked variants the app uses
+ * (xGlobClrDiffProc::CalcGlobalColorDiffM, xGlobClrDiff.h:42; xPSNR::calcPicMSEM /
+ * calcPicPSNRM, xPSNR.h:37-39; xWSPSNR::calcPicWSPSNRM, xWSPSNR.h:23;
+ * xIVPSNRM::calcPicIVPSNRM, xIVPSNR.h:55-56), ssd[] holds the masked SSD, and
+ * the structural-similarity metrics and SCP generation return IVQM_EINVAL (the
+ * app rejects that combination, xAppQMIV.cpp:360). A mask without any non-zero
+ * sample makes the metric calls return IVQM_EINVAL. */
+int ivqm_bind_mask(ivqm_ctx* ctx, const ivqm_mask* dev_mask);
+/* Copies a host mask into the context's own buffer, then binds it. */
+int ivqm_upload_mask(ivqm_ctx* ctx, const ivqm_mask* host_mask);
+
 /* ---- fused frame pass ----------------------------------------------------- */
 /* All requested metrics of the bound pair in one device pass (stats, GCD,
  * search with fused SCP, SSIM jobs, FP64 totals) and one result download;

@@ -268,6 +268,14 @@ class Oracle:
         m = np.ascontiguousarray(m, np.uint16)
         return self.L.or_count_nonzero(m.ctypes.data_as(_u16p), C.c_int64(m.size))
 
+    def weighted_ssd(self, a, b, msk):
+        a = np.ascontiguousarray(a, np.uint16)
+        b = np.ascontiguousarray(b, np.uint16)
+        m = np.ascontiguousarray(msk, np.uint16)
+        self.L.or_weighted_ssd.restype = C.c_uint64
+        return self.L.or_weighted_ssd(a.ctypes.data_as(_u16p), b.ctypes.data_as(_u16p), m.ctypes.data_as(_u16p),
+                                      C.c_int64(m.size))
+
     def gcd_masked(self, p0, p1, msk, bd, bdm, coef=(0.01, 0.01, 0.01, 0.0)):
         a, pa = _planes(p0)
         b, pb = _planes(p1)

@@ -28,6 +28,7 @@ EXPORTED = [
     "ivqm_gen_scp", "ivqm_ssim", "ivqm_msssim", "ivqm_ivssim", "ivqm_ivmsssim", "ivqm_frame",
     "ivqm_frame_enqueue", "ivqm_frame_collect", "ivqm_get_rows", "ivqm_get_scp", "ivqm_enable_best_map",
     "ivqm_get_best_map", "ivqm_upsample420", "ivqm_downsample", "ivqm_profile", "ivqm_upload_scp",
+    "ivqm_bind_mask", "ivqm_upload_mask", "ivqm_batch_enqueue_masked",
     "ivqm_batch_create", "ivqm_batch_destroy", "ivqm_batch_set_stream", "ivqm_batch_set_tuning",
     "ivqm_batch_get_tuning", "ivqm_batch_autotune", "ivqm_batch_run", "ivqm_batch_enqueue", "ivqm_batch_collect",
     "ivqm_batch_profile",
@@ -53,6 +54,11 @@ class Config(C.Structure):
 class Picture(C.Structure):
     _fields_ = [("plane", C.c_void_p * 3), ("stride", C.c_int32 * 3), ("chroma_420", C.c_int32),
                 ("bytes_per_sample", C.c_int32)]
+
+
+class Mask(C.Structure):
+    _fields_ = [("plane", C.c_void_p), ("stride", C.c_int32), ("bytes_per_sample", C.c_int32),
+                ("bit_depth", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Pair(C.Structure):
@@ -165,6 +171,10 @@ def load():
     L.ivqm_batch_run.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.POINTER(FrameResult)]
     L.ivqm_batch_enqueue.argtypes = [C.c_void_p, C.POINTER(Pair), C.c_int32, C.c_uint32, C.c_int32]
     L.ivqm_batch_collect.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FrameResult)]
+    L.ivqm_bind_mask.argtypes = [C.c_void_p, C.POINTER(Mask)]
+    L.ivqm_upload_mask.argtypes = [C.c_void_p, C.POINTER(Mask)]
+    L.ivqm_batch_enqueue_masked.argtypes = [C.c_void_p, C.POINTER(Pair), C.POINTER(Mask), C.c_int32, C.c_uint32,
+                                            C.c_int32]
     L.ivqm_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.ivqm_server_config_init.argtypes = [C.POINTER(ServerConfig)]
     L.ivqm_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]

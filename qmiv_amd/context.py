@@ -93,6 +93,23 @@ class IvqmContext:
         self.upload(0, p0, chroma_420)
         self.upload(1, p1, chroma_420)
 
+    # -- mask mode -------------------------------------------------------------
+    def upload_mask(self, plane, bit_depth):
+        """plane: 2-D numpy uint8 / uint16 weights at luma resolution."""
+        a = np.ascontiguousarray(plane)
+        m = _lib.Mask(a.ctypes.data, a.shape[1], a.dtype.itemsize, bit_depth, 0)
+        check(self.L.ivqm_upload_mask(self.h, C.byref(m)))
+
+    def bind_mask(self, plane, bit_depth=8):
+        """plane: 2-D torch CUDA tensor (int16 / uint8 storage of the weights), or None to unbind."""
+        if plane is None:
+            self._mask = None
+            check(self.L.ivqm_bind_mask(self.h, None))
+            return
+        m = _lib.Mask(plane.data_ptr(), plane.stride(0), plane.element_size(), bit_depth, 0)
+        self._mask = plane
+        check(self.L.ivqm_bind_mask(self.h, C.byref(m)))
+
     # -- per-metric calls ---------------------------------------------------------
     def gcd(self):
         g = (C.c_int32 * 4)()
@@ -225,6 +242,21 @@ class IvqmBatch:
             arr[i].p1 = _picture_from_torch(p1, c420)
             arr[i].width, arr[i].height, arr[i].bit_depth = w, h, bd
         return arr
+
+    @staticmethod
+    def make_masks(items):
+        """items: list of (torch CUDA 2-D plane or None, bit_depth)."""
+        arr = (_lib.Mask * len(items))()
+        for i, (t, bd) in enumerate(items):
+            if t is not None:
+                arr[i] = _lib.Mask(t.data_ptr(), t.stride(0), t.element_size(), bd, 0)
+        return arr
+
+    def run_masked(self, pairs, masks, metrics):
+        res = (FrameResult * len(pairs))()
+        check(self.L.ivqm_batch_enqueue_masked(self.h, pairs, masks, len(pairs), metrics, 0))
+        check(self.L.ivqm_batch_collect(self.h, 0, res))
+        return list(res)
 
     def set_tuning(self, search_tile_rows=0, ssim_band_rows=0, use_graph=False):
         check(self.L.ivqm_batch_set_tuning(self.h, search_tile_rows, ssim_band_rows, int(use_graph)))

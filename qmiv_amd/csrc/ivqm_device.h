@@ -38,6 +38,15 @@ struct PairDev {
   int32_t res_idx;        // index of gcd[4], sd[3], ssd[3]
   int32_t gcd_in[4];      // the caller's GCD when gcd_given
   int32_t pad;
+  // Mask mode (weights 0 .. msk_max at luma resolution), or msk == null:
+  const void* msk;
+  int32_t msk_stride;     // in samples
+  int32_t msk_bps;        // 1 or 2
+  int32_t msk_max;        // 2^BitDepthM - 1
+  int32_t mtile0;         // first k_search_masked tile of this pair
+  int64_t aux_off;        // [4][h] int64: samples that differ per 4:4:4 row of each component; non-zero
+                          // mask samples per row
+  int64_t rows64_off;     // [2][3][h] uint64 masked search row sums (direction 0: P0 + GCD searched in P1)
 };
 
 // One SSIM evaluation (a picture pair at one pyramid level).
@@ -76,10 +85,17 @@ struct DownJob {
 
 // Launchers (ivqm_kernels.hip). All are asynchronous on `st`; tables live in
 // device memory. `nblk` is the total block count of the flattened grid.
+// Masked pairs take 3 x h stats rows (every component at 4:4:4), unmasked pairs h + 2 x (h >> cs).
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* rows, hipStream_t st);
+                        uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st);
+// mcount: per pair {non-zero mask samples, differing samples of Y, Cb, Cr} (masked pairs only).
 hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
-                               int32_t* gcd, int64_t* sd, uint64_t* ssd, hipStream_t st);
+                               const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,
+                               hipStream_t st);
+// Masked search of both directions: tiles of 64 x 4 pixels per direction (ivqm_masked.hip).
+constexpr int kMaskTileW = 64, kMaskTileH = 4;
+hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
+                                const int32_t* gcd, uint64_t* rows64, hipStream_t st);
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st);

@@ -37,6 +37,9 @@ struct PairSpec {
   bool has_gcd = false;
   int32_t gcd[4] = {0, 0, 0, 0};
   uint32_t metrics = 0;  // this pair's IVQM_M_* set; 0 = the batch's set
+  // mask mode: weight plane at luma resolution (null = no mask)
+  const void* msk = nullptr;
+  int msk_stride = 0, msk_bps = 0, msk_bd = 0;
 };
 
 // Per-pair bookkeeping the epilogue needs.
@@ -48,6 +51,9 @@ struct PairPlan {
   int ssim = -1, iv_t2r = -1, iv_r2t = -1;
   int ms[5] = {-1, -1, -1, -1, -1}, ivms_t2r[5] = {-1, -1, -1, -1, -1}, ivms_r2t[5] = {-1, -1, -1, -1, -1};
   int erp_h = -1;  // height whose ERP weights apply
+  bool masked = false;
+  int bdm = 0;            // mask bit depth
+  int64_t rows64_off = 0;
 };
 
 // A picture of a plan: a caller's picture, or three planes in one of the
@@ -72,6 +78,9 @@ struct BatchPlan {
   std::vector<PicRef> down_src[5], down_dst[5];
   int stat_blocks = 0, tiles = 0, ssim_blocks = 0, down_blocks[5] = {0, 0, 0, 0, 0};
   bool ssim_gen = false;  // SSIM jobs run k_ssim_gen (any configuration but block-averaged 8 / 4)
+  bool any_mask = false;
+  int mtiles = 0;                              // k_search_masked tiles
+  int64_t aux_total = 0, rows64_total = 0;     // masked pairs' aux rows and uint64 search rows
   int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
   int py = 8, nb = 8;
   std::string signature;
@@ -88,7 +97,9 @@ struct RingSlot {
   uint64_t* r_row_ssd = nullptr;
   uint32_t* r_rows = nullptr;
   double* r_ssim = nullptr;
-  int64_t c_pairs = 0, c_rows = 0, c_rowsd = 0, c_jobs = 0, c_down = 0;  // capacities of the buffers above
+  int64_t* r_mcount = nullptr;   // per pair: non-zero mask samples, differing samples of Y, Cb, Cr
+  uint64_t* r_rows64 = nullptr;  // masked search rows
+  int64_t c_pairs = 0, c_rows = 0, c_rowsd = 0, c_jobs = 0, c_down = 0, c_rows64 = 0;  // capacities of the above
   hipEvent_t ev = nullptr;
   bool busy = false;
   BatchPlan plan;
@@ -148,6 +159,7 @@ class Engine {
   int materialize(const PicRef& r, PicDev& out) const;
   void drop_graphs();
   int erp_offset(int h, int64_t* off);
+  int masked_epilogue(const RingSlot& S, const PairPlan& pp, int i, ivqm_frame_result& r);
   int search_mode(int maxv, int32_t gcd_abs_max) const;
 
   ivqm_config cfg_;
@@ -177,6 +189,10 @@ class Engine {
   double* d_ssim_out_ = nullptr;
   double* d_erp_ = nullptr;
   double* d_wts_ = nullptr;  // SSIM window weights of the configured mode (Gaussian modes)
+  int64_t* d_aux_ = nullptr;      // masked stats: differing / non-zero-mask samples per row
+  int64_t* d_mcount_ = nullptr;   // 4 per pair
+  uint64_t* d_rows64_ = nullptr;  // masked search rows
+  int64_t cap_aux_ = 0, cap_rows64_ = 0;
   int64_t erp_used_ = 0, erp_cap_ = 0;
   std::map<int, int64_t> erp_off_;
 

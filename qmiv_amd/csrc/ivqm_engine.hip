@@ -138,12 +138,13 @@ Engine::~Engine() {
   for (auto& S : ring_) {
     if (S.graph) (void)hipGraphExecDestroy(S.graph);
     if (S.ev) (void)hipEventDestroy(S.ev);
-    void* hs[] = {S.t_pairs, S.t_jobs, S.t_down, S.r_gcd, S.r_sd, S.r_ssd, S.r_row_ssd, S.r_rows, S.r_ssim};
+    void* hs[] = {S.t_pairs, S.t_jobs, S.t_down, S.r_gcd, S.r_sd, S.r_ssd, S.r_row_ssd, S.r_rows, S.r_ssim, S.r_mcount,
+                  S.r_rows64};
     for (void* h : hs)
       if (h) (void)hipHostFree(h);
   }
   void* ds[] = {d_pairs_, d_jobs_, d_down_, d_row_sd_, d_row_ssd_, d_rows_, d_gcd_, d_sd_, d_ssd_,
-                d_scp_,   d_best_, d_pyr_,  d_partial_, d_ssim_out_, d_erp_, d_wts_};
+                d_scp_,   d_best_, d_pyr_,  d_partial_, d_ssim_out_, d_erp_, d_wts_, d_aux_, d_mcount_, d_rows64_};
   for (void* d : ds)
     if (d) (void)hipFree(d);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -298,11 +299,33 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.gcd_given = s.has_gcd ? 1 : 0;
     for (int k = 0; k < 4; k++) d.gcd_in[k] = s.has_gcd ? s.gcd[k] : 0;
     d.res_idx = i;
+    if (s.msk) {  // mask mode (xAppQMIV.cpp:360-365: no structural-similarity metric with a mask)
+      if (mp & (IVQM_M_SSIM | IVQM_M_MSSSIM | ivs))
+        return set_error(IVQM_EINVAL, "structural similarity metrics cannot be combined with a mask");
+      if (scp_only) return set_error(IVQM_EINVAL, "shift-compensated pictures are not generated with a mask");
+      d.msk = s.msk;
+      d.msk_stride = s.msk_stride;
+      d.msk_bps = s.msk_bps;
+      d.msk_max = (1 << s.msk_bd) - 1;
+      d.aux_off = P.aux_total;
+      P.aux_total += 4 * (int64_t)s.h;
+      d.rows64_off = P.rows64_total;
+      pp.rows64_off = d.rows64_off;
+      P.rows64_total += 6 * (int64_t)s.h;
+      pp.masked = true;
+      pp.bdm = s.msk_bd;
+      P.any_mask = true;
+    }
     d.stat_blk0 = P.stat_blocks;
-    P.stat_blocks += s.h + 2 * (s.h >> s.p0.cs);
+    P.stat_blocks += s.msk ? 3 * s.h : s.h + 2 * (s.h >> s.p0.cs);  // masked: every component at 4:4:4
     d.tile0 = P.tiles;  // pairs without a search own no tiles (the table lookup skips them)
-    if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only)))
+    d.mtile0 = P.mtiles;
+    if (s.msk) {
+      if (mp & IVQM_M_IVPSNR)
+        P.mtiles += 2 * ((s.w + kMaskTileW - 1) / kMaskTileW) * ((s.h + kMaskTileH - 1) / kMaskTileH);
+    } else if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only))) {
       P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
+    }
     d.rowsd_off = P.rowsd_total;
     pp.rowsd_off = d.rowsd_off;
     P.rowsd_total += 3 * (int64_t)s.h;
@@ -328,7 +351,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.best_off = (best_on_ && n == 1) ? 0 : -1;
     pp.ns = ms_num_scales(s.w, s.h);
     if (cfg_.erp) pp.erp_h = s.h;
-    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps, mp);
+    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x:%d|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps, mp,
+                  s.msk ? s.msk_bps : 0);
     P.signature += sigbuf;
   }
   // pyramid offsets: per pair, pictures 0..3 (P0, P1, SCP0, SCP1), levels 1..ns-1
@@ -489,7 +513,8 @@ int Engine::ensure_capacity(const BatchPlan& P) {
   for (int l = 1; l < 5; l++) ndown += (int64_t)P.down[l].size();
   const bool need = P.n > cap_pairs_ || P.rows_total > cap_rows_ || P.rowsd_total > cap_rowsd_ ||
                     P.scp_total > cap_scp_ || P.pyr_total > cap_pyr_ || P.partial_total > cap_partial_ ||
-                    njobs > cap_jobs_ || ndown > cap_down_;
+                    njobs > cap_jobs_ || ndown > cap_down_ || P.aux_total > cap_aux_ ||
+                    P.rows64_total > cap_rows64_;
   const int64_t best_need = best_on_ && P.n == 1 ? 2 * (int64_t)P.pp[0].w * P.pp[0].h : 0;
   if (!need && best_need <= cap_best_) return IVQM_OK;
   RCCHK(sync());
@@ -499,6 +524,12 @@ int Engine::ensure_capacity(const BatchPlan& P) {
     RCCHK(dmalloc(&d_gcd_, 4 * cap_pairs_));
     RCCHK(dmalloc(&d_sd_, 3 * cap_pairs_));
     RCCHK(dmalloc(&d_ssd_, 3 * cap_pairs_));
+    RCCHK(dmalloc(&d_mcount_, 4 * cap_pairs_));
+  }
+  if (P.aux_total > cap_aux_) { cap_aux_ = grow(P.aux_total, cap_aux_); RCCHK(dmalloc(&d_aux_, cap_aux_)); }
+  if (P.rows64_total > cap_rows64_) {
+    cap_rows64_ = grow(P.rows64_total, cap_rows64_);
+    RCCHK(dmalloc(&d_rows64_, cap_rows64_));
   }
   if (P.rows_total > cap_rows_) { cap_rows_ = grow(P.rows_total, cap_rows_); RCCHK(dmalloc(&d_rows_, cap_rows_)); }
   if (P.rowsd_total > cap_rowsd_) {
@@ -537,6 +568,12 @@ int Engine::ensure_slot_capacity(RingSlot& S, const BatchPlan& P) {
     RCCHK(hmalloc(&S.r_gcd, 4 * S.c_pairs));
     RCCHK(hmalloc(&S.r_sd, 3 * S.c_pairs));
     RCCHK(hmalloc(&S.r_ssd, 3 * S.c_pairs));
+    RCCHK(hmalloc(&S.r_mcount, 4 * S.c_pairs));
+    moved = true;
+  }
+  if (P.rows64_total > S.c_rows64 || !S.r_rows64) {
+    S.c_rows64 = grow(std::max<int64_t>(P.rows64_total, 1), S.c_rows64);
+    RCCHK(hmalloc(&S.r_rows64, S.c_rows64));
     moved = true;
   }
   if (nj > S.c_jobs || !S.t_jobs) {
@@ -625,9 +662,12 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
   if (nj) HIPCHK(hipMemcpyAsync(d_jobs_, S.t_jobs, sizeof(SsimJobDev) * nj, hipMemcpyHostToDevice, st));
   if (nd) HIPCHK(hipMemcpyAsync(d_down_, S.t_down, sizeof(DownJob) * nd, hipMemcpyHostToDevice, st));
   if (P.need_stats) {
-    HIPCHK(launch_stats(d_pairs_, n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, st));
-    HIPCHK(launch_gcd_finalize(d_pairs_, n, d_row_sd_, d_row_ssd_, d_gcd_, d_sd_, d_ssd_, st));
+    HIPCHK(launch_stats(d_pairs_, n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, st));
+    HIPCHK(launch_gcd_finalize(d_pairs_, n, d_row_sd_, d_row_ssd_, d_aux_, d_gcd_, d_sd_, d_ssd_, d_mcount_, st));
   }
+  if (P.mtiles)
+    HIPCHK(launch_search_masked(d_pairs_, n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows64_,
+                                st));
   if (P.need_search)
     HIPCHK(launch_search(d_pairs_, n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
                          d_scp_, d_best_, st));
@@ -651,6 +691,10 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
   }
   if (P.metrics & IVQM_M_IVPSNR)
     HIPCHK(hipMemcpyAsync(S.r_rows, d_rows_, sizeof(uint32_t) * P.rows_total, hipMemcpyDeviceToHost, st));
+  if (P.any_mask && P.need_stats)
+    HIPCHK(hipMemcpyAsync(S.r_mcount, d_mcount_, sizeof(int64_t) * 4 * n, hipMemcpyDeviceToHost, st));
+  if (P.mtiles)
+    HIPCHK(hipMemcpyAsync(S.r_rows64, d_rows64_, sizeof(uint64_t) * P.rows64_total, hipMemcpyDeviceToHost, st));
   if (nj) HIPCHK(hipMemcpyAsync(S.r_ssim, d_ssim_out_, sizeof(double) * 6 * nj, hipMemcpyDeviceToHost, st));
   return IVQM_OK;
 }
@@ -737,6 +781,58 @@ int Engine::query(int ring) {
   return hip_error(e, "hipEventQuery");
 }
 
+// Mask mode as the app computes it (xAppQMIV.cpp:948-1029): the result's ssd[] holds the masked
+// SSD (sum of (P0 - P1)^2 x mask); MSE = SSDM / (NNM x MaxM); PSNR uses the mask's bit depth for
+// the picture peak, as calcPicPSNRM passes it (xPSNR.cpp:60), and so does WS-PSNR without ERP
+// (xWSPSNR.cpp:61-64); WS-PSNR with ERP and IV-PSNR use the picture's (xWSPSNR.cpp:128-133,
+// xIVPSNR.cpp:190). Exact components get the app's fake PSNR. IV-PSNR labels follow
+// calcPicIVPSNRM (xIVPSNR.cpp:161-162): R2T is P1 - GCD searched in P0.
+int Engine::masked_epilogue(const RingSlot& S, const PairPlan& pp, int i, ivqm_frame_result& r) {
+  const uint32_t m = pp.m;
+  const int H = pp.h, bd = pp.bd, bdm = pp.bdm;
+  const int64_t area = (int64_t)pp.w * H;
+  const int32_t* cwa = cfg_.cmp_weights_average;
+  const int64_t nnm = S.r_mcount[4 * i];
+  if (nnm <= 0) return set_error(IVQM_EINVAL, "the mask has no non-zero sample");
+  const int64_t maxm = (1 << bdm) - 1;
+  const double fake = host::psnr_from_ssd(1.0, area, bd);
+  if (m & (IVQM_M_PSNR | IVQM_M_MSE)) {
+    for (int k = 0; k < 4; k++) { r.mse[k] = DBL_MAX; r.psnr[k] = DBL_MAX; }
+    for (int k = 0; k < 3; k++) {
+      const double ssdm = (double)r.ssd[k];
+      r.mse[k] = ssdm / (double)(nnm * maxm);
+      r.psnr[k] = host::psnr_masked(ssdm, nnm, bdm, bdm);
+      if (cfg_.fake_vals_for_exact && r.exact[k]) r.psnr[k] = fake;
+    }
+    r.psnr_pic = host::pic_value(r.psnr, cwa);
+  }
+  if (m & IVQM_M_WSPSNR) {
+    for (int k = 0; k < 4; k++) r.wspsnr[k] = DBL_MAX;
+    if (!cfg_.erp) {
+      for (int k = 0; k < 3; k++) r.wspsnr[k] = host::psnr_masked((double)r.ssd[k], nnm, bdm, bdm);
+    } else {
+      const auto& hw = erp_weights(H);
+      const uint64_t* rs = S.r_row_ssd + pp.rowsd_off;  // masked rows are 4:4:4 rows of every component
+      for (int k = 0; k < 3; k++) {
+        host::Kbns kb;
+        for (int y = 0; y < H; y++) kb.acc((double)rs[(size_t)k * H + y] * hw.first[y]);
+        r.wspsnr[k] = host::psnr_masked(kb.result() * hw.second, nnm, bd, bdm);
+      }
+    }
+    for (int k = 0; k < 3; k++)
+      if (cfg_.fake_vals_for_exact && r.exact[k]) r.wspsnr[k] = fake;
+    r.wspsnr_pic = host::pic_value(r.wspsnr, cwa);
+  }
+  if (m & IVQM_M_IVPSNR) {
+    const uint64_t* rows = S.r_rows64 + pp.rows64_off;
+    const double* ew = cfg_.erp ? erp_weights(H).first.data() : nullptr;
+    r.ivpsnr_t2r = host::ivpsnr_quality_masked(rows, H, nnm, bd, bdm, ew, cwa);                   // P0 + GCD in P1
+    r.ivpsnr_r2t = host::ivpsnr_quality_masked(rows + (size_t)3 * H, H, nnm, bd, bdm, ew, cwa);  // P1 - GCD in P0
+    r.ivpsnr = std::min(r.ivpsnr_r2t, r.ivpsnr_t2r);
+  }
+  return IVQM_OK;
+}
+
 int Engine::collect(int ring, ivqm_frame_result* out) {
   if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
   RingSlot& S = ring_[ring];
@@ -757,10 +853,15 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
       for (int k = 0; k < 4; k++) r.gcd_r2t[k] = S.r_gcd[4 * i + k];
       for (int k = 0; k < 3; k++) {
         r.ssd[k] = S.r_ssd[3 * i + k];
-        r.exact[k] = r.ssd[k] == 0;
+        r.exact[k] = pp.masked ? S.r_mcount[4 * i + 1 + k] == 0 : r.ssd[k] == 0;
       }
     }
     const double fake = host::psnr_from_ssd(1.0, area, bd);
+    if (pp.masked) {
+      RCCHK(masked_epilogue(S, pp, i, r));
+      if (out) out[i] = r;
+      continue;
+    }
     if (m & (IVQM_M_PSNR | IVQM_M_MSE)) {
       // libs/pmbb-ivqm/src/xPSNR.cpp:131-151; exact components -> fake PSNR (xAppQMIV.cpp:974-981)
       for (int k = 0; k < 4; k++) { r.mse[k] = DBL_MAX; r.psnr[k] = DBL_MAX; }
@@ -848,7 +949,9 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
   const int nj = (int)P.jobs.size();
   for (int i = 0; i < iters; i++) {
     switch (kernel) {
-      case 0: HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, stream_)); break;
+      case 0:
+        HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, stream_));
+        break;
       case 1:
         HIPCHK(launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
                              d_scp_, d_best_, stream_));

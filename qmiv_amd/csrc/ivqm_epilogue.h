@@ -89,5 +89,34 @@ inline double ivpsnr_quality(const uint32_t* rows, int w, int h, int bit_depth, 
   return weighted_mean4(q, cw_avg);
 }
 
+// Masked PSNR: libs/pmbb-ivqm/src/xPSNR.cpp:164-179 / :199-206,
+// MAX = NumNonMasked * MaxPic^2 * MaxMsk in int64.
+inline double psnr_masked(double ssdm, int64_t nnm, int bit_depth_pic, int bit_depth_msk) {
+  const int64_t maxp = (1 << bit_depth_pic) - 1, maxm = (1 << bit_depth_msk) - 1;
+  const int64_t MAX = nnm * (maxp * maxp) * maxm;
+  return ssdm > 0 ? 10.0 * log10((double)MAX / ssdm) : DBL_MAX;
+}
+
+// Masked asymmetric IV-PSNR quality from uint64 rows [3][h]:
+// libs/pmbb-ivqm/src/xIVPSNR.cpp:172-199.
+inline double ivpsnr_quality_masked(const uint64_t* rows, int h, int64_t nnm, int bit_depth, int bit_depth_msk,
+                                    const double* erp_w, const int32_t cw_avg[4]) {
+  double err[4] = {0, 0, 0, 0};
+  if (erp_w) {
+    Kbns k[3];
+    for (int y = 0; y < h; y++)
+      for (int c = 0; c < 3; c++) k[c].acc((double)rows[(size_t)c * h + y] * erp_w[y]);
+    for (int c = 0; c < 3; c++) err[c] = k[c].result();
+  } else {
+    uint64_t s[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++)
+      for (int y = 0; y < h; y++) s[c] += rows[(size_t)c * h + y];
+    for (int c = 0; c < 3; c++) err[c] = (double)s[c];
+  }
+  double q[4] = {0, 0, 0, 0};
+  for (int c = 0; c < 3; c++) q[c] = psnr_masked(err[c] > 0 ? err[c] : 1.0, nnm, bit_depth, bit_depth_msk);
+  return weighted_mean4(q, cw_avg);
+}
+
 }  // namespace host
 }  // namespace ivqm

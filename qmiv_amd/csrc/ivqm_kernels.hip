@@ -147,17 +147,68 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
   }
 }
 
-// nrows = sum over pairs of H + 2 * Hc; a pair's rows start at stat_blk0.
+// Mask mode: one 4:4:4 row of component c (chroma of 4:2:0 read at (x >> 1, y >> 1)), weighted by
+// the mask at (x, y). xDistortionSTD::CalcWeightedSD / CalcWeightedSSD
+// (libs/pmbb-core/src/xDistortionSTD.cpp:45-68): sum of d * M with the product in int32 (exact for
+// masks up to 14 bits), and sum of (uint64)(int32 square of d) * M. aux gets the samples that
+// differ (exact components) and, for luma, the non-zero mask samples (xPixelOps::CountNonZero).
+template <typename T, typename TM>
+__device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64_t* __restrict__ row_sd,
+                                 uint64_t* __restrict__ row_ssd, int64_t* __restrict__ aux) {
+  const int W = P.w, H = P.h;
+  const int cs = c ? P.p0.cs : 0;
+  const T* a = (const T*)plane_ptr(P.p0, c) + (size_t)(y >> cs) * plane_stride(P.p0, c);
+  const T* r = (const T*)plane_ptr(P.p1, c) + (size_t)(y >> cs) * plane_stride(P.p1, c);
+  const TM* m = (const TM*)P.msk + (size_t)y * P.msk_stride;
+  int64_t sd = 0, nd = 0, nz = 0;
+  uint64_t ssd = 0;
+  for (int x = lane; x < W; x += 64) {
+    const int d = (int)r[x >> cs] - (int)a[x >> cs];
+    const int w = (int)m[x];
+    sd += (int64_t)(d * w);
+    ssd += (uint64_t)(int64_t)(int32_t)((uint32_t)d * (uint32_t)d) * (uint64_t)w;
+    nd += d != 0;
+    nz += w != 0;
+  }
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) {
+    sd += __shfl_xor(sd, k, 64);
+    ssd += __shfl_xor(ssd, k, 64);
+    nd += __shfl_xor(nd, k, 64);
+    nz += __shfl_xor(nz, k, 64);
+  }
+  if (lane == 0) {
+    row_sd[P.rowsd_off + (int64_t)c * H + y] = sd;
+    row_ssd[P.rowsd_off + (int64_t)c * H + y] = ssd;
+    aux[P.aux_off + (int64_t)c * H + y] = nd;
+    if (c == 0) aux[P.aux_off + 3 * (int64_t)H + y] = nz;
+  }
+}
+
+// nrows = sum over pairs of H + 2 * Hc (masked pairs: 3 H); a pair's rows start at stat_blk0.
 // Each luma row y also clears the six search row sums of y.
 __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs, int nrows,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
-                                               uint32_t* __restrict__ rows) {
+                                               uint32_t* __restrict__ rows, int64_t* __restrict__ aux,
+                                               uint64_t* __restrict__ rows64) {
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * 4;
   for (int rw = blockIdx.x * 4 + (threadIdx.x >> 6); rw < nrows; rw += nw) {
     const int pi = find_entry(pairs, npairs, rw, [](const PairDev& p) { return p.stat_blk0; });
     const PairDev& P = pairs[pi];
     const int lb = rw - P.stat_blk0;
+    if (P.msk) {
+      const int c = lb / P.h, y = lb - c * P.h;
+      if (c == 0 && lane < 6) rows64[P.rows64_off + (int64_t)lane * P.h + y] = 0u;
+      if (P.p0.bps == 2) {
+        if (P.msk_bps == 2) stats_row_masked<uint16_t, uint16_t>(P, c, y, lane, row_sd, row_ssd, aux);
+        else stats_row_masked<uint16_t, uint8_t>(P, c, y, lane, row_sd, row_ssd, aux);
+      } else {
+        if (P.msk_bps == 2) stats_row_masked<uint8_t, uint16_t>(P, c, y, lane, row_sd, row_ssd, aux);
+        else stats_row_masked<uint8_t, uint8_t>(P, c, y, lane, row_sd, row_ssd, aux);
+      }
+      continue;
+    }
     if (lb < P.h && lane < 6) rows[P.rows_off + (int64_t)lane * P.h + lb] = 0u;
     if (P.p0.bps == 2) stats_row<uint16_t>(P, lb, lane, row_sd, row_ssd);
     else stats_row<uint8_t>(P, lb, lane, row_sd, row_ssd);
@@ -168,31 +219,66 @@ __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs
 // k_gcd_finalize: one block per pair. libs/pmbb-ivqm/src/xGlobClrDiff.cpp:23-56
 // with the app's argument order (sum of P1 - P0); chroma sums of 4:2:0 x4.
 // ---------------------------------------------------------------------------
+// Mask mode: xGlobClrDiff::CalcGlobalColorDiffM (xGlobClrDiff.cpp:60-91): mean = SD / (NNM * MaxM);
+// the row sums are already 4:4:4 totals; mcount receives NNM and the differing-sample counts.
 __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict__ pairs,
                                                       const int64_t* __restrict__ row_sd,
-                                                      const uint64_t* __restrict__ row_ssd, int32_t* __restrict__ gcd,
-                                                      int64_t* __restrict__ sd_out, uint64_t* __restrict__ ssd_out) {
+                                                      const uint64_t* __restrict__ row_ssd,
+                                                      const int64_t* __restrict__ aux, int32_t* __restrict__ gcd,
+                                                      int64_t* __restrict__ sd_out, uint64_t* __restrict__ ssd_out,
+                                                      int64_t* __restrict__ mcount) {
   const PairDev P = pairs[blockIdx.x];
-  __shared__ int64_t s_sd[4];
+  __shared__ int64_t s_sd[4], s_nd[4], s_nz[4];
   __shared__ uint64_t s_ssd[4];
   const int tid = threadIdx.x;
   const int W = P.w, H = P.h, cs = P.p0.cs;
+  const bool masked = P.msk != nullptr;
+  int64_t nnm = 0;
+  if (masked) {  // non-zero mask samples
+    int64_t nz = 0;
+    for (int y = tid; y < H; y += 256) nz += aux[P.aux_off + 3 * (int64_t)H + y];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nz += __shfl_xor(nz, m, 64);
+    if ((tid & 63) == 0) s_nz[tid >> 6] = nz;
+    __syncthreads();
+    nnm = s_nz[0] + s_nz[1] + s_nz[2] + s_nz[3];
+    if (tid == 0) mcount[4 * P.res_idx] = nnm;
+  }
   for (int c = 0; c < 3; c++) {
-    const int rows = c ? (H >> cs) : H;
-    int64_t sd = 0;
+    const int rows = (c && !masked) ? (H >> cs) : H;
+    int64_t sd = 0, nd = 0;
     uint64_t ssd = 0;
     for (int y = tid; y < rows; y += 256) {
       sd += row_sd[P.rowsd_off + (int64_t)c * H + y];
       ssd += row_ssd[P.rowsd_off + (int64_t)c * H + y];
+      if (masked) nd += aux[P.aux_off + (int64_t)c * H + y];
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
       sd += __shfl_xor(sd, m, 64);
       ssd += __shfl_xor(ssd, m, 64);
+      nd += __shfl_xor(nd, m, 64);
     }
-    if ((tid & 63) == 0) { s_sd[tid >> 6] = sd; s_ssd[tid >> 6] = ssd; }
+    if ((tid & 63) == 0) { s_sd[tid >> 6] = sd; s_ssd[tid >> 6] = ssd; s_nd[tid >> 6] = nd; }
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0 && masked) {
+      const int64_t t = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
+      sd_out[3 * P.res_idx + c] = t;
+      ssd_out[3 * P.res_idx + c] = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
+      mcount[4 * P.res_idx + 1 + c] = s_nd[0] + s_nd[1] + s_nd[2] + s_nd[3];
+      if (P.gcd_given) {
+        gcd[4 * P.res_idx + c] = P.gcd_in[c];
+      } else {
+        const double avg = (double)t / (double)(nnm * (int64_t)P.msk_max);
+        const double rnd = avg + copysign(0.5, avg);
+        int g = nnm > 0 ? (int)rnd : 0;  // no weighted sample: reported by the host
+        const int md = P.md[c];
+        g = g < -md ? -md : (g > md ? md : g);
+        gcd[4 * P.res_idx + c] = g;
+      }
+      if (c == 2) gcd[4 * P.res_idx + 3] = 0;
+    }
+    if (tid == 0 && !masked) {
       int64_t t = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
       uint64_t q = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
       if (c && cs) { t *= 4; q *= 4; }
@@ -1039,18 +1125,20 @@ __global__ __launch_bounds__(256) void k_upsample420(PicDev S, int w, int h, uin
 // ------------------------------- launchers ---------------------------------
 
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* rows, hipStream_t st) {
+                        uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st) {
   if (npairs <= 0 || nblk <= 0) return hipSuccess;
   // nblk = total plane rows; one wave per row, at most 8 blocks per CU resident
   const int grid = std::min((nblk + 3) / 4, 2048);
-  hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows);
+  hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
   return hipGetLastError();
 }
 
 hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
-                               int32_t* gcd, int64_t* sd, uint64_t* ssd, hipStream_t st) {
+                               const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,
+                               hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gcd_finalize, dim3(npairs), dim3(256), 0, st, pairs, row_sd, row_ssd, gcd, sd, ssd);
+  hipLaunchKernelGGL(k_gcd_finalize, dim3(npairs), dim3(256), 0, st, pairs, row_sd, row_ssd, aux, gcd, sd, ssd,
+                     mcount);
   return hipGetLastError();
 }
 

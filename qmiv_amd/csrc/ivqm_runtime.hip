@@ -48,6 +48,11 @@ struct ivqm_ctx {
   void* own[2] = {nullptr, nullptr};
   size_t own_bytes[2] = {0, 0};
   bool best_enabled = false;
+  // mask mode
+  const void* mask = nullptr;
+  int mask_stride = 0, mask_bps = 0, mask_bd = 0;
+  void* own_mask = nullptr;
+  size_t own_mask_bytes = 0;
 };
 
 struct ivqm_batch {
@@ -141,6 +146,22 @@ int copy_into_slot(ivqm_ctx* c, int slot, const ivqm_picture* p, hipMemcpyKind k
   return IVQM_OK;
 }
 
+int validate_mask(int W, int H, const ivqm_mask* m) {
+  if (!m || !m->plane) return set_error(IVQM_EINVAL, "mask is NULL");
+  if (m->bit_depth < 8 || m->bit_depth > 14) return set_error(IVQM_EINVAL, "mask bit_depth must be 8..14 (BitDepthM)");
+  if (m->bytes_per_sample != 1 && m->bytes_per_sample != 2) return set_error(IVQM_EINVAL, "mask bytes_per_sample");
+  if (m->bytes_per_sample == 1 && m->bit_depth != 8) return set_error(IVQM_EINVAL, "8-bit mask samples require bit_depth 8");
+  if (m->stride < W || W <= 0 || H <= 0) return set_error(IVQM_EINVAL, "mask stride smaller than the width");
+  return IVQM_OK;
+}
+
+void mask_into_spec(const ivqm_mask& m, PairSpec& s) {
+  s.msk = m.plane;
+  s.msk_stride = m.stride;
+  s.msk_bps = m.bytes_per_sample;
+  s.msk_bd = m.bit_depth;
+}
+
 int ctx_pair(ivqm_ctx* c, PairSpec& s, const int32_t* gcd) {
   if (!c->bound[0] || !c->bound[1]) return set_error(IVQM_ESTATE, "both picture slots must be bound");
   if (c->pic[0].cs != c->pic[1].cs || c->pic[0].bps != c->pic[1].bps)
@@ -153,6 +174,12 @@ int ctx_pair(ivqm_ctx* c, PairSpec& s, const int32_t* gcd) {
   s.has_gcd = gcd != nullptr;
   if (gcd)
     for (int k = 0; k < 4; k++) s.gcd[k] = k < 3 ? gcd[k] : 0;
+  if (c->mask) {
+    s.msk = c->mask;
+    s.msk_stride = c->mask_stride;
+    s.msk_bps = c->mask_bps;
+    s.msk_bd = c->mask_bd;
+  }
   return IVQM_OK;
 }
 
@@ -236,6 +263,7 @@ int ivqm_destroy(ivqm_ctx* c) {
     if (c->eng) (void)c->eng->sync();
     for (int s = 0; s < 2; s++)
       if (c->own[s]) (void)hipFree(c->own[s]);
+    if (c->own_mask) (void)hipFree(c->own_mask);
     delete c;
     return IVQM_OK;
   });
@@ -266,6 +294,46 @@ int ivqm_bind(ivqm_ctx* c, int32_t slot, const ivqm_picture* p) {
     c->bound[slot] = true;
     c->eng->set_scp_valid(false);
     return IVQM_OK;
+  });
+}
+
+int ivqm_bind_mask(ivqm_ctx* c, const ivqm_mask* m) {
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL context");
+    if (!m) {
+      c->mask = nullptr;
+      return IVQM_OK;
+    }
+    RCCHK(validate_mask(c->cfg.width, c->cfg.height, m));
+    c->mask = m->plane;
+    c->mask_stride = m->stride;
+    c->mask_bps = m->bytes_per_sample;
+    c->mask_bd = m->bit_depth;
+    return IVQM_OK;
+  });
+}
+
+int ivqm_upload_mask(ivqm_ctx* c, const ivqm_mask* m) {
+  return guarded([&]() -> int {
+    if (!c) return set_error(IVQM_EINVAL, "NULL context");
+    RCCHK(validate_mask(c->cfg.width, c->cfg.height, m));
+    const int W = c->cfg.width, H = c->cfg.height, bps = m->bytes_per_sample;
+    const size_t bytes = (size_t)W * H * bps;
+    RCCHK(c->eng->sync());
+    if (c->own_mask_bytes < bytes) {
+      if (c->own_mask) (void)hipFree(c->own_mask);
+      c->own_mask = nullptr;
+      c->own_mask_bytes = 0;
+      HIPCHK(hipMalloc(&c->own_mask, bytes));
+      c->own_mask_bytes = bytes;
+    }
+    HIPCHK(hipMemcpy2DAsync(c->own_mask, (size_t)W * bps, m->plane, (size_t)m->stride * bps, (size_t)W * bps, H,
+                            hipMemcpyHostToDevice, c->eng->stream()));
+    c->mask = c->own_mask;
+    c->mask_stride = W;
+    c->mask_bps = bps;
+    c->mask_bd = m->bit_depth;
+    return c->eng->sync();
   });
 }
 
@@ -572,6 +640,20 @@ int ivqm_batch_enqueue(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_
     if (!b) return set_error(IVQM_EINVAL, "NULL argument");
     std::vector<PairSpec> specs;
     RCCHK(batch_specs(pairs, n, specs));
+    return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
+  });
+}
+int ivqm_batch_enqueue_masked(ivqm_batch* b, const ivqm_pair* pairs, const ivqm_mask* masks, int32_t n,
+                              uint32_t metrics, int32_t ring) {
+  return guarded([&]() -> int {
+    if (!b || !masks) return set_error(IVQM_EINVAL, "NULL argument");
+    std::vector<PairSpec> specs;
+    RCCHK(batch_specs(pairs, n, specs));
+    for (int i = 0; i < n; i++) {
+      if (!masks[i].plane) continue;
+      RCCHK(validate_mask(pairs[i].width, pairs[i].height, &masks[i]));
+      mask_into_spec(masks[i], specs[i]);
+    }
     return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
   });
 }
