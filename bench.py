@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "batch", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="approximate CPU work of the baseline sample")
-    ap.add_argument("--sections", default="plain_sync,ssim_family,ssim_modes,mixed,served,stress,frontend",
-                    help="extra sections: plain_sync, ssim_family, ssim_modes (regular / block SSIM modes), "
+    ap.add_argument("--sections", default="plain_sync,ssim_family,ssim_modes,mask,mixed,served,stress,frontend",
+                    help="extra sections: plain_sync, ssim_family, ssim_modes (regular / block SSIM modes), mask, "
                          "mixed (64 mixed pairs), served (Poisson loop), "
                          "stress (8192x4096 16-bit R=4), frontend (raw files through qmiv_amd/bin/ivqm_qmiv)")
     ap.add_argument("--stress-frames", type=int, default=8)
@@ -259,6 +259,8 @@ def main():
         out["ssim_family"] = ssim_family(Q, torch, pairs, args)
     if "ssim_modes" in sections:
         out["ssim_modes"] = ssim_modes(Q, torch, pairs, args)
+    if "mask" in sections:
+        out["mask"] = mask_mode(Q, synth, torch, np, pairs, args)
     del pairs
     torch.cuda.empty_cache()
     if "mixed" in sections:
@@ -340,6 +342,30 @@ def ssim_modes(Q, torch, pairs, args):
                                "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 3)}
         out.append(rec)
     return out
+
+
+def mask_mode(Q, synth, torch, np, pairs, args):
+    """Mask mode (MSE + PSNR + WS-PSNR + IV-PSNR with a 10-bit weight mask per pair, about 40 % of the
+    samples weighted 0) on 4 ERP pairs, batch form; the masked search kernel's own time (HIP events).
+    Algorithmic work of the masked search: per weighted pixel and direction 25 candidates x 12 int ops."""
+    n = 4
+    m = Q.IVQM_M_MSE | Q.IVQM_M_PSNR | Q.IVQM_M_WSPSNR | Q.IVQM_M_IVPSNR
+    dev = torch.device("cuda:0")
+    planes = [synth.make_mask(W, H, 10, seed=i) for i in range(n)]
+    masks = [torch.from_numpy(p.view(np.int16)).to(dev) for p in planes]
+    weighted = sum(int(np.count_nonzero(p)) for p in planes)
+    with Q.IvqmBatch(erp=1) as bt:
+        arr = Q.IvqmBatch.make_pairs([(a, b, W, H, BD, True) for a, b in pairs[:n]])
+        marr = Q.IvqmBatch.make_masks([(t, 10) for t in masks])
+        ms = _time_steps(torch, lambda: bt.run_masked(arr, marr, m), 4, 1)
+        k_ms = bt.profile(3, 10)
+    ops = weighted * 2 * 25 * 12
+    tops = ops / (k_ms / 1e3) / 1e12
+    return {"metrics": "MSE+PSNR+WS-PSNR+IV-PSNR with mask", "pairs": n, "ms_per_step": round(ms, 3),
+            "value": round(n * W * H / (ms / 1e3) / 1e6, 1), "unit": "Mpixel-pairs/s",
+            "weighted_fraction": round(weighted / (n * W * H), 3), "k_search_masked_ms": round(k_ms, 3),
+            "roofline": {"bound": "valu", "achieved": round(tops, 2), "peak": PEAK_VALU_OPS_TOPS, "unit": "TOP/s",
+                         "frac": round(tops / PEAK_VALU_OPS_TOPS, 4)}}
 
 
 def mixed_batch(Q, synth, torch, np, args):

@@ -16,7 +16,8 @@ extern "C" {
 
 #define IVQM_K_STATS 0  /* k_stats: SD / SSD rows (streaming) */
 #define IVQM_K_SEARCH 1 /* k_search: both directions + SCP */
-#define IVQM_K_SSIM 2   /* k_ssim: all SSIM jobs of the pass (streaming) */
+#define IVQM_K_SSIM 2   /* k_ssim (or k_ssim_static / k_ssim_gen in the other SSIM modes): all SSIM jobs */
+#define IVQM_K_SEARCH_MASKED 3 /* k_search_masked: both directions of the masked pairs */
 
 /* Relaunches `kernel` `iters` times for the last enqueued pair / batch;
  * *avg_ms = mean duration (HIP events on the context's stream). */

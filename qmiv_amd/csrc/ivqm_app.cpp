@@ -55,12 +55,20 @@ const MetricInfo kInfo[kNumMetrics] = {
     {"IVSSIM", true, false, true, true, "  ", IVQM_M_IVSSIM},
     {"IVMSSSIM", false, false, true, true, "  ", IVQM_M_IVMSSSIM},
 };
+// metric name suffixes (apps/QMIV/src/xMetricQMIV.cpp:61-82; mask mode adds "-M")
 const char* kSuffixCmp = " Y:Cb:Cr  ";
 const char* kSuffixPicPerPic = "          ";
 const char* kSuffixPicPerCmp = "-YCbCr    ";
+void use_mask_suffixes() {
+  kSuffixCmp = "-M Y:Cb:Cr";
+  kSuffixPicPerPic = "-M        ";
+  kSuffixPicPerCmp = "-M-YCbCr  ";
+}
 
 struct Options {
   std::string in0, in1, result_file;
+  std::string inm;               // mask file (-im), apps/QMIV/src/xAppQMIV.cpp:69-71
+  int bdm = -1, cfm = -1;        // mask bit depth / chroma format (default: the pictures')
   int width = -1, height = -1, bit_depth = 8, chroma = 420;
   int start0 = 0, start1 = 0, num_frames = -1;
   bool calc[kNumMetrics] = {};
@@ -117,7 +125,8 @@ void usage() {
       " -erp  -lor LonRangeDeg  -lar LatRangeDeg  -sr SearchRange (1..4)  -cws / -cwa \"4:1:1:0\"\n"
       " -unc \"0.01:0.01:0.01:0\"  -ipa SKIP|WARN|STOP|CNCL  -v VerboseLevel\n"
       " -ssm RegularGaussianFlt|RegularGaussianInt|RegularAveraged|BlockGaussianInt|BlockAveraged\n"
-      " -ssb None|Nearest|Reflect|Mirror|Constant|Zero (regular modes)  -sss Stride  -ssw Window (block modes)\n");
+      " -ssb None|Nearest|Reflect|Mirror|Constant|Zero (regular modes)  -sss Stride  -ssw Window (block modes)\n"
+      " -im MaskFile  -bdm BitDepthM (8..14)  -cfm 400|420|422|444 (mask mode: MSE, PSNR, WSPSNR, IVPSNR)\n");
 }
 
 // Returns false (after printing the problems) when the options are not usable.
@@ -207,8 +216,10 @@ bool parse_options(int argc, char** argv, Options& o) {
     else if (a == "-ipa") o.ipa = upper(need(i));
     else if (a == "-v") int_opt(o.verbose);
     else if (a == "-nth") (void)need(i);  // the GPU path has no worker-thread count
-    else if (a == "-ff" || a == "-im" || a == "-bdm" || a == "-cfm" || a == "-csi" || a == "-csm" || a == "-c" ||
-             a == "-nma") {
+    else if (a == "-im") o.inm = need(i);
+    else if (a == "-bdm") int_opt(o.bdm);
+    else if (a == "-cfm") int_opt(o.cfm);
+    else if (a == "-ff" || a == "-csi" || a == "-csm" || a == "-c" || a == "-nma") {
       const std::string v = need(i);
       if (!(a == "-ff" && upper(v) == "RAW") && !(a == "-nma")) errs += "!  unsupported option " + a + " " + v + "\n";
     } else errs += "!  unknown option " + a + "\n";
@@ -242,6 +253,15 @@ bool parse_options(int argc, char** argv, Options& o) {
     errs += "!  BlockGaussianInt supports StructSimWindow 8 and 16\n";
   if (o.ss_mode_id == 4 && (o.ss_window < 1 || o.ss_window > 64)) errs += "!  StructSimWindow must be 1..64\n";
   if (o.ipa != "SKIP" && o.ipa != "WARN" && o.ipa != "STOP" && o.ipa != "CNCL") errs += "!  invalid InvalidPelActn\n";
+  // mask mode: xAppQMIV.cpp:284-288, :360
+  if (o.bdm < 0) o.bdm = o.bit_depth;
+  if (o.cfm < 0) o.cfm = o.chroma;
+  if (!o.inm.empty()) {
+    if (o.bdm < 8 || o.bdm > 14) errs += "!  Invalid or unsuported BitDepthM value\n";
+    if (o.cfm != 400 && o.cfm != 420 && o.cfm != 422 && o.cfm != 444) errs += "!  Invalid or unsuported ChromaFormatM value\n";
+    if (o.calc[kSSIM] || o.calc[kMSSSIM] || o.calc[kIVSSIM] || o.calc[kIVMSSSIM])
+      errs += "! Structural Similarity metrics cannot be combined with Mask mode\n";
+  }
   if (o.start0 < 0 || o.start1 < 0) errs += "!  invalid StartFrame\n";
   if (!errs.empty()) {
     fputs(errs.c_str(), stderr);
@@ -321,6 +341,8 @@ struct Ring {
 
 }  // namespace
 
+size_t chroma_of(int W, int H, int cs) { return (size_t)(W >> cs) * (H >> cs); }
+
 int main(int argc, char** argv) {
   Options o;
   for (int i = 1; i < argc; i++)
@@ -335,6 +357,13 @@ int main(int argc, char** argv) {
   const int W = o.width, H = o.height, cs = o.chroma == 420 ? 1 : 0, bps = o.bit_depth > 8 ? 2 : 1;
   const size_t luma = (size_t)W * H, chroma = (size_t)(W >> cs) * (H >> cs);
   const size_t frame_bytes = (luma + 2 * chroma) * bps;
+  // mask frames: luma plane of InputFileM (BitDepthM, ChromaFormatM); only the luma plane is used
+  const bool use_mask = !o.inm.empty();
+  const int bpsm = o.bdm > 8 ? 2 : 1;
+  const size_t chroma_m = o.cfm == 400 ? 0 : o.cfm == 420 ? 2 * chroma_of(W, H, 1) : o.cfm == 422 ? (size_t)W * H : 2 * luma;
+  const size_t frame_bytes_m = (luma + chroma_m) * bpsm, mask_bytes = use_mask ? luma * bpsm : 0;
+  const size_t slot_bytes = 2 * frame_bytes + mask_bytes;
+  if (use_mask) use_mask_suffixes();
 
   FILE* f0 = fopen(o.in0.c_str(), "rb");
   FILE* f1 = fopen(o.in1.c_str(), "rb");
@@ -355,6 +384,20 @@ int main(int argc, char** argv) {
     return 1;
   }
   const int N = (int)nf;
+  if (use_mask) {  // the mask sequence is read from its first frame (xAppQMIV.cpp:622-627)
+    FILE* fm = fopen(o.inm.c_str(), "rb");
+    if (!fm) {
+      fprintf(stderr, "ERROR --> InputFileM does not exist (%s)\n", o.inm.c_str());
+      return 1;
+    }
+    fseeko(fm, 0, SEEK_END);
+    const long long nm = (long long)(ftello(fm) / (off_t)frame_bytes_m);
+    fclose(fm);
+    if (N > nm) {
+      fprintf(stderr, "ERROR --> FramesToProcess > NumOfFramesM\n");
+      return 1;
+    }
+  }
 
   uint32_t metrics = 0;
   for (int m = 0; m < kNumMetrics; m++)
@@ -390,8 +433,8 @@ int main(int argc, char** argv) {
   std::vector<uint8_t*> host(kSlots), dev(kSlots);
   std::vector<hipEvent_t> copied(kSlots);
   for (int s = 0; s < kSlots; s++) {
-    HIPX(hipHostMalloc((void**)&host[s], 2 * frame_bytes, hipHostMallocDefault));
-    HIPX(hipMalloc((void**)&dev[s], 2 * frame_bytes));
+    HIPX(hipHostMalloc((void**)&host[s], slot_bytes, hipHostMallocDefault));
+    HIPX(hipMalloc((void**)&dev[s], slot_bytes));
     HIPX(hipEventCreateWithFlags(&copied[s], hipEventDisableTiming));
   }
 
@@ -405,7 +448,8 @@ int main(int argc, char** argv) {
   // sample check run in parallel; the slot order is kept by frame index)
   const int kReaders = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
   auto read_frames = [&](int t) {
-    FILE* fs[2] = {fopen(o.in0.c_str(), "rb"), fopen(o.in1.c_str(), "rb")};
+    FILE* fs[3] = {fopen(o.in0.c_str(), "rb"), fopen(o.in1.c_str(), "rb"),
+                   use_mask ? fopen(o.inm.c_str(), "rb") : nullptr};
     const int st[2] = {o.start0, o.start1};
     for (int f = t; f < N; f += kReaders) {
       const int s = f % kSlots;
@@ -421,6 +465,10 @@ int main(int argc, char** argv) {
             fread(dst + i * frame_bytes, 1, frame_bytes, fs[i]) != frame_bytes)
           err = 1;
       }
+      if (!err && use_mask &&
+          (!fs[2] || fseeko(fs[2], (off_t)f * (off_t)frame_bytes_m, SEEK_SET) != 0 ||
+           fread(dst + 2 * frame_bytes, 1, mask_bytes, fs[2]) != mask_bytes))
+        err = 1;
       // samples above the bit depth's maximum are invalid (InvalidPelActn,
       // apps/QMIV/src/xAppQMIV.cpp:770-775): a vectorised max first, the
       // per-sample action only when one is found
@@ -479,6 +527,7 @@ int main(int argc, char** argv) {
     if (g < ngroups) {
       const int fb = g * kGroup, fe = std::min(N, fb + kGroup);
       std::vector<ivqm_pair> pairs;
+      std::vector<ivqm_mask> masks;
       for (int f = fb; f < fe; f++) {
         const int s = f % kSlots;
         int err;
@@ -495,7 +544,7 @@ int main(int argc, char** argv) {
           fprintf(stderr, "Frame %08d ERROR --> sample value above %u (InvalidPelActn=STOP)\n", f, maxv);
           return fail(1);
         }
-        if (hipMemcpyAsync(dev[s], host[s], 2 * frame_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess ||
+        if (hipMemcpyAsync(dev[s], host[s], slot_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess ||
             hipEventRecord(copied[s], xstream) != hipSuccess)
           return fail(2);
         ivqm_pair pr;
@@ -505,9 +554,12 @@ int main(int argc, char** argv) {
         pr.height = H;
         pr.bit_depth = o.bit_depth;
         pairs.push_back(pr);
+        masks.push_back(ivqm_mask{use_mask ? dev[s] + 2 * frame_bytes : nullptr, W, bpsm, o.bdm, 0});
       }
       if (hipStreamWaitEvent(cstream, copied[(fe - 1) % kSlots], 0) != hipSuccess) return fail(2);
-      const int rc = ivqm_batch_enqueue(batch, pairs.data(), (int)pairs.size(), metrics, g % 2);
+      const int rc = use_mask ? ivqm_batch_enqueue_masked(batch, pairs.data(), masks.data(), (int)pairs.size(), metrics,
+                                                          g % 2)
+                              : ivqm_batch_enqueue(batch, pairs.data(), (int)pairs.size(), metrics, g % 2);
       if (rc != IVQM_OK) {
         fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(rc), ivqm_last_error());
         return fail(2);
@@ -575,7 +627,9 @@ int main(int argc, char** argv) {
     char tb[64];
     const time_t now = time(nullptr);
     strftime(tb, sizeof tb, "%Y-%m-%d  %H:%M:%S", localtime(&now));
-    fprintf(rf, "FILE0  \"%s\"\nFILE1  \"%s\"\nTIME   %s\n", o.in0.c_str(), o.in1.c_str(), tb);
+    fprintf(rf, "FILE0  \"%s\"\nFILE1  \"%s\"\n", o.in0.c_str(), o.in1.c_str());
+    if (use_mask) fprintf(rf, "FILEM  \"%s\"\n", o.inm.c_str());  // xAppQMIV.cpp:1140
+    fprintf(rf, "TIME   %s\n", tb);
     for (int m = 0; m < kNumMetrics; m++)
       if (o.calc[m]) fprintf(rf, "%s\n", fmt_avg(m, st[m], N, "").c_str());
     fclose(rf);

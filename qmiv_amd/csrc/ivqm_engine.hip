@@ -963,6 +963,10 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
         else
           HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, stream_));
         break;
+      case 3:
+        HIPCHK(launch_search_masked(d_pairs_, P.n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_,
+                                    d_rows64_, stream_));
+        break;
       default: return set_error(IVQM_EINVAL, "unknown kernel id");
     }
   }

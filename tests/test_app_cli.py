@@ -36,7 +36,10 @@ def test_help():
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "RegularAveraged", "-sss", "12"], "StructSimStride"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ssm", "BlockGaussianInt", "-ssw", "32"], "BlockGaussianInt"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-sr", "5"], "SearchRange"),
-    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv"], "unsupported option -im"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv"], "cannot be combined with Mask mode"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-bdm", "16"], "BitDepthM"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-cfm", "411"], "ChromaFormatM"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "PNG"], "unsupported option -ff"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cws", "4:1:1"], "component weights"),
     (["-i0", "a", "-i1", "b", "-pf", "rgb24", "-ps", "64x64"], "PictureFormat"),
 ])
@@ -57,6 +60,19 @@ def test_accepted_ssim_options(tmp_path, args):
     a.write_bytes(b"")
     r = _run("-i0", str(a), "-i1", str(a), "-ps", "64x64", *args)
     assert r.returncode == 1 and "no frames to process" in r.stderr, r.stderr
+
+
+def test_mask_file_checks(tmp_path):
+    """The mask sequence must exist and hold at least the frames to process (xAppQMIV.cpp:627)."""
+    a = tmp_path / "a.yuv"
+    a.write_bytes(bytes(64 * 64 * 3 // 2) * 3)  # three 8-bit 4:2:0 frames
+    base = ["-i0", str(a), "-i1", str(a), "-ps", "64x64", "-ml", "PSNR, IVPSNR"]
+    r = _run(*base, "-im", str(tmp_path / "none.yuv"))
+    assert r.returncode == 1 and "InputFileM does not exist" in r.stderr
+    m = tmp_path / "m.yuv"
+    m.write_bytes(bytes(64 * 64) * 2)  # two luma-only frames
+    r = _run(*base, "-im", str(m), "-cfm", "400")
+    assert r.returncode == 1 and "FramesToProcess > NumOfFramesM" in r.stderr
 
 
 def test_missing_input_file(tmp_path):

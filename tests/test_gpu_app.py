@@ -158,3 +158,46 @@ def test_qmiv_front_end_regular_ssim_mode(tmp_path, oracle):
             got = _numbers(line)
             assert len(got) == len(vals)
             assert np.max(np.abs(np.array(got) - np.array(vals))) <= 1.5e-6, (line, vals)
+
+
+def test_qmiv_front_end_mask_mode(tmp_path, oracle):
+    """-im / -bdm / -cfm: a 10-bit luma-only mask sequence; MSE / PSNR / WS-PSNR / IV-PSNR lines with
+    the "-M" suffixes (xMetricQMIV.cpp:61-82) and the masked values of the reference (the oracle is
+    bit-exact with it), the exact-component fake PSNR included; FILEM in the result file."""
+    if not os.path.exists(EXE):
+        pytest.fail("front end not built (python -m qmiv_amd.build)")
+    wl = synth.Workload(cfg=412, width=160, height=96, global_shift=1)
+    n, bd, bdm = 4, 10, 10
+    pairs = [synth.make_pair(wl, i) for i in range(n)]
+    pairs[1] = (pairs[1][0], [pairs[1][1][0], pairs[1][0][1].copy(), pairs[1][1][2]])  # exact Cb
+    masks = [synth.make_mask(wl.width, wl.height, bdm, seed=20 + i) for i in range(n)]
+    f0, f1, fm, rf = tmp_path / "a.yuv", tmp_path / "b.yuv", tmp_path / "m.yuv", tmp_path / "r.txt"
+    raw = lambda pic: b"".join(np.ascontiguousarray(p, np.uint16).tobytes() for p in pic)
+    f0.write_bytes(b"".join(raw(p[0]) for p in pairs))
+    f1.write_bytes(b"".join(raw(p[1]) for p in pairs))
+    fm.write_bytes(b"".join(m.tobytes() for m in masks) + masks[0].tobytes())  # one spare mask frame
+    out = subprocess.run([EXE, "-i0", str(f0), "-i1", str(f1), "-ps", f"{wl.width}x{wl.height}", "-bd", str(bd),
+                          "-cf", "420", "-ml", "MSE, PSNR, WSPSNR, IVPSNR", "-erp", "-im", str(fm), "-bdm", str(bdm),
+                          "-cfm", "400", "-v", "2", "-r", str(rf)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    area = wl.width * wl.height
+    fake = oracle.psnr_from_ssd(1, area, bd)
+    for f, ((p0n, p1n), m) in enumerate(zip(pairs, masks)):
+        p0, p1 = oracle.to444(p0n, True), oracle.to444(p1n, True)
+        exact = [bool(np.array_equal(p0[c], p1[c])) for c in range(3)]
+        g = oracle.gcd_masked(p0, p1, m, bd, bdm)
+        mse, psnr, ws = oracle.psnr_family_masked(p0, p1, m, bd, bdm, erp=True)
+        psnr = [fake if exact[c] else v for c, v in enumerate(psnr)]
+        ws = [fake if exact[c] else v for c, v in enumerate(ws)]
+        iv = oracle.ivpsnr_masked(p0, p1, m, bd, bdm, g, R=2, erp=True)[0]
+        for name, vals in (("MSE", mse), ("PSNR", psnr), ("WSPSNR", ws)):
+            line = "Frame %08d " % f + "%8s%s " % (name, "-M Y:Cb:Cr") + "".join("%8.4f " % v for v in vals)
+            line += "".join(" Exact" + "YUV"[c] for c in range(3) if exact[c])
+            assert line in lines, (line, [l for l in lines if name in l][:3])
+            pic = "Frame %08d " % f + "%8s%s " % (name, "-M-YCbCr  ") + "%8.4f " % oracle.pic_value(vals) + " " * 18
+            assert pic in lines, pic
+        line = "Frame %08d " % f + "%8s%s " % ("IVPSNR", "-M        ") + "%8.4f " % iv + " " * 18
+        assert line in lines, line
+    res = rf.read_text().splitlines()
+    assert res[0].startswith("FILE0") and res[1].startswith("FILE1") and res[2] == f'FILEM  "{fm}"'
