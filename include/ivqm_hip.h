@@ -193,6 +193,14 @@ int ivqm_get_best_map(ivqm_ctx* ctx, int32_t dir, uint8_t* map);
  * dst = three width x height uint16 planes with row stride dst_stride. */
 int ivqm_upsample420(const ivqm_picture* src, int32_t width, int32_t height, uint16_t* dst, int32_t dst_stride,
                      void* hip_stream);
+/* Colour-space conversion into three width x height uint16 4:4:4 planes (plane k at
+ * dst + k * dst_stride * height), as the app's preprocessFrames does it
+ * (apps/QMIV/src/xAppQMIV.cpp:853-876) with xColorSpace::ConvertYCbCr2RGB /
+ * ConvertRGB2YCbCr (libs/pmbb-core/src/xColorSpace.h, fixed-point kernels).
+ * direction 0: YCbCr -> RGB (src 4:2:0 or 4:4:4), 1: RGB -> YCbCr (src planes
+ * R, G, B); space: 0 BT.601 / SMPTE170M, 1 BT.709, 2 SMPTE240M, 3 BT.2020. */
+int ivqm_convert_colour(const ivqm_picture* src, int32_t width, int32_t height, int32_t bit_depth,
+                        int32_t direction, int32_t space, uint16_t* dst, int32_t dst_stride, void* hip_stream);
 /* 2x2 rounded-mean downsampling (xPixelOpsSTD.cpp:50-67, xSSIM.cpp:418-424):
  * dst = three (width/2) x (height/2) uint16 planes, tightly packed. */
 int ivqm_downsample(const ivqm_picture* src, int32_t width, int32_t height, uint16_t* dst, void* hip_stream);

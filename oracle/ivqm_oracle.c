@@ -837,6 +837,60 @@ OR_EXPORT double or_ivpsnr_masked(const uint16_t* const p0[3], const uint16_t* c
   return q[1] < q[0] ? q[1] : q[0];
 }
 
+/* ---------------------------------------------------------------------------
+ * Colour-space conversion, fixed point with 16 fractional bits, as the
+ * reference dispatches it (xColorSpace.h: the _I32 kernels):
+ * xColorSpaceSTD::ConvertRGB2YCbCr_I32 / ConvertYCbCr2RGB_I32
+ * (libs/pmbb-core/src/xColorSpaceSTD.cpp), coefficients
+ * xColorSpaceCoeffYCbCr::c_RGB2YCbCr_I32 / c_YCbCr2RGB_I32
+ * (xColorSpaceCoeff.h:59-73) for BT.601 (= SMPTE170M, JPEG), BT.709,
+ * SMPTE240M, BT.2020. The int32 arithmetic wraps for 16-bit samples (the
+ * products and iy << 16 exceed 2^31); it is computed here in uint32 and
+ * shifted as int32, which is what the reference's code does on x86.
+ * Planes are converted in place; the pictures are 4:4:4.
+ * ------------------------------------------------------------------------- */
+/* [space][row][col]: RGB -> YCbCr rows Y, Cb, Cr over R, G, B. */
+static const int32_t or_rgb2ycc[4][3][3] = {
+    {{19595, 38470, 7471}, {-11058, -21710, 32768}, {32768, -27439, -5329}},  /* BT.601 */
+    {{13933, 46871, 4732}, {-7509, -25259, 32768}, {32768, -29763, -3005}},   /* BT.709 */
+    {{13894, 45941, 5701}, {-7609, -25159, 32768}, {32768, -29150, -3618}},   /* SMPTE240M */
+    {{17216, 44434, 3886}, {-9151, -23617, 32768}, {32768, -30133, -2635}}};  /* BT.2020 */
+/* YCbCr -> RGB: R = Y + rv Cr, G = Y + gu Cb + gv Cr, B = Y + bu Cb; {rv, gu, gv, bu}. */
+static const int32_t or_ycc2rgb[4][4] = {
+    {91881, -22553, -46802, 116130}, {103206, -12276, -30679, 121609},
+    {103285, -14852, -31236, 119669}, {96639, -10784, -37444, 123299}};
+
+static inline int32_t or_w32(uint32_t v) { return (int32_t)v; }
+static inline int32_t or_clipu(int32_t v, int32_t mx) { return v < 0 ? 0 : (v > mx ? mx : v); }
+
+OR_EXPORT void or_rgb2ycbcr(uint16_t* c0, uint16_t* c1, uint16_t* c2, int64_t n, int32_t bit_depth, int32_t space) {
+  const int32_t(*k)[3] = or_rgb2ycc[space];
+  const int32_t mid = 1 << (bit_depth - 1), mx = (1 << bit_depth) - 1;
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t r = c0[i], g = c1[i], b = c2[i];
+    const int32_t ty = or_w32((uint32_t)k[0][0] * r + (uint32_t)k[0][1] * g + (uint32_t)k[0][2] * b + 32768u) >> 16;
+    const int32_t tu = or_w32((uint32_t)k[1][0] * r + (uint32_t)k[1][1] * g + (b << 15) + 32768u) >> 16;
+    const int32_t tv = or_w32((r << 15) + (uint32_t)k[2][1] * g + (uint32_t)k[2][2] * b + 32768u) >> 16;
+    c0[i] = (uint16_t)or_clipu(ty, mx);
+    c1[i] = (uint16_t)or_clipu(tu + mid, mx);
+    c2[i] = (uint16_t)or_clipu(tv + mid, mx);
+  }
+}
+OR_EXPORT void or_ycbcr2rgb(uint16_t* c0, uint16_t* c1, uint16_t* c2, int64_t n, int32_t bit_depth, int32_t space) {
+  const int32_t* k = or_ycc2rgb[space];
+  const int32_t mid = 1 << (bit_depth - 1), mx = (1 << bit_depth) - 1;
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t iy = c0[i], iu = (int32_t)c1[i] - mid, iv = (int32_t)c2[i] - mid;
+    const uint32_t sy = ((uint32_t)iy << 16) + 32768u;
+    const int32_t r = or_w32(sy + (uint32_t)k[0] * (uint32_t)iv) >> 16;
+    const int32_t g = or_w32(sy + (uint32_t)k[1] * (uint32_t)iu + (uint32_t)k[2] * (uint32_t)iv) >> 16;
+    const int32_t b = or_w32(sy + (uint32_t)k[3] * (uint32_t)iu) >> 16;
+    c0[i] = (uint16_t)or_clipu(r, mx);
+    c1[i] = (uint16_t)or_clipu(g, mx);
+    c2[i] = (uint16_t)or_clipu(b, mx);
+  }
+}
+
 /* Component-weighted mean exported for the picture value of per-component
  * metrics (xMetricQMIV.cpp:91-97 uses v0*w0 + v1*w1 + v2*w2). */
 OR_EXPORT double or_pic_value(const double v[3], const int32_t cw[4]) {

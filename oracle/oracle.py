@@ -320,6 +320,13 @@ class Oracle:
                                     C.byref(r2t), C.byref(t2r))
         return v, r2t.value, t2r.value
 
+    # -- colour space (dir 0: YCbCr -> RGB, 1: RGB -> YCbCr; space 0 BT.601, 1 BT.709, 2 SMPTE240M, 3 BT.2020) --
+    def convert_colour(self, planes, bd, direction, space):
+        out = [np.ascontiguousarray(p, np.uint16).copy() for p in planes]
+        fn = self.L.or_ycbcr2rgb if direction == 0 else self.L.or_rgb2ycbcr
+        fn(*[o.ctypes.data_as(_u16p) for o in out], C.c_int64(out[0].size), bd, space)
+        return out
+
     # -- structural-similarity modes (mode: xSSIM::eMode 0..4, ext: eMrgExt 0..5) --
     def ssim_weights(self, which):
         n = {0: 121, 1: 121, 2: 64, 3: 256}[which]
@@ -543,6 +550,12 @@ class Reference:
         self.L.ref_ivpsnr_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, _i32x4(gcd), R, _i32x4(cws),
                                  _i32x4(cwa), int(erp), lat, C.byref(v), C.byref(r2t), C.byref(t2r))
         return v.value, r2t.value, t2r.value
+
+    def convert_colour(self, planes, bd, direction, space):
+        out = [np.ascontiguousarray(p, np.uint16).copy() for p in planes]
+        h, w = out[0].shape
+        self.L.ref_convert_colour(*[o.ctypes.data_as(_u16p) for o in out], w, h, bd, direction, space)
+        return out
 
     def bench_pairs(self, pairs, w, h, bd, chroma420, erp, metrics, threads):
         """Times the app-order frame loop over in-memory pairs; returns (seconds, ivpsnr[], ivssim[])."""

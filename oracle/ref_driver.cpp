@@ -27,6 +27,7 @@
 #include "xThreadPool.h"
 #include "xWSPSNR.h"
 #include "xDistortion.h"
+#include "xColorSpace.h"
 
 using namespace PMBB_NAMESPACE;
 
@@ -366,6 +367,15 @@ int ref_ivpsnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], 
   if (r2t) *r2t = lr;
   if (t2r) *t2r = lt;
   return M.nnm;
+}
+
+// xColorSpace::ConvertRGB2YCbCr / ConvertYCbCr2RGB in place on 4:4:4 planes (xAppQMIV.cpp:856-876
+// calls them the same way); dir 0 = YCbCr -> RGB, 1 = RGB -> YCbCr; space = eClrSpcLC 0..3.
+int ref_convert_colour(uint16_t* c0, uint16_t* c1, uint16_t* c2, int32_t w, int32_t h, int32_t bd, int32_t dir,
+                       int32_t space) {
+  if (dir == 0) xColorSpace::ConvertYCbCr2RGB(c0, c1, c2, c0, c1, c2, w, w, w, h, bd, (eClrSpcLC)space);
+  else xColorSpace::ConvertRGB2YCbCr(c0, c1, c2, c0, c1, c2, w, w, w, h, bd, (eClrSpcLC)space);
+  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -28,7 +28,7 @@ EXPORTED = [
     "ivqm_gen_scp", "ivqm_ssim", "ivqm_msssim", "ivqm_ivssim", "ivqm_ivmsssim", "ivqm_frame",
     "ivqm_frame_enqueue", "ivqm_frame_collect", "ivqm_get_rows", "ivqm_get_scp", "ivqm_enable_best_map",
     "ivqm_get_best_map", "ivqm_upsample420", "ivqm_downsample", "ivqm_profile", "ivqm_upload_scp",
-    "ivqm_bind_mask", "ivqm_upload_mask", "ivqm_batch_enqueue_masked",
+    "ivqm_bind_mask", "ivqm_upload_mask", "ivqm_batch_enqueue_masked", "ivqm_convert_colour",
     "ivqm_batch_create", "ivqm_batch_destroy", "ivqm_batch_set_stream", "ivqm_batch_set_tuning",
     "ivqm_batch_get_tuning", "ivqm_batch_autotune", "ivqm_batch_run", "ivqm_batch_enqueue", "ivqm_batch_collect",
     "ivqm_batch_profile",
@@ -160,6 +160,8 @@ def load():
     L.ivqm_get_best_map.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]
     L.ivqm_upsample420.argtypes = [C.POINTER(Picture), C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
     L.ivqm_downsample.argtypes = [C.POINTER(Picture), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    L.ivqm_convert_colour.argtypes = [C.POINTER(Picture), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_void_p, C.c_int32, C.c_void_p]
     L.ivqm_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.ivqm_batch_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
     L.ivqm_batch_destroy.argtypes = [C.c_void_p]

@@ -59,11 +59,29 @@ const MetricInfo kInfo[kNumMetrics] = {
 const char* kSuffixCmp = " Y:Cb:Cr  ";
 const char* kSuffixPicPerPic = "          ";
 const char* kSuffixPicPerCmp = "-YCbCr    ";
-void use_mask_suffixes() {
-  kSuffixCmp = "-M Y:Cb:Cr";
-  kSuffixPicPerPic = "-M        ";
-  kSuffixPicPerCmp = "-M-YCbCr  ";
+void set_suffixes(bool mask, bool rgb) {
+  kSuffixCmp = mask ? (rgb ? "-M R:G:B  " : "-M Y:Cb:Cr") : (rgb ? " R:G:B    " : " Y:Cb:Cr  ");
+  kSuffixPicPerPic = mask ? (rgb ? "-M-RGB    " : "-M        ") : (rgb ? "-RGB      " : "          ");
+  kSuffixPicPerCmp = mask ? (rgb ? "-M-RGB    " : "-M-YCbCr  ") : (rgb ? "-RGB      " : "-YCbCr    ");
 }
+
+// Colour spaces of the app (apps/QMIV/src/xUtilsQMIV.h:12-27, xUtilsQMIV.cpp:13-25).
+enum ClrSpc { kRGB = 2, kBGR = 3, kGBR = 4, kYCbCr = 16, kBT601 = 17, kSMPTE170M = 18, kBT709 = 19, kSMPTE240M = 20,
+              kBT2020 = 21, kInvalidClrSpc = -1 };
+int clrspc_from(const std::string& s) {
+  static const std::pair<const char*, int> names[] = {
+      {"RGB", kRGB}, {"BGR", kBGR}, {"GBR", kGBR}, {"YCbCr", kYCbCr}, {"YCbCr_BT601", kBT601},
+      {"YCbCr_SMPTE170M", kSMPTE170M}, {"YCbCr_BT709", kBT709}, {"YCbCr_SMPTE240M", kSMPTE240M},
+      {"YCbCr_BT2020", kBT2020}};
+  for (const auto& n : names)
+    if (s == n.first) return n.second;
+  return kInvalidClrSpc;
+}
+bool is_rgb(int c) { return c == kRGB || c == kBGR || c == kGBR; }
+bool is_ycbcr(int c) { return c >= kYCbCr; }
+bool is_defined_ycbcr(int c) { return c > kYCbCr; }
+// eClrSpcLC of a defined YCbCr (xUtilsQMIV.cpp:42-53): 0 BT.601 / SMPTE170M, 1 BT.709, 2 SMPTE240M, 3 BT.2020
+int clrspc_lc(int c) { return c == kBT601 || c == kSMPTE170M ? 0 : c == kBT709 ? 1 : c == kSMPTE240M ? 2 : 3; }
 
 struct Options {
   std::string in0, in1, result_file;
@@ -80,6 +98,8 @@ struct Options {
   int ss_stride = 4, ss_window = -1;  // window default: 11 for regular modes, else 8 (xAppQMIV.cpp:330)
   std::string ss_mode = "BlockAveraged", ss_border = "None";
   int ss_mode_id = 4, ss_border_id = 0;
+  std::string csi_s, csm_s;      // -csi / -csm (default YCbCr, then ColorSpaceInput)
+  int csi = kYCbCr, csm = kYCbCr;
   std::string ipa = "STOP";
   int verbose = 1;
 };
@@ -126,7 +146,8 @@ void usage() {
       " -unc \"0.01:0.01:0.01:0\"  -ipa SKIP|WARN|STOP|CNCL  -v VerboseLevel\n"
       " -ssm RegularGaussianFlt|RegularGaussianInt|RegularAveraged|BlockGaussianInt|BlockAveraged\n"
       " -ssb None|Nearest|Reflect|Mirror|Constant|Zero (regular modes)  -sss Stride  -ssw Window (block modes)\n"
-      " -im MaskFile  -bdm BitDepthM (8..14)  -cfm 400|420|422|444 (mask mode: MSE, PSNR, WSPSNR, IVPSNR)\n");
+      " -im MaskFile  -bdm BitDepthM (8..14)  -cfm 400|420|422|444 (mask mode: MSE, PSNR, WSPSNR, IVPSNR)\n"
+      " -csi / -csm RGB|BGR|GBR|YCbCr|YCbCr_BT601|YCbCr_SMPTE170M|YCbCr_BT709|YCbCr_SMPTE240M|YCbCr_BT2020\n");
 }
 
 // Returns false (after printing the problems) when the options are not usable.
@@ -219,7 +240,9 @@ bool parse_options(int argc, char** argv, Options& o) {
     else if (a == "-im") o.inm = need(i);
     else if (a == "-bdm") int_opt(o.bdm);
     else if (a == "-cfm") int_opt(o.cfm);
-    else if (a == "-ff" || a == "-csi" || a == "-csm" || a == "-c" || a == "-nma") {
+    else if (a == "-csi") o.csi_s = need(i);
+    else if (a == "-csm") o.csm_s = need(i);
+    else if (a == "-ff" || a == "-c" || a == "-nma") {
       const std::string v = need(i);
       if (!(a == "-ff" && upper(v) == "RAW") && !(a == "-nma")) errs += "!  unsupported option " + a + " " + v + "\n";
     } else errs += "!  unknown option " + a + "\n";
@@ -253,6 +276,15 @@ bool parse_options(int argc, char** argv, Options& o) {
     errs += "!  BlockGaussianInt supports StructSimWindow 8 and 16\n";
   if (o.ss_mode_id == 4 && (o.ss_window < 1 || o.ss_window > 64)) errs += "!  StructSimWindow must be 1..64\n";
   if (o.ipa != "SKIP" && o.ipa != "WARN" && o.ipa != "STOP" && o.ipa != "CNCL") errs += "!  invalid InvalidPelActn\n";
+  // colour spaces: xAppQMIV.cpp:297-311
+  o.csi = o.csi_s.empty() ? kYCbCr : clrspc_from(trim(o.csi_s));
+  o.csm = o.csm_s.empty() ? o.csi : clrspc_from(trim(o.csm_s));
+  if (o.csi == kInvalidClrSpc) errs += "!  ColorSpaceInput value is not valid\n";
+  if (o.csm == kInvalidClrSpc) errs += "!  ColorSpaceMetric value is not valid\n";
+  if (o.csi != kInvalidClrSpc && o.csm != kInvalidClrSpc && o.csi != o.csm) {
+    if (is_ycbcr(o.csi) && is_ycbcr(o.csm)) errs += "!  YCbCr to YCbCr conversion is not supported.\n";
+    if (o.csi == kYCbCr || o.csm == kYCbCr) errs += "!  Generic YCbCr cannot be used for colorspace conversion.\n";
+  }
   // mask mode: xAppQMIV.cpp:284-288, :360
   if (o.bdm < 0) o.bdm = o.bit_depth;
   if (o.cfm < 0) o.cfm = o.chroma;
@@ -363,7 +395,13 @@ int main(int argc, char** argv) {
   const size_t chroma_m = o.cfm == 400 ? 0 : o.cfm == 420 ? 2 * chroma_of(W, H, 1) : o.cfm == 422 ? (size_t)W * H : 2 * luma;
   const size_t frame_bytes_m = (luma + chroma_m) * bpsm, mask_bytes = use_mask ? luma * bpsm : 0;
   const size_t slot_bytes = 2 * frame_bytes + mask_bytes;
-  if (use_mask) use_mask_suffixes();
+  set_suffixes(use_mask, is_rgb(o.csm));
+  // colour processing of the app's preprocessFrames (xAppQMIV.cpp:308-311, :853-892): conversions run
+  // on the device into 4:4:4 planes; BGR / GBR reorders (metrics in RGB) are plane swaps
+  const bool cvt_y2r = is_defined_ycbcr(o.csi) && is_rgb(o.csm), cvt_r2y = is_rgb(o.csi) && is_defined_ycbcr(o.csm);
+  const bool reorder = is_rgb(o.csi) && o.csi != kRGB && o.csm == kRGB;
+  const bool colour = cvt_y2r || cvt_r2y || reorder;
+  const int space = clrspc_lc(cvt_y2r ? o.csi : o.csm);
 
   FILE* f0 = fopen(o.in0.c_str(), "rb");
   FILE* f1 = fopen(o.in1.c_str(), "rb");
@@ -431,10 +469,12 @@ int main(int argc, char** argv) {
   IVQMX(ivqm_batch_set_stream(batch, (void*)cstream));
   IVQMX(ivqm_batch_set_tuning(batch, 0, 0, 1));
   std::vector<uint8_t*> host(kSlots), dev(kSlots);
+  std::vector<uint16_t*> conv(kSlots, nullptr);  // converted 4:4:4 pictures (colour processing)
   std::vector<hipEvent_t> copied(kSlots);
   for (int s = 0; s < kSlots; s++) {
     HIPX(hipHostMalloc((void**)&host[s], slot_bytes, hipHostMallocDefault));
     HIPX(hipMalloc((void**)&dev[s], slot_bytes));
+    if (colour) HIPX(hipMalloc((void**)&conv[s], 2 * 3 * luma * sizeof(uint16_t)));
     HIPX(hipEventCreateWithFlags(&copied[s], hipEventDisableTiming));
   }
 
@@ -544,12 +584,38 @@ int main(int argc, char** argv) {
           fprintf(stderr, "Frame %08d ERROR --> sample value above %u (InvalidPelActn=STOP)\n", f, maxv);
           return fail(1);
         }
-        if (hipMemcpyAsync(dev[s], host[s], slot_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess ||
-            hipEventRecord(copied[s], xstream) != hipSuccess)
-          return fail(2);
+        if (hipMemcpyAsync(dev[s], host[s], slot_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess) return fail(2);
         ivqm_pair pr;
         pr.p0 = picture(dev[s]);
         pr.p1 = picture(dev[s] + frame_bytes);
+        if (colour) {
+          ivqm_picture* pp[2] = {&pr.p0, &pr.p1};
+          for (int i = 0; i < 2; i++) {
+            uint16_t* d = conv[s] + (size_t)i * 3 * luma;
+            const int rc = reorder ? ivqm_upsample420(pp[i], W, H, d, W, (void*)xstream)
+                                   : ivqm_convert_colour(pp[i], W, H, o.bit_depth, cvt_y2r ? 0 : 1, space, d, W,
+                                                         (void*)xstream);
+            if (rc != IVQM_OK) {
+              fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(rc), ivqm_last_error());
+              return fail(2);
+            }
+            uint16_t* pl[3] = {d, d + luma, d + 2 * luma};
+            if (reorder && o.csi == kBGR) std::swap(pl[0], pl[2]);  // BGR -> RGB
+            if (reorder && o.csi == kGBR) {                         // GBR -> BGR -> RGB
+              std::swap(pl[0], pl[1]);
+              std::swap(pl[0], pl[2]);
+            }
+            ivqm_picture q;
+            for (int k = 0; k < 3; k++) {
+              q.plane[k] = pl[k];
+              q.stride[k] = W;
+            }
+            q.chroma_420 = 0;
+            q.bytes_per_sample = 2;
+            *pp[i] = q;
+          }
+        }
+        if (hipEventRecord(copied[s], xstream) != hipSuccess) return fail(2);
         pr.width = W;
         pr.height = H;
         pr.bit_depth = o.bit_depth;
@@ -635,6 +701,7 @@ int main(int argc, char** argv) {
     fclose(rf);
   }
   for (int s = 0; s < kSlots; s++) {
+    if (conv[s]) (void)hipFree(conv[s]);
     (void)hipHostFree(host[s]);
     (void)hipFree(dev[s]);
     (void)hipEventDestroy(copied[s]);

@@ -110,6 +110,9 @@ hipError_t launch_downsample(const DownJob* jobs, int njobs, int nblk, hipStream
 // Standalone pixel operations on one picture.
 hipError_t launch_downsample_one(const PicDev& src, int w, int h, uint16_t* dst, int dst_pitch, hipStream_t st);
 hipError_t launch_upsample420(const PicDev& src, int w, int h, uint16_t* dst, int dst_stride, hipStream_t st);
+// direction 0: YCbCr -> RGB, 1: RGB -> YCbCr; space 0 BT.601, 1 BT.709, 2 SMPTE240M, 3 BT.2020.
+hipError_t launch_colour(const PicDev& src, int w, int h, int bit_depth, int direction, int space, uint16_t* dst,
+                         int dst_stride, hipStream_t st);
 
 constexpr int kSearchTileW = 64;
 // The search runs waves of 4 pixel rows (64 x 16 tiles) for every search range;

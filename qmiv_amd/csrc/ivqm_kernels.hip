@@ -22,6 +22,7 @@
 //                    for IV-SSIM).
 //   k_downsample     2x2 rounded-mean pyramid level (MS-SSIM).
 //   k_upsample420    4:2:0 -> 4:4:4 nearest replication (standalone).
+//   k_colour         YCbCr <-> RGB fixed-point conversion to 4:4:4 (standalone).
 //
 // 4:2:0 input is never materialised at 4:4:4 by the metric kernels: a chroma
 // sample at luma-grid (x, y) is read at (x >> 1, y >> 1), exactly the value
@@ -1120,9 +1121,73 @@ __global__ __launch_bounds__(256) void k_upsample420(PicDev S, int w, int h, uin
   dst[(size_t)c * dst_stride * h + (size_t)y * dst_stride + x] = (uint16_t)ldpix<T>(S, c, x, y);
 }
 
+// Colour-space conversion to 4:4:4 planes (standalone, before the metrics): the fixed-point
+// kernels the reference dispatches, xColorSpaceSTD::ConvertYCbCr2RGB_I32 / ConvertRGB2YCbCr_I32
+// (libs/pmbb-core/src/xColorSpaceSTD.cpp), coefficients xColorSpaceCoeff.h:59-73. The int32
+// arithmetic wraps (16-bit samples) as the reference's two's-complement code does.
+struct ColourCoef {
+  int32_t k[9];  // RGB -> YCbCr: Y_R, Y_G, Y_B, U_R, U_G, V_G, V_B (7 used); YCbCr -> RGB: R_V, G_U, G_V, B_U
+};
+__device__ __forceinline__ int32_t clipu(int32_t v, int32_t mx) { return min(max(v, 0), mx); }
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(256) void k_colour(PicDev S, int w, int h, int bit_depth, ColourCoef K,
+                                                uint16_t* __restrict__ dst, int dst_stride) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const int32_t mid = 1 << (bit_depth - 1), mx = (1 << bit_depth) - 1;
+  const uint32_t a = (uint32_t)ldpix<T>(S, 0, x, y), b = (uint32_t)ldpix<T>(S, 1, x, y),
+                 c = (uint32_t)ldpix<T>(S, 2, x, y);
+  int32_t o0, o1, o2;
+  if (DIR == 1) {  // RGB -> YCbCr (a, b, c = R, G, B)
+    const int32_t ty = (int32_t)((uint32_t)K.k[0] * a + (uint32_t)K.k[1] * b + (uint32_t)K.k[2] * c + 32768u) >> 16;
+    const int32_t tu = (int32_t)((uint32_t)K.k[3] * a + (uint32_t)K.k[4] * b + (c << 15) + 32768u) >> 16;
+    const int32_t tv = (int32_t)((a << 15) + (uint32_t)K.k[5] * b + (uint32_t)K.k[6] * c + 32768u) >> 16;
+    o0 = clipu(ty, mx);
+    o1 = clipu(tu + mid, mx);
+    o2 = clipu(tv + mid, mx);
+  } else {  // YCbCr -> RGB
+    const int32_t iu = (int32_t)b - mid, iv = (int32_t)c - mid;
+    const uint32_t sy = (a << 16) + 32768u;
+    o0 = clipu((int32_t)(sy + (uint32_t)K.k[0] * (uint32_t)iv) >> 16, mx);
+    o1 = clipu((int32_t)(sy + (uint32_t)K.k[1] * (uint32_t)iu + (uint32_t)K.k[2] * (uint32_t)iv) >> 16, mx);
+    o2 = clipu((int32_t)(sy + (uint32_t)K.k[3] * (uint32_t)iu) >> 16, mx);
+  }
+  const size_t plane = (size_t)dst_stride * h, o = (size_t)y * dst_stride + x;
+  dst[o] = (uint16_t)o0;
+  dst[plane + o] = (uint16_t)o1;
+  dst[2 * plane + o] = (uint16_t)o2;
+}
+
 }  // namespace
 
 // ------------------------------- launchers ---------------------------------
+
+hipError_t launch_colour(const PicDev& src, int w, int h, int bit_depth, int direction, int space, uint16_t* dst,
+                         int dst_stride, hipStream_t st) {
+  // xColorSpaceCoeffYCbCr::c_RGB2YCbCr_I32 / c_YCbCr2RGB_I32 (xColorSpaceCoeff.h:59-73): BT.601
+  // (= SMPTE170M, JPEG), BT.709, SMPTE240M, BT.2020
+  static const ColourCoef kFwd[4] = {{{19595, 38470, 7471, -11058, -21710, -27439, -5329}},
+                                     {{13933, 46871, 4732, -7509, -25259, -29763, -3005}},
+                                     {{13894, 45941, 5701, -7609, -25159, -29150, -3618}},
+                                     {{17216, 44434, 3886, -9151, -23617, -30133, -2635}}};
+  static const ColourCoef kInv[4] = {{{91881, -22553, -46802, 116130}},
+                                     {{103206, -12276, -30679, 121609}},
+                                     {{103285, -14852, -31236, 119669}},
+                                     {{96639, -10784, -37444, 123299}}};
+  if (w <= 0 || h <= 0 || space < 0 || space > 3 || (direction != 0 && direction != 1)) return hipErrorInvalidValue;
+  const dim3 grid((w + 63) / 64, (h + 3) / 4);
+  const ColourCoef K = direction ? kFwd[space] : kInv[space];
+  if (src.bps == 2) {
+    if (direction) hipLaunchKernelGGL((k_colour<uint16_t, 1>), grid, dim3(256), 0, st, src, w, h, bit_depth, K, dst, dst_stride);
+    else hipLaunchKernelGGL((k_colour<uint16_t, 0>), grid, dim3(256), 0, st, src, w, h, bit_depth, K, dst, dst_stride);
+  } else {
+    if (direction) hipLaunchKernelGGL((k_colour<uint8_t, 1>), grid, dim3(256), 0, st, src, w, h, bit_depth, K, dst, dst_stride);
+    else hipLaunchKernelGGL((k_colour<uint8_t, 0>), grid, dim3(256), 0, st, src, w, h, bit_depth, K, dst, dst_stride);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st) {

@@ -555,6 +555,20 @@ int ivqm_upsample420(const ivqm_picture* src, int32_t w, int32_t h, uint16_t* ds
     return IVQM_OK;
   });
 }
+int ivqm_convert_colour(const ivqm_picture* src, int32_t w, int32_t h, int32_t bd, int32_t direction, int32_t space,
+                        uint16_t* dst, int32_t dst_stride, void* stream) {
+  return guarded([&]() -> int {
+    PicDev d;
+    RCCHK(pic_from(src, d));
+    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && ((w & 1) || (h & 1))))
+      return set_error(IVQM_EINVAL, "bad size");
+    if (bd < 8 || bd > 16 || (d.bps == 1 && bd != 8)) return set_error(IVQM_EINVAL, "bad bit depth");
+    if (direction != 0 && direction != 1) return set_error(IVQM_EINVAL, "direction must be 0 or 1");
+    if (space < 0 || space > 3) return set_error(IVQM_EINVAL, "space must be 0..3");
+    HIPCHK(launch_colour(d, w, h, bd, direction, space, dst, dst_stride, (hipStream_t)stream));
+    return IVQM_OK;
+  });
+}
 int ivqm_downsample(const ivqm_picture* src, int32_t w, int32_t h, uint16_t* dst, void* stream) {
   return guarded([&]() -> int {
     PicDev d;

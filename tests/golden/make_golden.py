@@ -183,6 +183,25 @@ def make_mask_cases(ref: Reference, orc: Oracle):
     return out
 
 
+# Colour-space conversion (xColorSpace::ConvertYCbCr2RGB / ConvertRGB2YCbCr, fixed point): the
+# planes of a 4:4:4 synthetic picture at 8..14 bits, each direction and colour space; SHA-1 of the
+# converted planes. 16-bit conversions overflow int32 in the reference and its builds disagree.
+COLOUR_WORKLOADS = [synth.Workload(cfg=190 + bd, width=96, height=40, bit_depth=bd, chroma_420=False,
+                                   sigma_r=(1 << bd) / 8, sigma_t=(1 << bd) / 16) for bd in (8, 10, 12, 14)]
+
+
+def make_colour_cases(ref: Reference):
+    out = []
+    for wl in COLOUR_WORKLOADS:
+        p0, _ = synth.make_pair(wl, 0)
+        for direction in (0, 1):
+            for space in range(4):
+                conv = ref.convert_colour(p0, wl.bit_depth, direction, space)
+                out.append({"workload": dataclasses.asdict(wl), "direction": direction, "space": space,
+                            "sha1": [sha1(c) for c in conv], "head": [c.ravel()[:8].tolist() for c in conv]})
+    return out
+
+
 def main():
     ref = Reference()
     orc = Oracle()
@@ -198,6 +217,7 @@ def main():
         "ssim_mode_reference_build": "oracle/Makefile (x86-64-v3), pictures with a margin of 8",
         "ssim_mode_cases": make_mode_cases(orc),
         "mask_cases": make_mask_cases(ref, orc),
+        "colour_cases": make_colour_cases(ref),
     }
     with open(OUT, "w") as fh:
         json.dump(doc, fh, indent=1)

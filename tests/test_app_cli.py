@@ -40,6 +40,10 @@ def test_help():
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-bdm", "16"], "BitDepthM"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-cfm", "411"], "ChromaFormatM"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "PNG"], "unsupported option -ff"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-csi", "XYZ"], "ColorSpaceInput value is not valid"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-csi", "YCbCr_BT709", "-csm", "YCbCr_BT601"],
+     "YCbCr to YCbCr conversion is not supported"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-csi", "YCbCr", "-csm", "RGB"], "Generic YCbCr cannot be used"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cws", "4:1:1"], "component weights"),
     (["-i0", "a", "-i1", "b", "-pf", "rgb24", "-ps", "64x64"], "PictureFormat"),
 ])

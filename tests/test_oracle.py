@@ -547,3 +547,32 @@ def test_masked_gcd_closed_form(oracle):
     want = 3 * int(m.astype(np.int64).sum()) / (np.count_nonzero(m) * 255)
     g = oracle.gcd_masked(p0, p1, m, 10, bdm)
     assert g[0] == int(want + 0.5) and g[3] == 0
+
+
+# ---------------------------------------------------------------------------
+# colour-space conversion
+# ---------------------------------------------------------------------------
+def test_colour_conversion_matches_reference_golden(golden, oracle):
+    """Fixed-point YCbCr <-> RGB (xColorSpaceSTD.cpp _I32 kernels, the ones xColorSpace.h dispatches)
+    for BT.601 / BT.709 / SMPTE240M / BT.2020 at 8..14 bits: identical planes to the reference."""
+    for c in golden["colour_cases"]:
+        w = dict(c["workload"])
+        w["offsets"] = tuple(w["offsets"])
+        wl = synth.Workload(**w)
+        p0, _ = synth.make_pair(wl, 0)
+        got = oracle.convert_colour(p0, wl.bit_depth, c["direction"], c["space"])
+        assert [sha1(g) for g in got] == c["sha1"], (wl.bit_depth, c["direction"], c["space"])
+
+
+def test_colour_conversion_known_values(oracle):
+    """Closed forms: grey (R = G = B = v) maps to Y = v, Cb = Cr = mid in every space (the Y row sums
+    to 65536, the chroma rows to 0), and Y with neutral chroma maps back to R = G = B = Y; white and
+    black clip to the range."""
+    for bd in (8, 10):
+        mx, mid = (1 << bd) - 1, 1 << (bd - 1)
+        v = np.array([[0, 1, mid, mx - 1, mx]], np.uint16)
+        for space in range(4):
+            y, u, w = oracle.convert_colour([v, v, v], bd, 1, space)
+            assert np.array_equal(y, v) and np.all(u == mid) and np.all(w == mid)
+            r, g, b = oracle.convert_colour([v, np.full_like(v, mid), np.full_like(v, mid)], bd, 0, space)
+            assert np.array_equal(r, v) and np.array_equal(g, v) and np.array_equal(b, v)
