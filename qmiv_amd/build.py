@@ -78,8 +78,8 @@ def build_app(lib: str) -> str:
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "ivqm_qmiv")
     _run([HIPCC, "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "ivqm_app.cpp"), "-L" + os.path.dirname(lib),
-          "-livqm_hip", "-lpthread", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "ivqm_app.cpp"), os.path.join(CSRC, "ivqm_images.cpp"),
+          "-L" + os.path.dirname(lib), "-livqm_hip", "-lz", "-lpthread", "-Wl,-rpath,$ORIGIN/..", "-o", out])
     return out
 
 

@@ -34,6 +34,7 @@
 
 #include "../../include/ivqm_batch.h"
 #include "ivqm_epilogue.h"
+#include "ivqm_images.h"
 
 namespace {
 
@@ -86,6 +87,7 @@ int clrspc_lc(int c) { return c == kBT601 || c == kSMPTE170M ? 0 : c == kBT709 ?
 struct Options {
   std::string in0, in1, result_file;
   std::string inm;               // mask file (-im), apps/QMIV/src/xAppQMIV.cpp:69-71
+  std::string ff = "RAW";        // file format (-ff RAW | PNG | BMP), xAppQMIV.cpp:222-224
   int bdm = -1, cfm = -1;        // mask bit depth / chroma format (default: the pictures')
   int width = -1, height = -1, bit_depth = 8, chroma = 420;
   int start0 = 0, start1 = 0, num_frames = -1;
@@ -147,7 +149,8 @@ void usage() {
       " -ssm RegularGaussianFlt|RegularGaussianInt|RegularAveraged|BlockGaussianInt|BlockAveraged\n"
       " -ssb None|Nearest|Reflect|Mirror|Constant|Zero (regular modes)  -sss Stride  -ssw Window (block modes)\n"
       " -im MaskFile  -bdm BitDepthM (8..14)  -cfm 400|420|422|444 (mask mode: MSE, PSNR, WSPSNR, IVPSNR)\n"
-      " -csi / -csm RGB|BGR|GBR|YCbCr|YCbCr_BT601|YCbCr_SMPTE170M|YCbCr_BT709|YCbCr_SMPTE240M|YCbCr_BT2020\n");
+      " -csi / -csm RGB|BGR|GBR|YCbCr|YCbCr_BT601|YCbCr_SMPTE170M|YCbCr_BT709|YCbCr_SMPTE240M|YCbCr_BT2020\n"
+      " -ff RAW|PNG|BMP (PNG / BMP: a file name pattern with an index field, e.g. frame_{:04d}.png; 8-bit RGB)\n");
 }
 
 // Returns false (after printing the problems) when the options are not usable.
@@ -242,7 +245,8 @@ bool parse_options(int argc, char** argv, Options& o) {
     else if (a == "-cfm") int_opt(o.cfm);
     else if (a == "-csi") o.csi_s = need(i);
     else if (a == "-csm") o.csm_s = need(i);
-    else if (a == "-ff" || a == "-c" || a == "-nma") {
+    else if (a == "-ff") o.ff = upper(trim(need(i)));
+    else if (a == "-c" || a == "-nma") {
       const std::string v = need(i);
       if (!(a == "-ff" && upper(v) == "RAW") && !(a == "-nma")) errs += "!  unsupported option " + a + " " + v + "\n";
     } else errs += "!  unknown option " + a + "\n";
@@ -276,8 +280,11 @@ bool parse_options(int argc, char** argv, Options& o) {
     errs += "!  BlockGaussianInt supports StructSimWindow 8 and 16\n";
   if (o.ss_mode_id == 4 && (o.ss_window < 1 || o.ss_window > 64)) errs += "!  StructSimWindow must be 1..64\n";
   if (o.ipa != "SKIP" && o.ipa != "WARN" && o.ipa != "STOP" && o.ipa != "CNCL") errs += "!  invalid InvalidPelActn\n";
+  // file format: PNG / BMP lists hold 8-bit RGB pictures (xAppQMIV.cpp:222-224, :297, :361-362)
+  const bool images = o.ff == "PNG" || o.ff == "BMP";
+  if (o.ff != "RAW" && !images) errs += "!  FileFormat is invalid\n";
   // colour spaces: xAppQMIV.cpp:297-311
-  o.csi = o.csi_s.empty() ? kYCbCr : clrspc_from(trim(o.csi_s));
+  o.csi = o.csi_s.empty() ? (images ? kRGB : kYCbCr) : clrspc_from(trim(o.csi_s));
   o.csm = o.csm_s.empty() ? o.csi : clrspc_from(trim(o.csm_s));
   if (o.csi == kInvalidClrSpc) errs += "!  ColorSpaceInput value is not valid\n";
   if (o.csm == kInvalidClrSpc) errs += "!  ColorSpaceMetric value is not valid\n";
@@ -285,6 +292,14 @@ bool parse_options(int argc, char** argv, Options& o) {
     if (is_ycbcr(o.csi) && is_ycbcr(o.csm)) errs += "!  YCbCr to YCbCr conversion is not supported.\n";
     if (o.csi == kYCbCr || o.csm == kYCbCr) errs += "!  Generic YCbCr cannot be used for colorspace conversion.\n";
   }
+  if (images && o.csi != kRGB)
+    errs += "! Input FileFormat=" + o.ff + " contains data in RGB color space whitch conflicts with defined ColorSpaceInput=" +
+            o.csi_s + "\n";
+  if (images && o.bit_depth != 8)
+    errs += "! Input FileFormat=" + o.ff + " contains 8-bit per pixel data whitch conflicts with defined BitDepth=" +
+            std::to_string(o.bit_depth) + "\n";
+  if (images) o.chroma = 444;  // R, G, B planes at full resolution
+  if (images && !o.inm.empty() && o.bdm != 8) errs += "!  PNG / BMP masks are 8-bit (BitDepthM must be 8)\n";
   // mask mode: xAppQMIV.cpp:284-288, :360
   if (o.bdm < 0) o.bdm = o.bit_depth;
   if (o.cfm < 0) o.cfm = o.chroma;
@@ -395,6 +410,11 @@ int main(int argc, char** argv) {
   const size_t chroma_m = o.cfm == 400 ? 0 : o.cfm == 420 ? 2 * chroma_of(W, H, 1) : o.cfm == 422 ? (size_t)W * H : 2 * luma;
   const size_t frame_bytes_m = (luma + chroma_m) * bpsm, mask_bytes = use_mask ? luma * bpsm : 0;
   const size_t slot_bytes = 2 * frame_bytes + mask_bytes;
+  // device slots keep rows at a pitch of a multiple of 4 samples (the batch ABI's alignment for
+  // full-resolution planes); the upload converts the pitch with 2-D copies
+  const int Wc = W >> cs, Hc = H >> cs, Wp = (W + 3) & ~3, Wcp = (Wc + 3) & ~3;
+  const size_t dframe = ((size_t)Wp * H + 2 * (size_t)Wcp * Hc) * bps;
+  const size_t dslot = 2 * dframe + mask_bytes;
   set_suffixes(use_mask, is_rgb(o.csm));
   // colour processing of the app's preprocessFrames (xAppQMIV.cpp:308-311, :853-892): conversions run
   // on the device into 4:4:4 planes; BGR / GBR reorders (metrics in RGB) are plane swaps
@@ -403,18 +423,32 @@ int main(int argc, char** argv) {
   const bool colour = cvt_y2r || cvt_r2y || reorder;
   const int space = clrspc_lc(cvt_y2r ? o.csi : o.csm);
 
-  FILE* f0 = fopen(o.in0.c_str(), "rb");
-  FILE* f1 = fopen(o.in1.c_str(), "rb");
-  if (!f0 || !f1) {
+  const bool images = o.ff != "RAW";
+  int first_idx[3] = {0, 0, 0};  // image lists: index of each sequence's first file (0 or 1)
+  long long nfiles[3] = {0, 0, 0};
+  if (images) {
+    const std::string* in[3] = {&o.in0, &o.in1, &o.inm};
+    for (int i = 0; i < (o.inm.empty() ? 2 : 3); i++) {
+      nfiles[i] = ivqm::images::count_frames(*in[i], &first_idx[i]);
+      if (nfiles[i] <= 0) {
+        fprintf(stderr, "ERROR --> InputFile opening failure (%s) First Idx=(0 or 1) not found\n", in[i]->c_str());
+        return 1;
+      }
+    }
+  }
+  FILE* f0 = images ? nullptr : fopen(o.in0.c_str(), "rb");
+  FILE* f1 = images ? nullptr : fopen(o.in1.c_str(), "rb");
+  if (!images && (!f0 || !f1)) {
     fprintf(stderr, "ERROR --> InputFile%d does not exist (%s)\n", f0 ? 1 : 0, (f0 ? o.in1 : o.in0).c_str());
     return 1;
   }
-  auto frames_in = [&](FILE* f) {
+  auto frames_in = [&](FILE* f, int i) {
+    if (images) return nfiles[i];
     fseeko(f, 0, SEEK_END);
     const long long n = (long long)(ftello(f) / (off_t)frame_bytes);
     return n;
   };
-  const long long n0 = frames_in(f0) - o.start0, n1 = frames_in(f1) - o.start1;
+  const long long n0 = frames_in(f0, 0) - o.start0, n1 = frames_in(f1, 1) - o.start1;
   long long nf = std::min(n0, n1);
   if (o.num_frames >= 0) nf = std::min<long long>(nf, o.num_frames);
   if (nf <= 0) {
@@ -422,7 +456,11 @@ int main(int argc, char** argv) {
     return 1;
   }
   const int N = (int)nf;
-  if (use_mask) {  // the mask sequence is read from its first frame (xAppQMIV.cpp:622-627)
+  if (use_mask && images && N > nfiles[2]) {
+    fprintf(stderr, "ERROR --> FramesToProcess > NumOfFramesM\n");
+    return 1;
+  }
+  if (use_mask && !images) {  // the mask sequence is read from its first frame (xAppQMIV.cpp:622-627)
     FILE* fm = fopen(o.inm.c_str(), "rb");
     if (!fm) {
       fprintf(stderr, "ERROR --> InputFileM does not exist (%s)\n", o.inm.c_str());
@@ -473,8 +511,8 @@ int main(int argc, char** argv) {
   std::vector<hipEvent_t> copied(kSlots);
   for (int s = 0; s < kSlots; s++) {
     HIPX(hipHostMalloc((void**)&host[s], slot_bytes, hipHostMallocDefault));
-    HIPX(hipMalloc((void**)&dev[s], slot_bytes));
-    if (colour) HIPX(hipMalloc((void**)&conv[s], 2 * 3 * luma * sizeof(uint16_t)));
+    HIPX(hipMalloc((void**)&dev[s], dslot));
+    if (colour) HIPX(hipMalloc((void**)&conv[s], 2 * 3 * (size_t)Wp * H * sizeof(uint16_t)));
     HIPX(hipEventCreateWithFlags(&copied[s], hipEventDisableTiming));
   }
 
@@ -488,8 +526,9 @@ int main(int argc, char** argv) {
   // sample check run in parallel; the slot order is kept by frame index)
   const int kReaders = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
   auto read_frames = [&](int t) {
-    FILE* fs[3] = {fopen(o.in0.c_str(), "rb"), fopen(o.in1.c_str(), "rb"),
-                   use_mask ? fopen(o.inm.c_str(), "rb") : nullptr};
+    FILE* fs[3] = {images ? nullptr : fopen(o.in0.c_str(), "rb"), images ? nullptr : fopen(o.in1.c_str(), "rb"),
+                   use_mask && !images ? fopen(o.inm.c_str(), "rb") : nullptr};
+    std::vector<uint8_t> mrgb(images && use_mask ? 3 * luma : 0);
     const int st[2] = {o.start0, o.start1};
     for (int f = t; f < N; f += kReaders) {
       const int s = f % kSlots;
@@ -500,12 +539,32 @@ int main(int argc, char** argv) {
       }
       int err = 0;
       uint8_t* dst = host[s];
-      for (int i = 0; i < 2 && !err; i++) {
+      for (int i = 0; i < 2 && !err && images; i++) {  // one image file per frame (xSeqLST.cpp:189-241, :337-415)
+        const std::string name = ivqm::images::format_name(i ? o.in1 : o.in0, first_idx[i] + st[i] + f);
+        const std::string e = o.ff == "PNG" ? ivqm::images::read_png(name, W, H, dst + i * frame_bytes)
+                                            : ivqm::images::read_bmp(name, W, H, dst + i * frame_bytes);
+        if (!e.empty()) {
+          fprintf(stderr, "Frame %08d ERROR --> %s\n", f, e.c_str());
+          err = 1;
+        }
+      }
+      if (!err && images && use_mask) {  // the mask is the first (R) plane of its image
+        const std::string name = ivqm::images::format_name(o.inm, first_idx[2] + f);
+        const std::string e = o.ff == "PNG" ? ivqm::images::read_png(name, W, H, mrgb.data())
+                                            : ivqm::images::read_bmp(name, W, H, mrgb.data());
+        if (!e.empty()) {
+          fprintf(stderr, "Frame %08d ERROR --> %s\n", f, e.c_str());
+          err = 1;
+        } else {
+          memcpy(dst + 2 * frame_bytes, mrgb.data(), luma);
+        }
+      }
+      for (int i = 0; i < 2 && !err && !images; i++) {
         if (!fs[i] || fseeko(fs[i], (off_t)(st[i] + f) * (off_t)frame_bytes, SEEK_SET) != 0 ||
             fread(dst + i * frame_bytes, 1, frame_bytes, fs[i]) != frame_bytes)
           err = 1;
       }
-      if (!err && use_mask &&
+      if (!err && use_mask && !images &&
           (!fs[2] || fseeko(fs[2], (off_t)f * (off_t)frame_bytes_m, SEEK_SET) != 0 ||
            fread(dst + 2 * frame_bytes, 1, mask_bytes, fs[2]) != mask_bytes))
         err = 1;
@@ -545,16 +604,29 @@ int main(int argc, char** argv) {
   };
 
   std::vector<ivqm_frame_result> res(N);
-  auto picture = [&](uint8_t* base) {
+  auto picture = [&](uint8_t* base) {  // a picture of a device slot (pitched rows)
     ivqm_picture p;
     p.plane[0] = base;
-    p.plane[1] = base + luma * bps;
-    p.plane[2] = base + (luma + chroma) * bps;
-    p.stride[0] = W;
-    p.stride[1] = p.stride[2] = W >> cs;
+    p.plane[1] = base + (size_t)Wp * H * bps;
+    p.plane[2] = base + ((size_t)Wp * H + (size_t)Wcp * Hc) * bps;
+    p.stride[0] = Wp;
+    p.stride[1] = p.stride[2] = Wcp;
     p.chroma_420 = cs;
     p.bytes_per_sample = bps;
     return p;
+  };
+  // host frame (tight rows) -> device slot (pitched rows), on the copy stream
+  auto upload = [&](uint8_t* dst, const uint8_t* src) -> bool {
+    const int pw[3] = {W, Wc, Wc}, ph[3] = {H, Hc, Hc}, pp[3] = {Wp, Wcp, Wcp};
+    size_t so = 0, doff = 0;
+    for (int k = 0; k < 3; k++) {
+      if (hipMemcpy2DAsync(dst + doff, (size_t)pp[k] * bps, src + so, (size_t)pw[k] * bps, (size_t)pw[k] * bps, ph[k],
+                           hipMemcpyHostToDevice, xstream) != hipSuccess)
+        return false;
+      so += (size_t)pw[k] * ph[k] * bps;
+      doff += (size_t)pp[k] * ph[k] * bps;
+    }
+    return true;
   };
   const int ngroups = (N + kGroup - 1) / kGroup;
   const auto t_begin = std::chrono::steady_clock::now();
@@ -584,22 +656,25 @@ int main(int argc, char** argv) {
           fprintf(stderr, "Frame %08d ERROR --> sample value above %u (InvalidPelActn=STOP)\n", f, maxv);
           return fail(1);
         }
-        if (hipMemcpyAsync(dev[s], host[s], slot_bytes, hipMemcpyHostToDevice, xstream) != hipSuccess) return fail(2);
+        if (!upload(dev[s], host[s]) || !upload(dev[s] + dframe, host[s] + frame_bytes) ||
+            (use_mask && hipMemcpyAsync(dev[s] + 2 * dframe, host[s] + 2 * frame_bytes, mask_bytes,
+                                        hipMemcpyHostToDevice, xstream) != hipSuccess))
+          return fail(2);
         ivqm_pair pr;
         pr.p0 = picture(dev[s]);
-        pr.p1 = picture(dev[s] + frame_bytes);
+        pr.p1 = picture(dev[s] + dframe);
         if (colour) {
           ivqm_picture* pp[2] = {&pr.p0, &pr.p1};
           for (int i = 0; i < 2; i++) {
-            uint16_t* d = conv[s] + (size_t)i * 3 * luma;
-            const int rc = reorder ? ivqm_upsample420(pp[i], W, H, d, W, (void*)xstream)
-                                   : ivqm_convert_colour(pp[i], W, H, o.bit_depth, cvt_y2r ? 0 : 1, space, d, W,
+            uint16_t* d = conv[s] + (size_t)i * 3 * Wp * H;
+            const int rc = reorder ? ivqm_upsample420(pp[i], W, H, d, Wp, (void*)xstream)
+                                   : ivqm_convert_colour(pp[i], W, H, o.bit_depth, cvt_y2r ? 0 : 1, space, d, Wp,
                                                          (void*)xstream);
             if (rc != IVQM_OK) {
               fprintf(stderr, "ERROR --> %s (%s)\n", ivqm_strerror(rc), ivqm_last_error());
               return fail(2);
             }
-            uint16_t* pl[3] = {d, d + luma, d + 2 * luma};
+            uint16_t* pl[3] = {d, d + (size_t)Wp * H, d + 2 * (size_t)Wp * H};
             if (reorder && o.csi == kBGR) std::swap(pl[0], pl[2]);  // BGR -> RGB
             if (reorder && o.csi == kGBR) {                         // GBR -> BGR -> RGB
               std::swap(pl[0], pl[1]);
@@ -608,7 +683,7 @@ int main(int argc, char** argv) {
             ivqm_picture q;
             for (int k = 0; k < 3; k++) {
               q.plane[k] = pl[k];
-              q.stride[k] = W;
+              q.stride[k] = Wp;
             }
             q.chroma_420 = 0;
             q.bytes_per_sample = 2;
@@ -620,7 +695,7 @@ int main(int argc, char** argv) {
         pr.height = H;
         pr.bit_depth = o.bit_depth;
         pairs.push_back(pr);
-        masks.push_back(ivqm_mask{use_mask ? dev[s] + 2 * frame_bytes : nullptr, W, bpsm, o.bdm, 0});
+        masks.push_back(ivqm_mask{use_mask ? dev[s] + 2 * dframe : nullptr, W, bpsm, o.bdm, 0});
       }
       if (hipStreamWaitEvent(cstream, copied[(fe - 1) % kSlots], 0) != hipSuccess) return fail(2);
       const int rc = use_mask ? ivqm_batch_enqueue_masked(batch, pairs.data(), masks.data(), (int)pairs.size(), metrics,
@@ -709,7 +784,7 @@ int main(int argc, char** argv) {
   ivqm_batch_destroy(batch);
   (void)hipStreamDestroy(cstream);
   (void)hipStreamDestroy(xstream);
-  fclose(f0);
-  fclose(f1);
+  if (f0) fclose(f0);  // image lists open their files per frame
+  if (f1) fclose(f1);
   return 0;
 }

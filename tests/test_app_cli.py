@@ -39,7 +39,9 @@ def test_help():
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv"], "cannot be combined with Mask mode"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-bdm", "16"], "BitDepthM"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-im", "m.yuv", "-ml", "PSNR", "-cfm", "411"], "ChromaFormatM"),
-    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "PNG"], "unsupported option -ff"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "TIFF"], "FileFormat is invalid"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "PNG", "-bd", "10"], "contains 8-bit per pixel data"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ff", "BMP", "-csi", "YCbCr_BT709"], "contains data in RGB color space"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-csi", "XYZ"], "ColorSpaceInput value is not valid"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-csi", "YCbCr_BT709", "-csm", "YCbCr_BT601"],
      "YCbCr to YCbCr conversion is not supported"),
@@ -77,6 +79,12 @@ def test_mask_file_checks(tmp_path):
     m.write_bytes(bytes(64 * 64) * 2)  # two luma-only frames
     r = _run(*base, "-im", str(m), "-cfm", "400")
     assert r.returncode == 1 and "FramesToProcess > NumOfFramesM" in r.stderr
+
+
+def test_image_list_not_found(tmp_path):
+    r = _run("-i0", str(tmp_path / "a_{:03d}.png"), "-i1", str(tmp_path / "b_{}.png"), "-ps", "64x64", "-ff", "PNG",
+             "-ml", "PSNR")
+    assert r.returncode == 1 and "First Idx=(0 or 1) not found" in r.stderr
 
 
 def test_missing_input_file(tmp_path):

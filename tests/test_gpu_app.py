@@ -133,7 +133,7 @@ def test_qmiv_front_end_regular_ssim_mode(tmp_path, oracle):
     per-frame SSIM / MS-SSIM / IV-SSIM lines against the oracle in that mode."""
     if not os.path.exists(EXE):
         pytest.fail("front end not built (python -m qmiv_amd.build)")
-    wl = synth.Workload(cfg=411, width=160, height=128)
+    wl = synth.Workload(cfg=411, width=150, height=126)  # width not a multiple of 4: pitched device slots
     n, bd = 3, 10
     pairs = [synth.make_pair(wl, i) for i in range(n)]
     f0, f1 = tmp_path / "a.yuv", tmp_path / "b.yuv"
