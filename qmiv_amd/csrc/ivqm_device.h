@@ -47,7 +47,17 @@ struct PairDev {
   int64_t aux_off;        // [4][h] int64: samples that differ per 4:4:4 row of each component; non-zero
                           // mask samples per row
   int64_t rows64_off;     // [2][3][h] uint64 masked search row sums (direction 0: P0 + GCD searched in P1)
+  // Fused IV-SSIM: instead of the SCP pictures, the search writes the exact 4x4 sub-block moments of
+  // (test picture, SCP) -- {S_a | S_b << 16, S_aa, S_bb, S_ab} as 4 x uint32 -- at mom_off (in uint16
+  // units of the SCP arena, 256-byte aligned): [direction][component][mom_rows][mom_pitch], direction 0
+  // pairing P0 with SCP1. Sub-blocks (sby, sbx) with sby < mom_rows, sbx < mom_cols are written; -1: off.
+  // Keyed search modes (1, 2) only.
+  int64_t mom_off;
+  int32_t mom_rows, mom_cols, mom_pitch, pad2;
 };
+// Largest bit depth of the fused IV-SSIM moments: 16 * 4095 fits the 16-bit halves of S_a | S_b << 16
+// and 64 * 4095^2 the 32-bit window sums.
+constexpr int kMomMaxBits = 12;
 
 // One SSIM evaluation (a picture pair at one pyramid level).
 struct SsimJobDev {
@@ -72,6 +82,11 @@ struct SsimJobDev {
   double kscale;                  // sums -> moments: 1 (float weights), 2^-18 (Q18), 1/area (averaged)
   const double* wt;               // wnd x wnd weights (row pitch wnd) followed by their transpose, or null
                                   // for averaged modes
+  // Block-averaged 8 / 4 only: the search's 4x4 sub-block moments of (a, b) ([3][mom_rows][mom_pitch] x
+  // 4 uint32, see PairDev::mom_off) replace reading the pictures; null: read a and b.
+  const uint32_t* mom;
+  int64_t mom_off;                // offset in the SCP arena (uint16 units) the host turns into `mom`
+  int32_t mom_pitch, mom_rows;
 };
 
 // One downsampling job of a pyramid level.

@@ -54,13 +54,14 @@ struct PairPlan {
   bool masked = false;
   int bdm = 0;            // mask bit depth
   int64_t rows64_off = 0;
+  bool fused = false;     // IV-SSIM from the search's sub-block moments (no SCP pictures)
 };
 
 // A picture of a plan: a caller's picture, or three planes in one of the
 // engine's arenas (turned into device pointers, with a bounds check, when the
 // batch is issued).
 struct PicRef {
-  int arena = 0;        // 0: caller picture, 1: SCP arena, 2: pyramid arena
+  int arena = 0;        // 0: caller picture, 1: SCP arena, 2: pyramid arena, 3: none (fused moments)
   int64_t off = 0;      // element offset of plane 0 in the arena
   int pitch = 0, w = 0, h = 0;
   PicDev user;

@@ -276,6 +276,11 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   const bool ms_any = (m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) != 0;
   const bool regular = cfg_.ssim_mode <= 2;
   P.ssim_gen = !(cfg_.ssim_mode == 4 && wnd == 8 && str == 4);
+  // Fused IV-SSIM (block-averaged 8 / 4): the search writes 4x4 sub-block moments
+  // instead of the SCP pictures when nothing else reads them -- batches of two or
+  // more pairs (a single pair keeps its SCP for ivqm_get_scp and the IV-SSIM
+  // entry points that reuse it), no IV-MS-SSIM pyramid, 32-bit sums.
+  const bool fuse_ok = n > 1 && !reuse && !scp_only && !P.ssim_gen && !cfg_.ssim_wide_sums;
   for (int i = 0; i < n; i++) {
     const PairSpec& s = pairs[i];
     PairPlan& pp = P.pp[i];
@@ -341,13 +346,28 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
     }
     if ((int64_t)d.scp_pitch * s.h * 2 >= (1ll << 32)) return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
-    if ((P.need_scp && ((mp & ivs) || scp_only)) || (reuse && (mp & ivs))) {
+    // the keyed search variants (1 narrow, 2 dot) write the moments
+    const bool fused = fuse_ok && (mp & IVQM_M_IVSSIM) && !(mp & IVQM_M_IVMSSSIM) && !s.msk &&
+                       s.bd <= kMomMaxBits && (d.mode == 1 || d.mode == 2) && num_blocks(s.w, 8, 4) > 0 &&
+                       num_blocks(s.h, 8, 4) > 0;
+    d.mom_off = -1;
+    if (fused) {
+      // sub-blocks 0 .. num_blocks of each axis: those the 8 / 4 windows cover, all inside the picture
+      d.scp_off = -1;
+      d.mom_rows = num_blocks(s.h, 8, 4) + 1;
+      d.mom_cols = num_blocks(s.w, 8, 4) + 1;
+      d.mom_pitch = (d.mom_cols + 15) & ~15;  // a 64-column search tile stores 16 sub-blocks = 256 bytes
+      P.scp_total = (P.scp_total + 127) & ~(int64_t)127;  // 256-byte aligned
+      d.mom_off = P.scp_total;
+      P.scp_total += 2 * 3 * (int64_t)d.mom_rows * d.mom_pitch * 8;  // 4 x uint32 = 8 uint16 per sub-block
+    } else if ((P.need_scp && ((mp & ivs) || scp_only)) || (reuse && (mp & ivs))) {
       d.scp_off = P.scp_total;
       P.scp_total += 6 * (int64_t)d.scp_pitch * s.h;
     } else {
       d.scp_off = -1;
     }
     pp.scp_off = d.scp_off;
+    pp.fused = fused;
     d.best_off = (best_on_ && n == 1) ? 0 : -1;
     pp.ns = ms_num_scales(s.w, s.h);
     if (cfg_.erp) pp.erp_h = s.h;
@@ -469,6 +489,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     J.kscale = J.mode == 0 ? 1.0 : (J.mode == 1 || J.mode == 3) ? (double)1.0 / (double)262144
                : J.mode == 2 ? (double)1.0 / (double)121 : (double)1.0 / (double)(wnd * wnd);
     J.wt = d_wts_;
+    J.mom_off = -1;
     *slot = J.out_idx;
     P.jobs.push_back(J);
     P.job_a.push_back(a);
@@ -482,6 +503,19 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if (mp & IVQM_M_IVSSIM) {
       RCCHK(add_job(i, &pp.iv_t2r, ref_for(i, 0, 0), ref_for(i, 3, 0), pp.w, pp.h, 1, cfg_.erp));
       RCCHK(add_job(i, &pp.iv_r2t, ref_for(i, 1, 0), ref_for(i, 2, 0), pp.w, pp.h, 1, cfg_.erp));
+      if (pp.fused) {  // search direction 0 pairs P0 with SCP1 (T2R), direction 1 P1 with SCP0 (R2T)
+        const PairDev& d = P.pairs[i];
+        for (int dir = 0; dir < 2; dir++) {
+          const int j = dir ? pp.iv_r2t : pp.iv_t2r;
+          SsimJobDev& J = P.jobs[j];
+          if (J.nux + 1 != d.mom_cols || J.nuy + 1 != d.mom_rows)
+            return set_error(IVQM_EINVAL, "internal: fused moment grid mismatch");
+          J.mom_off = d.mom_off + (int64_t)dir * 3 * d.mom_rows * d.mom_pitch * 8;
+          J.mom_pitch = d.mom_pitch;
+          J.mom_rows = d.mom_rows;
+          P.job_b[j].arena = 3;
+        }
+      }
     }
     if (mp & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) {
       for (int l = 0; l < pp.ns; l++) {
@@ -610,6 +644,10 @@ int Engine::materialize(const PicRef& r, PicDev& out) const {
     out = r.user;
     return IVQM_OK;
   }
+  if (r.arena == 3) {  // fused IV-SSIM: the job reads moments, not this picture
+    std::memset(&out, 0, sizeof(out));
+    return IVQM_OK;
+  }
   uint16_t* base = r.arena == 1 ? d_scp_ : d_pyr_;
   const int64_t cap = r.arena == 1 ? cap_scp_ : cap_pyr_;
   const int64_t plane = (int64_t)r.pitch * r.h;
@@ -632,6 +670,13 @@ int Engine::fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_fi
     S.t_jobs[j] = P.jobs[j];
     RCCHK(materialize(P.job_a[j], S.t_jobs[j].a));
     RCCHK(materialize(P.job_b[j], S.t_jobs[j].b));
+    SsimJobDev& J = S.t_jobs[j];
+    J.mom = nullptr;
+    if (J.mom_off >= 0) {
+      if (!d_scp_ || J.mom_off % 128 || J.mom_off + 3 * (int64_t)J.mom_rows * J.mom_pitch * 8 > cap_scp_)
+        return set_error(IVQM_EINVAL, "internal: fused moments outside the SCP arena");
+      J.mom = reinterpret_cast<const uint32_t*>(d_scp_ + J.mom_off);
+    }
   }
   int nd = 0;
   for (int l = 0; l < 5; l++) down_first[l] = 0;

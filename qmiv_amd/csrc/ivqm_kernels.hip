@@ -450,6 +450,22 @@ __device__ __forceinline__ void row16_sum3_u32_dpp(uint32_t v[3]) {
   dpp3_add<0x118>(v);
 }
 
+// Cross-row sums with the gfx950 lane-swap instructions (no LDS): v_permlane32_swap
+// exchanges lanes 32-63 of its first operand with lanes 0-31 of its second, so
+// x' + y' is x's 2-lane sum (lanes l, l ^ 32) in lanes 0-31 and y's in lanes
+// 32-63; v_permlane16_swap exchanges the odd 16-lane rows of the first operand
+// with the even rows of the second: x' + y' is x's sum over rows (0, 1) in row 0
+// and over rows (2, 3) in row 2, y's in rows 1 and 3.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint32_t swap_sum32(uint32_t x, uint32_t y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+  return r[0] + r[1];
+}
+__device__ __forceinline__ uint32_t swap_sum16(uint32_t x, uint32_t y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+  return r[0] + r[1];
+}
+
 template <typename T, int R, int MODE>
 __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
                                             SEnt (*lds2)[SearchTile<R>::LH][SearchTile<R>::LP], int wY, int wCb,
@@ -779,41 +795,91 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     atomicAdd(rsp + H, q[1]);
     atomicAdd(rsp + 2 * H, q[2]);
   }
-  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0). The SCP
-  // pitch is a multiple of the 64-column tile width, so a lane's 4 samples
-  // stay inside the row; columns >= W are padding.
-  // SCP planes are below 4 GiB (host-checked): a uniform plane base plus a
-  // 32-bit byte offset per lane.
-  if (Pr.scp_off >= 0 && row_ok) {
-    typedef __attribute__((address_space(1))) uint8_t GU8;
-    typedef __attribute__((address_space(1))) u32x2_t GU32x2;
-    const size_t plane_bytes = (size_t)Pr.scp_pitch * H * 2;
-    GU8* pic = (GU8*)(scp + Pr.scp_off) + (size_t)(1 - dir) * 3 * plane_bytes;
-    const uint32_t eo = ((uint32_t)y * (uint32_t)Pr.scp_pitch + (uint32_t)x) * 2u;
-    const int gc[3] = {g0, g1, g2};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      // clip(r - g, 0, Max) for r in [0, Max]: saturating u16 subtract of
-      // max(g, 0), saturating add of max(-g, 0), min Max
-      const u16x2_t gp = (unsigned short)max(gc[c], 0), gn = (unsigned short)max(-gc[c], 0);
-      const u16x2_t mx = (unsigned short)Pr.maxv;
-      u16x2_t v[2];
-#pragma unroll
-      for (int h = 0; h < 2; h++)
-        v[h] = __builtin_elementwise_min(
-            __builtin_elementwise_add_sat(__builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2_t, rp[c][h]), gp), gn),
-            mx);
-      const u32x2_t o = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
-      // streaming store: the SCP is read once, by the SSIM pass, after the whole search
-      __builtin_nontemporal_store(o, (GU32x2*)(pic + c * plane_bytes + eo));
-    }
-  }
   if (Pr.best_off >= 0 && row_ok) {
     typedef __attribute__((address_space(1))) uint8_t GU8;
     GU8* bm = (GU8*)best + Pr.best_off + (size_t)dir * W * H + (size_t)y * W + x;
 #pragma unroll
     for (int k = 0; k < 4; k++)
       if (x + k < W) bm[k] = (uint8_t)idx[k];
+  }
+  // dir 0 writes SCP1 (picture 1), dir 1 writes SCP0 (picture 0). The SCP
+  // pitch is a multiple of the 64-column tile width, so a lane's 4 samples
+  // stay inside the row; columns >= W are padding.
+  // SCP planes are below 4 GiB (host-checked): a uniform plane base plus a
+  // 32-bit byte offset per lane.
+  const bool want_scp = Pr.scp_off >= 0, want_mom = Pr.mom_off >= 0;  // block-uniform
+  if (want_scp || want_mom) {
+    const int gc[3] = {g0, g1, g2};
+    // SCP samples of pixels (0, 1) and (2, 3) per component: clip(r - g, 0, Max)
+    // for r in [0, Max] as a saturating u16 subtract of max(g, 0), a saturating
+    // add of max(-g, 0) and a min with Max
+    u16x2_t sv[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const u16x2_t gp = (unsigned short)max(gc[c], 0), gn = (unsigned short)max(-gc[c], 0);
+      const u16x2_t mx = (unsigned short)Pr.maxv;
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        sv[c][h] = __builtin_elementwise_min(
+            __builtin_elementwise_add_sat(__builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2_t, rp[c][h]), gp), gn),
+            mx);
+    }
+    if (want_scp && row_ok) {
+      typedef __attribute__((address_space(1))) uint8_t GU8;
+      typedef __attribute__((address_space(1))) u32x2_t GU32x2;
+      const size_t plane_bytes = (size_t)Pr.scp_pitch * H * 2;
+      GU8* pic = (GU8*)(scp + Pr.scp_off) + (size_t)(1 - dir) * 3 * plane_bytes;
+      const uint32_t eo = ((uint32_t)y * (uint32_t)Pr.scp_pitch + (uint32_t)x) * 2u;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const u32x2_t o = {__builtin_bit_cast(uint32_t, sv[c][0]), __builtin_bit_cast(uint32_t, sv[c][1])};
+        // streaming store: the SCP is read once, by the SSIM pass, after the whole search
+        __builtin_nontemporal_store(o, (GU32x2*)(pic + c * plane_bytes + eo));
+      }
+    }
+    // Keyed variants only (host-checked): the moments' registers fit beside the
+    // reference-form variants' larger state without lowering occupancy.
+    if (KEYED && want_mom) {
+      // Fused IV-SSIM: the wave's 4 rows x 64 columns are 16 sub-blocks of
+      // 4 x 4 pixels; lanes l, l ^ 16, l ^ 32, l ^ 48 hold the 4 rows of one.
+      // a = the test picture's sample (t - g), b = the SCP sample. Per lane and
+      // component {S_a | S_b << 16, S_aa, S_bb, S_ab} over its 4 pixels (host:
+      // at most 12-bit samples, so no half overflows), summed over the 4 rows
+      // by lane-swap exchanges that leave component g's totals in row group g.
+      u16x2_t ta[3][2];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) ta[c][h] = __builtin_bit_cast(u16x2_t, tp[c][h]) - (unsigned short)gc[c];
+      // one component's {S_a | S_b << 16, S_aa, S_bb, S_ab} over the lane's 4 pixels
+      auto moments = [&](int c, uint32_t (&m)[4]) {
+        const u16x2_t xa = ta[c][0] + ta[c][1], xb = sv[c][0] + sv[c][1];
+        const u16x2_t lo = {xa.x, xb.x}, hi = {xa.y, xb.y};
+        m[0] = __builtin_bit_cast(uint32_t, lo + hi);
+        m[1] = __builtin_amdgcn_udot2(ta[c][1], ta[c][1], __builtin_amdgcn_udot2(ta[c][0], ta[c][0], 0u, false), false);
+        m[2] = __builtin_amdgcn_udot2(sv[c][1], sv[c][1], __builtin_amdgcn_udot2(sv[c][0], sv[c][0], 0u, false), false);
+        m[3] = __builtin_amdgcn_udot2(ta[c][1], sv[c][1], __builtin_amdgcn_udot2(ta[c][0], sv[c][0], 0u, false), false);
+      };
+      uint32_t s0[4], s2[4], p[4];
+      moments(0, s0);
+      moments(2, s2);
+#pragma unroll
+      for (int k = 0; k < 4; k++) p[k] = swap_sum32(s0[k], s2[k]);  // lanes 0-31: component 0, 32-63: component 2
+      uint32_t s1[4];
+      moments(1, s1);
+      u32x4_t fin;
+#pragma unroll
+      for (int k = 0; k < 4; k++) fin[k] = swap_sum16(p[k], swap_sum32(s1[k], s1[k]));  // rows 0, 1, 2: components 0, 1, 2
+      const int g = lane >> 4;
+      const int sby = (y0 >> 2) + wd, sbx = (x0 >> 2) + (lane & 15);
+      if (g < 3 && sby < Pr.mom_rows && sbx < Pr.mom_cols) {
+        typedef __attribute__((address_space(1))) u32x4_t GU32x4;
+        GU32x4* dst = (GU32x4*)(scp + Pr.mom_off) +
+                      (((int64_t)dir * 3 + g) * Pr.mom_rows + sby) * (int64_t)Pr.mom_pitch + sbx;
+        // read once, by the SSIM pass, after the whole search
+        __builtin_nontemporal_store(fin, dst);
+      }
+    }
   }
 }
 
@@ -921,9 +987,12 @@ __device__ __forceinline__ float i64_to_f32(int64_t n) {
 
 // Sum widths of the window moments (host picks by bit depth): SM 0 up to 12-bit, everything in
 // uint32 (64 * 4095^2 < 2^32); SM 1 up to 14-bit, sub-blocks uint32, windows uint64; SM 2 uint64.
-template <typename TA, int CSA, typename TB, int CSB, int SM, int NB>
+// MOM: the sub-block sums come from the search's fused moments (J.mom; SM 0) instead of the pictures --
+// the same integers, so the same scores bit for bit.
+template <typename TA, int CSA, typename TB, int CSB, int SM, int NB, bool MOM = false>
 __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
                           double* __restrict__ partial) {
+  static_assert(!MOM || SM == 0, "fused moments are 32-bit");
   using SQ = typename std::conditional<SM == 2, uint64_t, uint32_t>::type;  // sub-block squared sums
   using WQ = typename std::conditional<SM == 0, uint32_t, uint64_t>::type;  // window squared sums
   const int wy0 = band * NB;
@@ -943,7 +1012,18 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
     if (by > J.nuy) break;
     uint32_t A = 0, B = 0;
     SQ AA = 0, BB = 0, AB = 0;
-    if (col_ok) {
+    if (MOM) {
+      if (col_ok) {
+        typedef __attribute__((address_space(1))) const u32x4_t GM;
+        const u32x4_t m = __builtin_nontemporal_load(
+            (GM*)J.mom + ((int64_t)c * J.mom_rows + by) * (int64_t)J.mom_pitch + bx);
+        A = m.x & 0xffffu;
+        B = m.x >> 16;
+        AA = (SQ)m.y;
+        BB = (SQ)m.z;
+        AB = (SQ)m.w;
+      }
+    } else if (col_ok) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         uint32_t a[4], b[4];
@@ -1039,7 +1119,8 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
   // band is wave-uniform: row pointers stay scalar and every load adds only the lane's column
   const int lane = threadIdx.x & 63, band = bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (J.wsq == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  if (J.mom) ssim_wave<uint16_t, 0, uint16_t, 0, 0, NB, true>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  else if (J.wsq == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
 }
