@@ -53,7 +53,10 @@ struct PairDev {
   // pairing P0 with SCP1. Sub-blocks (sby, sbx) with sby < mom_rows, sbx < mom_cols are written; -1: off.
   // Keyed search modes (1, 2) only.
   int64_t mom_off;
-  int32_t mom_rows, mom_cols, mom_pitch, pad2;
+  int32_t mom_rows, mom_cols, mom_pitch;
+  // Masked search in keyed form: the key of a masked-out candidate's entry (host-checked to exceed every
+  // considered candidate's key without wrapping), or 0 for the reference-form masked search.
+  int32_t mkey;
 };
 // Largest bit depth of the fused IV-SSIM moments: 16 * 4095 fits the 16-bit halves of S_a | S_b << 16
 // and 64 * 4095^2 the 32-bit window sums.
@@ -108,7 +111,7 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
                                const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,
                                hipStream_t st);
 // Masked search of both directions: tiles of 64 x 4 pixels per direction (ivqm_masked.hip).
-constexpr int kMaskTileW = 64, kMaskTileH = 4;
+constexpr int kMaskTileW = 64, kMaskTileH = 16;
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                                 const int32_t* gcd, uint64_t* rows64, hipStream_t st);
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],

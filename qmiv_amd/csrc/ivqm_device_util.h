@@ -101,6 +101,26 @@ __device__ __forceinline__ double wave_sum_f64_dpp(double v) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
   return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
 }
+// uint64 (mod 2^64) sum over the 64 lanes with DPP moves of both halves, same steps as
+// wave_sum_u32_dpp; uniform result read from lane 63.
+__device__ __forceinline__ uint64_t wave_sum_u64_dpp(uint64_t v) {
+#define IVQM_DPPQ(ctrl, rmask, bc)                                                                      \
+  {                                                                                                    \
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, ctrl, rmask, 0xf, bc); \
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), ctrl, rmask, 0xf, bc); \
+    v += (uint64_t)lo | ((uint64_t)hi << 32);                                                          \
+  }
+  IVQM_DPPQ(0x111, 0xf, true)
+  IVQM_DPPQ(0x112, 0xf, true)
+  IVQM_DPPQ(0x114, 0xf, true)
+  IVQM_DPPQ(0x118, 0xf, true)
+  IVQM_DPPQ(0x142, 0xa, false)
+  IVQM_DPPQ(0x143, 0xc, false)
+#undef IVQM_DPPQ
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
 // Index of the last table entry whose first-block index is <= blk.
 template <typename E, typename F>
 __device__ __forceinline__ int find_entry(const E* __restrict__ t, int n, int blk, F first) {

@@ -162,6 +162,7 @@ class Engine {
   int erp_offset(int h, int64_t* off);
   int masked_epilogue(const RingSlot& S, const PairPlan& pp, int i, ivqm_frame_result& r);
   int search_mode(int maxv, int32_t gcd_abs_max) const;
+  int32_t masked_key_base(int maxv, int32_t gcd_abs_max) const;
 
   ivqm_config cfg_;
   hipStream_t stream_ = nullptr;

@@ -215,6 +215,30 @@ int Engine::search_mode(int maxv, int32_t gmax) const {
   return 1;
 }
 
+// Keyed masked search (ivqm_masked.hip): key = S * (Q(r) - 2 sum_c w_c t_c r_c) + offset, a masked-out
+// candidate's entry holds the constant B instead of S * Q(r) + offset. With |t| <= A = Max + |gcd|,
+// r <= Max and T = 2 S wsum A Max bounding the per-pixel term, B = 2^31 - 1 - T never wraps and exceeds
+// every considered key (<= S wsum Max^2 + T + offset) when 2^31 - 1 - 2 T > that bound. Returns B, or 0
+// for the reference-form masked search.
+int32_t Engine::masked_key_base(int maxv, int32_t gmax) const {
+  if (cfg_.search_mode_cap == 1) return 0;
+  const int32_t* w = cfg_.cmp_weights_search;
+  for (int k = 0; k < 3; k++)
+    if (w[k] < 0) return 0;
+  const int R = cfg_.search_range, D = 2 * R + 1;
+  const int64_t S = 1 << key_bits(R);
+  const int64_t A = (int64_t)maxv + gmax;
+  const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
+  const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
+  if (A >= 32768 || 2 * S * wmax * A >= (1ll << 23) || wmax * maxv >= (1ll << 24)) return 0;
+  const int64_t off_max = (int64_t)(kMaskTileH + 2 * R) * D + kMaskTileW + 2 * R;
+  const int64_t T = 2 * S * wsum * A * maxv;
+  const int64_t vmax = S * wsum * (int64_t)maxv * maxv + T + off_max;
+  const int64_t B = (1ll << 31) - 1 - T;
+  if (B - T <= vmax) return 0;
+  return (int32_t)B;
+}
+
 const std::pair<std::vector<double>, double>& Engine::erp_weights(int h) {
   auto it = erp_host.find(h);
   if (it != erp_host.end()) return it->second;
@@ -319,6 +343,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       P.rows64_total += 6 * (int64_t)s.h;
       pp.masked = true;
       pp.bdm = s.msk_bd;
+      d.mkey = masked_key_base(d.maxv, gmax);
       P.any_mask = true;
     }
     d.stat_blk0 = P.stat_blocks;
