@@ -40,7 +40,10 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
                         int32_t* search_tile_rows, int32_t* ssim_band_rows);
 /* Synchronous: results[i] for pairs[i]. */
 int ivqm_batch_run(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, ivqm_frame_result* results);
-/* Asynchronous over a ring of 8 result slots. */
+/* Asynchronous over a ring of 8 result slots. Even and odd slots run on two
+ * device streams with their own buffers, so batches enqueued on consecutive
+ * slots overlap on the device (a stream given by ivqm_batch_set_stream
+ * serializes them). */
 int ivqm_batch_enqueue(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, int32_t ring);
 /* As ivqm_batch_enqueue, with one device-resident mask per pair (ivqm_hip.h,
  * mask mode); masks[i].plane == NULL leaves pair i unmasked. The app's mask

@@ -55,10 +55,15 @@ struct ivqm_ctx {
   size_t own_mask_bytes = 0;
 };
 
+// Two engines, each with its own stream, device buffers and captured graphs:
+// even ring slots run on the first, odd ones on the second, so consecutive
+// batches enqueued on slots k and k + 1 overlap on the device (one batch's
+// HBM-bound phases beside the other's VALU-bound search).
 struct ivqm_batch {
   ivqm_config cfg;
-  std::unique_ptr<Engine> eng;
+  std::unique_ptr<Engine> eng, eng2;
   bool use_graph = false;
+  Engine& lane(int ring) { return (ring & 1) ? *eng2 : *eng; }
 };
 
 namespace {
@@ -590,6 +595,8 @@ int ivqm_batch_create(const ivqm_config* cfg, ivqm_batch** out) {
     b->cfg = *cfg;
     b->eng.reset(new Engine(*cfg));
     RCCHK(b->eng->init());
+    b->eng2.reset(new Engine(*cfg));
+    RCCHK(b->eng2->init());
     *out = b.release();
     return IVQM_OK;
   });
@@ -598,6 +605,7 @@ int ivqm_batch_destroy(ivqm_batch* b) {
   return guarded([&]() -> int {
     if (!b) return IVQM_OK;
     if (b->eng) (void)b->eng->sync();
+    if (b->eng2) (void)b->eng2->sync();
     delete b;
     return IVQM_OK;
   });
@@ -605,13 +613,16 @@ int ivqm_batch_destroy(ivqm_batch* b) {
 int ivqm_batch_set_stream(ivqm_batch* b, void* s) {
   return guarded([&]() -> int {
     if (!b) return set_error(IVQM_EINVAL, "NULL argument");
-    return b->eng->set_stream((hipStream_t)s);
+    // a caller's stream serializes both lanes on it
+    RCCHK(b->eng->set_stream((hipStream_t)s));
+    return b->eng2->set_stream((hipStream_t)s);
   });
 }
 int ivqm_batch_set_tuning(ivqm_batch* b, int32_t search_tile_rows, int32_t ssim_band_rows, int32_t use_graph) {
   return guarded([&]() -> int {
     if (!b) return set_error(IVQM_EINVAL, "NULL argument");
     b->eng->set_tuning(search_tile_rows, ssim_band_rows);
+    b->eng2->set_tuning(search_tile_rows, ssim_band_rows);
     b->use_graph = use_graph != 0;
     return IVQM_OK;
   });
@@ -654,7 +665,8 @@ int ivqm_batch_enqueue(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_
     if (!b) return set_error(IVQM_EINVAL, "NULL argument");
     std::vector<PairSpec> specs;
     RCCHK(batch_specs(pairs, n, specs));
-    return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
+    if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+    return b->lane(ring).enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
   });
 }
 int ivqm_batch_enqueue_masked(ivqm_batch* b, const ivqm_pair* pairs, const ivqm_mask* masks, int32_t n,
@@ -668,13 +680,15 @@ int ivqm_batch_enqueue_masked(ivqm_batch* b, const ivqm_pair* pairs, const ivqm_
       RCCHK(validate_mask(pairs[i].width, pairs[i].height, &masks[i]));
       mask_into_spec(masks[i], specs[i]);
     }
-    return b->eng->enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
+    if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+    return b->lane(ring).enqueue(specs.data(), n, metrics & 0xFFu, ring, b->use_graph);
   });
 }
 int ivqm_batch_collect(ivqm_batch* b, int32_t ring, ivqm_frame_result* results) {
   return guarded([&]() -> int {
     if (!b || !results) return set_error(IVQM_EINVAL, "NULL argument");
-    return b->eng->collect(ring, results);
+    if (ring < 0 || ring >= kRing) return set_error(IVQM_EINVAL, "ring slot out of range");
+    return b->lane(ring).collect(ring, results);
   });
 }
 int ivqm_batch_run(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32_t metrics, ivqm_frame_result* results) {
@@ -720,6 +734,7 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
     if (t < best_j) { best_j = t; nb_best = nbs[ni]; }
   }
   b->eng->set_tuning(py_best, nb_best);
+  b->eng2->set_tuning(py_best, nb_best);
   if (search_tile_rows) *search_tile_rows = py_best;
   if (ssim_band_rows) *ssim_band_rows = nb_best;
   return IVQM_OK;
