@@ -72,27 +72,41 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // four 16-byte loads per picture in flight per lane; wave shuffles, no LDS.
 // ---------------------------------------------------------------------------
 // Adds the differences r - a of one 16-byte vector of samples (8 x 16-bit or
-// 16 x 8-bit) to the row sums; a square fits uint32 (<= 65535^2).
+// 16 x 8-bit) to 32-bit partial sums; the caller widens them once per 4
+// vectors (32 or 64 samples): up to 12-bit samples a square is < 2^24 and 64 of
+// them fit uint32. 16-bit pairs use v_dot2 on the packed int16 differences
+// (|d| < 2^15); bytes use v_dot4: sum (r - a)^2 = r.r + a.a - 2 r.a per word.
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
 template <typename T>
-__device__ __forceinline__ void stats_vec(const uint4& va, const uint4& vr, int64_t& sd, uint64_t& ssd) {
+__device__ __forceinline__ void stats_vec32(const uint4& va, const uint4& vr, int32_t& s, uint32_t& q) {
+  const uint32_t wa[4] = {va.x, va.y, va.z, va.w}, wr[4] = {vr.x, vr.y, vr.z, vr.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (sizeof(T) == 2) {
+      const s16x2_t d = __builtin_bit_cast(s16x2_t, wr[k]) - __builtin_bit_cast(s16x2_t, wa[k]);
+      s = __builtin_amdgcn_sdot2(d, s16x2_t{1, 1}, s, false);
+      q = (uint32_t)__builtin_amdgcn_sdot2(d, d, (int)q, false);
+    } else {
+      const uint32_t rr = __builtin_amdgcn_udot4(wr[k], wr[k], 0u, false);
+      const uint32_t aa = __builtin_amdgcn_udot4(wa[k], wa[k], 0u, false);
+      const uint32_t ra = __builtin_amdgcn_udot4(wr[k], wa[k], 0u, false);
+      q += rr + aa - 2u * ra;
+      s += (int32_t)__builtin_amdgcn_udot4(wr[k], 0x01010101u, 0u, false) -
+           (int32_t)__builtin_amdgcn_udot4(wa[k], 0x01010101u, 0u, false);
+    }
+  }
+}
+// Reference-width form (13- to 16-bit): squares need 64-bit sums.
+__device__ __forceinline__ void stats_vec64(const uint4& va, const uint4& vr, int64_t& sd, uint64_t& ssd) {
   const uint32_t wa[4] = {va.x, va.y, va.z, va.w}, wr[4] = {vr.x, vr.y, vr.z, vr.w};
   int32_t s = 0;
   uint64_t q = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    if (sizeof(T) == 2) {
-      const int d0 = (int)(wr[k] & 0xffffu) - (int)(wa[k] & 0xffffu);
-      const int d1 = (int)(wr[k] >> 16) - (int)(wa[k] >> 16);
-      s += d0 + d1;
-      q += (uint64_t)((uint32_t)d0 * (uint32_t)d0) + (uint64_t)((uint32_t)d1 * (uint32_t)d1);
-    } else {
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const int d = (int)((wr[k] >> (8 * b)) & 0xffu) - (int)((wa[k] >> (8 * b)) & 0xffu);
-        s += d;
-        q += (uint32_t)(d * d);
-      }
-    }
+    const int d0 = (int)(wr[k] & 0xffffu) - (int)(wa[k] & 0xffffu);
+    const int d1 = (int)(wr[k] >> 16) - (int)(wa[k] >> 16);
+    s += d0 + d1;
+    q += (uint64_t)((uint32_t)d0 * (uint32_t)d0) + (uint64_t)((uint32_t)d1 * (uint32_t)d1);
   }
   sd += s;
   ssd += q;
@@ -116,19 +130,43 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
   int x_done = 0;
   if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(r)) & 15) == 0) {
     const int nv = wp / V;
-    const uint4* a4 = reinterpret_cast<const uint4*>(a);
-    const uint4* r4 = reinterpret_cast<const uint4*>(r);
+    // buffer resources over the row's whole vectors: loads past them return 0
+    // (zero differences), so no per-load bounds branch or 64-bit address math
+    const __amdgpu_buffer_rsrc_t ba =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(a), (short)0, nv * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(r), (short)0, nv * 16, 0x00020000);
     for (int i0 = 0; i0 < nv; i0 += 256) {
       uint4 va[4], vr[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int i = i0 + 64 * k + lane;
-        va[k] = make_uint4(0, 0, 0, 0);
-        vr[k] = va[k];
-        if (i < nv) { va[k] = a4[i]; vr[k] = r4[i]; }
+        const uint32_t off = (uint32_t)(i0 + 64 * k + lane) * 16u;
+        va[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ba, off, 0, 0));
+        vr[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(br, off, 0, 0));
       }
+      if (sizeof(T) == 1) {
+        // bytes: one vector at a time (a scheduling barrier keeps the compiler
+        // from hoisting all 80 dot products, which raised the kernel's registers)
+        int32_t s = 0;
+        uint32_t q = 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) stats_vec<T>(va[k], vr[k], sd, ssd);
+        for (int k = 0; k < 4; k++) {
+          stats_vec32<T>(va[k], vr[k], s, q);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        sd += s;
+        ssd += q;
+      } else if (P.maxv < 4096) {
+        int32_t s = 0;
+        uint32_t q = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) stats_vec32<T>(va[k], vr[k], s, q);
+        sd += s;
+        ssd += q;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) stats_vec64(va[k], vr[k], sd, ssd);
+      }
     }
     x_done = nv * V;
   }
@@ -137,11 +175,8 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
     sd += d;
     ssd += (uint64_t)((uint32_t)d * (uint32_t)d);
   }
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    sd += __shfl_xor(sd, m, 64);
-    ssd += __shfl_xor(ssd, m, 64);
-  }
+  sd = (int64_t)wave_sum_u64_dpp((uint64_t)sd);  // mod 2^64: two's complement sums
+  ssd = wave_sum_u64_dpp(ssd);
   if (lane == 0) {
     row_sd[P.rowsd_off + (int64_t)c * H + y] = sd;
     row_ssd[P.rowsd_off + (int64_t)c * H + y] = ssd;
@@ -163,6 +198,7 @@ __device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64
   const TM* m = (const TM*)P.msk + (size_t)y * P.msk_stride;
   int64_t sd = 0, nd = 0, nz = 0;
   uint64_t ssd = 0;
+#pragma unroll 2
   for (int x = lane; x < W; x += 64) {
     const int d = (int)r[x >> cs] - (int)a[x >> cs];
     const int w = (int)m[x];
@@ -171,13 +207,10 @@ __device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64
     nd += d != 0;
     nz += w != 0;
   }
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) {
-    sd += __shfl_xor(sd, k, 64);
-    ssd += __shfl_xor(ssd, k, 64);
-    nd += __shfl_xor(nd, k, 64);
-    nz += __shfl_xor(nz, k, 64);
-  }
+  sd = (int64_t)wave_sum_u64_dpp((uint64_t)sd);
+  ssd = wave_sum_u64_dpp(ssd);
+  nd = (int64_t)wave_sum_u64_dpp((uint64_t)nd);
+  nz = (int64_t)wave_sum_u64_dpp((uint64_t)nz);
   if (lane == 0) {
     row_sd[P.rowsd_off + (int64_t)c * H + y] = sd;
     row_ssd[P.rowsd_off + (int64_t)c * H + y] = ssd;
@@ -194,7 +227,9 @@ __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs
                                                uint64_t* __restrict__ rows64) {
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * 4;
-  for (int rw = blockIdx.x * 4 + (threadIdx.x >> 6); rw < nrows; rw += nw) {
+  // wave-uniform row index: the pair table entry and row pointers stay scalar
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int rw = blockIdx.x * 4 + wv; rw < nrows; rw += nw) {
     const int pi = find_entry(pairs, npairs, rw, [](const PairDev& p) { return p.stat_blk0; });
     const PairDev& P = pairs[pi];
     const int lb = rw - P.stat_blk0;
@@ -1273,8 +1308,9 @@ hipError_t launch_colour(const PicDev& src, int w, int h, int bit_depth, int dir
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st) {
   if (npairs <= 0 || nblk <= 0) return hipSuccess;
-  // nblk = total plane rows; one wave per row, at most 8 blocks per CU resident
-  const int grid = std::min((nblk + 3) / 4, 2048);
+  // nblk = total plane rows; one wave per row, grid-stride. 73 VGPRs: 6 waves per SIMD, so
+  // 256 CUs x 24 waves = 1536 blocks of 4 waves fill the chip once
+  const int grid = std::min((nblk + 3) / 4, 1536);
   hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
   return hipGetLastError();
 }
