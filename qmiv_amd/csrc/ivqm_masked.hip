@@ -30,6 +30,7 @@ typedef unsigned int u32x2m_t __attribute__((ext_vector_type(2)));
 
 template <int R> struct MaskTile {
   static constexpr int D = 2 * R + 1, LW = kMaskTileW + 2 * R, LH = kMaskTileH + 2 * R;
+  static constexpr int LWP = (LW + 3) & ~3;  // keyed planes' row pitch: 16-byte aligned rows
   static constexpr int KB = D * D <= 16 ? 4 : D * D <= 32 ? 5 : D * D <= 64 ? 6 : 7;  // S = 2^KB > D^2
 };
 static_assert(kMaskTileH == 16 && kMaskTileW == 64, "4 waves x 4 rows, one column per lane");
@@ -110,18 +111,64 @@ __device__ void masked_tile(const PairDev& P, int dir, int x0, int y0, u32x2m_t 
   }
 }
 
+// NE consecutive ints from a 16-byte aligned LDS address: 16-byte reads plus an 8-byte tail.
+template <int NE>
+__device__ __forceinline__ void lds_ints(const int* p, int (&v)[NE]) {
+  typedef int i32x4m_t __attribute__((ext_vector_type(4)));
+  typedef int i32x2m_t __attribute__((ext_vector_type(2)));
+  static_assert(NE % 2 == 0, "run length");
+  const i32x4m_t* q = reinterpret_cast<const i32x4m_t*>(p);
+#pragma unroll
+  for (int i = 0; i < NE / 4; i++) {
+    const i32x4m_t w = q[i];
+    v[4 * i] = w.x;
+    v[4 * i + 1] = w.y;
+    v[4 * i + 2] = w.z;
+    v[4 * i + 3] = w.w;
+  }
+  if (NE % 4) {
+    const i32x2m_t w = *reinterpret_cast<const i32x2m_t*>(q + NE / 4);
+    v[NE - 2] = w.x;
+    v[NE - 1] = w.y;
+  }
+}
+
+// Inclusive 64-bit scan within each 16-lane row (row_shr 1, 2, 4, 8; lanes
+// shifted in from outside the row read 0): lane 15 of a row holds its total.
+__device__ __forceinline__ uint64_t row16_sum_u64_dpp(uint64_t v) {
+#define IVQM_ROWQ(ctrl)                                                                                 \
+  {                                                                                                    \
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, ctrl, 0xf, 0xf, true);  \
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), ctrl, 0xf, 0xf, true); \
+    v += (uint64_t)lo | ((uint64_t)hi << 32);                                                          \
+  }
+  IVQM_ROWQ(0x111)
+  IVQM_ROWQ(0x112)
+  IVQM_ROWQ(0x114)
+  IVQM_ROWQ(0x118)
+#undef IVQM_ROWQ
+  return v;
+}
+
 // Keyed form (host-checked ranges, PairDev::mkey != 0): the unmasked search's
 // narrow key S * (Q(r) - 2 sum_c w_c t_c r_c) + offset with the tile-position
 // offset ly * D + lx in the entry, so a plain integer min is the first strict
 // minimum in raster order (nothing wraps in range, so argmin of the reference's
 // E = T + E' is argmin E'). A masked-out candidate's entry holds mkey, which
-// exceeds every considered key. Entry {S * Q + off | mkey, Y, Cb, Cr}: three
-// 24-bit multiply-adds and a min per candidate.
+// exceeds every considered key. As in k_search, a lane owns 4 adjacent pixels
+// of one row (16 lanes per row, a wave 4 rows): per candidate row it reads the
+// 4 + 2R entries its pixels share once, from four int planes (key base, Y, Cb,
+// Cr; 16-byte reads of lanes 16 bytes apart, no bank conflicts), and scores
+// 4 x (2R + 1) candidates with three 24-bit multiply-adds and a min each.
 template <typename T, typename TM, int R>
-__device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int4 (*lds)[MaskTile<R>::LW], int4 cw,
+__device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int* lds, int4 cw,
                                   const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64) {
   using MT = MaskTile<R>;
-  constexpr int D = MT::D, LW = MT::LW, LH = MT::LH, S = 1 << MT::KB;
+  constexpr int D = MT::D, LW = MT::LW, LH = MT::LH, LWP = MT::LWP, S = 1 << MT::KB, NE = 4 + 2 * R;
+  int (*lk)[LWP] = reinterpret_cast<int (*)[LWP]>(lds);
+  int (*ly_)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + LH * LWP);
+  int (*lb)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + 2 * LH * LWP);
+  int (*lr)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + 3 * LH * LWP);
   const int W = P.w, H = P.h;
   const PicDev& Tp = dir ? P.p1 : P.p0;
   const PicDev& Rp = dir ? P.p0 : P.p1;
@@ -129,71 +176,85 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
   const int ms = P.msk_stride;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int x = x0 + lane;
+  const int iy = wv * 4 + (lane >> 4), ix = 4 * (lane & 15);
+  const int y = y0 + iy, x = x0 + ix;
   const int sg = dir ? -1 : 1;
   const int g0 = sg * gcd[4 * P.res_idx], g1 = sg * gcd[4 * P.res_idx + 1], g2 = sg * gcd[4 * P.res_idx + 2];
-  int ts[4][3];
-  uint32_t mw[4];
+  // test samples t = s + sg * GCD and weights of the lane's pixels (weight 0 outside the picture),
+  // loaded before the region so both sets of loads are in flight together
+  int t[4][3];
+  uint32_t m[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int y = y0 + wv * 4 + r;
-    ts[r][0] = ts[r][1] = ts[r][2] = 0;
-    mw[r] = 0;
-    if (x < W && y < H) {
-      ts[r][0] = ldpix<T>(Tp, 0, x, y) + g0;
-      ts[r][1] = ldpix<T>(Tp, 1, x, y) + g1;
-      ts[r][2] = ldpix<T>(Tp, 2, x, y) + g2;
-      mw[r] = (uint32_t)msk[(size_t)y * ms + x];
+  for (int k = 0; k < 4; k++) {
+    t[k][0] = t[k][1] = t[k][2] = 0;
+    m[k] = 0;
+    if (y < H && x + k < W) {
+      t[k][0] = ldpix<T>(Tp, 0, x + k, y) + g0;
+      t[k][1] = ldpix<T>(Tp, 1, x + k, y) + g1;
+      t[k][2] = ldpix<T>(Tp, 2, x + k, y) + g2;
+      m[k] = (uint32_t)msk[(size_t)y * ms + x + k];
     }
   }
   const uint32_t uY = (uint32_t)cw.x, uCb = (uint32_t)cw.y, uCr = (uint32_t)cw.z;
   for (int e = (int)threadIdx.x; e < LW * LH; e += 256) {
-    const int ly = e / LW, lx = e - ly * LW;
-    const int rx = clampi(x0 - R + lx, 0, W - 1), ry = clampi(y0 - R + ly, 0, H - 1);
+    const int ry_ = e / LW, rx_ = e - ry_ * LW;
+    const int rx = clampi(x0 - R + rx_, 0, W - 1), ry = clampi(y0 - R + ry_, 0, H - 1);
     const uint32_t Y = (uint32_t)ldpix<T>(Rp, 0, rx, ry), Cb = (uint32_t)ldpix<T>(Rp, 1, rx, ry);
     const uint32_t Cr = (uint32_t)ldpix<T>(Rp, 2, rx, ry), M = (uint32_t)msk[(size_t)ry * ms + rx];
     // Q(r) = sum w r^2 exactly: w * r < 2^24 and S * Q + off < 2^31 (host-checked)
     const uint32_t q = __umul24(__umul24(uY, Y), Y) + __umul24(__umul24(uCb, Cb), Cb) + __umul24(__umul24(uCr, Cr), Cr);
-    const int kb = M ? (int)(q * (uint32_t)S + (uint32_t)(ly * D + lx)) : P.mkey;
-    lds[ly][lx] = int4{kb, (int)Y, (int)Cb, (int)Cr};
+    lk[ry_][rx_] = M ? (int)(q * (uint32_t)S + (uint32_t)(ry_ * D + rx_)) : P.mkey;
+    ly_[ry_][rx_] = (int)Y;
+    lb[ry_][rx_] = (int)Cb;
+    lr[ry_][rx_] = (int)Cr;
   }
   __syncthreads();
+  // multipliers -2 S w t (|value| < 2^23, host-checked)
+  int kY[4], kCb[4], kCr[4], bk[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int iy = wv * 4 + r, y = y0 + iy;  // wave-uniform
-    if (y >= H) break;
-    uint64_t v[3] = {0, 0, 0};
-    const uint32_t m = mw[r];
-    if (m != 0) {  // zero for x >= W
-      const int t0 = ts[r][0], t1 = ts[r][1], t2 = ts[r][2];
-      // multipliers -2 S w t (|value| < 2^23, host-checked)
-      const int kY = -2 * S * cw.x * t0, kCb = -2 * S * cw.y * t1, kCr = -2 * S * cw.z * t2;
-      int bk = INT_MAX;
-      constexpr int UY = R <= 2 ? D : 1;  // larger windows: one candidate row at a time (registers)
-#pragma unroll UY
-      for (int dy = 0; dy < D; dy++)
+  for (int k = 0; k < 4; k++) {
+    kY[k] = -2 * S * cw.x * t[k][0];
+    kCb[k] = -2 * S * cw.y * t[k][1];
+    kCr[k] = -2 * S * cw.z * t[k][2];
+    bk[k] = INT_MAX;
+  }
+#pragma unroll 1
+  for (int dy = 0; dy < D; dy++) {  // one candidate row at a time (registers)
+    int ek[NE], ey[NE], eb[NE], er[NE];
+    lds_ints<NE>(&lk[iy + dy][ix], ek);
+    lds_ints<NE>(&ly_[iy + dy][ix], ey);
+    lds_ints<NE>(&lb[iy + dy][ix], eb);
+    lds_ints<NE>(&lr[iy + dy][ix], er);
 #pragma unroll
-        for (int dx = 0; dx < D; dx++) {
-          const int4 c = lds[iy + dy][lane + dx];
-          int acc = __mul24(kY, c.y) + c.x;
-          acc = __mul24(kCb, c.z) + acc;
-          acc = __mul24(kCr, c.w) + acc;
-          bk = min(bk, acc);
-        }
-      const uint32_t idx = (uint32_t)(bk - (iy * D + lane)) & (uint32_t)(S - 1);
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int dx = 0; dx < D; dx++) {
+        int acc = __mul24(kY[k], ey[k + dx]) + ek[k + dx];
+        acc = __mul24(kCb[k], eb[k + dx]) + acc;
+        acc = __mul24(kCr[k], er[k + dx]) + acc;
+        bk[k] = min(bk[k], acc);
+      }
+  }
+  uint64_t v[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (m[k] != 0) {
+      const uint32_t idx = (uint32_t)(bk[k] - (iy * D + ix + k)) & (uint32_t)(S - 1);
       const uint32_t wdy = idx / (uint32_t)D, wdx = idx - wdy * (uint32_t)D;
-      const int4 c = lds[iy + wdy][lane + wdx];
-      const int d0 = t0 - c.y, d1 = t1 - c.z, d2 = t2 - c.w;  // |d| < 2^15: squares are positive int32
-      v[0] = (uint64_t)(int64_t)(d0 * d0) * (uint64_t)m;
-      v[1] = (uint64_t)(int64_t)(d1 * d1) * (uint64_t)m;
-      v[2] = (uint64_t)(int64_t)(d2 * d2) * (uint64_t)m;
+      const int ey = iy + (int)wdy, ex = ix + k + (int)wdx;
+      const int d0 = t[k][0] - ly_[ey][ex], d1 = t[k][1] - lb[ey][ex], d2 = t[k][2] - lr[ey][ex];
+      // |d| < 2^15: the squares are positive int32
+      v[0] += (uint64_t)(int64_t)(d0 * d0) * (uint64_t)m[k];
+      v[1] += (uint64_t)(int64_t)(d1 * d1) * (uint64_t)m[k];
+      v[2] += (uint64_t)(int64_t)(d2 * d2) * (uint64_t)m[k];
     }
-    const uint64_t s0 = wave_sum_u64_dpp(v[0]), s1 = wave_sum_u64_dpp(v[1]), s2 = wave_sum_u64_dpp(v[2]);
-    if (lane < 3) {
-      const uint64_t s = lane == 0 ? s0 : (lane == 1 ? s1 : s2);
-      atomicAdd((unsigned long long*)&rows64[P.rows64_off + (int64_t)(dir * 3 + lane) * H + y],
-                (unsigned long long)s);
-    }
+  }
+  const uint64_t s0 = row16_sum_u64_dpp(v[0]), s1 = row16_sum_u64_dpp(v[1]), s2 = row16_sum_u64_dpp(v[2]);
+  if ((lane & 15) == 15 && y < H) {
+    uint64_t* rp = rows64 + P.rows64_off + (int64_t)dir * 3 * H + y;
+    atomicAdd((unsigned long long*)rp, (unsigned long long)s0);
+    atomicAdd((unsigned long long*)(rp + H), (unsigned long long)s1);
+    atomicAdd((unsigned long long*)(rp + 2 * H), (unsigned long long)s2);
   }
 }
 
@@ -202,8 +263,9 @@ __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict
                                                        int4 cw, const int32_t* __restrict__ gcd,
                                                        uint64_t* __restrict__ rows64) {
   using MT = MaskTile<R>;
-  __shared__ int4 lds4[MT::LH][MT::LW];  // keyed entries; the reference form uses its first half as 8-byte entries
-  auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(&lds4[0][0]);
+  // keyed form: four int planes of LH x LWP; the reference form: LH x LW 8-byte entries
+  __shared__ __attribute__((aligned(16))) int ldsi[4 * MT::LH * MT::LWP];
+  auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(ldsi);
   const int b = blockIdx.x;
   if (b >= ntiles) return;
   const int pi = find_entry(pairs, npairs, b, [](const PairDev& p) { return p.mtile0; });
@@ -215,11 +277,11 @@ __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict
   const int x0 = tx * kMaskTileW, y0 = ty * kMaskTileH;
   if (P.mkey != 0) {
     if (P.p0.bps == 2) {
-      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R>(P, dir, x0, y0, lds4, cw, gcd, rows64);
-      else masked_tile_keyed<uint16_t, uint8_t, R>(P, dir, x0, y0, lds4, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      else masked_tile_keyed<uint16_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
     } else {
-      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R>(P, dir, x0, y0, lds4, cw, gcd, rows64);
-      else masked_tile_keyed<uint8_t, uint8_t, R>(P, dir, x0, y0, lds4, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      else masked_tile_keyed<uint8_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
     }
   } else if (P.p0.bps == 2) {
     if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
