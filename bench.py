@@ -367,7 +367,7 @@ def mask_mode(Q, synth, torch, np, pairs, args):
     with Q.IvqmBatch(erp=1) as bt:
         arr = Q.IvqmBatch.make_pairs([(a, b, W, H, BD, True) for a, b in pairs[:n]])
         marr = Q.IvqmBatch.make_masks([(t, 10) for t in masks])
-        ms = _time_steps(torch, lambda: bt.run_masked(arr, marr, m), 4, 1)
+        ms = _time_steps(torch, lambda: bt.run_masked(arr, marr, m), 12, 2)
         k_ms = bt.profile(3, 10)
     ops = weighted * 2 * 25 * 12
     tops = ops / (k_ms / 1e3) / 1e12

@@ -57,6 +57,8 @@ struct PairDev {
   // Masked search in keyed form: the key of a masked-out candidate's entry (host-checked to exceed every
   // considered candidate's key without wrapping), or 0 for the reference-form masked search.
   int32_t mkey;
+  int32_t mdot;  // keyed masked search with the packed int16 {Y, Cb} key (host-checked ranges)
+  int32_t pad3;
 };
 // Largest bit depth of the fused IV-SSIM moments: 16 * 4095 fits the 16-bit halves of S_a | S_b << 16
 // and 64 * 4095^2 the 32-bit window sums.
@@ -110,10 +112,11 @@ hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row
 hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
                                const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,
                                hipStream_t st);
-// Masked search of both directions: tiles of 64 x 4 pixels per direction (ivqm_masked.hip).
+// Masked search of both directions: tiles of 64 x 16 pixels per direction (ivqm_masked.hip). forms: bit f
+// set when a pair of form f (0 reference form, 1 keyed narrow, 2 keyed dot) is in the batch.
 constexpr int kMaskTileW = 64, kMaskTileH = 16;
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                                const int32_t* gcd, uint64_t* rows64, hipStream_t st);
+                                const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st);
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st);

@@ -239,6 +239,17 @@ int32_t Engine::masked_key_base(int maxv, int32_t gmax) const {
   return (int32_t)B;
 }
 
+// The packed form of the keyed masked search: S * r and 2 * w * t are int16 operands of v_dot2
+// (as the unmasked search's dot variant); search_mode_cap 2 forbids it.
+bool Engine::masked_dot_ok(int maxv, int32_t gmax) const {
+  if (cfg_.search_mode_cap == 2) return false;
+  const int32_t* w = cfg_.cmp_weights_search;
+  const int64_t S = 1 << key_bits(cfg_.search_range);
+  const int64_t A = (int64_t)maxv + gmax;
+  const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
+  return S * maxv <= 32767 && 2 * wmax * A <= 32767;
+}
+
 const std::pair<std::vector<double>, double>& Engine::erp_weights(int h) {
   auto it = erp_host.find(h);
   if (it != erp_host.end()) return it->second;
@@ -344,6 +355,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       pp.masked = true;
       pp.bdm = s.msk_bd;
       d.mkey = masked_key_base(d.maxv, gmax);
+      d.mdot = (d.mkey != 0 && masked_dot_ok(d.maxv, gmax)) ? 1 : 0;
       P.any_mask = true;
     }
     d.stat_blk0 = P.stat_blocks;
@@ -351,8 +363,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.tile0 = P.tiles;  // pairs without a search own no tiles (the table lookup skips them)
     d.mtile0 = P.mtiles;
     if (s.msk) {
-      if (mp & IVQM_M_IVPSNR)
+      if (mp & IVQM_M_IVPSNR) {
         P.mtiles += 2 * ((s.w + kMaskTileW - 1) / kMaskTileW) * ((s.h + kMaskTileH - 1) / kMaskTileH);
+        P.mforms |= 1u << (d.mkey == 0 ? 0 : (d.mdot ? 2 : 1));
+      }
     } else if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only))) {
       P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
     }
@@ -737,7 +751,7 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
   }
   if (P.mtiles)
     HIPCHK(launch_search_masked(d_pairs_, n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows64_,
-                                st));
+                                P.mforms, st));
   if (P.need_search)
     HIPCHK(launch_search(d_pairs_, n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
                          d_scp_, d_best_, st));
@@ -1035,7 +1049,7 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
         break;
       case 3:
         HIPCHK(launch_search_masked(d_pairs_, P.n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_,
-                                    d_rows64_, stream_));
+                                    d_rows64_, P.mforms, stream_));
         break;
       default: return set_error(IVQM_EINVAL, "unknown kernel id");
     }

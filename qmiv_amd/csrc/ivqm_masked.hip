@@ -27,6 +27,7 @@ namespace ivqm {
 namespace {
 
 typedef unsigned int u32x2m_t __attribute__((ext_vector_type(2)));
+typedef short s16x2m_t __attribute__((ext_vector_type(2)));
 
 template <int R> struct MaskTile {
   static constexpr int D = 2 * R + 1, LW = kMaskTileW + 2 * R, LH = kMaskTileH + 2 * R;
@@ -160,15 +161,18 @@ __device__ __forceinline__ uint64_t row16_sum_u64_dpp(uint64_t v) {
 // 4 + 2R entries its pixels share once, from four int planes (key base, Y, Cb,
 // Cr; 16-byte reads of lanes 16 bytes apart, no bank conflicts), and scores
 // 4 x (2R + 1) candidates with three 24-bit multiply-adds and a min each.
-template <typename T, typename TM, int R>
+// DOT (PairDev::mdot): the Y and Cb planes become one plane of int16 pairs {Y * S, Cb * S} scored by one
+// v_dot2 against the pixel's {-2 wY t_Y, -2 wCb t_Cb}: one dot, one 24-bit multiply-add (Cr) and a min per
+// candidate, three planes.
+template <typename T, typename TM, int R, bool DOT>
 __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int* lds, int4 cw,
                                   const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64) {
   using MT = MaskTile<R>;
   constexpr int D = MT::D, LW = MT::LW, LH = MT::LH, LWP = MT::LWP, S = 1 << MT::KB, NE = 4 + 2 * R;
   int (*lk)[LWP] = reinterpret_cast<int (*)[LWP]>(lds);
   int (*ly_)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + LH * LWP);
-  int (*lb)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + 2 * LH * LWP);
-  int (*lr)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + 3 * LH * LWP);
+  int (*lb)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + 2 * LH * LWP);  // unused by DOT
+  int (*lr)[LWP] = reinterpret_cast<int (*)[LWP]>(lds + (DOT ? 2 : 3) * LH * LWP);
   const int W = P.w, H = P.h;
   const PicDev& Tp = dir ? P.p1 : P.p0;
   const PicDev& Rp = dir ? P.p0 : P.p1;
@@ -204,8 +208,12 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
     // Q(r) = sum w r^2 exactly: w * r < 2^24 and S * Q + off < 2^31 (host-checked)
     const uint32_t q = __umul24(__umul24(uY, Y), Y) + __umul24(__umul24(uCb, Cb), Cb) + __umul24(__umul24(uCr, Cr), Cr);
     lk[ry_][rx_] = M ? (int)(q * (uint32_t)S + (uint32_t)(ry_ * D + rx_)) : P.mkey;
-    ly_[ry_][rx_] = (int)Y;
-    lb[ry_][rx_] = (int)Cb;
+    if (DOT) {
+      ly_[ry_][rx_] = (int)((Y * (uint32_t)S) | ((Cb * (uint32_t)S) << 16));
+    } else {
+      ly_[ry_][rx_] = (int)Y;
+      lb[ry_][rx_] = (int)Cb;
+    }
     lr[ry_][rx_] = (int)Cr;
   }
   __syncthreads();
@@ -213,8 +221,13 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
   int kY[4], kCb[4], kCr[4], bk[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    kY[k] = -2 * S * cw.x * t[k][0];
-    kCb[k] = -2 * S * cw.y * t[k][1];
+    if (DOT) {  // {-2 wY t_Y, -2 wCb t_Cb} as an int16 pair (|value| <= 32767, host-checked)
+      kY[k] = (int)(((uint32_t)(-2 * cw.x * t[k][0]) & 0xffffu) | ((uint32_t)(-2 * cw.y * t[k][1]) << 16));
+      kCb[k] = 0;
+    } else {
+      kY[k] = -2 * S * cw.x * t[k][0];
+      kCb[k] = -2 * S * cw.y * t[k][1];
+    }
     kCr[k] = -2 * S * cw.z * t[k][2];
     bk[k] = INT_MAX;
   }
@@ -223,15 +236,20 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
     int ek[NE], ey[NE], eb[NE], er[NE];
     lds_ints<NE>(&lk[iy + dy][ix], ek);
     lds_ints<NE>(&ly_[iy + dy][ix], ey);
-    lds_ints<NE>(&lb[iy + dy][ix], eb);
+    if (!DOT) lds_ints<NE>(&lb[iy + dy][ix], eb);
     lds_ints<NE>(&lr[iy + dy][ix], er);
 #pragma unroll
     for (int k = 0; k < 4; k++)
 #pragma unroll
       for (int dx = 0; dx < D; dx++) {
-        int acc = __mul24(kY[k], ey[k + dx]) + ek[k + dx];
-        acc = __mul24(kCb[k], eb[k + dx]) + acc;
-        acc = __mul24(kCr[k], er[k + dx]) + acc;
+        int acc = __mul24(kCr[k], er[k + dx]) + ek[k + dx];
+        if (DOT) {
+          acc = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2m_t, kY[k]), __builtin_bit_cast(s16x2m_t, ey[k + dx]),
+                                       acc, false);
+        } else {
+          acc = __mul24(kY[k], ey[k + dx]) + acc;
+          acc = __mul24(kCb[k], eb[k + dx]) + acc;
+        }
         bk[k] = min(bk[k], acc);
       }
   }
@@ -242,7 +260,16 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
       const uint32_t idx = (uint32_t)(bk[k] - (iy * D + ix + k)) & (uint32_t)(S - 1);
       const uint32_t wdy = idx / (uint32_t)D, wdx = idx - wdy * (uint32_t)D;
       const int ey = iy + (int)wdy, ex = ix + k + (int)wdx;
-      const int d0 = t[k][0] - ly_[ey][ex], d1 = t[k][1] - lb[ey][ex], d2 = t[k][2] - lr[ey][ex];
+      int rY, rCb;
+      if (DOT) {
+        const uint32_t pk = (uint32_t)ly_[ey][ex];
+        rY = (int)((pk & 0xffffu) >> MT::KB);
+        rCb = (int)((pk >> 16) >> MT::KB);
+      } else {
+        rY = ly_[ey][ex];
+        rCb = lb[ey][ex];
+      }
+      const int d0 = t[k][0] - rY, d1 = t[k][1] - rCb, d2 = t[k][2] - lr[ey][ex];
       // |d| < 2^15: the squares are positive int32
       v[0] += (uint64_t)(int64_t)(d0 * d0) * (uint64_t)m[k];
       v[1] += (uint64_t)(int64_t)(d1 * d1) * (uint64_t)m[k];
@@ -258,54 +285,71 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
   }
 }
 
-template <int R>
+// Form of a pair's masked search: 0 reference form, 1 keyed (narrow), 2 keyed (packed dot).
+__device__ __forceinline__ int masked_form(const PairDev& P) { return P.mkey == 0 ? 0 : (P.mdot ? 2 : 1); }
+
+// One launch per form present in the batch (the forms' register needs differ: one kernel would run
+// every form at the largest); tiles of pairs of other forms exit at once.
+template <int R, int F>
 __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict__ pairs, int npairs, int ntiles,
                                                        int4 cw, const int32_t* __restrict__ gcd,
                                                        uint64_t* __restrict__ rows64) {
   using MT = MaskTile<R>;
-  // keyed form: four int planes of LH x LWP; the reference form: LH x LW 8-byte entries
-  __shared__ __attribute__((aligned(16))) int ldsi[4 * MT::LH * MT::LWP];
-  auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(ldsi);
+  // keyed forms: int planes of LH x LWP; the reference form: LH x LW 8-byte entries
+  __shared__ __attribute__((aligned(16))) int ldsi[(F == 2 ? 3 : 4) * MT::LH * MT::LWP];
   const int b = blockIdx.x;
   if (b >= ntiles) return;
   const int pi = find_entry(pairs, npairs, b, [](const PairDev& p) { return p.mtile0; });
   const PairDev& P = pairs[pi];
+  if (masked_form(P) != F) return;  // block-uniform
   const int nx = (P.w + kMaskTileW - 1) / kMaskTileW, ny = (P.h + kMaskTileH - 1) / kMaskTileH;
   const int lt = b - P.mtile0;
   const int dir = lt / (nx * ny), rem = lt - dir * nx * ny;
   const int ty = rem / nx, tx = rem - ty * nx;
   const int x0 = tx * kMaskTileW, y0 = ty * kMaskTileH;
-  if (P.mkey != 0) {
+  if (F == 0) {
+    auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(ldsi);
     if (P.p0.bps == 2) {
-      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
-      else masked_tile_keyed<uint16_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
+      else masked_tile<uint16_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
     } else {
-      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
-      else masked_tile_keyed<uint8_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile<uint8_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
+      else masked_tile<uint8_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
     }
-  } else if (P.p0.bps == 2) {
-    if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
-    else masked_tile<uint16_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
   } else {
-    if (P.msk_bps == 2) masked_tile<uint8_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
-    else masked_tile<uint8_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
+    constexpr bool DOT = F == 2;
+    if (P.p0.bps == 2) {
+      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      else masked_tile_keyed<uint16_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+    } else {
+      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      else masked_tile_keyed<uint8_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+    }
   }
+}
+
+template <int R>
+hipError_t masked_r(const PairDev* pairs, int npairs, int ntiles, int4 w, const int32_t* gcd, uint64_t* rows64,
+                    uint32_t forms, hipStream_t st) {
+  if (forms & 1u) hipLaunchKernelGGL((k_search_masked<R, 0>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
+  if (forms & 2u) hipLaunchKernelGGL((k_search_masked<R, 1>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
+  if (forms & 4u) hipLaunchKernelGGL((k_search_masked<R, 2>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                                const int32_t* gcd, uint64_t* rows64, hipStream_t st) {
-  if (npairs <= 0 || ntiles <= 0) return hipSuccess;
+                                const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st) {
+  if (npairs <= 0 || ntiles <= 0 || !forms) return hipSuccess;
   const int4 w = make_int4(cw[0], cw[1], cw[2], cw[3]);
   switch (R) {
-    case 1: hipLaunchKernelGGL(k_search_masked<1>, dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64); break;
-    case 2: hipLaunchKernelGGL(k_search_masked<2>, dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64); break;
-    case 3: hipLaunchKernelGGL(k_search_masked<3>, dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64); break;
-    case 4: hipLaunchKernelGGL(k_search_masked<4>, dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64); break;
+    case 1: return masked_r<1>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
+    case 2: return masked_r<2>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
+    case 3: return masked_r<3>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
+    case 4: return masked_r<4>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 }  // namespace ivqm

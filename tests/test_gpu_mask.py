@@ -73,20 +73,21 @@ def test_mask_against_oracle_640x360(oracle, c420, bdm, R):
 
 @pytest.mark.parametrize("bd,R,c420", [(10, 1, True), (10, 2, True), (8, 3, False), (8, 2, True), (10, 2, False)])
 def test_masked_search_keyed_form_equals_reference_form(oracle, bd, R, c420):
-    """The keyed masked search (narrow key, masked candidates keyed above every considered one) and the
-    reference-form masked search (search_mode_cap 1) give the same row sums, and both match the oracle."""
+    """The keyed masked search (packed dot key where the ranges allow, search_mode_cap 0; narrow key,
+    cap 2; masked candidates keyed above every considered one) and the reference-form masked search
+    (cap 1) give the same row sums, and they match the oracle."""
     wl = synth.Workload(cfg=170 + R, width=328, height=200, bit_depth=bd, chroma_420=c420, global_shift=1)
     p0n, p1n = synth.make_pair(wl, 0, as_u8=bd == 8)
     m = synth.make_mask(wl.width, wl.height, 10, seed=11 + R)
     out = []
-    for cap in (0, 1):
+    for cap in (0, 1, 2):
         with Q.IvqmContext(wl.width, wl.height, bd, search_range=R, search_mode_cap=cap) as ctx:
             ctx.upload_pair(p0n, p1n, c420)
             ctx.upload_mask(m, 10)
             r = ctx.frame(Q.IVQM_M_IVPSNR)
             out.append((r.ivpsnr, r.ivpsnr_r2t, r.ivpsnr_t2r, list(r.gcd_r2t)))
-    print("keyed", out[0], "reference form", out[1], flush=True)
-    assert out[0] == out[1]
+    print("keyed", out[0], "reference form", out[1], "narrow key", out[2], flush=True)
+    assert out[0] == out[1] == out[2]
     p0 = oracle.to444([np.asarray(p, np.uint16) for p in p0n], c420)
     p1 = oracle.to444([np.asarray(p, np.uint16) for p in p1n], c420)
     g = oracle.gcd_masked(p0, p1, m, bd, 10)
