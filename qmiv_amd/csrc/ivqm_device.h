@@ -119,7 +119,9 @@ hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, in
                                 const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st);
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
-hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st);
+// forms: bit 0 when jobs read pictures, bit 1 when jobs read fused moments (one launch each).
+hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
+                       hipStream_t st);
 // Any window mode / size / stride / border extension: one wave per 64 window columns and
 // kSsimGenRows window rows, 4 waves per block.
 constexpr int kSsimGenRows = 8;

@@ -570,6 +570,13 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   return IVQM_OK;
 }
 
+// Which k_ssim launches a plan needs: bit 0 jobs that read pictures, bit 1 jobs that read fused moments.
+static uint32_t ssim_forms(const BatchPlan& P) {
+  uint32_t f = 0;
+  for (const SsimJobDev& J : P.jobs) f |= J.mom_off >= 0 ? 2u : 1u;
+  return f;
+}
+
 int Engine::ensure_scp(int64_t elems) {
   if (elems <= cap_scp_) return IVQM_OK;
   RCCHK(sync());
@@ -763,7 +770,7 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
       HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
                              d_partial_, st));
     else
-      HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, st));
+      HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, ssim_forms(P), st));
     HIPCHK(launch_ssim_finalize(d_jobs_, nj, d_partial_, d_erp_, d_ssim_out_, st));
   }
   if (P.need_stats) {
@@ -1045,7 +1052,7 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
           HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
                                  d_partial_, stream_));
         else
-          HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, stream_));
+          HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, ssim_forms(P), stream_));
         break;
       case 3:
         HIPCHK(launch_search_masked(d_pairs_, P.n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_,
