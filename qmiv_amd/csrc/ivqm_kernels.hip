@@ -13,10 +13,13 @@
 //                    mean, round half away, clip), kept in device memory.
 //   k_search         corresponding-pixel-shift search, both directions, LDS
 //                    tile with a +-R halo, int32-wrapping row sums, fused
-//                    shift-compensated-picture epilogue.
+//                    shift-compensated-picture epilogue -- or, for IV-SSIM in
+//                    batches, the exact 4x4 sub-block moments of (test
+//                    picture, SCP) in place of the SCP pictures.
 //   k_ssim           block-averaged 8x8 / stride-4 SSIM: exact integer 4x4
-//                    sub-block moments, 2x2 combination, FP32 ratios, FP64
-//                    per-window-row partial sums.
+//                    sub-block moments (summed from the pictures, or read
+//                    from the search's fused moments), 2x2 combination, FP32
+//                    ratios, FP64 per-window-row partial sums.
 //   (k_ssim_gen*     every other SSIM configuration: ivqm_ssim_modes.hip)
 //   k_ssim_finalize  FP64 frame totals per job and component (ERP-weighted
 //                    for IV-SSIM).
