@@ -143,7 +143,7 @@ constexpr int kSearchTileW = 64;
 constexpr int kSearchRowsPerWave = 4;
 inline int search_tile_h(int py) { return 4 * py; }
 inline int search_rows(int /*R*/, int /*py*/) { return kSearchRowsPerWave; }
-constexpr int kDownBlkW = 64, kDownBlkH = 4;
+constexpr int kDownBlkW = 256, kDownBlkH = 4;  // k_downsample: outputs per block row (4 per lane), rows
 // MI355X: 8 XCDs, each with its own L2; consecutive workgroups go to consecutive XCDs.
 constexpr int kXcds = 8;
 

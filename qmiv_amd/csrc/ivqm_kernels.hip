@@ -1207,6 +1207,63 @@ __device__ __forceinline__ void down_px(const PicDev& S, int c, int x, int y, ui
   dst[(size_t)c * pitch * dh + (size_t)y * pitch + x] = (uint16_t)(v >> 2);
 }
 
+// Four adjacent outputs x4 .. x4 + 3 of row y (x4 + 3 < dw): the two source rows' 8 samples as two
+// vector loads each where the row is aligned for them (wave-uniform: one row per wave), one 8-byte
+// store (pyramid rows are 128-byte aligned). A 4:2:0 source's chroma is the level-1 chroma itself
+// (the 2x2 block replicates one sample), so it is copied.
+template <typename T>
+__device__ __forceinline__ void down_x4(const PicDev& S, int c, int x4, int y, uint16_t* dst, int pitch, int dh) {
+  typedef __attribute__((address_space(1))) const T GT;
+  typedef unsigned short u16x4d_t __attribute__((ext_vector_type(4)));
+  uint32_t o[4];
+  const int cs = c ? S.cs : 0;
+  if (cs) {
+    GT* row = (GT*)plane_ptr(S, c) + (size_t)y * plane_stride(S, c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = row[x4 + k];
+  } else {
+    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c);
+    GT* r1 = r0 + plane_stride(S, c);
+    uint32_t a[8], b[8];
+    constexpr int AL = 4 * (int)sizeof(T);  // a 4-sample vector
+    if (((reinterpret_cast<uintptr_t>(r0) | reinterpret_cast<uintptr_t>(r1)) & (AL - 1)) == 0) {
+      if (sizeof(T) == 2) {
+        typedef __attribute__((address_space(1))) const u16x4d_t GV;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const u16x4d_t va = *(GV*)(r0 + 2 * x4 + 4 * h), vb = *(GV*)(r1 + 2 * x4 + 4 * h);
+          a[4 * h] = va.x; a[4 * h + 1] = va.y; a[4 * h + 2] = va.z; a[4 * h + 3] = va.w;
+          b[4 * h] = vb.x; b[4 * h + 1] = vb.y; b[4 * h + 2] = vb.z; b[4 * h + 3] = vb.w;
+        }
+      } else {
+        typedef __attribute__((address_space(1))) const uint32_t GW;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint32_t wa = *(GW*)(r0 + 2 * x4 + 4 * h), wb = *(GW*)(r1 + 2 * x4 + 4 * h);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            a[4 * h + k] = (wa >> (8 * k)) & 0xffu;
+            b[4 * h + k] = (wb >> (8 * k)) & 0xffu;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        a[k] = r0[2 * x4 + k];
+        b[k] = r1[2 * x4 + k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (a[2 * k] + a[2 * k + 1] + b[2 * k] + b[2 * k + 1] + 2) >> 2;
+  }
+  typedef __attribute__((address_space(1))) u16x4d_t GO;
+  const u16x4d_t v = {(unsigned short)o[0], (unsigned short)o[1], (unsigned short)o[2], (unsigned short)o[3]};
+  *(GO*)(dst + (size_t)c * pitch * dh + (size_t)y * pitch + x4) = v;
+}
+
+// Block = 4 rows x 256 outputs (a lane makes 4 adjacent outputs); the last partial group of a row
+// takes the one-output form.
 __global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ jobs, int njobs) {
   const int b = blockIdx.x;
   const int ji = find_entry(jobs, njobs, b, [](const DownJob& j) { return j.blk0; });
@@ -1217,11 +1274,18 @@ __global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ 
   const int c = lb / (nbx * nby);
   lb -= c * nbx * nby;
   const int by = lb / nbx, bx = lb - by * nbx;
-  const int x = bx * kDownBlkW + (threadIdx.x & 63);
-  const int y = by * kDownBlkH + (threadIdx.x >> 6);
-  if (x >= dw || y >= dh) return;
-  if (J.src.bps == 2) down_px<uint16_t>(J.src, c, x, y, J.dst, J.pitch, dh);
-  else down_px<uint8_t>(J.src, c, x, y, J.dst, J.pitch, dh);
+  const int x4 = bx * kDownBlkW + 4 * (threadIdx.x & 63);
+  const int y = by * kDownBlkH + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (x4 >= dw || y >= dh) return;
+  if (x4 + 3 < dw) {
+    if (J.src.bps == 2) down_x4<uint16_t>(J.src, c, x4, y, J.dst, J.pitch, dh);
+    else down_x4<uint8_t>(J.src, c, x4, y, J.dst, J.pitch, dh);
+  } else {
+    for (int x = x4; x < dw; x++) {
+      if (J.src.bps == 2) down_px<uint16_t>(J.src, c, x, y, J.dst, J.pitch, dh);
+      else down_px<uint8_t>(J.src, c, x, y, J.dst, J.pitch, dh);
+    }
+  }
 }
 
 template <typename T>
