@@ -92,6 +92,9 @@ struct SsimJobDev {
   const uint32_t* mom;
   int64_t mom_off;                // offset in the SCP arena (uint16 units) the host turns into `mom`
   int32_t mom_pitch, mom_rows;
+  // Block-averaged 8 / 4 picture jobs: partials of a twin job over the same windows without the
+  // luminance term (MS-SSIM level 0 beside SSIM), written by this job; -1: none.
+  int64_t twin_part_off;
 };
 
 // One downsampling job of a pyramid level.

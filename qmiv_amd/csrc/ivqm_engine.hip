@@ -513,6 +513,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if (erp) RCCHK(erp_offset(h, &J.erp_off));
     J.blk0 = P.ssim_blocks;
     P.ssim_blocks += 3 * J.nchunk * J.nbg;
+    J.twin_part_off = -1;
     J.part_off = P.partial_total;
     P.partial_total += 3 * (int64_t)J.nuy * J.nchunk;
     J.out_idx = (int)P.jobs.size();
@@ -559,7 +560,21 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if (mp & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) {
       for (int l = 0; l < pp.ns; l++) {
         const int w = pp.w >> l, h = pp.h >> l, cl = (l == pp.ns - 1) ? 1 : 0;
-        if (mp & IVQM_M_MSSSIM) RCCHK(add_job(i, &pp.ms[l], ref_for(i, 0, l), ref_for(i, 1, l), w, h, cl, 0));
+        if (mp & IVQM_M_MSSSIM) {
+          RCCHK(add_job(i, &pp.ms[l], ref_for(i, 0, l), ref_for(i, 1, l), w, h, cl, 0));
+          // Level 0 of MS-SSIM reads the same (P0, P1) windows as SSIM: with the block-averaged 8 / 4
+          // kernel the SSIM job also writes this job's contrast-structure partials, and this job
+          // owns no blocks (a zero-block entry: the block lookup finds the next job).
+          if (l == 0 && cl == 0 && pp.ssim >= 0 && !P.ssim_gen) {
+            SsimJobDev& T = P.jobs[pp.ms[0]];
+            SsimJobDev& S = P.jobs[pp.ssim];
+            if (T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk && T.wsq == S.wsq &&
+                T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
+              P.ssim_blocks = T.blk0;
+              S.twin_part_off = T.part_off;
+            }
+          }
+        }
         if (mp & IVQM_M_IVMSSSIM) {
           RCCHK(add_job(i, &pp.ivms_t2r[l], ref_for(i, 0, l), ref_for(i, 3, l), w, h, cl, 0));
           RCCHK(add_job(i, &pp.ivms_r2t[l], ref_for(i, 1, l), ref_for(i, 2, l), w, h, cl, 0));

@@ -1095,7 +1095,7 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         nBB = (WQ)next_lane64((uint64_t)vBB);
         nAB = (WQ)next_lane64((uint64_t)vAB);
       }
-      double s = 0.0;
+      double s = 0.0, s_cs = 0.0;
       if (lane < 63 && bx < J.nux) {
         const uint32_t sA = vA + nA, sB = vB + nB;  // < 2^23
         const int64_t sAA = (int64_t)(uint64_t)(vAA + nAA), sBB = (int64_t)(uint64_t)(vBB + nBB);
@@ -1106,6 +1106,8 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         const int64_t vb = 64 * sBB - (int64_t)((uint64_t)sB * sB);
         float num = 2.0f * i64_to_f32(cov) + c2n;
         float den = i64_to_f32(va) + i64_to_f32(vb) + c2n;
+        // the twin's score is this job's without the luminance term: the same operations as its own job
+        if (!MOM && J.twin_part_off >= 0) s_cs = (double)(num * __builtin_amdgcn_rcpf(den));
         if (J.calc_l) {  // luminance term: positive terms only, FP32 products suffice
           const float fa = (float)sA, fb = (float)sB;
           num *= 2.0f * fa * fb + c1n;
@@ -1115,6 +1117,10 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
       }
       s = wave_sum_f64_dpp(s);
       if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
+      if (!MOM && J.twin_part_off >= 0) {  // block-uniform
+        s_cs = wave_sum_f64_dpp(s_cs);
+        if (lane == 0) partial[J.twin_part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s_cs;
+      }
     }
     pA = A; pB = B; pAA = AA; pBB = BB; pAB = AB;
   }
