@@ -104,6 +104,8 @@ struct DownJob {
   int32_t pitch;     // destination pitch
   int32_t blk0;
   uint16_t* dst;     // three planes, plane size pitch * (h >> 1)
+  int32_t ncomp;     // components written: 3, or 1 when the level takes a 4:2:0 source's chroma planes as they are
+  int32_t pad;
 };
 
 // Launchers (ivqm_kernels.hip). All are asynchronous on `st`; tables live in

@@ -65,6 +65,9 @@ struct PicRef {
   int64_t off = 0;      // element offset of plane 0 in the arena
   int pitch = 0, w = 0, h = 0;
   PicDev user;
+  // Pyramid level 1 of a 4:2:0 uint16 caller picture: the 2x2 mean of replicated chroma is the chroma
+  // plane itself, so planes 1 and 2 are the caller's chroma planes (no copy).
+  bool chroma_from_user = false;
 };
 
 struct BatchPlan {

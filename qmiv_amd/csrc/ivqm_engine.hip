@@ -447,6 +447,15 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       r.arena = 2;
       r.pitch = scp_pitch(d.w >> l);
       r.off = pyr[i][which][l];
+      const PicDev& u = which == 0 ? d.p0 : d.p1;
+      // the SSIM kernel reads a 4:4:4 picture's rows as 4-sample vectors: the chroma planes need the
+      // alignment the full-resolution planes have (8-byte base, stride a multiple of 4 samples)
+      const bool aligned = ((reinterpret_cast<uintptr_t>(u.p[1]) | reinterpret_cast<uintptr_t>(u.p[2])) & 7) == 0 &&
+                           u.stride[1] % 4 == 0 && u.stride[2] % 4 == 0;
+      if (l == 1 && which < 2 && u.cs == 1 && u.bps == 2 && aligned) {
+        r.chroma_from_user = true;
+        r.user = u;
+      }
     }
     return r;
   };
@@ -466,10 +475,12 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         dj.pitch = scp_pitch(pp.w >> l);
         const int dw = dj.w >> 1, dh = dj.h >> 1;
         dj.blk0 = P.down_blocks[l];
-        P.down_blocks[l] += 3 * ((dw + kDownBlkW - 1) / kDownBlkW) * ((dh + kDownBlkH - 1) / kDownBlkH);
+        const PicRef dst = ref_for(i, which, l);
+        dj.ncomp = dst.chroma_from_user ? 1 : 3;
+        P.down_blocks[l] += dj.ncomp * ((dw + kDownBlkW - 1) / kDownBlkW) * ((dh + kDownBlkH - 1) / kDownBlkH);
         P.down[l].push_back(dj);
         P.down_src[l].push_back(ref_for(i, which, l - 1));
-        P.down_dst[l].push_back(ref_for(i, which, l));
+        P.down_dst[l].push_back(dst);
       }
     }
   }
@@ -720,6 +731,13 @@ int Engine::materialize(const PicRef& r, PicDev& out) const {
   }
   out.cs = 0;
   out.bps = 2;
+  if (r.chroma_from_user) {  // level 1 of a 4:2:0 uint16 picture: its chroma planes at (w >> 1, h >> 1)
+    if (r.user.cs != 1 || r.user.bps != 2) return set_error(IVQM_EINVAL, "internal: pyramid chroma source");
+    for (int k = 1; k < 3; k++) {
+      out.p[k] = r.user.p[k];
+      out.stride[k] = r.user.stride[k];
+    }
+  }
   return IVQM_OK;
 }
 
