@@ -72,6 +72,16 @@ def build_dropin_check(lib: str) -> str:
     return out
 
 
+def build_arena_check() -> str:
+    """Host-only program over the batch engine's arena bounds rule (run by tests/test_plan_check.py)."""
+    out_dir = os.path.join(HERE, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "ivqm_arena_check")
+    _run([HIPCC, "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "plan_check", "arena_check.cpp"), "-o", out])
+    return out
+
+
 def build_app(lib: str) -> str:
     """ivqm_qmiv: the QMIV command-line front end (raw YUV files -> per-frame and average metrics)."""
     out_dir = os.path.join(HERE, "bin")
@@ -91,6 +101,7 @@ def build(oracle: bool = True) -> None:
     build_synth()
     lib = build_hip()
     build_dropin_check(lib)
+    build_arena_check()
     build_app(lib)
     if oracle:
         build_oracle()

@@ -70,6 +70,50 @@ struct PicRef {
   bool chroma_from_user = false;
 };
 
+// Device planes of a picture reference: the caller's picture, nothing (fused moments), or three planes
+// of pitch x h elements from element offset `off` of the SCP or pyramid arena. Every table entry that
+// names an arena picture goes through this check on the host before any launch reads the tables: an
+// arena picture that would leave its allocation is rejected with IVQM_EINVAL (*why says which rule).
+inline int resolve_picture(const PicRef& r, uint16_t* scp, int64_t cap_scp, uint16_t* pyr, int64_t cap_pyr,
+                           PicDev& out, const char** why) {
+  if (r.arena == 0) {
+    out = r.user;
+    return IVQM_OK;
+  }
+  if (r.arena == 3) {
+    out = PicDev{};
+    return IVQM_OK;
+  }
+  if (r.arena != 1 && r.arena != 2) {
+    *why = "internal: unknown picture arena";
+    return IVQM_EINVAL;
+  }
+  uint16_t* base = r.arena == 1 ? scp : pyr;
+  const int64_t cap = r.arena == 1 ? cap_scp : cap_pyr;
+  const int64_t plane = (int64_t)r.pitch * r.h;
+  if (!base || r.off < 0 || r.w <= 0 || r.h <= 0 || r.w > r.pitch || r.off + 3 * plane > cap) {
+    *why = "internal: arena picture outside its allocation";
+    return IVQM_EINVAL;
+  }
+  for (int k = 0; k < 3; k++) {
+    out.p[k] = base + r.off + (int64_t)k * plane;
+    out.stride[k] = r.pitch;
+  }
+  out.cs = 0;
+  out.bps = 2;
+  if (r.chroma_from_user) {  // level 1 of a 4:2:0 uint16 picture: its chroma planes at (w >> 1, h >> 1)
+    if (r.arena != 2 || r.user.cs != 1 || r.user.bps != 2) {
+      *why = "internal: pyramid chroma source";
+      return IVQM_EINVAL;
+    }
+    for (int k = 1; k < 3; k++) {
+      out.p[k] = r.user.p[k];
+      out.stride[k] = r.user.stride[k];
+    }
+  }
+  return IVQM_OK;
+}
+
 struct BatchPlan {
   uint32_t metrics = 0;
   bool need_stats = false, need_search = false, need_scp = false;

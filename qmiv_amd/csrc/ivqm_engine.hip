@@ -284,6 +284,13 @@ int Engine::erp_offset(int h, int64_t* off) {
   return IVQM_OK;
 }
 
+// Which k_ssim launches a plan needs: bit 0 jobs that read pictures, bit 1 jobs that read fused moments.
+static uint32_t ssim_forms(const BatchPlan& P) {
+  uint32_t f = 0;
+  for (const SsimJobDev& J : P.jobs) f |= J.mom_off >= 0 ? 2u : 1u;
+  return f;
+}
+
 int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   const bool reuse = (metrics & kReuseScp) != 0, scp_only = (metrics & kScpOnly) != 0;
   uint32_t m = metrics & 0xFFu;
@@ -593,15 +600,26 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       }
     }
   }
+  // A captured graph replays its launches with their captured grids and its copies with their captured
+  // sizes; only the tables are refreshed. Everything that shapes a launch or a copy therefore enters the
+  // signature: besides the pair shapes, the block counts of every phase, the masked search forms, the
+  // SSIM kernel forms and the pyramid jobs' component counts (level 1 of a 4:2:0 uint16 picture writes
+  // luma only when the picture's chroma planes are aligned, which depends on the caller's pointers).
+  std::string launch_sig;
+  char lb[160];
+  std::snprintf(lb, sizeof lb, "s%d t%d mt%d mf%x j%zu sb%d sf%x g%d am%d r%lld rd%lld r64%lld|", P.stat_blocks,
+                P.tiles, P.mtiles, P.mforms, P.jobs.size(), P.ssim_blocks, ssim_forms(P), P.ssim_gen ? 1 : 0,
+                P.any_mask ? 1 : 0, (long long)P.rows_total, (long long)P.rowsd_total, (long long)P.rows64_total);
+  launch_sig += lb;
+  for (int l = 1; l < 5; l++) {
+    std::snprintf(lb, sizeof lb, "L%d:%zu:%d:", l, P.down[l].size(), P.down_blocks[l]);
+    launch_sig += lb;
+    for (const DownJob& dj : P.down[l]) launch_sig += (char)('0' + dj.ncomp);
+  }
+  P.signature += launch_sig;
   return IVQM_OK;
 }
 
-// Which k_ssim launches a plan needs: bit 0 jobs that read pictures, bit 1 jobs that read fused moments.
-static uint32_t ssim_forms(const BatchPlan& P) {
-  uint32_t f = 0;
-  for (const SsimJobDev& J : P.jobs) f |= J.mom_off >= 0 ? 2u : 1u;
-  return f;
-}
 
 int Engine::ensure_scp(int64_t elems) {
   if (elems <= cap_scp_) return IVQM_OK;
@@ -712,33 +730,9 @@ int Engine::ensure_slot_capacity(RingSlot& S, const BatchPlan& P) {
 }
 
 int Engine::materialize(const PicRef& r, PicDev& out) const {
-  if (r.arena == 0) {
-    out = r.user;
-    return IVQM_OK;
-  }
-  if (r.arena == 3) {  // fused IV-SSIM: the job reads moments, not this picture
-    std::memset(&out, 0, sizeof(out));
-    return IVQM_OK;
-  }
-  uint16_t* base = r.arena == 1 ? d_scp_ : d_pyr_;
-  const int64_t cap = r.arena == 1 ? cap_scp_ : cap_pyr_;
-  const int64_t plane = (int64_t)r.pitch * r.h;
-  if (!base || r.off < 0 || r.w > r.pitch || r.off + 3 * plane > cap)
-    return set_error(IVQM_EINVAL, "internal: arena picture outside its allocation");
-  for (int k = 0; k < 3; k++) {
-    out.p[k] = base + r.off + (int64_t)k * plane;
-    out.stride[k] = r.pitch;
-  }
-  out.cs = 0;
-  out.bps = 2;
-  if (r.chroma_from_user) {  // level 1 of a 4:2:0 uint16 picture: its chroma planes at (w >> 1, h >> 1)
-    if (r.user.cs != 1 || r.user.bps != 2) return set_error(IVQM_EINVAL, "internal: pyramid chroma source");
-    for (int k = 1; k < 3; k++) {
-      out.p[k] = r.user.p[k];
-      out.stride[k] = r.user.stride[k];
-    }
-  }
-  return IVQM_OK;
+  const char* why = nullptr;
+  const int rc = resolve_picture(r, d_scp_, cap_scp_, d_pyr_, cap_pyr_, out, &why);
+  return rc ? set_error(rc, why) : IVQM_OK;
 }
 
 // Writes the slot's pinned tables (pairs, SSIM jobs, pyramid jobs) with device pointers.
@@ -835,8 +829,8 @@ int Engine::enqueue(const PairSpec* pairs, int n, uint32_t metrics, int ring, bo
   RCCHK(plan(pairs, n, metrics, P));
   RCCHK(ensure_capacity(P));
   RCCHK(ensure_slot_capacity(S, P));
-  // pointer identity enters the graph signature: a replay reuses the captured tables' device pointers
-  // only through the staging copy, so the signature needs shapes only
+  // Device pointers reach the kernels only through the tables, which a replay refreshes through the
+  // captured staging copy; the signature (plan()) covers everything that shapes a launch or a copy.
   if (use_graph) {
     if (S.graph && S.graph_sig != P.signature) {
       (void)hipGraphExecDestroy(S.graph);

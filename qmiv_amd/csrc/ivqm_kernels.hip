@@ -1278,6 +1278,7 @@ __global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ 
   const int nbx = (dw + kDownBlkW - 1) / kDownBlkW, nby = (dh + kDownBlkH - 1) / kDownBlkH;
   int lb = b - J.blk0;
   const int c = lb / (nbx * nby);
+  if (c >= J.ncomp) return;  // the job writes luma only (level 1 with the caller's chroma planes)
   lb -= c * nbx * nby;
   const int by = lb / nbx, bx = lb - by * nbx;
   const int x4 = bx * kDownBlkW + 4 * (threadIdx.x & 63);

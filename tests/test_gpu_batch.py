@@ -109,3 +109,55 @@ def test_batch_ring_results_survive_buffer_growth():
             for ring, i in enumerate(order):
                 (got,) = b.collect(ring, 1)
                 assert got.as_dict() == want[i], (use_graph, MIXED[i])
+
+
+def _chroma_layout(torch, plane, layout):
+    """A device copy of a 2-D uint16 plane: 'aligned' (contiguous), 'stride' (row stride not a multiple of
+    4 samples) or 'offset' (base 2 bytes past an 8-byte boundary)."""
+    h, w = plane.shape
+    src = torch.from_numpy(plane.view(np.int16)).cuda()
+    if layout == "aligned":
+        return src.clone()
+    if layout == "stride":
+        buf = torch.zeros((h, w + 2), dtype=torch.int16, device="cuda")
+        v = buf[:, :w]
+    else:
+        buf = torch.zeros(h * w + 8, dtype=torch.int16, device="cuda")
+        v = buf[1:1 + h * w].view(h, w)
+    v.copy_(src)
+    return v
+
+
+def test_batch_graph_replay_across_chroma_alignment(oracle):
+    """MS-SSIM level 1 of a 4:2:0 uint16 picture takes the caller's chroma planes when they are aligned
+    (luma-only pyramid job) and copies them otherwise. One captured graph must not be replayed across
+    that difference: batches of the same shapes alternate aligned and unaligned chroma with hipGraph on,
+    and every result equals the single-pair result (pictures uploaded into aligned slots)."""
+    import torch
+    wls = [synth.Workload(cfg=510, width=136, height=136), synth.Workload(cfg=511, width=136, height=136,
+                                                                          global_shift=1)]
+    host = [(wl, *synth.make_pair(wl, 0)) for wl in wls]
+    m = Q.IVQM_M_SSIM | Q.IVQM_M_MSSSIM | Q.IVQM_M_IVSSIM | Q.IVQM_M_IVMSSSIM
+    want = [_single(wl, p0, p1, m) for wl, p0, p1 in host]
+
+    def items(layout):
+        out = []
+        for wl, p0, p1 in host:
+            pics = []
+            for p in (p0, p1):
+                luma = torch.from_numpy(p[0].view(np.int16)).cuda()
+                pics.append([luma] + [_chroma_layout(torch, p[c], layout) for c in (1, 2)])
+            out.append((pics[0], pics[1], wl.width, wl.height, wl.bit_depth, True))
+        torch.cuda.synchronize()
+        return out
+
+    with IvqmBatch() as b:
+        b.set_tuning(0, 0, True)
+        for layout in ("aligned", "stride", "aligned", "offset", "stride", "aligned"):
+            it = items(layout)
+            got = [r.as_dict() for r in b.run(b.make_pairs(it), m)]
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert g == w, (layout, i)
+    from oracle_checks import check_pair
+    for (wl, p0, p1), r in zip(host, want):
+        check_pair(oracle, r, p0, p1, wl.bit_depth, True, m, label=str(wl))

@@ -153,6 +153,34 @@ def test_sd_wrap32_matches_x86_64_v4():
     assert o.sd(a, b, wrap32=True) == int(np.int32(np.uint32(exact % (1 << 32))))
 
 
+def test_gcd_past_2_31_matches_both_reference_builds(oracle):
+    """Component sums past 2^31 (4096x2048 4:4:4, 14-bit, P1 = P0 + 300 / -280 / +260): the x86-64-v4
+    reference build sums <= 14-bit differences in int32 lanes (xDistortionAVX512.cpp:14-73) and its GCD
+    follows the wrapped sum; the x86-64-v3 build sums in int64. The oracle's wrap32 switch reproduces
+    each, and the two differ in every component here."""
+    from oracle.oracle import Reference, host_has_avx512, reference_available
+    h, w, bd = 2048, 4096, 14
+    rng = np.random.default_rng(3)
+    p0 = [rng.integers(1000, 15000, (h, w), dtype=np.uint16) for _ in range(3)]
+    off = (300, -280, 260)
+    p1 = [(p0[c].astype(np.int32) + off[c]).astype(np.uint16) for c in range(3)]
+    exact = oracle.gcd(p0, p1, bd, wrap32=False)
+    wrapped = oracle.gcd(p0, p1, bd, wrap32=True)
+    md = round(0.01 * 16383)  # MaxDiff clip of 14-bit (164)
+    assert exact[:3] == [md, -md, md]
+    assert all(exact[c] != wrapped[c] for c in range(3))
+    for c in range(3):  # closed form of the wrapped mean
+        s = off[c] * h * w
+        s32 = (s + (1 << 31)) % (1 << 32) - (1 << 31)
+        m = s32 / (h * w)
+        g = int(math.copysign(math.floor(abs(m) + 0.5), m))
+        assert wrapped[c] == max(-md, min(md, g))
+    if reference_available("v3"):
+        assert Reference("v3").gcd(p0, p1, bd) == exact
+    if reference_available("v4") and host_has_avx512():
+        assert Reference("v4").gcd(p0, p1, bd) == wrapped
+
+
 def test_rounding_matches_std_round(oracle):
     """test-xCommon.cpp:62-113: round half away == std::round over i/1000."""
     for i in range(-65536, 65536, 7):
