@@ -156,6 +156,10 @@ struct RingSlot {
   std::string graph_sig;
 };
 
+// Search variant of a pair (PairDev::mode) from its sample maximum and the largest |GCD| it can see:
+// the keyed forms where the host-checked ranges allow, the reference-form arithmetic otherwise.
+int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax);
+
 // Checks the SSIM mode / window / stride / border of a configuration
 // (the rules of xAppQMIV.cpp:325-332 and xStructSimSTD.cpp:353-419).
 int validate_ssim_config(const ivqm_config& cfg);

@@ -192,9 +192,11 @@ int Engine::sync() {
   return IVQM_OK;
 }
 
-int Engine::search_mode(int maxv, int32_t gmax) const {
-  if (cfg_.search_mode_cap == 1) return 0;  // reference-form arithmetic only
-  const int32_t* w = cfg_.cmp_weights_search;
+int Engine::search_mode(int maxv, int32_t gmax) const { return pick_search_mode(cfg_, maxv, gmax); }
+
+int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax) {
+  if (cfg.search_mode_cap == 1) return 0;  // reference-form arithmetic only
+  const int32_t* w = cfg.cmp_weights_search;
   const int64_t A = (int64_t)maxv + gmax;
   int64_t wabs = 0;
   for (int k = 0; k < 3; k++) wabs = std::max<int64_t>(wabs, std::llabs((long long)w[k]));
@@ -202,7 +204,7 @@ int Engine::search_mode(int maxv, int32_t gmax) const {
   const int wide = (A * wabs < (1ll << 23)) ? 3 : 0;
   for (int k = 0; k < 3; k++)
     if (w[k] < 0) return wide;
-  const int64_t S = 1 << key_bits(cfg_.search_range);
+  const int64_t S = 1 << key_bits(cfg.search_range);
   const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
   const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
   // the keyed variants' epilogue takes t - r as int16
@@ -211,7 +213,7 @@ int Engine::search_mode(int maxv, int32_t gmax) const {
   if (3 * S * wsum * A * A >= (1ll << 31)) return wide;
   if (2 * S * wmax * A >= (1ll << 23)) return wide;
   // packed variant: S * r and 2 * w * t are int16 operands
-  if (cfg_.search_mode_cap != 2 && S * maxv <= 32767 && 2 * wmax * A <= 32767) return 2;
+  if (cfg.search_mode_cap != 2 && S * maxv <= 32767 && 2 * wmax * A <= 32767) return 2;
   return 1;
 }
 
@@ -974,54 +976,20 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
         r.exact[k] = pp.masked ? S.r_mcount[4 * i + 1 + k] == 0 : r.ssd[k] == 0;
       }
     }
-    const double fake = host::psnr_from_ssd(1.0, area, bd);
     if (pp.masked) {
       RCCHK(masked_epilogue(S, pp, i, r));
       if (out) out[i] = r;
       continue;
     }
-    if (m & (IVQM_M_PSNR | IVQM_M_MSE)) {
-      // libs/pmbb-ivqm/src/xPSNR.cpp:131-151; exact components -> fake PSNR (xAppQMIV.cpp:974-981)
-      for (int k = 0; k < 4; k++) { r.mse[k] = DBL_MAX; r.psnr[k] = DBL_MAX; }
-      for (int k = 0; k < 3; k++) {
-        const double ssd = (double)r.ssd[k];
-        r.mse[k] = ssd / (double)area;
-        r.psnr[k] = host::psnr_from_ssd(ssd, area, bd);
-        if (cfg_.fake_vals_for_exact && r.exact[k]) r.psnr[k] = fake;
-      }
-      r.psnr_pic = host::pic_value(r.psnr, cwa);
+    const double* ew = nullptr;
+    double corr = 1.0;
+    if (cfg_.erp && (m & (IVQM_M_WSPSNR | IVQM_M_IVPSNR))) {
+      const auto& hw = erp_weights(H);
+      ew = hw.first.data();
+      corr = hw.second;
     }
-    if (m & IVQM_M_WSPSNR) {
-      for (int k = 0; k < 4; k++) r.wspsnr[k] = DBL_MAX;
-      if (!cfg_.erp) {
-        for (int k = 0; k < 3; k++) r.wspsnr[k] = host::psnr_from_ssd((double)r.ssd[k], area, bd);
-      } else {
-        // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a 4:2:0 chroma plane row counts twice per luma row
-        const auto& hw = erp_weights(H);
-        const uint64_t* rs = S.r_row_ssd + pp.rowsd_off;
-        for (int k = 0; k < 3; k++) {
-          host::Kbns kb;
-          for (int y = 0; y < H; y++) {
-            const uint64_t row = (k && pp.cs) ? 2 * rs[(size_t)k * H + (y >> 1)] : rs[(size_t)k * H + y];
-            kb.acc((double)row * hw.first[y]);
-          }
-          const double err = kb.result() * hw.second;
-          r.wspsnr[k] = host::psnr_from_ssd(err, area, bd);
-          if (cfg_.fake_vals_for_exact && err == 0) r.wspsnr[k] = fake;
-        }
-      }
-      for (int k = 0; k < 3; k++)
-        if (cfg_.fake_vals_for_exact && r.exact[k]) r.wspsnr[k] = fake;
-      r.wspsnr_pic = host::pic_value(r.wspsnr, cwa);
-    }
-    if (m & IVQM_M_IVPSNR) {
-      const uint32_t* rows = S.r_rows + pp.rows_off;
-      const double* ew = nullptr;
-      if (cfg_.erp) ew = erp_weights(H).first.data();
-      r.ivpsnr_r2t = host::ivpsnr_quality(rows, W, H, bd, ew, cwa);
-      r.ivpsnr_t2r = host::ivpsnr_quality(rows + (size_t)3 * H, W, H, bd, ew, cwa);
-      r.ivpsnr = std::min(r.ivpsnr_r2t, r.ivpsnr_t2r);  // xIVPSNR.cpp:55 (xMin = std::min)
-    }
+    host::psnr_family(m, W, H, bd, pp.cs, cfg_.fake_vals_for_exact != 0, cwa, S.r_row_ssd + pp.rowsd_off,
+                      S.r_rows + pp.rows_off, ew, corr, r);
     const double* so = S.r_ssim;
     if ((m & IVQM_M_SSIM) && pp.ssim >= 0) {
       for (int k = 0; k < 3; k++) r.ssim[k] = ssim_value(P, so, pp.ssim, k);

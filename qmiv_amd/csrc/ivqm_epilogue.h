@@ -12,6 +12,8 @@
 
 #include <vector>
 
+#include "../../include/ivqm_hip.h"
+
 #pragma clang fp contract(off)
 
 namespace ivqm {
@@ -43,16 +45,20 @@ inline double psnr_from_ssd(double ssd, int64_t area, int bit_depth) {
   return ssd > 0 ? 10.0 * log10((double)MAX / ssd) : DBL_MAX;
 }
 
-// libs/pmbb-ivqm/src/xWeightedSpherically.cpp:13-29.
-inline void ws_weights(int h, int lat_range_deg, std::vector<double>& w, double& correction) {
-  w.resize(h);
+// libs/pmbb-ivqm/src/xWeightedSpherically.cpp:13-29: h row weights into w; returns the correction
+// factor h / sum(w).
+inline double ws_weights_into(int h, int lat_range_deg, double* w) {
   const double eh = 180.0 * (double)h / (double)lat_range_deg;
   const double eo = (eh - h) / 2.0;
   const double pi = 3.14159265358979323846;
   for (int y = 0; y < h; y++) w[y] = cos((y + eo - (eh / 2 - 0.5)) * pi / eh);
   Kbns k;
   for (int y = 0; y < h; y++) k.acc(w[y]);
-  correction = (double)h / k.result();
+  return (double)h / k.result();
+}
+inline void ws_weights(int h, int lat_range_deg, std::vector<double>& w, double& correction) {
+  w.resize(h);
+  correction = ws_weights_into(h, lat_range_deg, w.data());
 }
 
 // (v * w).getSum() * (1 / sum(w)): libs/pmbb-ivqm/src/xIVPSNR.cpp:120-123.
@@ -87,6 +93,53 @@ inline double ivpsnr_quality(const uint32_t* rows, int w, int h, int bit_depth, 
   double q[4] = {0, 0, 0, 0};
   for (int c = 0; c < 3; c++) q[c] = psnr_from_ssd(err[c] > 0 ? err[c] : 1.0, (int64_t)w * h, bit_depth);
   return weighted_mean4(q, cw_avg);
+}
+
+// The PSNR family of one unmasked pair from the device's exact integers; r.ssd and r.exact are set.
+// row_ssd: [3][plane rows] SSD rows (read for WS-PSNR with ERP weights); rows: the search's [2][3][H]
+// row sums (IV-PSNR); erp_w / erp_corr: row weights and correction of height H, or null without ERP.
+inline void psnr_family(uint32_t m, int W, int H, int bd, int cs, bool fake_exact, const int32_t cwa[4],
+                        const uint64_t* row_ssd, const uint32_t* rows, const double* erp_w, double erp_corr,
+                        ivqm_frame_result& r) {
+  const int64_t area = (int64_t)W * H;
+  const double fake = psnr_from_ssd(1.0, area, bd);
+  if (m & (IVQM_M_PSNR | IVQM_M_MSE)) {
+    // libs/pmbb-ivqm/src/xPSNR.cpp:131-151; exact components -> fake PSNR (xAppQMIV.cpp:974-981)
+    for (int k = 0; k < 4; k++) { r.mse[k] = DBL_MAX; r.psnr[k] = DBL_MAX; }
+    for (int k = 0; k < 3; k++) {
+      const double ssd = (double)r.ssd[k];
+      r.mse[k] = ssd / (double)area;
+      r.psnr[k] = psnr_from_ssd(ssd, area, bd);
+      if (fake_exact && r.exact[k]) r.psnr[k] = fake;
+    }
+    r.psnr_pic = pic_value(r.psnr, cwa);
+  }
+  if (m & IVQM_M_WSPSNR) {
+    for (int k = 0; k < 4; k++) r.wspsnr[k] = DBL_MAX;
+    if (!erp_w) {
+      for (int k = 0; k < 3; k++) r.wspsnr[k] = psnr_from_ssd((double)r.ssd[k], area, bd);
+    } else {
+      // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a 4:2:0 chroma plane row counts twice per luma row
+      for (int k = 0; k < 3; k++) {
+        Kbns kb;
+        for (int y = 0; y < H; y++) {
+          const uint64_t row = (k && cs) ? 2 * row_ssd[(size_t)k * H + (y >> 1)] : row_ssd[(size_t)k * H + y];
+          kb.acc((double)row * erp_w[y]);
+        }
+        const double err = kb.result() * erp_corr;
+        r.wspsnr[k] = psnr_from_ssd(err, area, bd);
+        if (fake_exact && err == 0) r.wspsnr[k] = fake;
+      }
+    }
+    for (int k = 0; k < 3; k++)
+      if (fake_exact && r.exact[k]) r.wspsnr[k] = fake;
+    r.wspsnr_pic = pic_value(r.wspsnr, cwa);
+  }
+  if (m & IVQM_M_IVPSNR) {
+    r.ivpsnr_r2t = ivpsnr_quality(rows, W, H, bd, erp_w, cwa);
+    r.ivpsnr_t2r = ivpsnr_quality(rows + (size_t)3 * H, W, H, bd, erp_w, cwa);
+    r.ivpsnr = (r.ivpsnr_t2r < r.ivpsnr_r2t) ? r.ivpsnr_t2r : r.ivpsnr_r2t;  // xIVPSNR.cpp:55 (xMin = std::min)
+  }
 }
 
 // Masked PSNR: libs/pmbb-ivqm/src/xPSNR.cpp:164-179 / :199-206,
