@@ -52,6 +52,31 @@ int ivqm_batch_enqueue_masked(ivqm_batch* b, const ivqm_pair* pairs, const ivqm_
                               uint32_t metrics, int32_t ring);
 int ivqm_batch_collect(ivqm_batch* b, int32_t ring, ivqm_frame_result* results);
 
+/* No-allocation form of ivqm_batch_run for the PSNR family (GCD, MSE, PSNR,
+ * WS-PSNR, IV-PSNR). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code that must not call hipMalloc /
+ * hipFree (fixed-memory serving loops, harnesses that forbid allocation).
+ * There is no batch object: cfg gives the shared parameters (its width,
+ * height and bit_depth are ignored), pairs are device-resident as for
+ * ivqm_batch_run, and results[i] equals what ivqm_batch_run returns for
+ * pairs[i] with the same metrics. The work runs on hip_stream (NULL: the null
+ * stream), ordered after what the caller enqueued there before; the call
+ * returns once that stream has finished it.
+ * Scratch memory (pair table, row partial sums, frame totals) is one fixed
+ * device array of IVQM_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends and reused by every call on that GPU; results are staged in fixed host
+ * arrays. A lock held over each whole call serializes calls from all threads,
+ * so they never overlap on the shared scratch. On success nothing is
+ * allocated. Limits (IVQM_EINVAL beyond them, before anything runs):
+ * 1..IVQM_NOALLOC_MAX_PAIRS pairs whose heights add up to at most
+ * IVQM_NOALLOC_MAX_ROWS, metrics within IVQM_M_PSNR | IVQM_M_MSE |
+ * IVQM_M_WSPSNR | IVQM_M_IVPSNR, no mask. */
+#define IVQM_NOALLOC_MAX_PAIRS 16
+#define IVQM_NOALLOC_MAX_ROWS 32768
+#define IVQM_NOALLOC_SCRATCH_BYTES (5u << 19) /* 2.5 MiB */
+int ivqm_batch_run_noalloc(const ivqm_config* cfg, const ivqm_pair* pairs, int32_t n, uint32_t metrics,
+                           ivqm_frame_result* results, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

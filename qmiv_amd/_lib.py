@@ -31,12 +31,14 @@ EXPORTED = [
     "ivqm_bind_mask", "ivqm_upload_mask", "ivqm_batch_enqueue_masked", "ivqm_convert_colour",
     "ivqm_batch_create", "ivqm_batch_destroy", "ivqm_batch_set_stream", "ivqm_batch_set_tuning",
     "ivqm_batch_get_tuning", "ivqm_batch_autotune", "ivqm_batch_run", "ivqm_batch_enqueue", "ivqm_batch_collect",
-    "ivqm_batch_profile",
+    "ivqm_batch_profile", "ivqm_batch_run_noalloc",
     "ivqm_server_config_init", "ivqm_server_create", "ivqm_server_destroy", "ivqm_server_add_sequence",
     "ivqm_server_remove_sequence", "ivqm_server_submit", "ivqm_server_poll", "ivqm_server_drain",
     "ivqm_server_get_stats",
 ]
 MAX_JOB_FRAMES = 8
+NOALLOC_MAX_PAIRS = 16
+NOALLOC_MAX_ROWS = 32768
 
 
 class Config(C.Structure):
@@ -178,6 +180,8 @@ def load():
     L.ivqm_batch_enqueue_masked.argtypes = [C.c_void_p, C.POINTER(Pair), C.POINTER(Mask), C.c_int32, C.c_uint32,
                                             C.c_int32]
     L.ivqm_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.ivqm_batch_run_noalloc.argtypes = [C.POINTER(Config), C.POINTER(Pair), C.c_int32, C.c_uint32,
+                                         C.POINTER(FrameResult), C.c_void_p]
     L.ivqm_server_config_init.argtypes = [C.POINTER(ServerConfig)]
     L.ivqm_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
     L.ivqm_server_destroy.argtypes = [C.c_void_p]

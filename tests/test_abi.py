@@ -115,3 +115,37 @@ def test_invalid_arguments_return_status_not_crash():
     else:
         assert rc <= -1000
         assert len(L.ivqm_last_error()) > 0
+
+
+def test_noalloc_rejects_requests_beyond_its_limits():
+    """ivqm_batch_run_noalloc checks its limits before any HIP call: too many pairs, heights adding up
+    past IVQM_NOALLOC_MAX_ROWS, metrics outside the PSNR family and empty batches return IVQM_EINVAL.
+    (The plane addresses are placeholders that nothing reads.)"""
+    from qmiv_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libivqm_hip.so not built")
+    from qmiv_amd.context import default_config
+    L = _lib.load()
+    cfg = default_config(64, 64)
+    fam = _lib.IVQM_M_PSNR | _lib.IVQM_M_MSE | _lib.IVQM_M_WSPSNR | _lib.IVQM_M_IVPSNR
+
+    def pairs(n, w, h):
+        arr = (_lib.Pair * n)()
+        for p in arr:
+            for pic in (p.p0, p.p1):
+                for k in range(3):
+                    pic.plane[k] = 0x10000 * (k + 1)
+                    pic.stride[k] = w if k == 0 else w // 2
+                pic.chroma_420, pic.bytes_per_sample = 1, 2
+            p.width, p.height, p.bit_depth = w, h, 10
+        return arr
+
+    res = (_lib.FrameResult * 32)()
+    n = _lib.NOALLOC_MAX_PAIRS + 1
+    assert L.ivqm_batch_run_noalloc(C.byref(cfg), pairs(n, 64, 64), n, fam, res, None) == -22
+    tall = _lib.NOALLOC_MAX_ROWS // 4 + 2  # four such heights pass the row limit
+    assert L.ivqm_batch_run_noalloc(C.byref(cfg), pairs(4, 64, tall), 4, fam, res, None) == -22
+    assert b"IVQM_NOALLOC_MAX_ROWS" in L.ivqm_last_error()
+    for m in (_lib.IVQM_M_IVSSIM, _lib.IVQM_M_SSIM | _lib.IVQM_M_PSNR, 0):
+        assert L.ivqm_batch_run_noalloc(C.byref(cfg), pairs(1, 64, 64), 1, m, res, None) == -22
+    assert L.ivqm_batch_run_noalloc(C.byref(cfg), pairs(1, 64, 64), 0, fam, res, None) == -22
