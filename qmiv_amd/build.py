@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["ivqm_kernels.hip", "ivqm_ssim_modes.hip", "ivqm_masked.hip", "ivqm_engine.hip", "ivqm_runtime.hip",
-               "ivqm_serve.hip", "ivqm_noalloc.hip"]
+               "ivqm_serve.hip", "ivqm_noalloc.hip", "ivqm_search420.hip"]
 OBJ_DIR = os.path.join(HERE, "_build")
 
 HIP_FLAGS = [

@@ -24,7 +24,7 @@ struct PairDev {
   PicDev p0, p1;
   int32_t w, h, maxv;
   int32_t mode;           // search variant: 0 reference form, 1 narrow key, 2 packed dot key, 3 reference form
-                          // with 24-bit multiplies (host-checked ranges)
+                          // with 24-bit multiplies, 4 keyed 4:2:0 search (k_search420) (host-checked ranges)
   int32_t md[3];          // GCD clip limits round_f32(coef * Max)
   int32_t wrap32;         // x86-64-v4 int32 SD accumulation
   int32_t stat_blk0;      // first k_stats block of this pair
@@ -37,7 +37,7 @@ struct PairDev {
   int64_t best_off;       // [2][h][w] uint8, or -1
   int32_t res_idx;        // index of gcd[4], sd[3], ssd[3]
   int32_t gcd_in[4];      // the caller's GCD when gcd_given
-  int32_t pad;
+  int32_t tile0b;         // first k_search420 tile of this pair (mode kSearchDot420)
   // Mask mode (weights 0 .. msk_max at luma resolution), or msk == null:
   const void* msk;
   int32_t msk_stride;     // in samples
@@ -122,7 +122,7 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 constexpr int kMaskTileW = 64, kMaskTileH = 16;
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                                 const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st);
-hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
+hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 // forms: bit 0 when jobs read pictures, bit 1 when jobs read fused moments (one launch each).
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
@@ -143,11 +143,23 @@ hipError_t launch_colour(const PicDev& src, int w, int h, int bit_depth, int dir
                          int dst_stride, hipStream_t st);
 
 constexpr int kSearchTileW = 64;
-// The search runs waves of 4 pixel rows (64 x 16 tiles) for every search range;
-// a requested tile height of 8 rows per wave is accepted and runs as 4.
+// k_search: waves of 4 pixel rows on 64 x 16 tiles for every search range and variant but the keyed
+// 4:2:0 one; k_search420 (ivqm_search420.hip): 64 x TH tiles, TH = 16 or 32 (tuning).
 constexpr int kSearchRowsPerWave = 4;
-inline int search_tile_h(int py) { return 4 * py; }
-inline int search_rows(int /*R*/, int /*py*/) { return kSearchRowsPerWave; }
+constexpr int kSearchTileH = 4 * kSearchRowsPerWave;
+constexpr int kSearchDot420 = 4;  // PairDev::mode of the keyed 4:2:0 search
+// Tile height of k_search420 from a tuning value: 16 or 32 (4 and 8, the earlier per-wave row counts, are
+// taken as 16 and 32); anything else keeps `current`.
+inline int search420_tile_rows(int v, int current) {
+  if (v == 4 || v == 16) return 16;
+  if (v == 8 || v == 32) return 32;
+  return current;
+}
+// Key scale Sk = 2^search420_key_bits(R) and sample scale Sy = 2^search420_sample_bits(R) of k_search420.
+int search420_key_bits(int R);
+int search420_sample_bits(int R);
+hipError_t launch_search420(const PairDev* pairs, int npairs, int ntiles, int R, int th, const int32_t cw[4],
+                            const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 constexpr int kDownBlkW = 256, kDownBlkH = 4;  // k_downsample: outputs per block row (4 per lane), rows
 // MI355X: 8 XCDs, each with its own L2; consecutive workgroups go to consecutive XCDs.
 constexpr int kXcds = 8;

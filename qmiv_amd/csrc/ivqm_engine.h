@@ -131,7 +131,8 @@ struct BatchPlan {
   uint32_t mforms = 0;                         // masked search forms present (bit 0 reference, 1 narrow, 2 dot)
   int64_t aux_total = 0, rows64_total = 0;     // masked pairs' aux rows and uint64 search rows
   int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
-  int py = 8, nb = 8;
+  int th420 = 32, nb = 8;  // k_search420 tile rows, SSIM band rows
+  int tiles420 = 0;         // k_search420 tiles
   std::string signature;
 };
 
@@ -159,6 +160,9 @@ struct RingSlot {
 // Search variant of a pair (PairDev::mode) from its sample maximum and the largest |GCD| it can see:
 // the keyed forms where the host-checked ranges allow, the reference-form arithmetic otherwise.
 int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax);
+// Whether a 4:2:0 pair with sample maximum maxv and largest |GCD| gmax can run the keyed 4:2:0 search
+// (k_search420): its int16 operands and int32 keys stay exact.
+bool search420_ok(const ivqm_config& cfg, int maxv, int32_t gmax);
 
 // Checks the SSIM mode / window / stride / border of a configuration
 // (the rules of xAppQMIV.cpp:325-332 and xStructSimSTD.cpp:353-419).
@@ -183,11 +187,11 @@ class Engine {
   int query(int ring);
   int sync();
 
-  void set_tuning(int py, int nb) {
-    if (py == 4 || py == 8) py_ = py;
+  void set_tuning(int th, int nb) {
+    th420_ = search420_tile_rows(th, th420_);
     if (nb == 4 || nb == 8 || nb == 16) nb_ = nb;
   }
-  int tuning_py() const { return search_rows(cfg_.search_range, py_); }
+  int tuning_py() const { return th420_; }
   int tuning_nb() const { return nb_; }
   void enable_best(bool on) { best_on_ = on; }
   // Times `iters` launches of one phase of the last enqueued batch (HIP events).
@@ -208,6 +212,7 @@ class Engine {
   int ensure_capacity(const BatchPlan& P);
   int ensure_slot_capacity(RingSlot& S, const BatchPlan& P);
   int issue(const BatchPlan& P, RingSlot& S);
+  hipError_t launch_search_all(const BatchPlan& P, hipStream_t st);
   int fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_first[5]);
   int materialize(const PicRef& r, PicDev& out) const;
   void drop_graphs();
@@ -220,7 +225,7 @@ class Engine {
   ivqm_config cfg_;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
-  int py_ = 8, nb_ = 8;
+  int th420_ = 32, nb_ = 8;
   bool best_on_ = false;
   bool scp_valid_ = false;
 

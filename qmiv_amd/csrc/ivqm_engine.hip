@@ -129,9 +129,7 @@ const char* last_error_message() { return g_last_error.c_str(); }
     if (_rc) return _rc;     \
   } while (0)
 
-Engine::Engine(const ivqm_config& cfg) : cfg_(cfg) {
-  if (cfg.search_tile_rows == 4 || cfg.search_tile_rows == 8) py_ = cfg.search_tile_rows;
-}
+Engine::Engine(const ivqm_config& cfg) : cfg_(cfg) { th420_ = search420_tile_rows(cfg.search_tile_rows, th420_); }
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
@@ -243,6 +241,25 @@ int32_t Engine::masked_key_base(int maxv, int32_t gmax) const {
 
 // The packed form of the keyed masked search: S * r and 2 * w * t are int16 operands of v_dot2
 // (as the unmasked search's dot variant); search_mode_cap 2 forbids it.
+// k_search420 operands (C0 = Max + max|gcd| <= A): samples below 2^12, weights below 128 and 2 C0 below
+// 2^14 (the kernel's 24-bit multiplies), chroma samples times Sy and the chroma multipliers
+// 2 F w (C0 - t) <= 2 F w 2A (F = Sk / Sy) are u16, the luma multiplier 2 Sk wY (C0 - t) takes a u16 scale
+// 2 Sk wY, and the keys Sk * E' + offset stay inside int32 (|E'| <= 3 wsum A^2 bounds Q(r) + 2 sum w |t| r).
+bool search420_ok(const ivqm_config& cfg, int maxv, int32_t gmax) {
+  const int32_t* w = cfg.cmp_weights_search;
+  for (int k = 0; k < 3; k++)
+    if (w[k] < 0) return false;
+  const int R = cfg.search_range;
+  const int64_t Sk = 1ll << search420_key_bits(R), Sy = 1ll << search420_sample_bits(R), F = Sk / Sy;
+  const int64_t A = (int64_t)maxv + gmax;
+  const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
+  const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
+  if (maxv >= 4096 || A >= 8192 || wmax >= 128) return false;  // the kernel's operand masks
+  if (Sy * maxv > 65535 || 2 * F * wmax * 2 * A > 65535 || 2 * Sk * wmax > 65535) return false;
+  if (2 * Sk * wmax * 2 * A >= (1ll << 24)) return false;  // luma multiplier: an operand of v_mad_u32_u24
+  return 3 * Sk * wsum * A * A + (1ll << 16) < (1ll << 31);
+}
+
 bool Engine::masked_dot_ok(int maxv, int32_t gmax) const {
   if (cfg_.search_mode_cap == 2) return false;
   const int32_t* w = cfg_.cmp_weights_search;
@@ -301,7 +318,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   P = BatchPlan();
   P.metrics = m;
   P.n = n;
-  P.py = search_rows(cfg_.search_range, py_);
+  P.th420 = th420_;
   P.nb = nb_;
   P.need_search = (m & IVQM_M_IVPSNR) || (!reuse && (m & ivs)) || scp_only;
   P.need_scp = (!reuse && (m & ivs)) || scp_only;
@@ -309,13 +326,13 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   if ((m & ivs) && reuse && !(n == 1 && scp_valid_))
     return set_error(IVQM_ESTATE, "IV-SSIM needs shift-compensated pictures (ivqm_gen_scp / ivqm_upload_scp)");
   const int R = cfg_.search_range;
-  const int TH = search_tile_h(P.py);
+  const int TH = kSearchTileH;
   const int wnd = cfg_.ssim_window, str = cfg_.ssim_stride;
   P.pp.resize(n);
   P.pairs.resize(n);
   char sigbuf[96];
   std::snprintf(sigbuf, sizeof sigbuf, "m%x py%d nb%d n%d|", m | (reuse ? 0x100u : 0u) | (scp_only ? 0x200u : 0u),
-                P.py, P.nb, n);
+                P.th420, P.nb, n);
   P.signature = sigbuf;
   const bool ms_any = (m & (IVQM_M_MSSSIM | IVQM_M_IVMSSSIM)) != 0;
   const bool regular = cfg_.ssim_mode <= 2;
@@ -370,6 +387,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.stat_blk0 = P.stat_blocks;
     P.stat_blocks += s.msk ? 3 * s.h : s.h + 2 * (s.h >> s.p0.cs);  // masked: every component at 4:4:4
     d.tile0 = P.tiles;  // pairs without a search own no tiles (the table lookup skips them)
+    d.tile0b = P.tiles420;
+    if (!s.msk && s.p0.cs == 1 && (d.mode == 1 || d.mode == 2) && cfg_.search_mode_cap == 0 &&
+        search420_ok(cfg_, d.maxv, gmax))
+      d.mode = kSearchDot420;
     d.mtile0 = P.mtiles;
     if (s.msk) {
       if (mp & IVQM_M_IVPSNR) {
@@ -377,7 +398,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         P.mforms |= 1u << (d.mkey == 0 ? 0 : (d.mdot ? 2 : 1));
       }
     } else if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only))) {
-      P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
+      if (d.mode == kSearchDot420)
+        P.tiles420 += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + P.th420 - 1) / P.th420);
+      else
+        P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
     }
     d.rowsd_off = P.rowsd_total;
     pp.rowsd_off = d.rowsd_off;
@@ -396,7 +420,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if ((int64_t)d.scp_pitch * s.h * 2 >= (1ll << 32)) return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
     // the keyed search variants (1 narrow, 2 dot) write the moments
     const bool fused = fuse_ok && (mp & IVQM_M_IVSSIM) && !(mp & IVQM_M_IVMSSSIM) && !s.msk &&
-                       s.bd <= kMomMaxBits && (d.mode == 1 || d.mode == 2) && num_blocks(s.w, 8, 4) > 0 &&
+                       s.bd <= kMomMaxBits && (d.mode == 1 || d.mode == 2 || d.mode == kSearchDot420) &&
+                       num_blocks(s.w, 8, 4) > 0 &&
                        num_blocks(s.h, 8, 4) > 0;
     d.mom_off = -1;
     if (fused) {
@@ -609,8 +634,8 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   // luma only when the picture's chroma planes are aligned, which depends on the caller's pointers).
   std::string launch_sig;
   char lb[160];
-  std::snprintf(lb, sizeof lb, "s%d t%d mt%d mf%x j%zu sb%d sf%x g%d am%d r%lld rd%lld r64%lld|", P.stat_blocks,
-                P.tiles, P.mtiles, P.mforms, P.jobs.size(), P.ssim_blocks, ssim_forms(P), P.ssim_gen ? 1 : 0,
+  std::snprintf(lb, sizeof lb, "s%d t%d/%d mt%d mf%x j%zu sb%d sf%x g%d am%d r%lld rd%lld r64%lld|", P.stat_blocks,
+                P.tiles, P.tiles420, P.mtiles, P.mforms, P.jobs.size(), P.ssim_blocks, ssim_forms(P), P.ssim_gen ? 1 : 0,
                 P.any_mask ? 1 : 0, (long long)P.rows_total, (long long)P.rowsd_total, (long long)P.rows64_total);
   launch_sig += lb;
   for (int l = 1; l < 5; l++) {
@@ -772,6 +797,15 @@ int Engine::fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_fi
   return IVQM_OK;
 }
 
+// Both search kernels of a batch: k_search over its tiles, k_search420 over the keyed 4:2:0 pairs' tiles.
+hipError_t Engine::launch_search_all(const BatchPlan& P, hipStream_t st) {
+  hipError_t e = launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows_,
+                               d_scp_, d_best_, st);
+  if (e != hipSuccess) return e;
+  return launch_search420(d_pairs_, P.n, P.tiles420, cfg_.search_range, P.th420, cfg_.cmp_weights_search, d_gcd_,
+                          d_rows_, d_scp_, d_best_, st);
+}
+
 int Engine::issue(const BatchPlan& P, RingSlot& S) {
   const hipStream_t st = stream_;
   const int n = P.n;
@@ -789,8 +823,7 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
     HIPCHK(launch_search_masked(d_pairs_, n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows64_,
                                 P.mforms, st));
   if (P.need_search)
-    HIPCHK(launch_search(d_pairs_, n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
-                         d_scp_, d_best_, st));
+    HIPCHK(launch_search_all(P, st));
   for (int l = 1; l < 5; l++)
     if (!P.down[l].empty())
       HIPCHK(launch_downsample(d_down_ + down_first[l], (int)P.down[l].size(), P.down_blocks[l], st));
@@ -1039,8 +1072,7 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
         HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, stream_));
         break;
       case 1:
-        HIPCHK(launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, P.py, cfg_.cmp_weights_search, d_gcd_, d_rows_,
-                             d_scp_, d_best_, stream_));
+        HIPCHK(launch_search_all(P, stream_));
         break;
       case 2:
         if (P.ssim_gen)

@@ -399,7 +399,7 @@ template <int R> struct KeyBits {
 template <int R> struct SearchTile {
   static constexpr int TW = kSearchTileW, TH = 16, LW = TW + 2 * R, LH = TH + 2 * R, LP = (LW + 3) & ~3;
 };
-static_assert(SearchTile<1>::TH == 4 * kSearchRowsPerWave, "tile height");
+static_assert(SearchTile<1>::TH == kSearchTileH, "tile height");
 
 struct SEnt {
   int x, y, z;
@@ -1410,10 +1410,9 @@ static hipError_t search_r(const PairDev* pairs, int npairs, int ntiles, const i
   return hipGetLastError();
 }
 
-hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, int py, const int32_t cw[4],
+hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
   if (npairs <= 0 || ntiles <= 0) return hipSuccess;
-  if (py != kSearchRowsPerWave) return hipErrorInvalidValue;
   switch (R) {
     case 1: return search_r<1>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
     case 2: return search_r<2>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);

@@ -119,8 +119,8 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
   uint32_t* d_rows = reinterpret_cast<uint32_t*>(base + kOffRows);
 
   // pair table: the batch engine's plan of an unmasked pair without SSIM metrics
-  const int TH = search_tile_h(kSearchRowsPerWave);
-  int stat_blocks = 0, tiles = 0;
+  const int TH = kSearchTileH, TH420 = search420_tile_rows(cfg.search_tile_rows, 32);
+  int stat_blocks = 0, tiles = 0, tiles420 = 0;
   int64_t rowsd_off = 0, rows_off = 0;
   for (int i = 0; i < n; i++) {
     const ivqm_pair& p = pairs[i];
@@ -142,7 +142,13 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
     d.stat_blk0 = stat_blocks;
     stat_blocks += p.height + 2 * (p.height >> d.p0.cs);
     d.tile0 = tiles;
-    if (m & IVQM_M_IVPSNR) tiles += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH - 1) / TH);
+    d.tile0b = tiles420;
+    if (d.p0.cs == 1 && (d.mode == 1 || d.mode == 2) && cfg.search_mode_cap == 0 && search420_ok(cfg, d.maxv, gmax))
+      d.mode = kSearchDot420;
+    if ((m & IVQM_M_IVPSNR) && d.mode == kSearchDot420)
+      tiles420 += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH420 - 1) / TH420);
+    else if (m & IVQM_M_IVPSNR)
+      tiles += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH - 1) / TH);
     d.rowsd_off = rowsd_off;
     rowsd_off += 3 * (int64_t)p.height;
     d.rows_off = rows_off;
@@ -160,8 +166,11 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
   NA_HIPCHK(launch_stats(d_pairs, n, stat_blocks, d_row_sd, d_row_ssd, d_rows, nullptr, nullptr, st));
   NA_HIPCHK(launch_gcd_finalize(d_pairs, n, d_row_sd, d_row_ssd, nullptr, d_gcd, d_sd, d_ssd, nullptr, st));
   if (tiles)
-    NA_HIPCHK(launch_search(d_pairs, n, tiles, cfg.search_range, kSearchRowsPerWave, cfg.cmp_weights_search, d_gcd,
-                            d_rows, nullptr, nullptr, st));
+    NA_HIPCHK(launch_search(d_pairs, n, tiles, cfg.search_range, cfg.cmp_weights_search, d_gcd, d_rows, nullptr,
+                            nullptr, st));
+  if (tiles420)
+    NA_HIPCHK(launch_search420(d_pairs, n, tiles420, cfg.search_range, TH420, cfg.cmp_weights_search, d_gcd, d_rows,
+                               nullptr, nullptr, st));
   NA_HIPCHK(hipMemcpyAsync(h_gcd, d_gcd, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st));
   NA_HIPCHK(hipMemcpyAsync(h_ssd, d_ssd, sizeof(uint64_t) * 3 * n, hipMemcpyDeviceToHost, st));
   if ((m & IVQM_M_WSPSNR) && cfg.erp)

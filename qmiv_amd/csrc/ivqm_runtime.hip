@@ -651,16 +651,16 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
   RCCHK(batch_specs(pairs, n, specs));
   double best_s = 1e30, best_j = 1e30;
   int py_best = b->eng->tuning_py(), nb_best = b->eng->tuning_nb();
-  const int pys[2] = {4, 8}, nbs[3] = {4, 8, 16};
+  // the keyed 4:2:0 search's tile rows (the other search variants run 64 x 16 tiles)
+  const int ths[2] = {16, 32}, nbs[3] = {4, 8, 16};
   for (int pi = 0; pi < 2; pi++) {
-    if (pi > 0 && search_rows(0, pys[pi]) == search_rows(0, pys[0])) break;  // same tile shape
-    b->eng->set_tuning(pys[pi], nb_best);
+    b->eng->set_tuning(ths[pi], nb_best);
     RCCHK(b->eng->enqueue(specs.data(), n, metrics & 0xFFu, 0, false));
     RCCHK(b->eng->collect(0, nullptr));
-    if (!b->eng->last_plan().need_search) break;
+    if (!b->eng->last_plan().need_search || !b->eng->last_plan().tiles420) break;
     double t = 0;
     RCCHK(b->eng->profile(1, 5, &t));
-    if (t < best_s) { best_s = t; py_best = search_rows(0, pys[pi]); }
+    if (t < best_s) { best_s = t; py_best = ths[pi]; }
   }
   for (int ni = 0; ni < 3; ni++) {
     b->eng->set_tuning(py_best, nbs[ni]);

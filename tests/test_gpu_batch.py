@@ -73,9 +73,9 @@ def test_batch_autotune_keeps_results():
         pairs = b.make_pairs(items)
         ref = [r.as_dict() for r in b.run(pairs, Q.IVQM_M_ALL)]
         py, nb = b.autotune(pairs, Q.IVQM_M_ALL)
-        assert py in (4, 8) and nb in (4, 8, 16)
+        assert py in (16, 32) and nb in (4, 8, 16)
         print("autotuned: search tile rows", py, "ssim band rows", nb)
-        for py2, nb2 in ((4, 4), (8, 16), (py, nb)):
+        for py2, nb2 in ((16, 4), (32, 16), (4, 8), (py, nb)):  # 4 and 8 are taken as 16 and 32
             b.set_tuning(py2, nb2, False)
             assert [r.as_dict() for r in b.run(pairs, Q.IVQM_M_ALL)] == ref
 
