@@ -74,6 +74,7 @@ template <int R, int TH> struct Lds420 {
   using P = S420<R, TH>;
   int key[2][P::LH][P::LPK];
   unsigned int ys[2][P::LH][P::LPK];     // Y
+  unsigned int cu[2][P::LH][P::LPK];     // {Cb, Cr} u16 pair of the entry's chroma sample (epilogue)
   int cw[2][P::LHC][P::LPC];             // {Cb * Sy, Cr * Sy} u16 pairs at chroma resolution
 };
 
@@ -213,6 +214,7 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
         if (ly >= 0 && ly < LH) {
           L.key[dir][ly][lane] = (int)(((uint32_t)(q_of(uwY, ra[k][h]) + qc) << KB) + (uint32_t)(ly * DP + lane));
           L.ys[dir][ly][lane] = (uint32_t)ra[k][h];
+          L.cu[dir][ly][lane] = (uint32_t)ra[k][2] | ((uint32_t)ra[k][3] << 16);
         }
       }
       if (cwriter(lane)) L.cw[dir][j][(lane + R1) >> 1] = chroma_word(ra[k][2], ra[k][3]);
@@ -226,6 +228,7 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
       const int q = q_of(uwY, rh[k][0]) + q_of(uwCb, rh[k][1]) + q_of(uwCr, rh[k][2]);
       L.key[dir][hy][lx] = (int)(((uint32_t)q << KB) + (uint32_t)(hy * DP + lx));
       L.ys[dir][hy][lx] = (uint32_t)rh[k][0];
+      L.cu[dir][hy][lx] = (uint32_t)rh[k][1] | ((uint32_t)rh[k][2] << 16);
       if (cwriter(lx)) L.cw[dir][(hy + R1) >> 1][(lx + R1) >> 1] = chroma_word(rh[k][1], rh[k][2]);
     }
   }
@@ -311,18 +314,22 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
   asm volatile("" ::: "memory");
   // ---- epilogue, one pixel row of the lane at a time: winners, squared differences, row sums, winner map,
   // SCP / fused moments ----
-  // The winner of pixel (iy0 + r, ix + k) at candidate (ddy, ddx) is the region entry (iy0 + r + ddy,
-  // ix + k + ddx); its chroma is chroma entry ((iy0 + r + ddy + R1) >> 1, (ix + k + ddx + R1) >> 1).
+  // The winner of pixel (iy0 + r, ix + k) is region entry (iy0 + r + ddy, ix + k + ddx) with
+  // u = ddy * DP + ddx = (key - pixel constant) mod Sk; its Y and chroma words sit at the entry's offset
+  // 4 (u + (LPK - DP) (u >> LOGDP)) from the pixel's own entry.
   const bool edge = x0 + 64 > W || y0 + TH > H;  // block-uniform
   const bool want_scp = Pr.scp_off >= 0, want_mom = Pr.mom_off >= 0;  // block-uniform
   const int gc[3] = {g0, g1, g2};
-  u16x2 gp[3], gn[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    gp[c] = (unsigned short)max(gc[c], 0);
-    gn[c] = (unsigned short)max(-gc[c], 0);
-  }
   const u16x2 mx = (unsigned short)Pr.maxv;
+  // test chroma + g of pixel pair h, per component, in both halves (shared by both rows of the lane)
+  uint32_t tcp[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t w = L.cu[1 - dir][iy0 + R][ix + 2 * h + R];
+    const u16x2 cb = (unsigned short)((w & 0xffffu) + (uint32_t)g1), cr = (unsigned short)((w >> 16) + (uint32_t)g2);
+    tcp[0][h] = __builtin_bit_cast(uint32_t, cb);
+    tcp[1][h] = __builtin_bit_cast(uint32_t, cr);
+  }
   // fused IV-SSIM partials over the lane's 8 pixels: per component sum a and sum b as u16 pairs (pixels
   // 2h, 2h + 1), S_aa, S_bb, S_ab; a = the test picture's sample (t - g), b = the SCP sample
   u16x2 sa[3], sb[3];
@@ -332,37 +339,34 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
     sa[c] = sb[c] = (unsigned short)0;
     saa[c] = sbb[c] = sab[c] = 0u;
   }
+  constexpr int UMAX = DP * 2 * R + 2 * R;
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     __builtin_amdgcn_sched_barrier(0);
     const int y = y0 + iy0 + r;
+    const unsigned int* ybase = &L.ys[dir][iy0 + r][ix];
+    const unsigned int* cbase = &L.cu[dir][iy0 + r][ix];
     uint32_t wy[4], wc[4];
-    int idx[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int u = (bk[r][k] - ((iy0 + r) * DP + ix + k)) & ((1 << KB) - 1);
-      // exact keys give ddy, ddx <= 2R; the clamp keeps the winner reads inside the region regardless
-      const int ddy = min(u >> LOGDP, 2 * R), ddx = min(u & (DP - 1), 2 * R);
-      idx[k] = ddy * D + ddx;
-      wy[k] = L.ys[dir][iy0 + r + ddy][ix + k + ddx];
-      wc[k] = (uint32_t)L.cw[dir][(iy0 >> 1) + ((r + R1 + ddy) >> 1)][(ix >> 1) + ((k + R1 + ddx) >> 1)];
+      // exact keys give u <= UMAX; the clamp keeps the reads inside the region regardless
+      const uint32_t u = min((uint32_t)(bk[r][k] - ((iy0 + r) * DP + ix + k)) & ((1u << KB) - 1u), (uint32_t)UMAX);
+      const uint32_t o = u + (uint32_t)(P::LPK - DP) * (u >> LOGDP);  // in entries
+      wy[k] = ybase[k + o];
+      wc[k] = cbase[k + o];
     }
     // u16 pairs of pixels (2h, 2h + 1): test + g and winner samples per component
     uint32_t tp[3][2], wp[3][2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      const uint32_t w = test_c(h);
-      const unsigned short tcb = (unsigned short)(((w & 0xffffu) >> YB) + g1), tcr = (unsigned short)((w >> (16 + YB)) + g2);
-      const u16x2 ty = {(unsigned short)(test_y(r, 2 * h) + g0), (unsigned short)(test_y(r, 2 * h + 1) + g0)};
-      tp[0][h] = __builtin_bit_cast(uint32_t, ty);
-      tp[1][h] = __builtin_bit_cast(uint32_t, (u16x2)tcb);
-      tp[2][h] = __builtin_bit_cast(uint32_t, (u16x2)tcr);
-      wp[0][h] = wy[2 * h] | (wy[2 * h + 1] << 16);
-      const u16x2 c0 = __builtin_bit_cast(u16x2, wc[2 * h]) >> (unsigned short)YB;
-      const u16x2 c1 = __builtin_bit_cast(u16x2, wc[2 * h + 1]) >> (unsigned short)YB;
-      const u16x2 cb = {c0.x, c1.x}, crr = {c0.y, c1.y};
-      wp[1][h] = __builtin_bit_cast(uint32_t, cb);
-      wp[2][h] = __builtin_bit_cast(uint32_t, crr);
+      const unsigned int* ty = &L.ys[1 - dir][iy0 + r + R][ix + 2 * h + R];
+      const u16x2 tyv = {(unsigned short)ty[0], (unsigned short)ty[1]};
+      tp[0][h] = __builtin_bit_cast(uint32_t, tyv + (unsigned short)g0);
+      tp[1][h] = tcp[0][h];
+      tp[2][h] = tcp[1][h];
+      wp[0][h] = __builtin_amdgcn_perm(wy[2 * h + 1], wy[2 * h], 0x05040100u);
+      wp[1][h] = __builtin_amdgcn_perm(wc[2 * h + 1], wc[2 * h], 0x05040100u);
+      wp[2][h] = __builtin_amdgcn_perm(wc[2 * h + 1], wc[2 * h], 0x07060302u);
     }
     // |t - r| <= Max + |gcd| < 2^15 (host-checked): int16 differences, v_dot2 adds their squares
     uint32_t q[3];
@@ -392,20 +396,23 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
       const int x = x0 + ix;
       GU8* bm = (GU8*)best + Pr.best_off + (size_t)dir * W * H + (size_t)y * W + x;
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (x + k < W) bm[k] = (uint8_t)idx[k];
+      for (int k = 0; k < 4; k++) {
+        const uint32_t u = min((uint32_t)(bk[r][k] - ((iy0 + r) * DP + ix + k)) & ((1u << KB) - 1u), (uint32_t)UMAX);
+        if (x + k < W) bm[k] = (uint8_t)((u >> LOGDP) * D + (u & (DP - 1)));
+      }
     }
     if (want_scp || want_mom) {
-      // SCP samples clip(r - g, 0, Max) as saturating u16 ops (r in [0, Max])
+      // SCP samples clip(r - g, 0, Max) for r in [0, Max]: r - g saturated at 0 when g >= 0,
+      // min(r + |g|, Max) otherwise (g is uniform)
       u16x2 sv[3][2];
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int h = 0; h < 2; h++)
-          sv[c][h] = __builtin_elementwise_min(
-              __builtin_elementwise_add_sat(__builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, wp[c][h]), gp[c]),
-                                            gn[c]),
-              mx);
+        for (int h = 0; h < 2; h++) {
+          const u16x2 w = __builtin_bit_cast(u16x2, wp[c][h]);
+          sv[c][h] = gc[c] >= 0 ? __builtin_elementwise_sub_sat(w, (u16x2)(unsigned short)gc[c])
+                                : __builtin_elementwise_min(w + (u16x2)(unsigned short)(-gc[c]), mx);
+        }
       if (want_scp && y < H) {
         typedef __attribute__((address_space(1))) uint8_t GU8;
         typedef __attribute__((address_space(1))) u32x2 GU32x2;
