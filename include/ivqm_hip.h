@@ -159,10 +159,13 @@ typedef struct ivqm_mask {
  * and ivqm_frame run the masked variants the app uses
  * (xGlobClrDiffProc::CalcGlobalColorDiffM, xGlobClrDiff.h:42; xPSNR::calcPicMSEM /
  * calcPicPSNRM, xPSNR.h:37-39; xWSPSNR::calcPicWSPSNRM, xWSPSNR.h:23;
- * xIVPSNRM::calcPicIVPSNRM, xIVPSNR.h:55-56), ssd[] holds the masked SSD, and
- * the structural-similarity metrics and SCP generation return IVQM_EINVAL (the
- * app rejects that combination, xAppQMIV.cpp:360). A mask without any non-zero
- * sample makes the metric calls return IVQM_EINVAL. */
+ * xIVPSNRM::calcPicIVPSNRM, xIVPSNR.h:55-56), ssd[] holds the masked SSD,
+ * ivqm_gen_scp runs xShftCompPicProc::GenShftCompPicsM (xShftCompPic.h:48-50:
+ * pixels whose mask sample is 0 are not searched and hold 0 in the SCP
+ * pictures; the reference leaves them unwritten), and the structural-similarity
+ * metrics return IVQM_EINVAL (the app rejects that combination,
+ * xAppQMIV.cpp:360). A mask without any non-zero sample makes the metric calls
+ * return IVQM_EINVAL. */
 int ivqm_bind_mask(ivqm_ctx* ctx, const ivqm_mask* dev_mask);
 /* Copies a host mask into the context's own buffer, then binds it. */
 int ivqm_upload_mask(ivqm_ctx* ctx, const ivqm_mask* host_mask);

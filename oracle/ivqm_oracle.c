@@ -766,6 +766,26 @@ OR_EXPORT void or_psnr_family_masked(const uint16_t* const p0[3], const uint16_t
  * picture is the Nearest-extended mask, xAppQMIV.cpp:899); the winner is the
  * first strict minimum of the int32 (wrapping) weighted error; the row sum is
  * uint64 of (uint64)(int32 square) * M per component. */
+static void or_masked_best(const uint16_t* const ref[3], const uint16_t* msk, int32_t w, int32_t h, int32_t x,
+                           int32_t y, const int32_t t[3], int32_t R, const int32_t cw[4], int32_t* bx, int32_t* by) {
+  int32_t best_err = INT32_MAX;
+  *bx = -1;
+  *by = -1;
+  for (int32_t dy = -R; dy <= R; dy++) {
+    const int32_t ry = or_clampi(y + dy, 0, h - 1);
+    for (int32_t dx = -R; dx <= R; dx++) {
+      const int32_t rx = or_clampi(x + dx, 0, w - 1);
+      if (msk[(size_t)ry * w + rx] == 0) continue;
+      uint32_t e = 0;
+      for (int c = 0; c < 3; c++) {
+        const uint32_t d = (uint32_t)(t[c] - (int32_t)ref[c][(size_t)ry * w + rx]);
+        e += d * d * (uint32_t)cw[c];
+      }
+      if ((int32_t)e < best_err) { best_err = (int32_t)e; *bx = rx; *by = ry; }
+    }
+  }
+}
+
 OR_EXPORT void or_cps_masked(const uint16_t* const tst[3], const uint16_t* const ref[3], const uint16_t* msk,
                              int32_t w, int32_t h, const int32_t gcs[4], int32_t R, const int32_t cw[4],
                              uint64_t* rows) {
@@ -774,26 +794,43 @@ OR_EXPORT void or_cps_masked(const uint16_t* const tst[3], const uint16_t* const
     for (int32_t x = 0; x < w; x++) {
       const uint16_t m = msk[(size_t)y * w + x];
       if (m == 0) continue;
-      int32_t t[3];
+      int32_t t[3], bx, by;
       for (int c = 0; c < 3; c++) t[c] = (int32_t)tst[c][(size_t)y * w + x] + gcs[c];
-      int32_t best_err = INT32_MAX, bx = -1, by = -1;
-      for (int32_t dy = -R; dy <= R; dy++) {
-        const int32_t ry = or_clampi(y + dy, 0, h - 1);
-        for (int32_t dx = -R; dx <= R; dx++) {
-          const int32_t rx = or_clampi(x + dx, 0, w - 1);
-          if (msk[(size_t)ry * w + rx] == 0) continue;
-          uint32_t e = 0;
-          for (int c = 0; c < 3; c++) {
-            const uint32_t d = (uint32_t)(t[c] - (int32_t)ref[c][(size_t)ry * w + rx]);
-            e += d * d * (uint32_t)cw[c];
-          }
-          if ((int32_t)e < best_err) { best_err = (int32_t)e; bx = rx; by = ry; }
-        }
-      }
+      or_masked_best(ref, msk, w, h, x, y, t, R, cw, &bx, &by);
       for (int c = 0; c < 3; c++) row[c] += or_wssd_term(t[c] - (int32_t)ref[c][(size_t)by * w + bx], m);
     }
     for (int c = 0; c < 4; c++) rows[(size_t)y * 4 + c] = row[c];
   }
+}
+
+/* Shift-compensated pictures with a mask: xShftCompPic::GenShftCompPicsM
+ * (libs/pmbb-ivqm/src/xShftCompPic.cpp:32-42, :80-117) ->
+ * xCorrespPixelShiftSTD::GenShftCompRowM (xCorrespPixelShiftSTD.cpp:268-291):
+ * for every pixel with M != 0, the masked search's winner in Ref minus the
+ * shift, clipped to [0, Max]; pixels with M = 0 are left as they are.
+ * scp1 = from (Ref = P1, Tst = P0, +GCD); scp0 = from (Ref = P0, Tst = P1, -GCD). */
+static void or_scp_masked_dir(const uint16_t* const tst[3], const uint16_t* const ref[3], const uint16_t* msk,
+                              int32_t w, int32_t h, int32_t bit_depth, const int32_t gcs[4], int32_t R,
+                              const int32_t cw[4], uint16_t* const dst[3]) {
+  const int32_t maxv = (1 << bit_depth) - 1;
+  for (int32_t y = 0; y < h; y++)
+    for (int32_t x = 0; x < w; x++) {
+      if (msk[(size_t)y * w + x] == 0) continue;
+      int32_t t[3], bx, by;
+      for (int c = 0; c < 3; c++) t[c] = (int32_t)tst[c][(size_t)y * w + x] + gcs[c];
+      or_masked_best(ref, msk, w, h, x, y, t, R, cw, &bx, &by);
+      for (int c = 0; c < 3; c++)
+        dst[c][(size_t)y * w + x] = (uint16_t)or_clampi((int32_t)ref[c][(size_t)by * w + bx] - gcs[c], 0, maxv);
+    }
+}
+
+OR_EXPORT void or_gen_scp_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk,
+                                 int32_t w, int32_t h, int32_t bit_depth, const int32_t gcd_r2t[4], int32_t R,
+                                 const int32_t cw_search[4], uint16_t* const scp0[3], uint16_t* const scp1[3]) {
+  int32_t ng[4];
+  for (int c = 0; c < 4; c++) ng[c] = -gcd_r2t[c];
+  or_scp_masked_dir(p0, p1, msk, w, h, bit_depth, gcd_r2t, R, cw_search, scp1);
+  or_scp_masked_dir(p1, p0, msk, w, h, bit_depth, ng, R, cw_search, scp0);
 }
 
 /* IV-PSNR with a mask: xIVPSNRM::calcPicIVPSNRM(TstI, RefI, Msk, NNM, GCD_R2T)

@@ -308,6 +308,19 @@ class Oracle:
                              rows.ctypes.data_as(_u64p))
         return rows
 
+    def scp_masked(self, p0, p1, msk, bd, gcd, R=2, cws=(4, 1, 1, 0), fill=0):
+        """(SCP0, SCP1) with a mask; pixels whose mask sample is 0 keep `fill`."""
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        s0 = [np.full((h, w), fill, np.uint16) for _ in range(3)]
+        s1 = [np.full((h, w), fill, np.uint16) for _ in range(3)]
+        self.L.or_gen_scp_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, _i32x4(gcd), R, _i32x4(cws),
+                                 (_u16p * 3)(*[s.ctypes.data_as(_u16p) for s in s0]),
+                                 (_u16p * 3)(*[s.ctypes.data_as(_u16p) for s in s1]))
+        return s0, s1
+
     def ivpsnr_masked(self, p0, p1, msk, bd, bdm, gcd, R=2, cws=(4, 1, 1, 0), cwa=(4, 1, 1, 0), erp=False, lat=180):
         a, pa = _planes(p0)
         b, pb = _planes(p1)
@@ -540,6 +553,19 @@ class Reference:
         mse, psnr, ws = (C.c_double * 4)(), (C.c_double * 4)(), (C.c_double * 4)()
         self.L.ref_psnr_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, bdm, int(erp), lat, mse, psnr, ws)
         return list(mse)[:3], list(psnr)[:3], list(ws)[:3]
+
+    def scp_masked(self, p0, p1, msk, bd, gcd, R=2, cws=(4, 1, 1, 0), fill=0):
+        """(SCP0, SCP1) with a mask; pixels whose mask sample is 0 keep `fill`."""
+        a, pa = _planes(p0)
+        b, pb = _planes(p1)
+        m = np.ascontiguousarray(msk, np.uint16)
+        h, w = a[0].shape
+        s0 = [np.full((h, w), fill, np.uint16) for _ in range(3)]
+        s1 = [np.full((h, w), fill, np.uint16) for _ in range(3)]
+        self.L.ref_scp_masked(pa, pb, m.ctypes.data_as(_u16p), w, h, bd, _i32x4(gcd), R, _i32x4(cws), int(fill),
+                              (_u16p * 3)(*[s.ctypes.data_as(_u16p) for s in s0]),
+                              (_u16p * 3)(*[s.ctypes.data_as(_u16p) for s in s1]))
+        return s0, s1
 
     def ivpsnr_masked(self, p0, p1, msk, bd, bdm, gcd, R=2, cws=(4, 1, 1, 0), cwa=(4, 1, 1, 0), erp=False, lat=180):
         a, pa = _planes(p0)

@@ -345,6 +345,38 @@ int ref_psnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], co
   return M.nnm;
 }
 
+// xShftCompPicProc::GenShftCompPicsM(SCI1, SCI0, I1, I0, Msk, GCD_R2T) + rearrangeToPlanar
+// (xAppQMIV.cpp:937-942). The destination pictures start filled with `fill`, which the
+// reference keeps at pixels whose mask sample is 0.
+int ref_scp_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w,
+                   int32_t h, int32_t bd, const int32_t gcd[4], int32_t R, const int32_t cw[4], int32_t fill,
+                   uint16_t* const scp0[3], uint16_t* const scp1[3]) {
+  const int32 mg = margin_for(R);
+  Masked M(p0, p1, msk, w, h, bd, bd, mg);
+  xPicI ai(int32V2(w, h), bd, mg), bi(int32V2(w, h), bd, mg), si0(int32V2(w, h), bd, mg), si1(int32V2(w, h), bd, mg);
+  ai.rearrangeFromPlanar(&M.a);
+  bi.rearrangeFromPlanar(&M.b);
+  xPicP f(int32V2(w, h), bd, mg), s0(int32V2(w, h), bd, mg), s1(int32V2(w, h), bd, mg);
+  for (int c = 0; c < 3; c++) {
+    uint16* dst = f.getAddr((eCmp)c);
+    for (int32 y = 0; y < h; y++)
+      for (int32 x = 0; x < w; x++) dst[(size_t)y * f.getStride() + x] = (uint16)fill;
+  }
+  si0.rearrangeFromPlanar(&f);
+  si1.rearrangeFromPlanar(&f);
+  Inline in;
+  xShftCompPicProc proc;
+  proc.setSearchRange(R);
+  proc.setCmpWeightsSearch(v4(cw));
+  proc.bindThrdPoolIntf(&in.tpi);
+  proc.GenShftCompPicsM(&si1, &si0, &bi, &ai, &M.m, v4(gcd));
+  si0.rearrangeToPlanar(&s0);
+  si1.rearrangeToPlanar(&s1);
+  store_pic(scp0, s0);
+  store_pic(scp1, s1);
+  return M.nnm;
+}
+
 int ref_ivpsnr_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* msk, int32_t w,
                       int32_t h, int32_t bd, int32_t bdm, const int32_t gcd[4], int32_t R, const int32_t cws[4],
                       const int32_t cwa[4], int32_t erp, int32_t lat, double* out, double* r2t, double* t2r) {

@@ -118,10 +118,11 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
                                const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,
                                hipStream_t st);
 // Masked search of both directions: tiles of 64 x 16 pixels per direction (ivqm_masked.hip). forms: bit f
-// set when a pair of form f (0 reference form, 1 keyed narrow, 2 keyed dot) is in the batch.
+// set when a pair of form f (0 reference form, 1 keyed narrow, 2 keyed dot) is in the batch. Pairs with
+// scp_off >= 0 also write their SCP pictures into scp.
 constexpr int kMaskTileW = 64, kMaskTileH = 16;
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                                const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st);
+                                const int32_t* gcd, uint64_t* rows64, uint16_t* scp, uint32_t forms, hipStream_t st);
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
 // forms: bit 0 when jobs read pictures, bit 1 when jobs read fused moments (one launch each).

@@ -368,7 +368,6 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if (s.msk) {  // mask mode (xAppQMIV.cpp:360-365: no structural-similarity metric with a mask)
       if (mp & (IVQM_M_SSIM | IVQM_M_MSSSIM | ivs))
         return set_error(IVQM_EINVAL, "structural similarity metrics cannot be combined with a mask");
-      if (scp_only) return set_error(IVQM_EINVAL, "shift-compensated pictures are not generated with a mask");
       d.msk = s.msk;
       d.msk_stride = s.msk_stride;
       d.msk_bps = s.msk_bps;
@@ -393,7 +392,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       d.mode = kSearchDot420;
     d.mtile0 = P.mtiles;
     if (s.msk) {
-      if (mp & IVQM_M_IVPSNR) {
+      if ((mp & IVQM_M_IVPSNR) || scp_only) {  // the masked search also writes the SCP pictures
         P.mtiles += 2 * ((s.w + kMaskTileW - 1) / kMaskTileW) * ((s.h + kMaskTileH - 1) / kMaskTileH);
         P.mforms |= 1u << (d.mkey == 0 ? 0 : (d.mdot ? 2 : 1));
       }
@@ -821,7 +820,7 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
   }
   if (P.mtiles)
     HIPCHK(launch_search_masked(d_pairs_, n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows64_,
-                                P.mforms, st));
+                                d_scp_, P.mforms, st));
   if (P.need_search)
     HIPCHK(launch_search_all(P, st));
   for (int l = 1; l < 5; l++)
@@ -1084,7 +1083,7 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
         break;
       case 3:
         HIPCHK(launch_search_masked(d_pairs_, P.n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_,
-                                    d_rows64_, P.mforms, stream_));
+                                    d_rows64_, d_scp_, P.mforms, stream_));
         break;
       default: return set_error(IVQM_EINVAL, "unknown kernel id");
     }

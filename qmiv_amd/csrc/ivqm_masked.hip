@@ -19,6 +19,12 @@
 // considered). Candidate (0, 0) carries the pixel's own non-zero weight and is
 // always considered. Row sums: a DPP wave reduction per row (no LDS), one
 // uint64 atomic per component.
+//
+// Shift-compensated pictures with a mask (PairDev::scp_off >= 0): the winner's
+// reference sample minus the direction's colour shift, clipped to [0, Max]
+// (xCorrespPixelShiftSTD.cpp:268-291, GenShftCompRowM), at every pixel whose
+// weight is non-zero; the reference leaves masked pixels unwritten, here they
+// hold 0. Direction 0 writes SCP1, direction 1 SCP0, as the unmasked search.
 #include "ivqm_device.h"
 #include "ivqm_device_util.h"
 
@@ -36,9 +42,19 @@ template <int R> struct MaskTile {
 };
 static_assert(kMaskTileH == 16 && kMaskTileW == 64, "4 waves x 4 rows, one column per lane");
 
+// One SCP sample triple of pixel (x, y): clip(r - g, 0, Max) per component, 0 for a masked pixel.
+__device__ __forceinline__ void masked_scp_store(const PairDev& P, int dir, int x, int y, bool keep, const int (&r)[3],
+                                                 const int (&g)[3], uint16_t* __restrict__ scp) {
+  const size_t plane = (size_t)P.scp_pitch * P.h;
+  uint16_t* pic = scp + P.scp_off + (size_t)(1 - dir) * 3 * plane + (size_t)y * P.scp_pitch + x;
+#pragma unroll
+  for (int c = 0; c < 3; c++) pic[c * plane] = keep ? (uint16_t)clampi(r[c] - g[c], 0, P.maxv) : (uint16_t)0;
+}
+
 template <typename T, typename TM, int R>
 __device__ void masked_tile(const PairDev& P, int dir, int x0, int y0, u32x2m_t (*lds)[MaskTile<R>::LW], int4 cw,
-                            const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64) {
+                            const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64,
+                            uint16_t* __restrict__ scp) {
   using MT = MaskTile<R>;
   constexpr int D = MT::D, LW = MT::LW, LH = MT::LH;
   const int W = P.w, H = P.h;
@@ -102,6 +118,14 @@ __device__ void masked_tile(const PairDev& P, int dir, int x0, int y0, u32x2m_t 
       v[0] = (uint64_t)(int64_t)(int32_t)(d0 * d0) * (uint64_t)m;
       v[1] = (uint64_t)(int64_t)(int32_t)(d1 * d1) * (uint64_t)m;
       v[2] = (uint64_t)(int64_t)(int32_t)(d2 * d2) * (uint64_t)m;
+      if (P.scp_off >= 0) {
+        const int r3[3] = {(int)(c.x & 0xffffu), (int)(c.x >> 16), (int)(c.y & 0xffffu)};
+        const int g3[3] = {g0, g1, g2};
+        masked_scp_store(P, dir, x, y, true, r3, g3, scp);
+      }
+    } else if (x < W && P.scp_off >= 0) {
+      const int z3[3] = {0, 0, 0};
+      masked_scp_store(P, dir, x, y, false, z3, z3, scp);
     }
     const uint64_t s0 = wave_sum_u64_dpp(v[0]), s1 = wave_sum_u64_dpp(v[1]), s2 = wave_sum_u64_dpp(v[2]);
     if (lane < 3) {
@@ -166,7 +190,8 @@ __device__ __forceinline__ uint64_t row16_sum_u64_dpp(uint64_t v) {
 // candidate, three planes.
 template <typename T, typename TM, int R, bool DOT>
 __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int* lds, int4 cw,
-                                  const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64) {
+                                  const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64,
+                                  uint16_t* __restrict__ scp) {
   using MT = MaskTile<R>;
   constexpr int D = MT::D, LW = MT::LW, LH = MT::LH, LWP = MT::LWP, S = 1 << MT::KB, NE = 4 + 2 * R;
   int (*lk)[LWP] = reinterpret_cast<int (*)[LWP]>(lds);
@@ -269,11 +294,20 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
         rY = ly_[ey][ex];
         rCb = lb[ey][ex];
       }
-      const int d0 = t[k][0] - rY, d1 = t[k][1] - rCb, d2 = t[k][2] - lr[ey][ex];
+      const int rCr = lr[ey][ex];
+      const int d0 = t[k][0] - rY, d1 = t[k][1] - rCb, d2 = t[k][2] - rCr;
       // |d| < 2^15: the squares are positive int32
       v[0] += (uint64_t)(int64_t)(d0 * d0) * (uint64_t)m[k];
       v[1] += (uint64_t)(int64_t)(d1 * d1) * (uint64_t)m[k];
       v[2] += (uint64_t)(int64_t)(d2 * d2) * (uint64_t)m[k];
+      if (P.scp_off >= 0) {
+        const int r3[3] = {rY, rCb, rCr};
+        const int g3[3] = {g0, g1, g2};
+        masked_scp_store(P, dir, x + k, y, true, r3, g3, scp);
+      }
+    } else if (P.scp_off >= 0 && y < H && x + k < W) {
+      const int z3[3] = {0, 0, 0};
+      masked_scp_store(P, dir, x + k, y, false, z3, z3, scp);
     }
   }
   const uint64_t s0 = row16_sum_u64_dpp(v[0]), s1 = row16_sum_u64_dpp(v[1]), s2 = row16_sum_u64_dpp(v[2]);
@@ -293,7 +327,7 @@ __device__ __forceinline__ int masked_form(const PairDev& P) { return P.mkey == 
 template <int R, int F>
 __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict__ pairs, int npairs, int ntiles,
                                                        int4 cw, const int32_t* __restrict__ gcd,
-                                                       uint64_t* __restrict__ rows64) {
+                                                       uint64_t* __restrict__ rows64, uint16_t* __restrict__ scp) {
   using MT = MaskTile<R>;
   // keyed forms: int planes of LH x LWP; the reference form: LH x LW 8-byte entries
   __shared__ __attribute__((aligned(16))) int ldsi[(F == 2 ? 3 : 4) * MT::LH * MT::LWP];
@@ -310,44 +344,44 @@ __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict
   if (F == 0) {
     auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(ldsi);
     if (P.p0.bps == 2) {
-      if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
-      else masked_tile<uint16_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64, scp);
+      else masked_tile<uint16_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64, scp);
     } else {
-      if (P.msk_bps == 2) masked_tile<uint8_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
-      else masked_tile<uint8_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile<uint8_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64, scp);
+      else masked_tile<uint8_t, uint8_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64, scp);
     }
   } else {
     constexpr bool DOT = F == 2;
     if (P.p0.bps == 2) {
-      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
-      else masked_tile_keyed<uint16_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile_keyed<uint16_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+      else masked_tile_keyed<uint16_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
     } else {
-      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
-      else masked_tile_keyed<uint8_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64);
+      if (P.msk_bps == 2) masked_tile_keyed<uint8_t, uint16_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+      else masked_tile_keyed<uint8_t, uint8_t, R, DOT>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
     }
   }
 }
 
 template <int R>
 hipError_t masked_r(const PairDev* pairs, int npairs, int ntiles, int4 w, const int32_t* gcd, uint64_t* rows64,
-                    uint32_t forms, hipStream_t st) {
-  if (forms & 1u) hipLaunchKernelGGL((k_search_masked<R, 0>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
-  if (forms & 2u) hipLaunchKernelGGL((k_search_masked<R, 1>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
-  if (forms & 4u) hipLaunchKernelGGL((k_search_masked<R, 2>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64);
+                    uint16_t* scp, uint32_t forms, hipStream_t st) {
+  if (forms & 1u) hipLaunchKernelGGL((k_search_masked<R, 0>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
+  if (forms & 2u) hipLaunchKernelGGL((k_search_masked<R, 1>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
+  if (forms & 4u) hipLaunchKernelGGL((k_search_masked<R, 2>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
   return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                                const int32_t* gcd, uint64_t* rows64, uint32_t forms, hipStream_t st) {
+                                const int32_t* gcd, uint64_t* rows64, uint16_t* scp, uint32_t forms, hipStream_t st) {
   if (npairs <= 0 || ntiles <= 0 || !forms) return hipSuccess;
   const int4 w = make_int4(cw[0], cw[1], cw[2], cw[3]);
   switch (R) {
-    case 1: return masked_r<1>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
-    case 2: return masked_r<2>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
-    case 3: return masked_r<3>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
-    case 4: return masked_r<4>(pairs, npairs, ntiles, w, gcd, rows64, forms, st);
+    case 1: return masked_r<1>(pairs, npairs, ntiles, w, gcd, rows64, scp, forms, st);
+    case 2: return masked_r<2>(pairs, npairs, ntiles, w, gcd, rows64, scp, forms, st);
+    case 3: return masked_r<3>(pairs, npairs, ntiles, w, gcd, rows64, scp, forms, st);
+    case 4: return masked_r<4>(pairs, npairs, ntiles, w, gcd, rows64, scp, forms, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -179,6 +179,10 @@ def make_mask_cases(ref: Reference, orc: Oracle):
             case["psnr" + sfx] = [f(x) for x in psnr]
             case["wspsnr" + sfx] = [f(x) for x in ws]
             case["ivpsnr" + sfx] = [f(x) for x in iv]
+        # GenShftCompPicsM with the masked GCD; masked pixels keep the destination's 0
+        s0, s1 = ref.scp_masked(p0, p1, m, bd, g, R=R, fill=0)
+        case["scp0_sha1"] = [sha1(p) for p in s0]
+        case["scp1_sha1"] = [sha1(p) for p in s1]
         out.append(case)
     return out
 

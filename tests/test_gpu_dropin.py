@@ -1,5 +1,8 @@
-"""The C++ drop-in classes (include/ivqm_dropin.hpp), driven like the QMIV frame loop,
-against the oracle: GCD and IV-PSNR / PSNR / WS-PSNR bit-exact, SSIM family within 1e-6."""
+"""The C++ drop-in classes (include/ivqm_dropin.hpp), driven like the QMIV app's set-up and
+frame loop (tests/dropin/dropin_check.cpp), against the oracle: GCD, SCP pictures and
+IV-PSNR / PSNR / WS-PSNR / MSE bit-exact, the SSIM family within 1e-6; the debug callbacks
+receive the values the reference's call sites pass (xIVPSNR.cpp:23-53, :145-163;
+xPSNR.cpp:63; xWSPSNR.cpp:74; xIVSSIM.cpp:58, :89); mask mode through the masked calls."""
 import json
 import os
 import subprocess
@@ -12,30 +15,88 @@ from qmiv_amd import synth
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "qmiv_amd", "bin", "ivqm_dropin_check")
+W, H, BD = 544, 528, 10
+
+
+def _pair(oracle):
+    wl = synth.Workload(cfg=400, width=W, height=H, global_shift=1)
+    p0n, p1n = synth.make_pair(wl, 0)
+    return oracle.to444(p0n, True), oracle.to444(p1n, True)
+
+
+def _raw(path, planes):
+    path.write_bytes(b"".join(np.ascontiguousarray(p, np.uint16).tobytes() for p in planes))
+
+
+def _scp(path):
+    a = np.fromfile(path, np.uint16).reshape(3, H, W)
+    return [a[c] for c in range(3)]
+
+
+def _run(tmp_path, p0, p1, erp, mask=None, bdm=0):
+    if not os.path.exists(EXE):
+        pytest.fail("drop-in check program not built (python -m qmiv_amd.build)")
+    _raw(tmp_path / "p0.raw", p0)
+    _raw(tmp_path / "p1.raw", p1)
+    args = [EXE, str(W), str(H), str(BD), str(tmp_path / "p0.raw"), str(tmp_path / "p1.raw"), str(erp), str(tmp_path)]
+    if mask is not None:
+        _raw(tmp_path / "m.raw", [mask])
+        args += [str(tmp_path / "m.raw"), str(bdm)]
+    print(" ".join(args[1:4]), "erp", erp, "mask", mask is not None, flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout), _scp(tmp_path / "scp0.raw"), _scp(tmp_path / "scp1.raw")
 
 
 @pytest.mark.parametrize("erp", [0, 1])
 def test_dropin_classes_match_oracle(tmp_path, oracle, erp):
-    if not os.path.exists(EXE):
-        pytest.fail("drop-in check program not built (python -m qmiv_amd.build)")
-    wl = synth.Workload(cfg=400, width=544, height=528, global_shift=1)
-    p0n, p1n = synth.make_pair(wl, 0)
-    p0, p1 = oracle.to444(p0n, True), oracle.to444(p1n, True)
-    f0, f1 = tmp_path / "p0.raw", tmp_path / "p1.raw"
-    f0.write_bytes(b"".join(np.ascontiguousarray(p).tobytes() for p in p0))
-    f1.write_bytes(b"".join(np.ascontiguousarray(p).tobytes() for p in p1))
-    out = subprocess.run([EXE, str(wl.width), str(wl.height), "10", str(f0), str(f1), str(erp)],
-                         capture_output=True, text=True, timeout=120)
-    assert out.returncode == 0, out.stderr
-    r = json.loads(out.stdout)
-    g = oracle.gcd(p0, p1, 10)
+    p0, p1 = _pair(oracle)
+    r, s0, s1 = _run(tmp_path, p0, p1, erp)
+    g = oracle.gcd(p0, p1, BD)
     assert r["gcd"] == g
-    assert r["psnr"] == oracle.psnr(p0, p1, 10)
-    assert r["wspsnr"] == oracle.wspsnr(p0, p1, 10, erp=bool(erp))
-    assert [r["ivpsnr"], r["r2t"], r["t2r"]] == list(oracle.ivpsnr(p0, p1, 10, g, erp=bool(erp)))
-    s0, s1 = oracle.scp(p0, p1, 10, g)
-    assert r["scp_sum"] == sum(int(s0[c].sum(dtype=np.uint64)) * 3 + int(s1[c].sum(dtype=np.uint64)) for c in range(3))
-    assert np.max(np.abs(np.array(r["ssim"]) - oracle.ssim(p0, p1, 10))) <= 1e-6
-    assert np.max(np.abs(np.array(r["msssim"]) - oracle.msssim(p0, p1, 10))) <= 1e-6
-    assert abs(r["ivssim"] - oracle.ivssim(p0, p1, s0, s1, 10, erp=bool(erp))[0]) <= 1e-6
-    assert abs(r["ivmsssim"] - oracle.ivmsssim(p0, p1, s0, s1, 10)) <= 1e-6
+    w0, w1 = oracle.scp(p0, p1, BD, g)
+    assert all(np.array_equal(a, b) for a, b in zip(s0 + s1, w0 + w1))  # interleaved SCP, rearranged
+    assert r["scp_planar_equals_interleaved"]
+    assert r["psnr"] == oracle.psnr(p0, p1, BD)
+    assert r["mse"] == [oracle.ssd(p1[c], p0[c]) / (W * H) for c in range(3)]
+    assert r["wspsnr"] == oracle.wspsnr(p0, p1, BD, erp=bool(erp))
+    want = list(oracle.ivpsnr(p0, p1, BD, g, erp=bool(erp)))
+    assert [r["ivpsnr"], r["r2t"], r["t2r"]] == want  # interleaved form, QAP callback
+    assert r["ivpsnr_planar"] == want[0] and r["ivpsnr_auto"] == want[0]
+    assert r["gcs_reports"] == [g]  # calcPicIVPSNR(P0, P1, I0, I1) reports the GCD it computes
+    assert r["msk_reports"] == []
+    assert r["fake_psnr"] == oracle.psnr_from_ssd(1.0, W * H, BD)
+    assert np.max(np.abs(np.array(r["ssim"]) - oracle.ssim(p0, p1, BD))) <= 1e-6
+    assert np.max(np.abs(np.array(r["msssim"]) - oracle.msssim(p0, p1, BD))) <= 1e-6
+    v, a, b = oracle.ivssim(p0, p1, w0, w1, BD, erp=bool(erp))
+    assert max(abs(r["ivssim"] - v), abs(r["ivssim_r2t"] - a), abs(r["ivssim_t2r"] - b)) <= 1e-6
+    assert abs(r["ivmsssim"] - oracle.ivmsssim(p0, p1, w0, w1, BD)) <= 1e-6
+    assert min(r["ivmsssim_r2t"], r["ivmsssim_t2r"]) == r["ivmsssim"]
+    assert r["ssim_mask_refused"]
+    assert r["mode_names"] == ["RegularGaussianInt", "INVALID"]
+
+
+@pytest.mark.parametrize("erp,bdm", [(0, 10), (1, 8)])
+def test_dropin_masked_calls_match_oracle(tmp_path, oracle, erp, bdm):
+    """The app's mask mode (xAppQMIV.cpp:930-1029): CalcGlobalColorDiffM, GenShftCompPicsM on
+    interleaved pictures (pixels of weight 0 keep the destination's 7), calcPicMSEM / PSNRM /
+    WSPSNRM and calcPicIVPSNRM in both forms; MSK reported where the reference reports it."""
+    p0, p1 = _pair(oracle)
+    m = synth.make_mask(W, H, bdm, seed=31)
+    nnm = int(np.count_nonzero(m))
+    r, s0, s1 = _run(tmp_path, p0, p1, erp, m, bdm)
+    g = oracle.gcd_masked(p0, p1, m, BD, bdm)
+    assert r["gcd"] == g and r["nnm"] == nnm
+    w0, w1 = oracle.scp_masked(p0, p1, m, BD, g, fill=7)
+    assert all(np.array_equal(a, b) for a, b in zip(s0 + s1, w0 + w1))
+    mse, psnr, ws = oracle.psnr_family_masked(p0, p1, m, BD, bdm, erp=bool(erp))
+    assert r["mse"] == mse and r["psnr"] == psnr and r["wspsnr"] == ws
+    want = list(oracle.ivpsnr_masked(p0, p1, m, BD, bdm, g, erp=bool(erp)))
+    assert [r["ivpsnr"], r["r2t"], r["t2r"]] == want
+    assert r["ivpsnr_auto"] == want[0]
+    # PSNRM reports once; WSPSNRM twice without ERP (it calls PSNRM), once with; the self-counting
+    # IV-PSNR form once; the refused call never
+    wsr = [nnm] if erp else [nnm, nnm]
+    assert r["msk_reports"] == [nnm] + wsr + [-1000, nnm]
+    assert r["gcs_reports"] == [g]
+    assert r["wrong_nnm_refused"]

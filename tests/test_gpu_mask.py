@@ -1,7 +1,7 @@
 """Mask mode on the GPU through the C ABI (apps/QMIV/src/xAppQMIV.cpp:930-1029): GCD,
 MSE / PSNR / WS-PSNR and IV-PSNR with a weight mask, bit-exact against the reference's golden
-values and the oracle; masked and unmasked pairs in one batch; the combinations the app
-rejects."""
+values and the oracle; shift-compensated pictures with a mask (GenShftCompPicsM); masked and
+unmasked pairs in one batch; the combinations the app rejects."""
 import json
 import os
 
@@ -93,6 +93,53 @@ def test_masked_search_keyed_form_equals_reference_form(oracle, bd, R, c420):
     g = oracle.gcd_masked(p0, p1, m, bd, 10)
     assert out[0][3] == g
     assert out[0][:3] == oracle.ivpsnr_masked(p0, p1, m, bd, 10, g, R=R)
+
+
+def _sha1(a):
+    import hashlib
+    return hashlib.sha1(np.ascontiguousarray(a, np.uint16).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", _golden_mask_cases(), ids=lambda c: c["name"])
+def test_masked_scp_matches_reference_golden(case):
+    """ivqm_gen_scp with a mask bound (xShftCompPicProc::GenShftCompPicsM, xShftCompPic.h:48-50) and
+    the masked GCD: both pictures hash to the reference's, whose masked pixels keep 0."""
+    wl = _wl(case)
+    bd, bdm = wl.bit_depth, case["bit_depth_m"]
+    print("case", case["name"], flush=True)
+    p0n, p1n = synth.make_pair(wl, 0)
+    m = synth.make_mask(wl.width, wl.height, bdm, seed=wl.cfg)
+    with Q.IvqmContext(wl.width, wl.height, bd, search_range=case["R"]) as ctx:
+        ctx.upload_pair(p0n, p1n, wl.chroma_420)
+        ctx.upload_mask(m if bdm > 8 else m.astype(np.uint8), bdm)
+        ctx.gen_scp(case["gcd"])
+        s0, s1 = ctx.scp(0), ctx.scp(1)
+    assert [_sha1(p) for p in s0] == case["scp0_sha1"]
+    assert [_sha1(p) for p in s1] == case["scp1_sha1"]
+
+
+@pytest.mark.parametrize("bd,R,c420", [(10, 2, True), (8, 3, False), (10, 4, True)])
+def test_masked_scp_every_search_form_against_oracle(oracle, bd, R, c420):
+    """Masked SCP generation in each masked search form (packed dot key, reference form, narrow key)
+    at 328x200: identical pictures, equal to the oracle's GenShftCompPicsM with 0 at masked pixels;
+    a masked IV-PSNR call on the same context afterwards is unchanged."""
+    wl = synth.Workload(cfg=190 + R, width=328, height=200, bit_depth=bd, chroma_420=c420, global_shift=1)
+    p0n, p1n = synth.make_pair(wl, 0, as_u8=bd == 8)
+    m = synth.make_mask(wl.width, wl.height, 10, seed=21 + R)
+    p0 = oracle.to444([np.asarray(p, np.uint16) for p in p0n], c420)
+    p1 = oracle.to444([np.asarray(p, np.uint16) for p in p1n], c420)
+    g = oracle.gcd_masked(p0, p1, m, bd, 10)
+    w0, w1 = oracle.scp_masked(p0, p1, m, bd, g, R=R, fill=0)
+    iv = oracle.ivpsnr_masked(p0, p1, m, bd, 10, g, R=R)
+    for cap in (0, 1, 2):
+        print("cap", cap, flush=True)
+        with Q.IvqmContext(wl.width, wl.height, bd, search_range=R, search_mode_cap=cap) as ctx:
+            ctx.upload_pair(p0n, p1n, c420)
+            ctx.upload_mask(m, 10)
+            ctx.gen_scp(g)
+            s0, s1 = ctx.scp(0), ctx.scp(1)
+            assert all(np.array_equal(x, y) for x, y in zip(s0 + s1, w0 + w1)), cap
+            assert ctx.ivpsnr(g) == iv
 
 
 def test_masked_and_unmasked_pairs_in_one_batch():

@@ -520,6 +520,58 @@ def test_mask_cases_match_reference_golden(golden, oracle):
             assert [repr(float(x)) for x in ws] == c["wspsnr" + sfx], c["name"]
             iv = oracle.ivpsnr_masked(p0, p1, m, bd, bdm, g, R=c["R"], erp=erp)
             assert [repr(float(x)) for x in iv] == c["ivpsnr" + sfx], (c["name"], erp, iv)
+        s0, s1 = oracle.scp_masked(p0, p1, m, bd, g, R=c["R"], fill=0)
+        assert [sha1(p) for p in s0] == c["scp0_sha1"] and [sha1(p) for p in s1] == c["scp1_sha1"], c["name"]
+
+
+def test_masked_scp_with_mask_of_ones_is_the_unmasked_scp(oracle):
+    """GenShftCompPicsM with an all-ones mask considers every candidate and writes every pixel: the
+    pictures equal GenShftCompPics' (xCorrespPixelShiftSTD.cpp:268-291 vs the unmasked row)."""
+    for R, bd in ((1, 8), (2, 10), (4, 12)):
+        wl = synth.Workload(cfg=160 + R, width=61, height=37, bit_depth=bd, chroma_420=False)
+        p0n, p1n = synth.make_pair(wl, 0)
+        p0, p1 = oracle.to444(p0n, False), oracle.to444(p1n, False)
+        g = oracle.gcd(p0, p1, bd)
+        ones = np.ones((wl.height, wl.width), np.uint16)
+        a0, a1 = oracle.scp_masked(p0, p1, ones, bd, g, R=R, fill=7)
+        b0, b1 = oracle.scp(p0, p1, bd, g, R=R)
+        assert all(np.array_equal(x, y) for x, y in zip(a0 + a1, b0 + b1)), (R, bd)
+
+
+def test_masked_scp_brute_force(oracle):
+    """numpy brute force of the masked SCP on a tiny picture: masked pixels keep the fill value, the
+    others take clip(winner - shift, 0, Max) of the masked search's first strict minimum."""
+    rng = np.random.default_rng(12)
+    h, w, R, bd = 9, 11, 2, 10
+    p0 = [rng.integers(0, 1024, (h, w)).astype(np.uint16) for _ in range(3)]
+    p1 = [rng.integers(0, 1024, (h, w)).astype(np.uint16) for _ in range(3)]
+    m = rng.integers(0, 3, (h, w)).astype(np.uint16)
+    g, cw = [5, -3, 2, 0], (4, 1, 1, 0)
+
+    def one(tst, ref, gs):
+        out = [np.full((h, w), 99, np.uint16) for _ in range(3)]
+        for y in range(h):
+            for x in range(w):
+                if m[y, x] == 0:
+                    continue
+                t = [int(tst[c][y, x]) + gs[c] for c in range(3)]
+                best, bxy = None, None
+                for dy in range(-R, R + 1):
+                    for dx in range(-R, R + 1):
+                        ry, rx = min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)
+                        if m[ry, rx] == 0:
+                            continue
+                        e = sum(cw[c] * (t[c] - int(ref[c][ry, rx])) ** 2 for c in range(3))
+                        if best is None or e < best:
+                            best, bxy = e, (ry, rx)
+                for c in range(3):
+                    out[c][y, x] = min(max(int(ref[c][bxy]) - gs[c], 0), 1023)
+        return out
+
+    s0, s1 = oracle.scp_masked(p0, p1, m, bd, g, R=R, fill=99)
+    want1 = one(p0, p1, g)
+    want0 = one(p1, p0, [-v for v in g])
+    assert all(np.array_equal(x, y) for x, y in zip(s0 + s1, want0 + want1))
 
 
 def test_mask_of_ones_is_the_unmasked_search(oracle):
