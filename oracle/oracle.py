@@ -417,7 +417,8 @@ class Reference:
         L.ref_ssd.restype = C.c_uint64
         L.ref_bench_pairs.restype = C.c_double
         L.ref_bench_pairs.argtypes = [C.c_int32, C.POINTER(_u16p), C.POINTER(_u16p), C.c_int32, C.c_int32,
-                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p]
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, _f64p]
+        L.ref_hardware_concurrency.restype = C.c_uint
 
     def upsample(self, src):
         src = np.ascontiguousarray(src, dtype=np.uint16)
@@ -583,8 +584,13 @@ class Reference:
         self.L.ref_convert_colour(*[o.ctypes.data_as(_u16p) for o in out], w, h, bd, direction, space)
         return out
 
-    def bench_pairs(self, pairs, w, h, bd, chroma420, erp, metrics, threads):
-        """Times the app-order frame loop over in-memory pairs; returns (seconds, ivpsnr[], ivssim[])."""
+    def hardware_concurrency(self):
+        return int(self.L.ref_hardware_concurrency())
+
+    def bench_pairs(self, pairs, w, h, bd, chroma420, erp, metrics, threads, stages=None):
+        """Times the app-order frame loop over in-memory pairs; returns (seconds, ivpsnr[], ivssim[]).
+        `stages`, a list, receives the seconds per stage (unpack, PREPROC, Rearrange, GCD, SCP, IV-PSNR,
+        IV-SSIM) summed over the pairs."""
         n = len(pairs)
         keep = []
         p0 = (_u16p * (3 * n))()
@@ -598,6 +604,9 @@ class Reference:
                 p1[3 * f + c] = bb.ctypes.data_as(_u16p)
         iv = np.zeros(n, np.float64)
         ss = np.zeros(n, np.float64)
+        st = np.zeros(7, np.float64)
         t = self.L.ref_bench_pairs(n, p0, p1, w, h, bd, int(chroma420), int(erp), metrics, threads,
-                                   iv.ctypes.data_as(_f64p), ss.ctypes.data_as(_f64p))
+                                   iv.ctypes.data_as(_f64p), ss.ctypes.data_as(_f64p), st.ctypes.data_as(_f64p))
+        if stages is not None:
+            stages[:] = st.tolist()
         return t, iv, ss

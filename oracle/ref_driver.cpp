@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "xGlobClrDiff.h"
@@ -418,9 +419,15 @@ int ref_convert_colour(uint16_t* c0, uint16_t* c1, uint16_t* c2, int32_t w, int3
 // rearrange (planar -> interleaved), GCD, SCP (if IV-SSIM), IV-PSNR, IV-SSIM.
 // metrics bit 0 = IV-PSNR, bit 1 = IV-SSIM. Returns total seconds.
 // ---------------------------------------------------------------------------
+// stage_s[7] (may be NULL) receives the seconds spent per stage, summed over the pairs, in the
+// app's stage order (xAppQMIV.cpp:761-827): 0 unpack (readFrame's 4:2:0 upsample, no file I/O),
+// 1 PREPROC (exact-component check, margins), 2 Rearrange, 3 GCD, 4 SCP, 5 IV-PSNR, 6 IV-SSIM.
+// unsigned hardware_concurrency() is what the app's thread policy reads (xAppQMIV.cpp:517).
+unsigned ref_hardware_concurrency() { return std::thread::hardware_concurrency(); }
+
 double ref_bench_pairs(int32_t n_pairs, const uint16_t* const* p0_planes, const uint16_t* const* p1_planes,
                        int32_t w, int32_t h, int32_t bd, int32_t chroma420, int32_t erp, int32_t metrics,
-                       int32_t threads, double* ivpsnr_out, double* ivssim_out) {
+                       int32_t threads, double* ivpsnr_out, double* ivssim_out, double* stage_s) {
   const int32 R = 2, m = margin_for(R);
   const int32_t cw[4] = {4, 1, 1, 0};
   xThreadPool* pool = nullptr;
@@ -455,8 +462,16 @@ double ref_bench_pairs(int32_t n_pairs, const uint16_t* const* p0_planes, const 
   setup_ssim(ivs, &tpi, w, h, bd, R, false, 8, 4, cw, erp, 180);
 
   const int32 cs = chroma420 ? 1 : 0;
+  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  auto lap = std::chrono::steady_clock::now();
+  auto mark = [&](int k) {
+    const auto now = std::chrono::steady_clock::now();
+    st[k] += std::chrono::duration<double>(now - lap).count();
+    lap = now;
+  };
   auto t0 = std::chrono::steady_clock::now();
   for (int32_t f = 0; f < n_pairs; f++) {
+    lap = std::chrono::steady_clock::now();
     const uint16_t* const* src[2] = {p0_planes + 3 * f, p1_planes + 3 * f};
     for (int i = 0; i < 2; i++)
       tpi.storeTask([&, i](int32) {
@@ -470,29 +485,38 @@ double ref_bench_pairs(int32_t n_pairs, const uint16_t* const* p0_planes, const 
         }
       });
     tpi.executeStoredTasks();
+    mark(0);
     bool exact[3];
     for (int c = 0; c < 3; c++) exact[c] = P[0].equalCmp(&P[1], (eCmp)c);
     (void)exact;
     for (int i = 0; i < 2; i++) tpi.storeTask([&, i](int32) { P[i].extend(); });
     tpi.executeStoredTasks();
+    mark(1);
     for (int i = 0; i < 2; i++) tpi.storeTask([&, i](int32) { I[i].rearrangeFromPlanar(&P[i]); });
     tpi.executeStoredTasks();
+    mark(2);
     int32V4 g = gcd.CalcGlobalColorDiff(&P[0], &P[1]);
+    mark(3);
     if (metrics & 2) {
       scp.GenShftCompPics(&SCI[1], &SCI[0], &I[1], &I[0], g);
       for (int i = 0; i < 2; i++) tpi.storeTask([&, i](int32) { SCI[i].rearrangeToPlanar(&SCP[i]); });
       tpi.executeStoredTasks();
     }
+    mark(4);
     if (metrics & 1) {
       double v = ivp.calcPicIVPSNR(&I[0], &I[1], g);
       if (ivpsnr_out) ivpsnr_out[f] = v;
     }
+    mark(5);
     if (metrics & 2) {
       double v = ivs.calcPicIVSSIM(&P[0], &P[1], &SCP[0], &SCP[1]);
       if (ivssim_out) ivssim_out[f] = v;
     }
+    mark(6);
   }
   auto t1 = std::chrono::steady_clock::now();
+  if (stage_s)
+    for (int k = 0; k < 7; k++) stage_s[k] = st[k];
   ivs.destroy();
   if (pool) {
     tpi.uninit();
