@@ -237,7 +237,7 @@ def main():
                                                  else " (one pair per enqueue)"),
                    "frames": FRAMES, "width": W, "height": H, "bit_depth": BD, "metrics": args.metrics,
                    "parallelism": "1 GPU"},
-        "roofline": {"kernel": "k_search (both directions + " + ("SSIM moments)" if head == "batch" else "SCP)"),
+        "roofline": {"kernel": "k_search420 (both directions + " + ("SSIM moments)" if head == "batch" else "SCP)"),
                      "bound": "valu",
                      "achieved": round(search_tops, 2), "peak": PEAK_VALU_OPS_TOPS, "unit": "TOP/s",
                      "frac": round(search_tops / PEAK_VALU_OPS_TOPS, 4),
