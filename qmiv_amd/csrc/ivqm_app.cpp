@@ -14,9 +14,10 @@
 // stream once their copies are done, and a group's results are collected after
 // the next group is enqueued, so reading, copying and computing overlap.
 //
-// Not supported (the reference supports them): PNG/BMP input, 4:2:2, mask
-// mode, colour-space conversion, regular SSIM modes, SSIM windows other than
-// 8/4, config files. They are rejected with a message.
+// Also built: PNG / BMP picture lists (-ff), mask mode (-im -bdm -cfm), colour
+// spaces (-csi -csm), every SSIM mode, window, stride and border extension
+// (-ssm -ssw -sss -ssb). Not supported (the reference supports them): 4:2:2
+// test pictures and config files (-c); they are rejected with a message.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
