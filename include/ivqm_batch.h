@@ -29,10 +29,11 @@ typedef struct ivqm_batch ivqm_batch;
 int ivqm_batch_create(const ivqm_config* cfg, ivqm_batch** out);
 int ivqm_batch_destroy(ivqm_batch* b);
 int ivqm_batch_set_stream(ivqm_batch* b, void* hip_stream);
-/* Kernel shapes: search tile rows per wave (4 or 8 accepted; the search runs
- * 4-row waves, 64 x 16 tiles), SSIM window rows per
- * wave (4, 8 or 16), and whether each ring slot replays a captured hipGraph
- * while the batch shape stays the same (re-captured when it changes). */
+/* Kernel shapes: the search tile height of 4:2:0 pairs (16 or 32 rows of 64
+ * pixels; 4 and 8 are taken as 16 and 32, 0 keeps the current value; 4:4:4
+ * pairs always use 64 x 16 tiles), SSIM window rows per wave (4, 8 or 16), and
+ * whether each ring slot replays a captured hipGraph while the batch shape
+ * stays the same (re-captured when it changes). */
 int ivqm_batch_set_tuning(ivqm_batch* b, int32_t search_tile_rows, int32_t ssim_band_rows, int32_t use_graph);
 int ivqm_batch_get_tuning(ivqm_batch* b, int32_t* search_tile_rows, int32_t* ssim_band_rows);
 /* Times the candidate shapes on this batch and keeps the fastest. */

@@ -54,7 +54,8 @@ typedef struct ivqm_config {
                                            ssim_window; stride 1..window */
   int32_t fake_vals_for_exact;    /* PSNR of an exact component = PSNR(SSD=1) (xPSNR.cpp:147, xAppQMIV.cpp:974-981) */
   int32_t gcd_sd_wrap32;          /* reproduce the x86-64-v4 <=14-bit int32 SD accumulation (xDistortionAVX512.cpp:14-73) */
-  int32_t search_tile_rows;       /* accepted: 0, 4 or 8; the search runs 4-row waves (64 x 16 tiles) */
+  int32_t search_tile_rows;       /* search tile height of 4:2:0 pairs: 16 or 32 rows (4 / 8 taken as
+                                     16 / 32; 0 = 32); 4:4:4 pairs use 64 x 16 tiles */
   int32_t ssim_wide_sums;         /* 64-bit squared sums in SSIM at any bit depth (always on above 14-bit) */
   int32_t search_mode_cap;        /* search arithmetic: 0 = fastest exact variant the ranges allow,
                                      1 = reference-form int32 arithmetic only, 2 = no packed-int16 variant
