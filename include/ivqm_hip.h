@@ -69,7 +69,10 @@ typedef struct ivqm_config {
 } ivqm_config;
 
 /* One picture: three planes of 8-bit (bytes_per_sample = 1) or 16-bit samples.
- * chroma_420 = 1: chroma planes are (width/2) x (height/2); 0: full size.
+ * chroma_420: the chroma format, as the reference reads it (libs/pmbb-core/src/xSeq.cpp:231-301):
+ * 1 = 4:2:0, chroma planes (width/2) x (height/2), even sizes; 2 = 4:2:2, chroma planes
+ * (width/2) x height, even width; 0 = 4:4:4, full size. Subsampled chroma is read as its
+ * nearest replication at 4:4:4, which is what the reference's upsampling stores.
  * Strides are in samples. */
 typedef struct ivqm_picture {
   const void* plane[3];

@@ -45,7 +45,7 @@ typedef struct ivqm_server_config {
  * (from host or device memory) into the server's own allocation. */
 typedef struct ivqm_sequence_desc {
   int32_t width, height, bit_depth;
-  int32_t chroma_420;          /* 1: 4:2:0 planes, 0: 4:4:4 */
+  int32_t chroma_420;          /* chroma format: 1 4:2:0, 2 4:2:2, 0 4:4:4 (as ivqm_picture) */
   int32_t bytes_per_sample;    /* 1 (8-bit only) or 2 */
   int32_t num_frames;
   const void* const* planes;   /* num_frames * 3 plane pointers, frame-major */

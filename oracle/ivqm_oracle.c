@@ -62,6 +62,13 @@ OR_EXPORT void or_upsample_hv(uint16_t* dst, const uint16_t* src, int32_t dst_w,
   for (int32_t y = 0; y < dst_h; y++)
     for (int32_t x = 0; x < dst_w; x++) dst[(size_t)y * dst_w + x] = src[(size_t)(y >> 1) * sw + (x >> 1)];
 }
+/* 4:2:2 -> 4:4:4: xPixelOpsSTD::UpsampleH (libs/pmbb-core/src/xPixelOpsSTD.cpp:107-120), each
+ * chroma sample replicated into two horizontally adjacent positions (xSeq.cpp:266-280). */
+OR_EXPORT void or_upsample_h(uint16_t* dst, const uint16_t* src, int32_t dst_w, int32_t dst_h) {
+  const int32_t sw = dst_w >> 1;
+  for (int32_t y = 0; y < dst_h; y++)
+    for (int32_t x = 0; x < dst_w; x++) dst[(size_t)y * dst_w + x] = src[(size_t)y * sw + (x >> 1)];
+}
 OR_EXPORT void or_downsample_hv(uint16_t* dst, const uint16_t* src, int32_t src_w, int32_t src_h) {
   const int32_t dw = src_w >> 1, dh = src_h >> 1;
   for (int32_t y = 0; y < dh; y++)

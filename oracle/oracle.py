@@ -110,10 +110,20 @@ class Oracle:
         self.L.or_downsample_hv(dst.ctypes.data_as(_u16p), src.ctypes.data_as(_u16p), w, h)
         return dst
 
-    def to444(self, planes, chroma420: bool):
-        if not chroma420:
+    def upsample_h(self, src):
+        src = np.ascontiguousarray(src, dtype=np.uint16)
+        h, w = src.shape
+        dst = np.empty((h, 2 * w), np.uint16)
+        self.L.or_upsample_h(dst.ctypes.data_as(_u16p), src.ctypes.data_as(_u16p), 2 * w, h)
+        return dst
+
+    def to444(self, planes, chroma420):
+        """chroma420: the chroma format code of the planes, 0 (or False) 4:4:4, 1 (or True) 4:2:0, 2 4:2:2."""
+        cf = int(chroma420)
+        if cf == 0:
             return [np.ascontiguousarray(p, np.uint16) for p in planes]
-        return [np.ascontiguousarray(planes[0], np.uint16), self.upsample(planes[1]), self.upsample(planes[2])]
+        up = self.upsample if cf == 1 else self.upsample_h
+        return [np.ascontiguousarray(planes[0], np.uint16), up(planes[1]), up(planes[2])]
 
     # -- scalar helpers ------------------------------------------------------
     def kbns(self, x):
@@ -425,6 +435,13 @@ class Reference:
         h, w = src.shape
         dst = np.empty((2 * h, 2 * w), np.uint16)
         self.L.ref_upsample_hv(dst.ctypes.data_as(_u16p), src.ctypes.data_as(_u16p), 2 * w, 2 * h)
+        return dst
+
+    def upsample_h(self, src):
+        src = np.ascontiguousarray(src, dtype=np.uint16)
+        h, w = src.shape
+        dst = np.empty((h, 2 * w), np.uint16)
+        self.L.ref_upsample_h(dst.ctypes.data_as(_u16p), src.ctypes.data_as(_u16p), 2 * w, h)
         return dst
 
     def downsample(self, src):

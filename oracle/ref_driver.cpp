@@ -78,6 +78,10 @@ int ref_upsample_hv(uint16_t* dst, const uint16_t* src, int32_t dst_w, int32_t d
   xPixelOps::UpsampleHV(dst, src, dst_w, dst_w >> 1, dst_w, dst_h);
   return 0;
 }
+int ref_upsample_h(uint16_t* dst, const uint16_t* src, int32_t dst_w, int32_t dst_h) {
+  xPixelOps::UpsampleH(dst, src, dst_w, dst_w >> 1, dst_w, dst_h);
+  return 0;
+}
 int ref_downsample_hv(uint16_t* dst, const uint16_t* src, int32_t src_w, int32_t src_h) {
   xPixelOps::DownsampleHV(dst, src, src_w >> 1, src_w, src_w >> 1, src_h >> 1);
   return 0;

@@ -36,7 +36,9 @@ inline int validate_picture(int W, int H, int bd, const ivqm_picture* p) {
   if (!p) return set_error(IVQM_EINVAL, "picture is NULL");
   if (p->bytes_per_sample != 1 && p->bytes_per_sample != 2) return set_error(IVQM_EINVAL, "bytes_per_sample must be 1 or 2");
   if (p->bytes_per_sample == 1 && bd != 8) return set_error(IVQM_EINVAL, "8-bit samples require bit_depth 8");
-  if (p->chroma_420 && ((W & 1) || (H & 1))) return set_error(IVQM_EINVAL, "4:2:0 requires even width and height");
+  if (p->chroma_420 < 0 || p->chroma_420 > 2) return set_error(IVQM_EINVAL, "chroma_420 must be 0 (4:4:4), 1 (4:2:0) or 2 (4:2:2)");
+  if (p->chroma_420 == 1 && ((W & 1) || (H & 1))) return set_error(IVQM_EINVAL, "4:2:0 requires even width and height");
+  if (p->chroma_420 == 2 && (W & 1)) return set_error(IVQM_EINVAL, "4:2:2 requires an even width");
   const int cs = p->chroma_420 ? 1 : 0;
   for (int k = 0; k < 3; k++) {
     if (!p->plane[k]) return set_error(IVQM_EINVAL, "plane pointer is NULL");
@@ -63,6 +65,7 @@ inline PicDev to_dev(const ivqm_picture* p) {
     d.stride[k] = p->stride[k];
   }
   d.cs = p->chroma_420 ? 1 : 0;
+  d.csy = p->chroma_420 == 1 ? 1 : 0;
   d.bps = p->bytes_per_sample;
   return d;
 }

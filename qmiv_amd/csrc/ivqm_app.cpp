@@ -16,8 +16,8 @@
 //
 // Also built: PNG / BMP picture lists (-ff), mask mode (-im -bdm -cfm), colour
 // spaces (-csi -csm), every SSIM mode, window, stride and border extension
-// (-ssm -ssw -sss -ssb). Not supported (the reference supports them): 4:2:2
-// test pictures and config files (-c); they are rejected with a message.
+// (-ssm -ssw -sss -ssb), 4:2:0, 4:2:2 and 4:4:4 input (-cf, -pf). Not supported
+// (the reference supports it): config files (-c); rejected with a message.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -142,7 +142,7 @@ bool parse_vec4(const std::string& s, T out[4], F conv) {
 void usage() {
   printf(
       "ivqm_qmiv: QMIV metrics of two raw YUV sequences on the GPU\n"
-      " -i0 InputFile0  -i1 InputFile1  -ps WxH | -pw W -ph H  -bd BitDepth (8..16)  -cf 420|444\n"
+      " -i0 InputFile0  -i1 InputFile1  -ps WxH | -pw W -ph H  -bd BitDepth (8..16)  -cf 420|422|444\n"
       " -pf pix_fmt (yuv420p, yuv420p10le, yuv444p16le, ...)  -s0 / -s1 StartFrame  -nf NumberOfFrames\n"
       " -r ResultFile  -ml \"PSNR, WSPSNR, IVPSNR, SSIM, MSSSIM, IVSSIM, IVMSSSIM, MSE\" | All\n"
       " -erp  -lor LonRangeDeg  -lar LatRangeDeg  -sr SearchRange (1..4)  -cws / -cwa \"4:1:1:0\"\n"
@@ -258,8 +258,10 @@ bool parse_options(int argc, char** argv, Options& o) {
   if (o.in1.empty()) errs += "!  InputFile1 is empty\n";
   if (o.width <= 0 || o.height <= 0) errs += "!  Invalid or missing PictureSize\n";
   if (o.bit_depth < 8 || o.bit_depth > 16) errs += "!  Invalid or unsuported BitDepth value\n";
-  if (o.chroma != 420 && o.chroma != 444) errs += "!  Invalid or unsuported ChromaFormat value (420 or 444)\n";
+  if (o.chroma != 420 && o.chroma != 422 && o.chroma != 444)
+    errs += "!  Invalid or unsuported ChromaFormat value (420, 422 or 444)\n";
   if (o.chroma == 420 && ((o.width | o.height) & 1)) errs += "!  4:2:0 needs even picture sizes\n";
+  if (o.chroma == 422 && (o.width & 1)) errs += "!  4:2:2 needs an even picture width\n";
   if (o.search_range < 1 || o.search_range > 4) errs += "!  SearchRange must be 1..4\n";
   // xSSIM::xStrToMode (xSSIM.cpp:17-26, case-insensitive), xStr2MrgExt (xMiscUtilsCORE.cpp:152-164)
   const char* kModes[] = {"REGULARGAUSSIANFLT", "REGULARGAUSSIANINT", "REGULARAVERAGED", "BLOCKGAUSSIANINT",
@@ -402,8 +404,10 @@ int main(int argc, char** argv) {
     usage();
     return 1;
   }
-  const int W = o.width, H = o.height, cs = o.chroma == 420 ? 1 : 0, bps = o.bit_depth > 8 ? 2 : 1;
-  const size_t luma = (size_t)W * H, chroma = (size_t)(W >> cs) * (H >> cs);
+  // chroma planes: 4:2:0 (W/2) x (H/2), 4:2:2 (W/2) x H (xSeq.cpp:231-301); cf is the ABI's format code
+  const int W = o.width, H = o.height, bps = o.bit_depth > 8 ? 2 : 1;
+  const int csx = o.chroma == 444 ? 0 : 1, csy = o.chroma == 420 ? 1 : 0, cf = o.chroma == 420 ? 1 : (o.chroma == 422 ? 2 : 0);
+  const size_t luma = (size_t)W * H, chroma = (size_t)(W >> csx) * (H >> csy);
   const size_t frame_bytes = (luma + 2 * chroma) * bps;
   // mask frames: luma plane of InputFileM (BitDepthM, ChromaFormatM); only the luma plane is used
   const bool use_mask = !o.inm.empty();
@@ -413,7 +417,7 @@ int main(int argc, char** argv) {
   const size_t slot_bytes = 2 * frame_bytes + mask_bytes;
   // device slots keep rows at a pitch of a multiple of 4 samples (the batch ABI's alignment for
   // full-resolution planes); the upload converts the pitch with 2-D copies
-  const int Wc = W >> cs, Hc = H >> cs, Wp = (W + 3) & ~3, Wcp = (Wc + 3) & ~3;
+  const int Wc = W >> csx, Hc = H >> csy, Wp = (W + 3) & ~3, Wcp = (Wc + 3) & ~3;
   const size_t dframe = ((size_t)Wp * H + 2 * (size_t)Wcp * Hc) * bps;
   const size_t dslot = 2 * dframe + mask_bytes;
   set_suffixes(use_mask, is_rgb(o.csm));
@@ -612,7 +616,7 @@ int main(int argc, char** argv) {
     p.plane[2] = base + ((size_t)Wp * H + (size_t)Wcp * Hc) * bps;
     p.stride[0] = Wp;
     p.stride[1] = p.stride[2] = Wcp;
-    p.chroma_420 = cs;
+    p.chroma_420 = cf;
     p.bytes_per_sample = bps;
     return p;
   };

@@ -15,9 +15,12 @@ namespace ivqm {
 struct PicDev {
   const void* p[3];
   int32_t stride[3];
-  int32_t cs;
+  int32_t cs;   // chroma column shift: 1 for 4:2:0 and 4:2:2, 0 for 4:4:4
   int32_t bps;  // bytes per sample: 1 or 2
+  int32_t csy;  // chroma row shift: 1 for 4:2:0, 0 for 4:2:2 and 4:4:4
 };
+// Chroma format of a picture: 0 4:4:4, 1 4:2:0, 2 4:2:2 (the ABI's ivqm_picture::chroma_420 values).
+__host__ __device__ inline int chroma_format(const PicDev& p) { return p.cs ? (p.csy ? 1 : 2) : 0; }
 
 // One frame pair of a batch and where its outputs live.
 struct PairDev {

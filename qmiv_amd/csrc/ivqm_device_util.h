@@ -20,14 +20,14 @@ __device__ __forceinline__ int plane_stride(const PicDev& P, int c) {
   return c == 0 ? s0 : (c == 1 ? s1 : s2);
 }
 
-// Sample (x, y) of plane c on the luma grid. Plane offsets are 32-bit with
+// Sample (x, y) of plane c on the luma grid (chroma at (x >> cs, y >> csy)). Plane offsets are 32-bit with
 // 24-bit operands (rows and strides stay below 2^24, planes below 2^32 samples).
 // Row y (luma grid) of plane c as a global pointer; with a wave-uniform y the
 // pointer lives in scalar registers and a load adds only the lane's column.
 template <typename T>
 __device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(const PicDev& P, int c, int y) {
   typedef __attribute__((address_space(1))) const T GT;
-  const int s = c ? P.cs : 0;
+  const int s = c ? P.csy : 0;
   return (const GT*)plane_ptr(P, c) + __umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c));
 }
 
@@ -37,7 +37,7 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* plane_row(
 // vector offset serves every row and no address arithmetic is per load.
 template <typename T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const PicDev& P, int c, int H) {
-  const int s = c ? P.cs : 0;
+  const int s = c ? P.csy : 0;
   const uint32_t bytes = (uint32_t)((H + s) >> s) * (uint32_t)plane_stride(P, c) * (uint32_t)sizeof(T);
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(plane_ptr(P, c)), (short)0, (int)bytes, 0x00020000);
 }
@@ -52,9 +52,9 @@ __device__ __forceinline__ int ld_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, u
 template <typename T>
 __device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
   typedef __attribute__((address_space(1))) const T GT;
-  const int s = c ? P.cs : 0;
+  const int sx = c ? P.cs : 0, sy = c ? P.csy : 0;
   const GT* p = (const GT*)plane_ptr(P, c);
-  return (int)p[__umul24((uint32_t)(y >> s), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> s)];
+  return (int)p[__umul24((uint32_t)(y >> sy), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> sx)];
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // v_med3_i32

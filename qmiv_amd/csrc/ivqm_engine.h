@@ -45,7 +45,7 @@ struct PairSpec {
 // Per-pair bookkeeping the epilogue needs.
 struct PairPlan {
   uint32_t m;  // metrics of this pair
-  int w, h, bd, cs;
+  int w, h, bd, cs, csy;  // chroma column / row shifts
   int64_t rows_off, rowsd_off, scp_off;
   int ns;
   int ssim = -1, iv_t2r = -1, iv_r2t = -1;
@@ -100,9 +100,10 @@ inline int resolve_picture(const PicRef& r, uint16_t* scp, int64_t cap_scp, uint
     out.stride[k] = r.pitch;
   }
   out.cs = 0;
+  out.csy = 0;
   out.bps = 2;
   if (r.chroma_from_user) {  // level 1 of a 4:2:0 uint16 picture: its chroma planes at (w >> 1, h >> 1)
-    if (r.arena != 2 || r.user.cs != 1 || r.user.bps != 2) {
+    if (r.arena != 2 || chroma_format(r.user) != 1 || r.user.bps != 2) {
       *why = "internal: pyramid chroma source";
       return IVQM_EINVAL;
     }

@@ -349,7 +349,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     std::memset(&d, 0, sizeof(d));
     const uint32_t mp = s.metrics ? (s.metrics & 0xFFu) : m;
     pp.m = mp;
-    pp.w = s.w; pp.h = s.h; pp.bd = s.bd; pp.cs = s.p0.cs;
+    pp.w = s.w; pp.h = s.h; pp.bd = s.bd; pp.cs = s.p0.cs; pp.csy = s.p0.csy;
     d.p0 = s.p0;
     d.p1 = s.p1;
     d.w = s.w;
@@ -384,10 +384,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       P.any_mask = true;
     }
     d.stat_blk0 = P.stat_blocks;
-    P.stat_blocks += s.msk ? 3 * s.h : s.h + 2 * (s.h >> s.p0.cs);  // masked: every component at 4:4:4
+    P.stat_blocks += s.msk ? 3 * s.h : s.h + 2 * (s.h >> s.p0.csy);  // masked: every component at 4:4:4
     d.tile0 = P.tiles;  // pairs without a search own no tiles (the table lookup skips them)
     d.tile0b = P.tiles420;
-    if (!s.msk && s.p0.cs == 1 && (d.mode == 1 || d.mode == 2) && cfg_.search_mode_cap == 0 &&
+    if (!s.msk && chroma_format(s.p0) == 1 && (d.mode == 1 || d.mode == 2) && cfg_.search_mode_cap == 0 &&
         search420_ok(cfg_, d.maxv, gmax))
       d.mode = kSearchDot420;
     d.mtile0 = P.mtiles;
@@ -411,7 +411,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.scp_pitch = scp_pitch(s.w);
     // the kernels address a plane with 32-bit byte offsets (SCP, buffer resources)
     for (int c = 0; c < 3; c++) {
-      const int64_t rows = c ? (s.h + s.p0.cs) >> s.p0.cs : s.h;
+      const int64_t rows = c ? (s.h + s.p0.csy) >> s.p0.csy : s.h;
       if ((int64_t)s.p0.stride[c] * rows * s.p0.bps >= (1ll << 32) ||
           (int64_t)s.p1.stride[c] * rows * s.p1.bps >= (1ll << 32))
         return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
@@ -443,7 +443,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.best_off = (best_on_ && n == 1) ? 0 : -1;
     pp.ns = ms_num_scales(s.w, s.h);
     if (cfg_.erp) pp.erp_h = s.h;
-    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x:%d|", s.w, s.h, s.bd, s.p0.cs, s.p0.bps, mp,
+    std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x:%d|", s.w, s.h, s.bd, chroma_format(s.p0), s.p0.bps, mp,
                   s.msk ? s.msk_bps : 0);
     P.signature += sigbuf;
   }
@@ -485,7 +485,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       // alignment the full-resolution planes have (8-byte base, stride a multiple of 4 samples)
       const bool aligned = ((reinterpret_cast<uintptr_t>(u.p[1]) | reinterpret_cast<uintptr_t>(u.p[2])) & 7) == 0 &&
                            u.stride[1] % 4 == 0 && u.stride[2] % 4 == 0;
-      if (l == 1 && which < 2 && u.cs == 1 && u.bps == 2 && aligned) {
+      if (l == 1 && which < 2 && chroma_format(u) == 1 && u.bps == 2 && aligned) {
         r.chroma_from_user = true;
         r.user = u;
       }
@@ -1020,7 +1020,7 @@ int Engine::collect(int ring, ivqm_frame_result* out) {
       ew = hw.first.data();
       corr = hw.second;
     }
-    host::psnr_family(m, W, H, bd, pp.cs, cfg_.fake_vals_for_exact != 0, cwa, S.r_row_ssd + pp.rowsd_off,
+    host::psnr_family(m, W, H, bd, pp.cs, pp.csy, cfg_.fake_vals_for_exact != 0, cwa, S.r_row_ssd + pp.rowsd_off,
                       S.r_rows + pp.rows_off, ew, corr, r);
     const double* so = S.r_ssim;
     if ((m & IVQM_M_SSIM) && pp.ssim >= 0) {

@@ -98,7 +98,8 @@ inline double ivpsnr_quality(const uint32_t* rows, int w, int h, int bit_depth, 
 // The PSNR family of one unmasked pair from the device's exact integers; r.ssd and r.exact are set.
 // row_ssd: [3][plane rows] SSD rows (read for WS-PSNR with ERP weights); rows: the search's [2][3][H]
 // row sums (IV-PSNR); erp_w / erp_corr: row weights and correction of height H, or null without ERP.
-inline void psnr_family(uint32_t m, int W, int H, int bd, int cs, bool fake_exact, const int32_t cwa[4],
+// cs / csy: chroma column / row shifts of the pair's pictures (4:2:0 1 / 1, 4:2:2 1 / 0, 4:4:4 0 / 0).
+inline void psnr_family(uint32_t m, int W, int H, int bd, int cs, int csy, bool fake_exact, const int32_t cwa[4],
                         const uint64_t* row_ssd, const uint32_t* rows, const double* erp_w, double erp_corr,
                         ivqm_frame_result& r) {
   const int64_t area = (int64_t)W * H;
@@ -119,11 +120,12 @@ inline void psnr_family(uint32_t m, int W, int H, int bd, int cs, bool fake_exac
     if (!erp_w) {
       for (int k = 0; k < 3; k++) r.wspsnr[k] = psnr_from_ssd((double)r.ssd[k], area, bd);
     } else {
-      // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a 4:2:0 chroma plane row counts twice per luma row
+      // libs/pmbb-ivqm/src/xWSPSNR.cpp:81-107; a subsampled chroma row counts 2^cs times per 4:4:4 row,
+      // and serves 2^csy of them
       for (int k = 0; k < 3; k++) {
         Kbns kb;
         for (int y = 0; y < H; y++) {
-          const uint64_t row = (k && cs) ? 2 * row_ssd[(size_t)k * H + (y >> 1)] : row_ssd[(size_t)k * H + y];
+          const uint64_t row = k ? ((uint64_t)1 << cs) * row_ssd[(size_t)k * H + (y >> csy)] : row_ssd[y];
           kb.acc((double)row * erp_w[y]);
         }
         const double err = kb.result() * erp_corr;

@@ -24,12 +24,13 @@
 //   k_ssim_finalize  FP64 frame totals per job and component (ERP-weighted
 //                    for IV-SSIM).
 //   k_downsample     2x2 rounded-mean pyramid level (MS-SSIM).
-//   k_upsample420    4:2:0 -> 4:4:4 nearest replication (standalone).
+//   k_upsample420    4:2:0 / 4:2:2 -> 4:4:4 nearest replication (standalone).
 //   k_colour         YCbCr <-> RGB fixed-point conversion to 4:4:4 (standalone).
 //
-// 4:2:0 input is never materialised at 4:4:4 by the metric kernels: a chroma
-// sample at luma-grid (x, y) is read at (x >> 1, y >> 1), exactly the value
-// 2x2 replication (libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49) stores there.
+// 4:2:0 and 4:2:2 input is never materialised at 4:4:4 by the metric kernels:
+// a chroma sample at luma-grid (x, y) is read at (x >> 1, y >> 1) (4:2:0) or
+// (x >> 1, y) (4:2:2), exactly the value 2x2 or horizontal replication
+// (libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49, :107-120) stores there.
 // The reference's Nearest margin (xMarginOps.h:40-73) is reproduced by
 // clamping coordinates.
 #include "ivqm_device.h"
@@ -119,7 +120,7 @@ __device__ __forceinline__ void stats_vec64(const uint4& va, const uint4& vr, in
 template <typename T>
 __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, int64_t* __restrict__ row_sd,
                                           uint64_t* __restrict__ row_ssd) {
-  const int H = P.h, W = P.w, Hc = H >> P.p0.cs, Wc = W >> P.p0.cs;
+  const int H = P.h, W = P.w, Hc = H >> P.p0.csy, Wc = W >> P.p0.cs;
   int c, y, wp;
   if (lb < H) { c = 0; y = lb; wp = W; }
   else if (lb < H + Hc) { c = 1; y = lb - H; wp = Wc; }
@@ -186,7 +187,7 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
   }
 }
 
-// Mask mode: one 4:4:4 row of component c (chroma of 4:2:0 read at (x >> 1, y >> 1)), weighted by
+// Mask mode: one 4:4:4 row of component c (subsampled chroma read at (x >> cs, y >> csy)), weighted by
 // the mask at (x, y). xDistortionSTD::CalcWeightedSD / CalcWeightedSSD
 // (libs/pmbb-core/src/xDistortionSTD.cpp:45-68): sum of d * M with the product in int32 (exact for
 // masks up to 14 bits), and sum of (uint64)(int32 square of d) * M. aux gets the samples that
@@ -195,9 +196,9 @@ template <typename T, typename TM>
 __device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64_t* __restrict__ row_sd,
                                  uint64_t* __restrict__ row_ssd, int64_t* __restrict__ aux) {
   const int W = P.w, H = P.h;
-  const int cs = c ? P.p0.cs : 0;
-  const T* a = (const T*)plane_ptr(P.p0, c) + (size_t)(y >> cs) * plane_stride(P.p0, c);
-  const T* r = (const T*)plane_ptr(P.p1, c) + (size_t)(y >> cs) * plane_stride(P.p1, c);
+  const int cs = c ? P.p0.cs : 0, csy = c ? P.p0.csy : 0;
+  const T* a = (const T*)plane_ptr(P.p0, c) + (size_t)(y >> csy) * plane_stride(P.p0, c);
+  const T* r = (const T*)plane_ptr(P.p1, c) + (size_t)(y >> csy) * plane_stride(P.p1, c);
   const TM* m = (const TM*)P.msk + (size_t)y * P.msk_stride;
   int64_t sd = 0, nd = 0, nz = 0;
   uint64_t ssd = 0;
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs
 
 // ---------------------------------------------------------------------------
 // k_gcd_finalize: one block per pair. libs/pmbb-ivqm/src/xGlobClrDiff.cpp:23-56
-// with the app's argument order (sum of P1 - P0); chroma sums of 4:2:0 x4.
+// with the app's argument order (sum of P1 - P0); chroma sums of 4:2:0 x4, of 4:2:2 x2.
 // ---------------------------------------------------------------------------
 // Mask mode: xGlobClrDiff::CalcGlobalColorDiffM (xGlobClrDiff.cpp:60-91): mean = SD / (NNM * MaxM);
 // the row sums are already 4:4:4 totals; mcount receives NNM and the differing-sample counts.
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
   __shared__ int64_t s_sd[4], s_nd[4], s_nz[4];
   __shared__ uint64_t s_ssd[4];
   const int tid = threadIdx.x;
-  const int W = P.w, H = P.h, cs = P.p0.cs;
+  const int W = P.w, H = P.h, cs = P.p0.cs, csy = P.p0.csy;
   const bool masked = P.msk != nullptr;
   int64_t nnm = 0;
   if (masked) {  // non-zero mask samples
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
     if (tid == 0) mcount[4 * P.res_idx] = nnm;
   }
   for (int c = 0; c < 3; c++) {
-    const int rows = (c && !masked) ? (H >> cs) : H;
+    const int rows = (c && !masked) ? (H >> csy) : H;
     int64_t sd = 0, nd = 0;
     uint64_t ssd = 0;
     for (int y = tid; y < rows; y += 256) {
@@ -320,7 +321,10 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
     if (tid == 0 && !masked) {
       int64_t t = s_sd[0] + s_sd[1] + s_sd[2] + s_sd[3];
       uint64_t q = s_ssd[0] + s_ssd[1] + s_ssd[2] + s_ssd[3];
-      if (c && cs) { t *= 4; q *= 4; }
+      if (c) {  // a subsampled chroma sample stands for 2 (4:2:2) or 4 (4:2:0) samples at 4:4:4
+        t *= (int64_t)1 << (cs + csy);
+        q *= (uint64_t)1 << (cs + csy);
+      }
       sd_out[3 * P.res_idx + c] = t;
       ssd_out[3 * P.res_idx + c] = q;
       if (P.gcd_given) {
@@ -540,9 +544,9 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   constexpr int NPT = (LH + PAR + 1) / 2;  // row pairs of the region
   constexpr int NPW = (NPT + 3) / 4;       // row pairs per wave
   const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);
-  // byte offsets of the lane's column in luma / 4:2:0 chroma rows, and the
-  // planes' buffer resources; row byte offsets are scalar
-  const uint32_t ol = cx * (uint32_t)sizeof(T), oc = (cx >> 1) * (uint32_t)sizeof(T);
+  // byte offsets of the lane's column in luma / chroma rows (4:2:0 and 4:2:2
+  // chroma at cx >> 1), and the planes' buffer resources; row byte offsets are scalar
+  const uint32_t ol = cx * (uint32_t)sizeof(T), oc = (cx >> Ref.cs) * (uint32_t)sizeof(T);
   const __amdgpu_buffer_rsrc_t rs0 = plane_rsrc<T>(Ref, 0, H), rs1 = plane_rsrc<T>(Ref, 1, H),
                                rs2 = plane_rsrc<T>(Ref, 2, H);
   const uint32_t st0 = (uint32_t)plane_stride(Ref, 0) * (uint32_t)sizeof(T);
@@ -559,19 +563,19 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
       const uint32_t ua = (uint32_t)ya, ub = (uint32_t)yb2;
       if (ly0 >= 0) ra[k][0][0] = ld_buf<T>(rs0, ol, ua * st0);
       if (ly1 < LH) ra[k][1][0] = ld_buf<T>(rs0, ol, ub * st0);
-      if (Ref.cs) {
+      if (Ref.csy) {  // 4:2:0: one chroma row serves the row pair
         const uint32_t yc = (ly0 >= 0 ? ua : ub) >> 1;
         const int cb = ld_buf<T>(rs1, oc, yc * st1), cr = ld_buf<T>(rs2, oc, yc * st2);
         ra[k][0][1] = ra[k][1][1] = cb;
         ra[k][0][2] = ra[k][1][2] = cr;
-      } else {
+      } else {  // 4:4:4 and 4:2:2: a chroma row per row (column oc)
         if (ly0 >= 0) {
-          ra[k][0][1] = ld_buf<T>(rs1, ol, ua * st1);
-          ra[k][0][2] = ld_buf<T>(rs2, ol, ua * st2);
+          ra[k][0][1] = ld_buf<T>(rs1, oc, ua * st1);
+          ra[k][0][2] = ld_buf<T>(rs2, oc, ua * st2);
         }
         if (ly1 < LH) {
-          ra[k][1][1] = ld_buf<T>(rs1, ol, ub * st1);
-          ra[k][1][2] = ld_buf<T>(rs2, ol, ub * st2);
+          ra[k][1][1] = ld_buf<T>(rs1, oc, ub * st1);
+          ra[k][1][2] = ld_buf<T>(rs2, oc, ub * st2);
         }
       }
     }
@@ -1066,8 +1070,8 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
       for (int r = 0; r < 4; r++) {
         uint32_t a[4], b[4];
         const int y = by * 4 + r;  // wave-uniform row
-        load4<TA, CSA>(pa + (size_t)(y >> CSA) * sa, bx * 4, a);
-        load4<TB, CSB>(pb + (size_t)(y >> CSB) * sb, bx * 4, b);
+        load4<TA, (CSA != 0)>(pa + (size_t)(y >> (CSA == 1 ? 1 : 0)) * sa, bx * 4, a);
+        load4<TB, (CSB != 0)>(pb + (size_t)(y >> (CSB == 1 ? 1 : 0)) * sb, bx * 4, b);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           A += a[j];
@@ -1126,24 +1130,33 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
   }
 }
 
+// CSA / CSB: chroma format of picture a / b for component c (0 4:4:4, 1 4:2:0, 2 4:2:2; luma 0). A job
+// pairs two caller pictures of one format, or a caller picture with a 4:4:4 SCP picture (either order).
 template <int SM, int NB>
 __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
                               double* partial) {
-  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
+  const int csa = c ? chroma_format(J.a) : 0, csb = c ? chroma_format(J.b) : 0;
+#define IVQM_SSIM(TA, A, TB, B) ssim_wave<TA, A, TB, B, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial)
   if (J.a.bps == 2 && J.b.bps == 2) {
-    if (csa == 0 && csb == 0) ssim_wave<uint16_t, 0, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 1) ssim_wave<uint16_t, 1, uint16_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else if (csa == 1 && csb == 0) ssim_wave<uint16_t, 1, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-    else ssim_wave<uint16_t, 0, uint16_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+    if (csa == 0 && csb == 0) IVQM_SSIM(uint16_t, 0, uint16_t, 0);
+    else if (csa == 1 && csb == 1) IVQM_SSIM(uint16_t, 1, uint16_t, 1);
+    else if (csa == 1 && csb == 0) IVQM_SSIM(uint16_t, 1, uint16_t, 0);
+    else if (csa == 0 && csb == 1) IVQM_SSIM(uint16_t, 0, uint16_t, 1);
+    else if (csa == 2 && csb == 2) IVQM_SSIM(uint16_t, 2, uint16_t, 2);
+    else if (csa == 2 && csb == 0) IVQM_SSIM(uint16_t, 2, uint16_t, 0);
+    else IVQM_SSIM(uint16_t, 0, uint16_t, 2);
   } else if (SM < 2) {
     if (J.a.bps == 1 && J.b.bps == 1) {
-      if (csa == 0 && csb == 0) ssim_wave<uint8_t, 0, uint8_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint8_t, 1, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0 && csb == 0) IVQM_SSIM(uint8_t, 0, uint8_t, 0);
+      else if (csa == 1) IVQM_SSIM(uint8_t, 1, uint8_t, 1);
+      else IVQM_SSIM(uint8_t, 2, uint8_t, 2);
     } else if (J.a.bps == 1) {
-      if (csa == 0) ssim_wave<uint8_t, 0, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
-      else ssim_wave<uint8_t, 1, uint16_t, 0, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial);
+      if (csa == 0) IVQM_SSIM(uint8_t, 0, uint16_t, 0);
+      else if (csa == 1) IVQM_SSIM(uint8_t, 1, uint16_t, 0);
+      else IVQM_SSIM(uint8_t, 2, uint16_t, 0);
     }
   }
+#undef IVQM_SSIM
 }
 
 // MOM: this launch runs the jobs that read fused moments (20 VGPRs, full occupancy), the other launch
@@ -1216,17 +1229,23 @@ __device__ __forceinline__ void down_px(const PicDev& S, int c, int x, int y, ui
 // Four adjacent outputs x4 .. x4 + 3 of row y (x4 + 3 < dw): the two source rows' 8 samples as two
 // vector loads each where the row is aligned for them (wave-uniform: one row per wave), one 8-byte
 // store (pyramid rows are 128-byte aligned). A 4:2:0 source's chroma is the level-1 chroma itself
-// (the 2x2 block replicates one sample), so it is copied.
+// (the 2x2 block replicates one sample), so it is copied; a 4:2:2 source's 2x2 block holds two
+// samples twice each, one above the other.
 template <typename T>
 __device__ __forceinline__ void down_x4(const PicDev& S, int c, int x4, int y, uint16_t* dst, int pitch, int dh) {
   typedef __attribute__((address_space(1))) const T GT;
   typedef unsigned short u16x4d_t __attribute__((ext_vector_type(4)));
   uint32_t o[4];
-  const int cs = c ? S.cs : 0;
-  if (cs) {
+  const int cf = c ? chroma_format(S) : 0;
+  if (cf == 1) {
     GT* row = (GT*)plane_ptr(S, c) + (size_t)y * plane_stride(S, c);
 #pragma unroll
     for (int k = 0; k < 4; k++) o[k] = row[x4 + k];
+  } else if (cf == 2) {
+    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c);
+    GT* r1 = r0 + plane_stride(S, c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (2 * (uint32_t)r0[x4 + k] + 2 * (uint32_t)r1[x4 + k] + 2) >> 2;
   } else {
     GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c);
     GT* r1 = r0 + plane_stride(S, c);

@@ -92,9 +92,9 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
       return set_error(IVQM_EINVAL, "pictures of a pair must share chroma format and sample size");
     if (!aligned_for_kernels(&p.p0) || !aligned_for_kernels(&p.p1))
       return set_error(IVQM_EINVAL, "batch pictures need 4-sample-aligned full-resolution planes and strides");
-    const int cs = p.p0.chroma_420 ? 1 : 0;
+    const int csy = p.p0.chroma_420 == 1 ? 1 : 0;
     for (int c = 0; c < 3; c++) {  // the kernels address a plane with 32-bit byte offsets
-      const int64_t rows = c ? (p.height + cs) >> cs : p.height;
+      const int64_t rows = c ? (p.height + csy) >> csy : p.height;
       if ((int64_t)p.p0.stride[c] * rows * p.p0.bytes_per_sample >= (1ll << 32) ||
           (int64_t)p.p1.stride[c] * rows * p.p1.bytes_per_sample >= (1ll << 32))
         return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
@@ -140,10 +140,11 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
     d.wrap32 = (cfg.gcd_sd_wrap32 && p.bit_depth <= 14) ? 1 : 0;
     d.res_idx = i;
     d.stat_blk0 = stat_blocks;
-    stat_blocks += p.height + 2 * (p.height >> d.p0.cs);
+    stat_blocks += p.height + 2 * (p.height >> d.p0.csy);
     d.tile0 = tiles;
     d.tile0b = tiles420;
-    if (d.p0.cs == 1 && (d.mode == 1 || d.mode == 2) && cfg.search_mode_cap == 0 && search420_ok(cfg, d.maxv, gmax))
+    if (chroma_format(d.p0) == 1 && (d.mode == 1 || d.mode == 2) && cfg.search_mode_cap == 0 &&
+        search420_ok(cfg, d.maxv, gmax))
       d.mode = kSearchDot420;
     if ((m & IVQM_M_IVPSNR) && d.mode == kSearchDot420)
       tiles420 += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH420 - 1) / TH420);
@@ -195,7 +196,7 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
       corr = host::ws_weights_into(d.h, cfg.lat_range_deg, h_erp);
       ew = h_erp;
     }
-    host::psnr_family(m, d.w, d.h, pairs[i].bit_depth, d.p0.cs, cfg.fake_vals_for_exact != 0,
+    host::psnr_family(m, d.w, d.h, pairs[i].bit_depth, d.p0.cs, d.p0.csy, cfg.fake_vals_for_exact != 0,
                       cfg.cmp_weights_average, h_row_ssd + d.rowsd_off, h_rows + d.rows_off, ew, corr, r);
     out[i] = r;
   }

@@ -72,9 +72,10 @@ namespace {
 // Copies a picture (host or device) into the slot's own pitched buffer.
 int copy_into_slot(ivqm_ctx* c, int slot, const ivqm_picture* p, hipMemcpyKind kind) {
   const int W = c->cfg.width, H = c->cfg.height;
-  const int cs = p->chroma_420 ? 1 : 0, bps = p->bytes_per_sample;
+  const PicDev f = to_dev(p);  // chroma shifts
+  const int cs = f.cs, csy = f.csy, bps = p->bytes_per_sample;
   const int pitch = Engine::scp_pitch(W), pitch_c = Engine::scp_pitch(W >> cs);
-  const size_t bytes = ((size_t)pitch * H + 2 * (size_t)pitch_c * (H >> cs)) * bps;
+  const size_t bytes = ((size_t)pitch * H + 2 * (size_t)pitch_c * (H >> csy)) * bps;
   RCCHK(c->eng->sync());
   if (c->own_bytes[slot] < bytes) {
     if (c->own[slot]) (void)hipFree(c->own[slot]);
@@ -87,7 +88,7 @@ int copy_into_slot(ivqm_ctx* c, int slot, const ivqm_picture* p, hipMemcpyKind k
   char* base = (char*)c->own[slot];
   size_t off = 0;
   for (int k = 0; k < 3; k++) {
-    const int wp = k ? (W >> cs) : W, hp = k ? (H >> cs) : H;
+    const int wp = k ? (W >> cs) : W, hp = k ? (H >> csy) : H;
     const int pp = k ? pitch_c : pitch;
     d.p[k] = base + off;
     d.stride[k] = pp;
@@ -96,6 +97,7 @@ int copy_into_slot(ivqm_ctx* c, int slot, const ivqm_picture* p, hipMemcpyKind k
     off += (size_t)pp * hp * bps;
   }
   d.cs = cs;
+  d.csy = csy;
   d.bps = bps;
   c->pic[slot] = d;
   c->bound[slot] = true;
@@ -121,7 +123,7 @@ void mask_into_spec(const ivqm_mask& m, PairSpec& s) {
 
 int ctx_pair(ivqm_ctx* c, PairSpec& s, const int32_t* gcd) {
   if (!c->bound[0] || !c->bound[1]) return set_error(IVQM_ESTATE, "both picture slots must be bound");
-  if (c->pic[0].cs != c->pic[1].cs || c->pic[0].bps != c->pic[1].bps)
+  if (chroma_format(c->pic[0]) != chroma_format(c->pic[1]) || c->pic[0].bps != c->pic[1].bps)
     return set_error(IVQM_EINVAL, "both pictures must share chroma format and sample size");
   s.p0 = c->pic[0];
   s.p1 = c->pic[1];
@@ -483,6 +485,7 @@ int ivqm_profile(ivqm_ctx* c, int32_t kernel, int32_t iters, double* avg_ms) {
 
 static int pic_from(const ivqm_picture* p, PicDev& d) {
   if (!p || (p->bytes_per_sample != 1 && p->bytes_per_sample != 2)) return set_error(IVQM_EINVAL, "bad picture");
+  if (p->chroma_420 < 0 || p->chroma_420 > 2) return set_error(IVQM_EINVAL, "bad chroma format");
   for (int k = 0; k < 3; k++)
     if (!p->plane[k]) return set_error(IVQM_EINVAL, "plane pointer is NULL");
   d = to_dev(p);
@@ -492,7 +495,7 @@ int ivqm_upsample420(const ivqm_picture* src, int32_t w, int32_t h, uint16_t* ds
   return guarded([&]() -> int {
     PicDev d;
     RCCHK(pic_from(src, d));
-    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && ((w & 1) || (h & 1))))
+    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && (w & 1)) || (d.csy && (h & 1)))
       return set_error(IVQM_EINVAL, "bad size");
     HIPCHK(launch_upsample420(d, w, h, dst, dst_stride, (hipStream_t)stream));
     return IVQM_OK;
@@ -503,7 +506,7 @@ int ivqm_convert_colour(const ivqm_picture* src, int32_t w, int32_t h, int32_t b
   return guarded([&]() -> int {
     PicDev d;
     RCCHK(pic_from(src, d));
-    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && ((w & 1) || (h & 1))))
+    if (!dst || w <= 0 || h <= 0 || dst_stride < w || (d.cs && (w & 1)) || (d.csy && (h & 1)))
       return set_error(IVQM_EINVAL, "bad size");
     if (bd < 8 || bd > 16 || (d.bps == 1 && bd != 8)) return set_error(IVQM_EINVAL, "bad bit depth");
     if (direction != 0 && direction != 1) return set_error(IVQM_EINVAL, "direction must be 0 or 1");

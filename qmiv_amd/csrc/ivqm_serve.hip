@@ -105,9 +105,9 @@ class Arena {
 };
 
 struct Format {
-  int w = 0, h = 0, bd = 0, cs = 0, bps = 2;
+  int w = 0, h = 0, bd = 0, cs = 0, bps = 2, csy = 0;  // chroma column / row shifts
   int plane_w(int k) const { return k ? (w >> cs) : w; }
-  int plane_h(int k) const { return k ? (h >> cs) : h; }
+  int plane_h(int k) const { return k ? (h >> csy) : h; }
   // staged copy of one picture: three planes with 64-sample pitches
   uint64_t staged_bytes() const {
     uint64_t b = 0;
@@ -232,6 +232,7 @@ int ivqm_server::admit(Job* j) {
         p.stride[k] = j->strides.empty() ? f.plane_w(k) : j->strides[3 * i + k];
       }
       p.cs = f.cs;
+      p.csy = f.csy;
       p.bps = f.bps;
     }
     j->inputs_ready = true;
@@ -252,6 +253,7 @@ int ivqm_server::admit(Job* j) {
       off += (uint64_t)pitch * hp * f.bps;
     }
     p.cs = f.cs;
+    p.csy = f.csy;
     p.bps = f.bps;
   }
   HIPCHK(hipEventCreateWithFlags(&j->uploaded, hipEventDisableTiming));
@@ -526,7 +528,9 @@ int ivqm_server_add_sequence(ivqm_server* s, const ivqm_sequence_desc* q, int32_
     if (q->bit_depth < 8 || q->bit_depth > 16) return set_error(IVQM_EINVAL, "bit_depth must be 8..16");
     if (q->bytes_per_sample != 1 && q->bytes_per_sample != 2) return set_error(IVQM_EINVAL, "bytes_per_sample must be 1 or 2");
     if (q->bytes_per_sample == 1 && q->bit_depth != 8) return set_error(IVQM_EINVAL, "8-bit samples require bit_depth 8");
-    if (q->chroma_420 && ((q->width & 1) || (q->height & 1))) return set_error(IVQM_EINVAL, "4:2:0 requires even sizes");
+    if (q->chroma_420 < 0 || q->chroma_420 > 2) return set_error(IVQM_EINVAL, "chroma_420 must be 0, 1 or 2");
+    if (q->chroma_420 == 1 && ((q->width & 1) || (q->height & 1))) return set_error(IVQM_EINVAL, "4:2:0 requires even sizes");
+    if (q->chroma_420 == 2 && (q->width & 1)) return set_error(IVQM_EINVAL, "4:2:2 requires an even width");
     if (q->num_frames < 1 || !q->planes) return set_error(IVQM_EINVAL, "no frames");
     auto seq = std::make_shared<Sequence>();
     Format& f = seq->f;
@@ -534,6 +538,7 @@ int ivqm_server_add_sequence(ivqm_server* s, const ivqm_sequence_desc* q, int32_
     f.h = q->height;
     f.bd = q->bit_depth;
     f.cs = q->chroma_420 ? 1 : 0;
+    f.csy = q->chroma_420 == 1 ? 1 : 0;
     f.bps = q->bytes_per_sample;
     for (int i = 0; i < 3 * q->num_frames; i++) {
       if (!q->planes[i]) return set_error(IVQM_EINVAL, "plane pointer is NULL");
@@ -562,6 +567,7 @@ int ivqm_server_add_sequence(ivqm_server* s, const ivqm_sequence_desc* q, int32_
         off += (uint64_t)pitch * hp * f.bps;
       }
       p.cs = f.cs;
+      p.csy = f.csy;
       p.bps = f.bps;
       seq->frames.push_back(p);
     }

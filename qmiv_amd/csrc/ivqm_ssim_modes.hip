@@ -55,7 +55,8 @@ __device__ void ssim_gen_wave(const SsimJobDev& J, int c, int chunk, int wr0, in
   GTA* pa = (GTA*)plane_ptr(J.a, c);
   GTB* pb = (GTB*)plane_ptr(J.b, c);
   const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
-  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;
+  const int csa = c ? J.a.cs : 0, csb = c ? J.b.cs : 0;      // chroma column shifts
+  const int rsa = c ? J.a.csy : 0, rsb = c ? J.b.csy : 0;    // chroma row shifts
   const double kfix = J.ext == 4 ? 65535.0 : 0.0;  // Constant border: xPic.cpp:94-101 passes uint16 max
   double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
 #pragma unroll
@@ -72,8 +73,8 @@ __device__ void ssim_gen_wave(const SsimJobDev& J, int c, int chunk, int wr0, in
       y = ext_index(y, J.h, J.ext);
       yfix = y < 0;
     }
-    GTA* ra = pa + (size_t)(uint32_t)((yfix ? 0 : y) >> csa) * (uint32_t)sa;
-    GTB* rb = pb + (size_t)(uint32_t)((yfix ? 0 : y) >> csb) * (uint32_t)sb;
+    GTA* ra = pa + (size_t)(uint32_t)((yfix ? 0 : y) >> rsa) * (uint32_t)sa;
+    GTB* rb = pb + (size_t)(uint32_t)((yfix ? 0 : y) >> rsb) * (uint32_t)sb;
 #pragma unroll 4
     for (int dx = 0; dx < n; dx++) {
       double a, b;
@@ -152,7 +153,8 @@ __device__ void ssim_static_wave(const SsimJobDev& J, const PicDev& A, const Pic
   GTA* pa = (GTA*)plane_ptr(A, c);
   GTB* pb = (GTB*)plane_ptr(B, c);
   const int sa = plane_stride(A, c), sb = plane_stride(B, c);
-  const int csa = c ? A.cs : 0, csb = c ? B.cs : 0;
+  const int csa = c ? A.cs : 0, csb = c ? B.cs : 0;      // chroma column shifts
+  const int rsa = c ? A.csy : 0, rsb = c ? B.csy : 0;    // chroma row shifts
   const double kfix = J.ext == 4 ? 65535.0 : 0.0;
   const double* __restrict__ wt_t = J.wt + N * N;  // transposed weights: [dx][dy]
   double mA[KY], mB[KY], mAA[KY], mBB[KY], mAB[KY];
@@ -177,8 +179,8 @@ __device__ void ssim_static_wave(const SsimJobDev& J, const PicDev& A, const Pic
         fix = fix || y < 0;
         y = y < 0 ? 0 : y;
       }
-      double a = (double)pa[(uint32_t)(y >> csa) * (uint32_t)sa + xa];
-      double b = (double)pb[(uint32_t)(y >> csb) * (uint32_t)sb + xb];
+      double a = (double)pa[(uint32_t)(y >> rsa) * (uint32_t)sa + xa];
+      double b = (double)pb[(uint32_t)(y >> rsb) * (uint32_t)sb + xb];
       if (EXT && fix) {
         a = kfix;
         b = kfix;

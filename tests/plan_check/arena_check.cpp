@@ -55,8 +55,10 @@ int main() {
   r.h = 8;
   r.chroma_from_user = true;
   r.user = PicDev{};
-  r.user.cs = 1; r.user.bps = 2;
+  r.user.cs = 1; r.user.csy = 1; r.user.bps = 2;
   expect("level 1 with the caller's chroma", r, scp, cap_scp, pyr, cap_pyr, IVQM_OK);
+  r.user.csy = 0;
+  expect("caller's chroma taken from a 4:2:2 picture", r, scp, cap_scp, pyr, cap_pyr, IVQM_EINVAL);
   r.user.cs = 0;
   expect("caller's chroma taken from a 4:4:4 picture", r, scp, cap_scp, pyr, cap_pyr, IVQM_EINVAL);
   r.arena = 7;

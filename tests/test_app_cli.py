@@ -24,7 +24,8 @@ def test_help():
 @pytest.mark.parametrize("args,msg", [
     (["-i0", "a", "-ps", "64x64"], "InputFile1 is empty"),
     (["-i0", "a", "-i1", "b"], "PictureSize"),
-    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cf", "422"], "ChromaFormat"),
+    (["-i0", "a", "-i1", "b", "-ps", "64x64", "-cf", "411"], "ChromaFormat"),
+    (["-i0", "a", "-i1", "b", "-ps", "65x64", "-cf", "422"], "4:2:2 needs an even picture width"),
     (["-i0", "a", "-i1", "b", "-ps", "65x64", "-cf", "420"], "even picture sizes"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-bd", "17"], "BitDepth"),
     (["-i0", "a", "-i1", "b", "-ps", "64x64", "-ml", "PSNR, FOO"], 'Token "FOO"'),

@@ -201,3 +201,41 @@ def test_qmiv_front_end_mask_mode(tmp_path, oracle):
         assert line in lines, line
     res = rf.read_text().splitlines()
     assert res[0].startswith("FILE0") and res[1].startswith("FILE1") and res[2] == f'FILEM  "{fm}"'
+
+
+@pytest.mark.parametrize("form", ["-cf", "-pf"])
+def test_qmiv_front_end_422(tmp_path, oracle, form):
+    """4:2:2 raw files (-cf 422 / -pf yuv422p10le; chroma planes (W/2) x H, xSeq.cpp:266-280): the
+    PSNR / WS-PSNR / IV-PSNR lines equal the oracle's on the replicated 4:4:4 pictures as text, the
+    SSIM-family lines within 1.5e-6."""
+    if not os.path.exists(EXE):
+        pytest.fail("front end not built (python -m qmiv_amd.build)")
+    wl = synth.Workload(cfg=413, width=176, height=99, chroma_420=False, global_shift=1)  # odd height: 4:2:2 allows it
+    n, bd = 3, 10
+    sub = lambda p: [p[0]] + [np.ascontiguousarray(c[:, ::2]) for c in p[1:]]
+    pairs = [tuple(sub(p) for p in synth.make_pair(wl, i)) for i in range(n)]
+    f0, f1 = tmp_path / "a.yuv", tmp_path / "b.yuv"
+    raw = lambda pic: b"".join(np.ascontiguousarray(p, np.uint16).tobytes() for p in pic)
+    f0.write_bytes(b"".join(raw(p[0]) for p in pairs))
+    f1.write_bytes(b"".join(raw(p[1]) for p in pairs))
+    fmt = ["-cf", "422", "-bd", str(bd)] if form == "-cf" else ["-pf", "yuv422p10le"]
+    out = subprocess.run([EXE, "-i0", str(f0), "-i1", str(f1), "-ps", f"{wl.width}x{wl.height}"] + fmt +
+                         ["-ml", "PSNR, WSPSNR, IVPSNR, SSIM, IVSSIM", "-erp", "-v", "2"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    for f, (p0n, p1n) in enumerate(pairs):
+        p0, p1 = oracle.to444(p0n, 2), oracle.to444(p1n, 2)
+        exact = [False] * 3
+        g = oracle.gcd(p0, p1, bd)
+        s0, s1 = oracle.scp(p0, p1, bd, g)
+        for name, vals in (("PSNR", oracle.psnr(p0, p1, bd)), ("WSPSNR", oracle.wspsnr(p0, p1, bd, erp=True))):
+            for line in _frame_lines(f, name, vals, oracle.pic_value(vals), exact):
+                assert line in lines, (line, [l for l in lines if name in l][:3])
+        for line in _frame_lines(f, "IVPSNR", None, oracle.ivpsnr(p0, p1, bd, g, erp=True)[0], exact):
+            assert line in lines, line
+        want = {"SSIM": oracle.ssim(p0, p1, bd), "IVSSIM": [oracle.ivssim(p0, p1, s0, s1, bd, erp=True)[0]]}
+        for name, vals in want.items():
+            tag = " Y:Cb:Cr  " if name != "IVSSIM" else " " * 10
+            line = next(l for l in lines if l.startswith("Frame %08d " % f + "%8s%s " % (name, tag)))
+            assert np.max(np.abs(np.array(_numbers(line)) - np.array(vals))) <= 1.5e-6, (line, vals)

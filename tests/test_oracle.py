@@ -210,6 +210,24 @@ def test_upsample_downsample_round_trip(oracle):
     assert np.array_equal(d, want.astype(np.uint16))
 
 
+def test_upsample_h_422(oracle):
+    """4:2:2 -> 4:4:4 (UpsampleH, xPixelOpsSTD.cpp:107-120): horizontal replication, so the samples
+    sum twice the source's and taking every other column gives the source back; equal to the
+    reference build's UpsampleH where that builds; to444 of a 4:2:2 picture leaves luma untouched."""
+    from oracle.oracle import Reference, reference_available
+    rng = np.random.default_rng(2)
+    src = rng.integers(0, 65536, (37, 27)).astype(np.uint16)
+    up = oracle.upsample_h(src)
+    assert up.shape == (37, 54)
+    assert int(up.sum(dtype=np.int64)) == 2 * int(src.sum(dtype=np.int64))
+    assert np.array_equal(up[:, ::2], src) and np.array_equal(up[:, 1::2], src)
+    if reference_available():
+        assert np.array_equal(Reference().upsample_h(src), up)
+    y = rng.integers(0, 1024, (37, 54)).astype(np.uint16)
+    p444 = oracle.to444([y, src, src[::-1].copy()], 2)
+    assert np.array_equal(p444[0], y) and np.array_equal(p444[1], up)
+
+
 @pytest.mark.parametrize("n", [8, 9, 15, 16, 17, 63, 64, 1080, 1920, 2048, 4096])
 def test_ssim_block_counts_brute_force(oracle, n):
     """test-xSSIM.cpp:29-77."""

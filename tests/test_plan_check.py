@@ -23,4 +23,4 @@ def test_arena_bounds_rule_rejects_pictures_outside_their_arena():
     assert out.returncode == 0, out.stdout + out.stderr
     lines = out.stdout.splitlines()
     assert lines[-1] == "OK"
-    assert sum("rc=-22" in ln for ln in lines) == 8
+    assert sum("rc=-22" in ln for ln in lines) == 9
