@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--stress-frames", type=int, default=8)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="mean job arrivals per second")
+    ap.add_argument("--serve-ramp-seconds", type=float, default=3.0,
+                    help="seconds per step of the arrival-rate ramp (x1.5 per step from --serve-rate); 0 = no ramp")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference dispatch for HBM counter scaling)")
     return ap.parse_args()
@@ -266,25 +268,36 @@ def main():
         "gen_seconds": round(t_gen, 1),
     }
     sections = set(args.sections.split(",")) if args.sections else set()
+
+    def note(name):  # progress on stderr; stdout carries the one JSON line
+        print(f"bench: section {name}", file=sys.stderr, flush=True)
     if "ssim_family" in sections:
+        note("ssim_family")
         out["ssim_family"] = ssim_family(Q, torch, pairs, args)
     if "ssim_modes" in sections:
+        note("ssim_modes")
         out["ssim_modes"] = ssim_modes(Q, torch, pairs, args)
     if "mask" in sections:
+        note("mask")
         out["mask"] = mask_mode(Q, synth, torch, np, pairs, args)
     del pairs
     torch.cuda.empty_cache()
     if "mixed" in sections:
+        note("mixed")
         out["mixed_batch"] = mixed_batch(Q, synth, torch, np, args)
         torch.cuda.empty_cache()
     if "served" in sections:
+        note("served")
         out["served"] = served(Q, synth, torch, np, args)
         torch.cuda.empty_cache()
     if "stress" in sections:
+        note("stress")
         out["stress"] = stress(Q, synth, torch, np, args)
     if "frontend" in sections:
+        note("frontend")
         out["frontend"] = frontend(pairs_np, np)
     if not args.no_cpu_baseline:
+        note("cpu_baseline")
         gpu = [(r.ivpsnr, r.ivssim) for r in res[-FRAMES:]]
         out["cpu_baseline"], out["parity"] = cpu_baseline(pairs_np, args.cpu_seconds, gpu)
     print(json.dumps(out), flush=True)
@@ -520,11 +533,10 @@ def stress(Q, synth, torch, np, args):
     return res
 
 
-def served(Q, synth, torch, np, args):
-    """Poisson arrivals of jobs (1-8 frame pairs, sizes and 8/10-bit depths uniform over the mixed set,
-    IV-PSNR or IV-PSNR+IV-SSIM 50/50) into the serving loop; reference sequences resident on the device,
-    test frames in pinned host memory, staged under a 64 GiB budget."""
-    rng = np.random.default_rng(2024)
+def _serve_setup(Q, synth, torch, np, device_tests=False):
+    """A server with one resident reference sequence per mixed-batch size and depth (8 frames each),
+    test frames in pinned host memory (staged by the server) or, with device_tests, on the device (read
+    in place), and a warm-up of the largest batches of both job kinds."""
     srv = Q.IvqmServer(memory_budget=64 << 30, max_batch_pairs=16, max_inflight=4)
     seqs = []
     for (w, h) in synth.BATCH_SIZES:
@@ -534,12 +546,11 @@ def served(Q, synth, torch, np, args):
             for f in range(8):
                 p0, p1 = synth.make_pair(wl, f, as_u8=(bd == 8))
                 ref.append(p1)
-                tst.append([torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).pin_memory()
-                            for p in p0])
+                planes = [torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p) for p in p0]
+                tst.append([t.cuda() for t in planes] if device_tests else [t.pin_memory() for t in planes])
             sid = srv.add_sequence(ref, w, h, bd)
             seqs.append((sid, w, h, tst))
     kinds = [Q.IVQM_M_IVPSNR, Q.IVQM_M_IVPSNR | Q.IVQM_M_IVSSIM]
-    # warm-up: largest batches of both kinds (device buffers reach their working size)
     tag = 1 << 40
     for kind in kinds:
         for sid, w, h, tst in seqs:
@@ -548,12 +559,14 @@ def served(Q, synth, torch, np, args):
     srv.drain()
     while srv.poll(64):
         pass
-    jobs = {}
-    done = []
+    return srv, seqs, kinds
+
+
+def _serve_run(srv, seqs, kinds, np, rate, seconds, rng, tag0):
+    """Poisson arrivals at `rate` jobs/s for `seconds`; the results of every job (after a drain)."""
+    jobs, done = {}, []
     t0 = time.perf_counter()
-    t_next = t0
-    tag = 0
-    end = t0 + args.serve_seconds
+    t_next, tag, end = t0, tag0, t0 + seconds
     while True:
         now = time.perf_counter()
         if now >= end:
@@ -566,11 +579,12 @@ def served(Q, synth, torch, np, args):
             srv.submit(tag, sid, first, tst[first:first + nf], kind)
             jobs[tag] = (nf * w * h, kind)
             tag += 1
-            t_next += rng.exponential(1.0 / args.serve_rate)
+            t_next += rng.exponential(1.0 / rate)
             continue
         done += srv.poll(16)
         if t_next - time.perf_counter() > 0.0003:
             time.sleep(0.0002)
+    t_sub = time.perf_counter()
     srv.drain()
     t_end = time.perf_counter()
     while True:
@@ -578,25 +592,81 @@ def served(Q, synth, torch, np, args):
         if not r:
             break
         done += r
-    st = srv.stats()
-    srv.close()
-    lat = np.array([r.latency_us for r in done]) / 1e3
-    ok = all(r.status == 0 for r in done)
-    px = sum(jobs[r.tag][0] for r in done)
+    return jobs, done, t0, t_sub, t_end
 
-    def pct(a, q):
-        return round(float(np.percentile(a, q)), 3) if len(a) else None
+
+def _pct(np, a, q):
+    return round(float(np.percentile(a, q)), 3) if len(a) else None
+
+
+def _serve_summary(np, jobs, done, t0, t_sub, t_end, rate, seconds):
+    lat = np.array([r.latency_us for r in done]) / 1e3
+    px = sum(jobs[r.tag][0] for r in done)
+    comp = {k: np.array([getattr(r, k) for r in done]) / 1e3 for k in ("queue_us", "stage_us", "wait_us", "device_us")}
+    p99 = np.percentile(lat, 99) if len(lat) else 0.0
+    tail = [i for i, x in enumerate(lat) if x >= p99]
+    return {"offered_jobs_per_s": rate, "submit_seconds": round(t_sub - t0, 2), "seconds": round(t_end - t0, 2),
+            "jobs": len(done), "all_ok": all(r.status == 0 for r in done),
+            "offered_mpx_pairs_per_s": round(px / seconds / 1e6, 1),
+            "served_mpx_pairs_per_s": round(px / (t_end - t0) / 1e6, 1),
+            "p50_ms": _pct(np, lat, 50), "p99_ms": _pct(np, lat, 99), "p999_ms": _pct(np, lat, 99.9),
+            "max_ms": _pct(np, lat, 100), "queue_p99_ms": _pct(np, comp["queue_us"], 99),
+            "queue_wait_p99_ms": _pct(np, comp["queue_us"] + comp["wait_us"], 99),
+            # the slowest 1 %: mean of each part of their latency (queue, staging, waiting for a batch slot,
+            # launches + kernels + result copies)
+            "p99_breakdown_ms": {k[:-3]: round(float(np.mean(comp[k][tail])), 3) if tail else None
+                                 for k in comp}}
+
+
+def served(Q, synth, torch, np, args):
+    """Poisson arrivals of jobs (1-8 frame pairs, sizes and 8/10-bit depths uniform over the mixed set,
+    IV-PSNR or IV-PSNR+IV-SSIM 50/50) into the serving loop; reference sequences resident on the device,
+    test frames in pinned host memory, staged under a 64 GiB budget. Then ramps of the arrival rate (x1.5
+    per step) until the queue grows, with test frames in host memory and with test frames on the device:
+    the largest rate the server sustains (served within 5 % of offered, p99 of admission queue plus wait
+    for a batch slot at most 5 ms), with its latency percentiles, where the slowest 1 % spend their time,
+    and the served rate once saturated."""
+    rng = np.random.default_rng(2024)
+    srv, seqs, kinds = _serve_setup(Q, synth, torch, np)
+    jobs, done, t0, t_sub, t_end = _serve_run(srv, seqs, kinds, np, args.serve_rate, args.serve_seconds, rng, 0)
+    st = srv.stats()
+    out = _serve_summary(np, jobs, done, t0, t_sub, t_end, args.serve_rate, args.serve_seconds)
     by_kind = {}
     for kind, name in ((kinds[0], "ivpsnr"), (kinds[1], "ivpsnr+ivssim")):
         a = np.array([r.latency_us for r in done if jobs[r.tag][1] == kind]) / 1e3
-        by_kind[name] = {"jobs": int(len(a)), "p50_ms": pct(a, 50), "p99_ms": pct(a, 99)}
-    return {"offered_jobs_per_s": args.serve_rate, "seconds": round(t_end - t0, 2), "jobs": len(done),
-            "all_ok": ok, "served_mpx_pairs_per_s": round(px / (t_end - t0) / 1e6, 1),
-            "offered_mpx_pairs_per_s": round(px / args.serve_seconds / 1e6, 1),
-            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "p999_ms": pct(lat, 99.9), "max_ms": pct(lat, 100),
-            "queue_p99_ms": pct(np.array([r.queue_us for r in done]) / 1e3, 99), "by_kind": by_kind,
-            "batches": st["batches"], "pairs": st["pairs_completed"], "max_batch_fill": st["max_batch_fill"],
-            "budget_peak_gib": round(st["budget_peak"] / 2**30, 3), "resident_gib": round(st["resident_bytes"] / 2**30, 3)}
+        by_kind[name] = {"jobs": int(len(a)), "p50_ms": _pct(np, a, 50), "p99_ms": _pct(np, a, 99)}
+    out.update({"by_kind": by_kind, "batches": st["batches"], "pairs": st["pairs_completed"],
+                "max_batch_fill": st["max_batch_fill"], "budget_peak_gib": round(st["budget_peak"] / 2**30, 3),
+                "resident_gib": round(st["resident_bytes"] / 2**30, 3)})
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+    def ramp(server, sq, label):
+        steps, best, rate, tag = [], None, args.serve_rate, 1 << 30
+        while args.serve_ramp_seconds > 0 and rate <= 64 * args.serve_rate:
+            print(f"bench: served ramp ({label}) {rate:.0f} jobs/s", file=sys.stderr, flush=True)
+            j, d, a0, a1, a2 = _serve_run(server, sq, kinds, np, rate, args.serve_ramp_seconds, rng, tag)
+            tag += len(j)
+            step = _serve_summary(np, j, d, a0, a1, a2, rate, args.serve_ramp_seconds)
+            # sustained: served keeps up with offered and jobs do not queue (admission + waiting for a
+            # batch slot, p99) for more than 5 ms
+            step["sustained"] = bool(step["all_ok"] and step["served_mpx_pairs_per_s"] >= 0.95 *
+                                     step["offered_mpx_pairs_per_s"] and (step["queue_wait_p99_ms"] or 0) <= 5.0)
+            steps.append(step)
+            if not step["sustained"]:
+                break
+            best = step
+            rate *= 1.5
+        # past the last sustained rate the server runs saturated: its served rate is the capacity
+        cap = max((st["served_mpx_pairs_per_s"] for st in steps), default=None)
+        return {"steps": steps, "max_sustained": best, "saturated_mpx_pairs_per_s": cap}
+    out["ramp_host_frames"] = ramp(srv, seqs, "host test frames")
+    srv.close()
+    # the same jobs with test frames already on the device: the server's own capacity, no staging copies
+    srv, seqs, kinds = _serve_setup(Q, synth, torch, np, device_tests=True)
+    out["ramp_device_frames"] = ramp(srv, seqs, "device test frames")
+    srv.close()
+    return out
 
 
 CPU_POOL_CAP = 16  # worker threads at most: the host share of one GPU on the measured box

@@ -65,6 +65,15 @@ typedef struct ivqm_job {
   int32_t reserved;
   const void* const* planes;   /* num_frames * 3 plane pointers; must stay valid until the result is polled */
   const int32_t* strides;      /* num_frames * 3 strides in samples; NULL = tight */
+  /* Mask mode, the app's InputFileM (apps/QMIV/src/xAppQMIV.cpp:899-906, :930-1029): NULL, or num_frames
+   * weight planes at luma resolution, one per test frame, in host or device memory as planes_on_device
+   * says (staged or read in place like the test planes). The job's metrics are then the masked MSE,
+   * PSNR, WS-PSNR and IV-PSNR of ivqm_bind_mask; structural-similarity metrics are rejected, as the app
+   * rejects them (xAppQMIV.cpp:360). */
+  const void* const* mask_planes;
+  const int32_t* mask_strides; /* num_frames strides in samples; NULL = tight */
+  int32_t mask_bit_depth;      /* BitDepthM, 8..14 */
+  int32_t mask_bytes_per_sample; /* 1 (bit depth 8 only) or 2 */
 } ivqm_job;
 
 typedef struct ivqm_job_result {
@@ -74,6 +83,10 @@ typedef struct ivqm_job_result {
   double latency_us;           /* submit -> results ready on the host */
   double queue_us;             /* submit -> admitted (memory granted) */
   ivqm_frame_result frames[IVQM_MAX_JOB_FRAMES];
+  /* Where the rest of latency_us went (queue_us + stage_us + wait_us + device_us <= latency_us): */
+  double stage_us;             /* admitted -> test frames (and masks) on the device (0 for device planes) */
+  double wait_us;              /* on the device -> the job's first frame put into a batch */
+  double device_us;            /* first frame batched -> the job's last batch collected (launches, kernels, copies) */
 } ivqm_job_result;
 
 typedef struct ivqm_server_stats {

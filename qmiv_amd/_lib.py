@@ -101,12 +101,15 @@ class SequenceDesc(C.Structure):
 class Job(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("seq_id", C.c_int32), ("first_frame", C.c_int32), ("num_frames", C.c_int32),
                 ("metrics", C.c_uint32), ("planes_on_device", C.c_int32), ("reserved", C.c_int32),
-                ("planes", C.POINTER(C.c_void_p)), ("strides", C.POINTER(C.c_int32))]
+                ("planes", C.POINTER(C.c_void_p)), ("strides", C.POINTER(C.c_int32)),
+                ("mask_planes", C.POINTER(C.c_void_p)), ("mask_strides", C.POINTER(C.c_int32)),
+                ("mask_bit_depth", C.c_int32), ("mask_bytes_per_sample", C.c_int32)]
 
 
 class JobResult(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("num_frames", C.c_int32), ("latency_us", C.c_double),
-                ("queue_us", C.c_double), ("frames", FrameResult * MAX_JOB_FRAMES)]
+                ("queue_us", C.c_double), ("frames", FrameResult * MAX_JOB_FRAMES), ("stage_us", C.c_double),
+                ("wait_us", C.c_double), ("device_us", C.c_double)]
 
 
 class ServerStats(C.Structure):

@@ -366,10 +366,21 @@ class IvqmServer:
     def remove_sequence(self, seq_id):
         check(self.L.ivqm_server_remove_sequence(self.h, seq_id))
 
-    def submit(self, tag, seq_id, first_frame, frames, metrics):
-        """frames: test pictures (host numpy or device torch); kept referenced until polled."""
+    def submit(self, tag, seq_id, first_frame, frames, metrics, masks=None, mask_bit_depth=8):
+        """frames: test pictures (host numpy or device torch); kept referenced until polled.
+        masks: optional weight planes, one per frame, on the same side as the frames (mask mode)."""
         ptrs, strides, on_dev, keep = _plane_ptrs(frames)
         j = _lib.Job(tag, seq_id, first_frame, len(frames), metrics, int(on_dev), 0, ptrs, strides)
+        if masks is not None:
+            mp, ms, _, mkeep = _plane_ptrs([[m, m, m] for m in masks])
+            n = len(masks)
+            mptrs = (C.c_void_p * n)(*[mp[3 * i] for i in range(n)])
+            mstr = (C.c_int32 * n)(*[ms[3 * i] for i in range(n)])
+            j.mask_planes, j.mask_strides = mptrs, mstr
+            j.mask_bit_depth = mask_bit_depth
+            m0 = mkeep[0]
+            j.mask_bytes_per_sample = m0.element_size() if hasattr(m0, "element_size") else m0.dtype.itemsize
+            keep = keep + [mptrs, mstr, mkeep]
         check(self.L.ivqm_server_submit(self.h, C.byref(j)))
         self._jobs[tag] = (ptrs, strides, keep)
 

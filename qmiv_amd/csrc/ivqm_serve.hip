@@ -128,8 +128,13 @@ struct Job {
   ivqm_job d;
   std::vector<const void*> planes;
   std::vector<int32_t> strides;
+  std::vector<const void*> mask_planes;  // empty: no mask
+  std::vector<int32_t> mask_strides;
+  std::vector<const void*> mask_dev;     // per frame, on the device
+  std::vector<int32_t> mask_dev_stride;
   std::shared_ptr<Sequence> seq;
-  Clock::time_point t_submit, t_admit;
+  Clock::time_point t_submit, t_admit, t_ready, t_batched;
+  bool batched = false;  // a frame of the job went into a batch
   bool staged = false;
   uint64_t arena_off = 0, arena_bytes = 0;
   hipEvent_t uploaded = nullptr;
@@ -204,6 +209,19 @@ int validate_job(ivqm_server* s, const ivqm_job* j, std::shared_ptr<Sequence>* s
     if (!j->planes[i]) return set_error(IVQM_EINVAL, "plane pointer is NULL");
     if (j->strides && j->strides[i] < q.f.plane_w(i % 3)) return set_error(IVQM_EINVAL, "stride smaller than plane width");
   }
+  if (j->mask_planes) {
+    if (m & (IVQM_M_SSIM | IVQM_M_MSSSIM | IVQM_M_IVSSIM | IVQM_M_IVMSSSIM))
+      return set_error(IVQM_EINVAL, "structural similarity metrics cannot be combined with a mask");
+    if (j->mask_bit_depth < 8 || j->mask_bit_depth > 14) return set_error(IVQM_EINVAL, "mask_bit_depth must be 8..14");
+    if (j->mask_bytes_per_sample != 1 && j->mask_bytes_per_sample != 2)
+      return set_error(IVQM_EINVAL, "mask_bytes_per_sample must be 1 or 2");
+    if (j->mask_bytes_per_sample == 1 && j->mask_bit_depth != 8)
+      return set_error(IVQM_EINVAL, "8-bit mask samples require mask_bit_depth 8");
+    for (int f = 0; f < j->num_frames; f++) {
+      if (!j->mask_planes[f]) return set_error(IVQM_EINVAL, "mask plane pointer is NULL");
+      if (j->mask_strides && j->mask_strides[f] < q.f.w) return set_error(IVQM_EINVAL, "mask stride smaller than the width");
+    }
+  }
   if (j->planes_on_device) {
     for (int f = 0; f < j->num_frames; f++) {
       PicDev p;
@@ -235,6 +253,13 @@ int ivqm_server::admit(Job* j) {
       p.csy = f.csy;
       p.bps = f.bps;
     }
+    j->t_ready = j->t_admit;
+    if (!j->mask_planes.empty()) {
+      j->mask_dev = j->mask_planes;
+      j->mask_dev_stride.resize(j->d.num_frames);
+      for (int i = 0; i < j->d.num_frames; i++)
+        j->mask_dev_stride[i] = j->mask_strides.empty() ? f.w : j->mask_strides[i];
+    }
     j->inputs_ready = true;
     return IVQM_OK;
   }
@@ -256,6 +281,18 @@ int ivqm_server::admit(Job* j) {
     p.csy = f.csy;
     p.bps = f.bps;
   }
+  if (!j->mask_planes.empty()) {  // the masks after the test frames, 64-sample pitch
+    const int mbps = j->d.mask_bytes_per_sample, pitch = Engine::scp_pitch(f.w);
+    j->mask_dev.resize(j->d.num_frames);
+    j->mask_dev_stride.assign(j->d.num_frames, pitch);
+    for (int i = 0; i < j->d.num_frames; i++) {
+      const int sstride = j->mask_strides.empty() ? f.w : j->mask_strides[i];
+      j->mask_dev[i] = base + off;
+      HIPCHK(hipMemcpy2DAsync(base + off, (size_t)pitch * mbps, j->mask_planes[i], (size_t)sstride * mbps,
+                              (size_t)f.w * mbps, f.h, hipMemcpyHostToDevice, copy_stream));
+      off += (uint64_t)pitch * f.h * mbps;
+    }
+  }
   HIPCHK(hipEventCreateWithFlags(&j->uploaded, hipEventDisableTiming));
   HIPCHK(hipEventRecord(j->uploaded, copy_stream));
   return IVQM_OK;
@@ -267,7 +304,14 @@ void ivqm_server::finish_job(Job* j) {
   r.status = j->status;
   r.num_frames = j->d.num_frames;
   r.latency_us = us_between(j->t_submit, Clock::now());
+  const Clock::time_point now = Clock::now();
   r.queue_us = j->staged || j->d.planes_on_device ? us_between(j->t_submit, j->t_admit) : r.latency_us;
+  r.stage_us = r.wait_us = r.device_us = 0;
+  if (j->batched) {
+    r.stage_us = us_between(j->t_admit, j->t_ready);
+    r.wait_us = us_between(j->t_ready, j->t_batched);
+    r.device_us = us_between(j->t_batched, now);
+  }
   if (j->uploaded) (void)hipEventDestroy(j->uploaded);
   j->uploaded = nullptr;
   std::lock_guard<std::mutex> lk(mu);
@@ -330,6 +374,9 @@ int ivqm_server::step(bool* progress) {
     Job* j = pending.front().get();
     if (!j->d.planes_on_device) {
       j->arena_bytes = j->seq->f.staged_bytes() * j->d.num_frames;
+      if (!j->mask_planes.empty())
+        j->arena_bytes += (uint64_t)Engine::scp_pitch(j->seq->f.w) * j->seq->f.h * j->d.mask_bytes_per_sample *
+                          j->d.num_frames;
       if (j->arena_bytes > arena.cap()) {
         j->status = set_error(IVQM_ENOMEM, "job larger than the memory budget");
         j->t_admit = Clock::now();
@@ -374,6 +421,11 @@ int ivqm_server::step(bool* progress) {
         if (e == hipErrorNotReady) continue;
         if (e != hipSuccess) return hip_error(e, "hipEventQuery");
         j->inputs_ready = true;
+        j->t_ready = Clock::now();
+      }
+      if (!j->batched) {
+        j->batched = true;
+        j->t_batched = Clock::now();
       }
       while (j->next_frame < j->d.num_frames && (int)specs.size() < cfg.max_batch_pairs) {
         const int f = j->next_frame++;
@@ -384,6 +436,12 @@ int ivqm_server::step(bool* progress) {
         s.h = j->seq->f.h;
         s.bd = j->seq->f.bd;
         s.metrics = j->d.metrics & IVQM_M_ALL;
+        if (!j->mask_dev.empty()) {
+          s.msk = j->mask_dev[f];
+          s.msk_stride = j->mask_dev_stride[f];
+          s.msk_bps = j->d.mask_bytes_per_sample;
+          s.msk_bd = j->d.mask_bit_depth;
+        }
         specs.push_back(s);
         F.pairs.emplace_back(j, f);
       }
@@ -613,6 +671,10 @@ int ivqm_server_submit(ivqm_server* s, const ivqm_job* job) {
     j->d = *job;
     j->planes.assign(job->planes, job->planes + 3 * job->num_frames);
     if (job->strides) j->strides.assign(job->strides, job->strides + 3 * job->num_frames);
+    if (job->mask_planes) {
+      j->mask_planes.assign(job->mask_planes, job->mask_planes + job->num_frames);
+      if (job->mask_strides) j->mask_strides.assign(job->mask_strides, job->mask_strides + job->num_frames);
+    }
     std::memset(&j->result, 0, sizeof(j->result));
     j->t_submit = Clock::now();
     {

@@ -175,3 +175,71 @@ def test_serve_rejects_invalid_requests(data):
         assert srv.poll(4)[0].status == 0
         if not removed:
             srv.remove_sequence(sid)
+
+
+def test_serve_mask_mode_alone_mixed_and_oracle(data, oracle):
+    """Jobs with per-frame weight masks (host-staged 16-bit masks and device-resident 8-bit masks)
+    mixed with unmasked IV-PSNR / IV-SSIM jobs: each masked frame equals the masked context call on
+    the same pictures and mask, and the oracle; unmasked jobs are unchanged by the masked ones;
+    structural-similarity metrics with a mask are rejected."""
+    import torch
+    M4 = Q.IVQM_M_MSE | Q.IVQM_M_PSNR | Q.IVQM_M_WSPSNR | Q.IVQM_M_IVPSNR
+    with Q.IvqmServer(memory_budget=256 << 20, max_batch_pairs=8) as srv:
+        sids = [srv.add_sequence(ref, wl.width, wl.height, wl.bit_depth) for wl, u8, ref, tst in data]
+        jobs = []  # (tag, si, first, n, metrics, masks, mask bit depth, masks on the device)
+        for tag, si in enumerate([0, 1, 2, 0, 2]):
+            wl = data[si][0]
+            bdm = 10 if tag % 2 == 0 else 8
+            masks = [synth.make_mask(wl.width, wl.height, bdm, seed=40 + 4 * tag + f) for f in range(2)]
+            jobs.append((tag, si, 1, 2, M4, masks, bdm, tag == 3))
+        unmasked = [(50 + k, si, 0, 2, IVS if k % 2 else IV, None, 0, False) for k, si in enumerate([0, 1, 2])]
+        keep = []
+        for tag, si, first, n, m, masks, bdm, on_dev in jobs + unmasked:
+            tst = data[si][3][first:first + n]
+            if masks is not None and on_dev:  # device planes: test frames and masks both on the device
+                tst = [[torch.from_numpy(np.ascontiguousarray(p).astype(np.uint16).view(np.int16)).cuda()
+                        if p.dtype != np.uint8 else torch.from_numpy(p).cuda() for p in fr] for fr in tst]
+                masks = [torch.from_numpy(mk.astype(np.uint8)).cuda() if bdm == 8 else
+                         torch.from_numpy(mk.view(np.int16)).cuda() for mk in masks]
+                torch.cuda.synchronize()
+                keep += [tst, masks]
+            srv.submit(tag, sids[si], first, tst, m, masks=masks, mask_bit_depth=bdm)
+        with pytest.raises(Q.IvqmError):  # SSIM family with a mask
+            srv.submit(99, sids[0], 0, data[0][3][:1], IVS, masks=jobs[0][5][:1], mask_bit_depth=10)
+        srv.drain()
+        res = {}
+        while True:
+            r = srv.poll(64)
+            if not r:
+                break
+            res.update({x.tag: x for x in r})
+        st = srv.stats()
+    assert st["max_batch_fill"] > 1
+    for tag, si, first, n, m, masks, bdm, on_dev in jobs:
+        wl, u8, ref, tst = data[si]
+        r = res[tag]
+        assert r.status == 0 and r.num_frames == n
+        for f in range(n):
+            p0n, p1n, mk = tst[first + f], ref[first + f], masks[f]
+            with Q.IvqmContext(wl.width, wl.height, wl.bit_depth) as ctx:
+                ctx.upload_pair(p0n, p1n, True)
+                ctx.upload_mask(mk if bdm > 8 else mk.astype(np.uint8), bdm)
+                want = ctx.frame(m).as_dict()
+            got = r.frames[f].as_dict()
+            for k in ("gcd_r2t", "mse", "psnr", "wspsnr", "ivpsnr", "ivpsnr_r2t", "ivpsnr_t2r"):
+                assert repr(got[k]) == repr(want[k]), (tag, f, k)
+            p0 = oracle.to444([np.asarray(p, np.uint16) for p in p0n], True)
+            p1 = oracle.to444([np.asarray(p, np.uint16) for p in p1n], True)
+            g = oracle.gcd_masked(p0, p1, mk, wl.bit_depth, bdm)
+            assert list(got["gcd_r2t"]) == g
+            mse, psnr, ws = oracle.psnr_family_masked(p0, p1, mk, wl.bit_depth, bdm)
+            assert list(got["mse"][:3]) == mse and list(got["psnr"][:3]) == psnr
+            assert (got["ivpsnr"], got["ivpsnr_r2t"], got["ivpsnr_t2r"]) == \
+                oracle.ivpsnr_masked(p0, p1, mk, wl.bit_depth, bdm, g)
+    for tag, si, first, n, m, masks, bdm, on_dev in unmasked:
+        wl, u8, ref, tst = data[si]
+        for f in range(n):
+            want = _single(wl, tst[first + f], ref[first + f], m)
+            got = res[tag].frames[f].as_dict()
+            for k in _fields(m):
+                assert repr(got[k]) == repr(want[k]), (tag, f, k)
