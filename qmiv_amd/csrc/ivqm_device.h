@@ -27,7 +27,8 @@ struct PairDev {
   PicDev p0, p1;
   int32_t w, h, maxv;
   int32_t mode;           // search variant: 0 reference form, 1 narrow key, 2 packed dot key, 3 reference form
-                          // with 24-bit multiplies, 4 keyed 4:2:0 search (k_search420) (host-checked ranges)
+                          // with 24-bit multiplies, 4 keyed 4:2:0 search (k_search420), 5 mode 3 with the
+                          // default weights {4, 1, 1} (host-checked ranges)
   int32_t md[3];          // GCD clip limits round_f32(coef * Max)
   int32_t wrap32;         // x86-64-v4 int32 SD accumulation
   int32_t stat_blk0;      // first k_stats block of this pair

@@ -198,8 +198,10 @@ int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax) {
   const int64_t A = (int64_t)maxv + gmax;
   int64_t wabs = 0;
   for (int k = 0; k < 3; k++) wabs = std::max<int64_t>(wabs, std::llabs((long long)w[k]));
-  // (d*w)*d by 24-bit multiplies: |d| <= A and |d * w| < 2^23
-  const int wide = (A * wabs < (1ll << 23)) ? 3 : 0;
+  // (d*w)*d by 24-bit multiplies: |d| <= A and |d * w| < 2^23; with the default weights {4, 1, 1}
+  // the luma term is (2d)^2 (mode 5, kSearchWide24D in the kernels): |2d| < 2^23
+  const bool w411 = w[0] == 4 && w[1] == 1 && w[2] == 1;
+  const int wide = (A * wabs < (1ll << 23)) ? (w411 ? 5 : 3) : 0;
   for (int k = 0; k < 3; k++)
     if (w[k] < 0) return wide;
   const int64_t S = 1 << key_bits(cfg.search_range);

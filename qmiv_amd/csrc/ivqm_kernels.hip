@@ -391,7 +391,10 @@ __global__ __launch_bounds__(256) void k_gcd_finalize(const PairDev* __restrict_
 typedef short short2_t __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3 };
+// kSearchWide24D: kSearchWide24 with the default component weights {4, 1, 1}: the LDS holds 2Y, so the
+// luma term 4 dY^2 is one 24-bit multiply (2dY)^2 -- the same value mod 2^32 as the reference's int32
+// 4 * dY^2 -- and a candidate costs three multiply-adds instead of six multiplies.
+enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3, kSearchWide24D = 5 };
 template <int R> struct KeyBits {
   static constexpr int n = (2 * R + 1) * (2 * R + 1);
   static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
@@ -446,7 +449,7 @@ __device__ __forceinline__ void sent_raw(const SEnt& e, int& Y, int& Cb, int& Cr
     Y = (int)((uint32_t)e.y & 0xffffu);
     Cb = (int)((uint32_t)e.y >> 16);
   } else {
-    Y = e.x;
+    Y = MODE == kSearchWide24D ? (e.x >> 1) : e.x;
     Cb = e.y;
   }
   Cr = e.z;
@@ -614,7 +617,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
                          __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr);
       return SEnt{(int)((q << KB) + (uint32_t)off), (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
     }
-    return SEnt{rY, rCb, rCr};
+    return SEnt{MODE == kSearchWide24D ? 2 * rY : rY, rCb, rCr};
   };
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
@@ -646,6 +649,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   // < 2^23 host-checked) and {-2 wY t0, -2 wCb t1} (DOT: one packed 16-bit
   // multiply-add, exact mod 2^16 and the value fits int16, host-checked).
   int t0[4], t1[4], t2[4];
+  int t0d[4];  // kSearchWide24D: 2 t0, against the doubled luma entries
   uint32_t tp[3][2];
   int m0[4], m1[4], m2[4];
   SEnt te[4];
@@ -688,6 +692,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
       t0[k] = s0 + g0;
       t1[k] = s1 + g1;
       t2[k] = s2 + g2;
+      t0d[k] = 2 * t0[k];
     }
   }
 
@@ -736,7 +741,14 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
         for (int dx = 0; dx < D; dx++) {
           const SEnt& c = e[k + dx];
           int v;
-          if (MODE == kSearchWide24) {
+          if (MODE == kSearchWide24D) {
+            // |2 dY| < 2^23: exact 24-bit operands, products mod 2^32 as the reference's int32
+            const int d0 = t0d[k] - c.x, d1 = t1[k] - c.y, d2 = t2[k] - c.z;
+            uint32_t acc = (uint32_t)__mul24(d0, d0);
+            acc += (uint32_t)__mul24(d1, d1);
+            acc += (uint32_t)__mul24(d2, d2);
+            v = (int)acc;
+          } else if (MODE == kSearchWide24) {
             const int d0 = t0[k] - c.x, d1 = t1[k] - c.y, d2 = t2[k] - c.z;
             uint32_t acc = (uint32_t)__mul24(__mul24(d0, wY), d0);
             acc += (uint32_t)__mul24(__mul24(d1, wCb), d1);
@@ -949,6 +961,8 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
       search_tile<uint16_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
       search_tile<uint16_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchWide24D)
+      search_tile<uint16_t, R, kSearchWide24D>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
       search_tile<uint16_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
@@ -958,6 +972,8 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
       search_tile<uint8_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchNarrow)
       search_tile<uint8_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    else if (P.mode == kSearchWide24D)
+      search_tile<uint8_t, R, kSearchWide24D>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else if (P.mode == kSearchWide24)
       search_tile<uint8_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
     else
