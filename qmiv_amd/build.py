@@ -82,6 +82,16 @@ def build_arena_check() -> str:
     return out
 
 
+def build_image_dump() -> str:
+    """Host-only program over the front end's PNG / BMP readers (run by tests/test_images.py)."""
+    out_dir = os.path.join(HERE, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "ivqm_image_dump")
+    _run(["g++", "-std=c++17", "-O2", os.path.join(ROOT, "tests", "images_check", "image_dump.cpp"),
+          os.path.join(CSRC, "ivqm_images.cpp"), "-lz", "-o", out])
+    return out
+
+
 def build_app(lib: str) -> str:
     """ivqm_qmiv: the QMIV command-line front end (raw YUV files -> per-frame and average metrics)."""
     out_dir = os.path.join(HERE, "bin")
@@ -102,6 +112,7 @@ def build(oracle: bool = True) -> None:
     lib = build_hip()
     build_dropin_check(lib)
     build_arena_check()
+    build_image_dump()
     build_app(lib)
     if oracle:
         build_oracle()

@@ -98,45 +98,67 @@ std::string read_png(const std::string& file, int w, int h, uint8_t* rgb) {
     p += 12 + (size_t)len;
   }
   if ((int)W != w || (int)H != h) return "PNG size does not match: " + file;
-  if (depth != 8 || interlace != 0) return "only 8-bit, non-interlaced PNG is supported: " + file;
+  if (depth != 8) return "only 8-bit PNG is supported: " + file;
+  if (interlace != 0 && interlace != 1) return "unknown PNG interlace method: " + file;
   const int ch = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
   if (!ch) return "unsupported PNG colour type: " + file;
   if (ctype == 3 && plte.size() < 3) return "PNG palette missing: " + file;
-  const size_t stride = (size_t)w * ch;
-  std::vector<uint8_t> raw((stride + 1) * (size_t)h);
+  // Passes: the whole image, or the seven Adam7 sub-images (PNG specification,
+  // section 8.2): pass p holds the pixels at (x0 + i*dx, y0 + j*dy), each pass
+  // stored as its own filtered image; an empty pass stores nothing.
+  static const int kAdam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                   {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+  static const int kWhole[1][4] = {{0, 0, 1, 1}};
+  const int npass = interlace ? 7 : 1;
+  const int(*pass)[4] = interlace ? kAdam7 : kWhole;
+  size_t total = 0;
+  for (int p = 0; p < npass; p++) {
+    const size_t pw = ((size_t)w + pass[p][2] - 1 - pass[p][0]) / pass[p][2];
+    const size_t ph = ((size_t)h + pass[p][3] - 1 - pass[p][1]) / pass[p][3];
+    if (pw && ph) total += (pw * ch + 1) * ph;
+  }
+  std::vector<uint8_t> raw(total);
   uLongf rawlen = (uLongf)raw.size();
   if (uncompress(raw.data(), &rawlen, idat.data(), (uLong)idat.size()) != Z_OK || rawlen != raw.size())
     return "PNG data does not inflate to the image size: " + file;
-  std::vector<uint8_t> prev(stride, 0), cur(stride);
   const size_t plane = (size_t)w * h;
-  for (int y = 0; y < h; y++) {
-    const uint8_t f = raw[(stride + 1) * y];
-    const uint8_t* s = &raw[(stride + 1) * y + 1];
-    for (size_t i = 0; i < stride; i++) {
-      const int a = i >= (size_t)ch ? cur[i - ch] : 0, b = prev[i], c = i >= (size_t)ch ? prev[i - ch] : 0;
-      int v = s[i];
-      if (f == 1) v += a;
-      else if (f == 2) v += b;
-      else if (f == 3) v += (a + b) >> 1;
-      else if (f == 4) v += paeth(a, b, c);
-      else if (f != 0) return "bad PNG filter: " + file;
-      cur[i] = (uint8_t)v;
+  size_t pos = 0;
+  for (int p = 0; p < npass; p++) {
+    const int x0 = pass[p][0], y0 = pass[p][1], dx = pass[p][2], dy = pass[p][3];
+    const size_t pw = ((size_t)w + dx - 1 - x0) / dx, ph = ((size_t)h + dy - 1 - y0) / dy;
+    if (!pw || !ph) continue;
+    const size_t stride = pw * ch;
+    std::vector<uint8_t> prev(stride, 0), cur(stride);
+    for (size_t j = 0; j < ph; j++, pos += stride + 1) {
+      const uint8_t f = raw[pos];
+      const uint8_t* s = &raw[pos + 1];
+      for (size_t i = 0; i < stride; i++) {
+        const int a = i >= (size_t)ch ? cur[i - ch] : 0, b = prev[i], c = i >= (size_t)ch ? prev[i - ch] : 0;
+        int v = s[i];
+        if (f == 1) v += a;
+        else if (f == 2) v += b;
+        else if (f == 3) v += (a + b) >> 1;
+        else if (f == 4) v += paeth(a, b, c);
+        else if (f != 0) return "bad PNG filter: " + file;
+        cur[i] = (uint8_t)v;
+      }
+      const size_t y = y0 + j * dy;
+      for (size_t i = 0; i < pw; i++) {
+        uint8_t r, g, bl;
+        const uint8_t* px = &cur[i * ch];
+        if (ctype == 0 || ctype == 4) r = g = bl = px[0];
+        else if (ctype == 3) {
+          const size_t k = (size_t)px[0] * 3;
+          if (k + 2 >= plte.size()) return "PNG palette index out of range: " + file;
+          r = plte[k]; g = plte[k + 1]; bl = plte[k + 2];
+        } else { r = px[0]; g = px[1]; bl = px[2]; }
+        const size_t o = y * w + x0 + i * dx;
+        rgb[o] = r;
+        rgb[plane + o] = g;
+        rgb[2 * plane + o] = bl;
+      }
+      prev.swap(cur);
     }
-    for (int x = 0; x < w; x++) {
-      uint8_t r, g, bl;
-      const uint8_t* px = &cur[(size_t)x * ch];
-      if (ctype == 0 || ctype == 4) r = g = bl = px[0];
-      else if (ctype == 3) {
-        const size_t k = (size_t)px[0] * 3;
-        if (k + 2 >= plte.size()) return "PNG palette index out of range: " + file;
-        r = plte[k]; g = plte[k + 1]; bl = plte[k + 2];
-      } else { r = px[0]; g = px[1]; bl = px[2]; }
-      const size_t o = (size_t)y * w + x;
-      rgb[o] = r;
-      rgb[plane + o] = g;
-      rgb[2 * plane + o] = bl;
-    }
-    prev.swap(cur);
   }
   return "";
 }

@@ -24,7 +24,8 @@ int count_frames(const std::string& pattern, int* first_idx);
 // Decodes one file into planes R, G, B (each w x h bytes). Returns an empty
 // string on success, else the reason.
 //   PNG: colour types gray, gray + alpha, RGB, RGBA and palette at 8 bits per
-//        sample, not interlaced; alpha is dropped (as a decode to RGB8 does).
+//        sample, not interlaced or Adam7-interlaced; alpha is dropped (as a
+//        decode to RGB8 does).
 //   BMP: BI_RGB 24 or 32 bits per pixel, bottom-up or top-down rows padded
 //        to 4 bytes (xSeqLST.cpp:337-415).
 std::string read_png(const std::string& file, int w, int h, uint8_t* rgb);

@@ -11,6 +11,7 @@ import pytest
 
 import qmiv_amd as Q
 from qmiv_amd import _lib, synth
+from tests.imagefiles import bmp_bytes, png_bytes
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -99,71 +100,25 @@ def test_front_end_colour_spaces(tmp_path, oracle, csi, csm, c420):
         assert any(l.startswith("Frame %08d " % f + "%8s%s " % ("SSIM", sfx_pic)) for l in lines)
 
 
-def _png_bytes(rgb, filters):
-    """PNG (8-bit RGB, one filter type per row cycling through `filters`) written with zlib."""
-    import struct
-    import zlib
-    h, w, _ = rgb.shape
-    raw = bytearray()
-    prev = np.zeros(w * 3, np.int32)
-    for y in range(h):
-        cur = rgb[y].reshape(-1).astype(np.int32)
-        f = filters[y % len(filters)]
-        left = np.concatenate([np.zeros(3, np.int32), cur[:-3]])
-        upleft = np.concatenate([np.zeros(3, np.int32), prev[:-3]])
-        if f == 0:
-            out = cur
-        elif f == 1:
-            out = cur - left
-        elif f == 2:
-            out = cur - prev
-        elif f == 3:
-            out = cur - (left + prev) // 2
-        else:
-            p = left + prev - upleft
-            pa, pb, pc = np.abs(p - left), np.abs(p - prev), np.abs(p - upleft)
-            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, upleft))
-            out = cur - pred
-        raw.append(f)
-        raw += bytes((out & 0xFF).astype(np.uint8))
-        prev = cur
-
-    def chunk(t, d):
-        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
-    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
-            chunk(b"IDAT", zlib.compress(bytes(raw), 6)[:40]) + chunk(b"IDAT", zlib.compress(bytes(raw), 6)[40:]) +
-            chunk(b"IEND", b""))
-
-
-def _bmp_bytes(rgb, top_down):
-    import struct
-    h, w, _ = rgb.shape
-    line = (w * 3 + 3) // 4 * 4
-    rows = []
-    for r in range(h):
-        y = r if top_down else h - 1 - r
-        row = rgb[y][:, ::-1].reshape(-1).tobytes()  # B, G, R
-        rows.append(row + bytes(line - len(row)))
-    body = b"".join(rows)
-    info = struct.pack("<IiiHHIIiiII", 40, w, -h if top_down else h, 1, 24, 0, len(body), 2835, 2835, 0, 0)
-    return struct.pack("<HIHHI", 0x4D42, 54 + len(body), 0, 0, 54) + info + body
-
-
-@pytest.mark.parametrize("fmt", ["PNG", "BMP"])
+@pytest.mark.parametrize("fmt", ["PNG", "PNG-Adam7", "BMP"])
 def test_front_end_image_lists(tmp_path, oracle, fmt):
     """-ff PNG / BMP: frames decoded from image files named by a pattern (first index 1; PNG rows use
-    every filter type and the data spans two IDAT chunks; BMP top-down and bottom-up) give the PSNR /
-    IV-PSNR of the RGB planes, with R:G:B suffixes (xSeqLST.cpp, xAppQMIV.cpp:297)."""
+    every filter type and the data spans two IDAT chunks, plain and Adam7-interlaced; BMP top-down and
+    bottom-up) give the PSNR / IV-PSNR of the RGB planes, with R:G:B suffixes (xSeqLST.cpp,
+    xAppQMIV.cpp:297)."""
     if not os.path.exists(EXE):
         pytest.fail("front end not built (python -m qmiv_amd.build)")
     wl = synth.Workload(cfg=414, width=90, height=60, bit_depth=8, chroma_420=False, global_shift=1)
     n = 2
     pairs = [synth.make_pair(wl, i) for i in range(n)]
+    adam7 = fmt == "PNG-Adam7"
+    fmt = fmt.split("-")[0]
     ext = fmt.lower()
     for f, (p0n, p1n) in enumerate(pairs):
         for tag, pic in (("a", p0n), ("b", p1n)):
             rgb = np.stack([np.asarray(p, np.uint8) for p in pic], axis=-1)
-            data = _png_bytes(rgb, [0, 1, 2, 3, 4]) if fmt == "PNG" else _bmp_bytes(rgb, top_down=(f % 2 == 1))
+            data = (png_bytes(rgb, 2, [0, 1, 2, 3, 4], interlace=adam7) if fmt == "PNG"
+                    else bmp_bytes(rgb, top_down=(f % 2 == 1)))
             (tmp_path / f"{tag}_{f + 1:03d}.{ext}").write_bytes(data)
     out = subprocess.run([EXE, "-i0", str(tmp_path / f"a_{{:03d}}.{ext}"), "-i1", str(tmp_path / f"b_{{:03d}}.{ext}"),
                           "-ps", f"{wl.width}x{wl.height}", "-ff", fmt, "-ml", "PSNR, IVPSNR", "-v", "2"],
