@@ -28,7 +28,8 @@ struct PairDev {
   int32_t w, h, maxv;
   int32_t mode;           // search variant: 0 reference form, 1 narrow key, 2 packed dot key, 3 reference form
                           // with 24-bit multiplies, 4 keyed 4:2:0 search (k_search420), 5 mode 3 with the
-                          // default weights {4, 1, 1} (host-checked ranges)
+                          // default weights {4, 1, 1}, 6 reference score in expanded form (host-checked
+                          // ranges)
   int32_t md[3];          // GCD clip limits round_f32(coef * Max)
   int32_t wrap32;         // x86-64-v4 int32 SD accumulation
   int32_t stat_blk0;      // first k_stats block of this pair
@@ -127,8 +128,15 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 constexpr int kMaskTileW = 64, kMaskTileH = 16;
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                                 const int32_t* gcd, uint64_t* rows64, uint16_t* scp, uint32_t forms, hipStream_t st);
+// forms: one k_search kernel per form of the search arithmetic present among the batch's k_search pairs,
+// each over all tiles: bit 0 keyed (modes 1, 2), bit 1 reference forms (modes 0, 3, 5), bit 2 mode 6.
+constexpr int kSearchQuadMode = 6;
+__host__ __device__ inline uint32_t search_form_bit(int mode) {
+  return mode == 1 || mode == 2 ? 1u : (mode == kSearchQuadMode ? 4u : 2u);
+}
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
+                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, uint32_t forms,
+                         hipStream_t st);
 // forms: bit 0 when jobs read pictures, bit 1 when jobs read fused moments (one launch each).
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
                        hipStream_t st);

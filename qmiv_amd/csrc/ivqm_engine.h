@@ -130,6 +130,7 @@ struct BatchPlan {
   bool any_mask = false;
   int mtiles = 0;                              // k_search_masked tiles
   uint32_t mforms = 0;                         // masked search forms present (bit 0 reference, 1 narrow, 2 dot)
+  uint32_t sforms = 0;                         // k_search kernels needed (search_form_bit of each pair's mode)
   int64_t aux_total = 0, rows64_total = 0;     // masked pairs' aux rows and uint64 search rows
   int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
   int th420 = 32, nb = 8;  // k_search420 tile rows, SSIM band rows

@@ -199,9 +199,11 @@ int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax) {
   int64_t wabs = 0;
   for (int k = 0; k < 3; k++) wabs = std::max<int64_t>(wabs, std::llabs((long long)w[k]));
   // (d*w)*d by 24-bit multiplies: |d| <= A and |d * w| < 2^23; with the default weights {4, 1, 1}
-  // the luma term is (2d)^2 (mode 5, kSearchWide24D in the kernels): |2d| < 2^23
+  // the luma term is (2d)^2 (mode 5, kSearchWide24D in the kernels): |2d| < 2^23. The expanded form
+  // T + Q(r) - 2 sum w t r mod 2^32 (mode 6, kSearchQuad) takes the multipliers -2 w t as 24-bit
+  // operands: 2 |w| A < 2^23.
   const bool w411 = w[0] == 4 && w[1] == 1 && w[2] == 1;
-  const int wide = (A * wabs < (1ll << 23)) ? (w411 ? 5 : 3) : 0;
+  const int wide = (2 * A * wabs < (1ll << 23)) ? 6 : (A * wabs < (1ll << 23)) ? (w411 ? 5 : 3) : 0;
   for (int k = 0; k < 3; k++)
     if (w[k] < 0) return wide;
   const int64_t S = 1 << key_bits(cfg.search_range);
@@ -401,8 +403,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     } else if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only))) {
       if (d.mode == kSearchDot420)
         P.tiles420 += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + P.th420 - 1) / P.th420);
-      else
+      else {
         P.tiles += ((s.w + kSearchTileW - 1) / kSearchTileW) * ((s.h + TH - 1) / TH);
+        P.sforms |= search_form_bit(d.mode);
+      }
     }
     d.rowsd_off = P.rowsd_total;
     pp.rowsd_off = d.rowsd_off;
@@ -635,8 +639,9 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   // luma only when the picture's chroma planes are aligned, which depends on the caller's pointers).
   std::string launch_sig;
   char lb[160];
-  std::snprintf(lb, sizeof lb, "s%d t%d/%d mt%d mf%x j%zu sb%d sf%x g%d am%d r%lld rd%lld r64%lld|", P.stat_blocks,
-                P.tiles, P.tiles420, P.mtiles, P.mforms, P.jobs.size(), P.ssim_blocks, ssim_forms(P), P.ssim_gen ? 1 : 0,
+  std::snprintf(lb, sizeof lb, "s%d t%d/%d/%x mt%d mf%x j%zu sb%d sf%x g%d am%d r%lld rd%lld r64%lld|",
+                P.stat_blocks, P.tiles, P.tiles420, P.sforms, P.mtiles, P.mforms, P.jobs.size(), P.ssim_blocks,
+                ssim_forms(P), P.ssim_gen ? 1 : 0,
                 P.any_mask ? 1 : 0, (long long)P.rows_total, (long long)P.rowsd_total, (long long)P.rows64_total);
   launch_sig += lb;
   for (int l = 1; l < 5; l++) {
@@ -801,7 +806,7 @@ int Engine::fill_tables(const BatchPlan& P, RingSlot& S, int* ndown, int down_fi
 // Both search kernels of a batch: k_search over its tiles, k_search420 over the keyed 4:2:0 pairs' tiles.
 hipError_t Engine::launch_search_all(const BatchPlan& P, hipStream_t st) {
   hipError_t e = launch_search(d_pairs_, P.n, P.tiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_, d_rows_,
-                               d_scp_, d_best_, st);
+                               d_scp_, d_best_, P.sforms, st);
   if (e != hipSuccess) return e;
   return launch_search420(d_pairs_, P.n, P.tiles420, cfg_.search_range, P.th420, cfg_.cmp_weights_search, d_gcd_,
                           d_rows_, d_scp_, d_best_, st);

@@ -394,7 +394,18 @@ typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 // kSearchWide24D: kSearchWide24 with the default component weights {4, 1, 1}: the LDS holds 2Y, so the
 // luma term 4 dY^2 is one 24-bit multiply (2dY)^2 -- the same value mod 2^32 as the reference's int32
 // 4 * dY^2 -- and a candidate costs three multiply-adds instead of six multiplies.
-enum SearchMode { kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3, kSearchWide24D = 5 };
+// kSearchQuad (host-checked 2 |w| (Max + |gcd|) < 2^23): the reference's wrapping score in expanded form.
+// sum_c w_c (t_c - r_c)^2 = T + Q(r) - 2 sum_c w_c t_c r_c holds over the integers, so it holds mod 2^32:
+// with Q(r) = sum_c w_c r_c^2 mod 2^32 stored per entry, the multipliers m_c = -2 w_c t_c (24-bit) and
+// T = sum_c w_c t_c^2 mod 2^32 per pixel, three 24-bit multiply-adds and an add give exactly the int32
+// value the reference compares, wrapped or not. Candidates are reduced per candidate row (min3 folds, no
+// index), a strictly smaller row minimum moves the pixel's best row, and after the search the winning
+// row is scored once more to find its first column holding the minimum: the first strict minimum in
+// raster order, at about half the instructions of the explicit compare-and-select per candidate.
+enum SearchMode {
+  kSearchWide = 0, kSearchNarrow = 1, kSearchDot = 2, kSearchWide24 = 3, kSearchWide24D = 5, kSearchQuad = 6
+};
+static_assert(kSearchQuad == kSearchQuadMode, "mode number shared with the host");
 template <int R> struct KeyBits {
   static constexpr int n = (2 * R + 1) * (2 * R + 1);
   static constexpr int v = n <= 16 ? 4 : n <= 32 ? 5 : n <= 64 ? 6 : 7;
@@ -445,7 +456,7 @@ __device__ __forceinline__ void sent_raw(const SEnt& e, int& Y, int& Cb, int& Cr
   if (MODE == kSearchDot) {
     Y = (int)(((uint32_t)e.y & 0xffffu) >> KB);
     Cb = (int)((uint32_t)e.y >> (16 + KB));
-  } else if (MODE == kSearchNarrow) {
+  } else if (MODE == kSearchNarrow || MODE == kSearchQuad) {
     Y = (int)((uint32_t)e.y & 0xffffu);
     Cb = (int)((uint32_t)e.y >> 16);
   } else {
@@ -617,6 +628,12 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
                          __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr);
       return SEnt{(int)((q << KB) + (uint32_t)off), (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
     }
+    if (MODE == kSearchQuad) {
+      // |w r| < 2^23 (host-checked): 24-bit operands, products mod 2^32
+      const uint32_t q = (uint32_t)__mul24(__mul24(wY, rY), rY) + (uint32_t)__mul24(__mul24(wCb, rCb), rCb) +
+                         (uint32_t)__mul24(__mul24(wCr, rCr), rCr);
+      return SEnt{(int)q, (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
+    }
     return SEnt{MODE == kSearchWide24D ? 2 * rY : rY, rCb, rCr};
   };
 #pragma unroll
@@ -695,15 +712,60 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
       t0d[k] = 2 * t0[k];
     }
   }
+  // kSearchQuad: m_c = -2 w_c t_c and T = sum_c w_c t_c^2 mod 2^32 (|w t| < 2^22, host-checked)
+  int q0[4], q1[4], q2[4];
+  uint32_t qT[4];
+  if (MODE == kSearchQuad) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      q0[k] = -2 * wY * t0[k];
+      q1[k] = -2 * wCb * t1[k];
+      q2[k] = -2 * wCr * t2[k];
+      qT[k] = (uint32_t)__mul24(__mul24(wY, t0[k]), t0[k]) + (uint32_t)__mul24(__mul24(wCb, t1[k]), t1[k]) +
+              (uint32_t)__mul24(__mul24(wCr, t2[k]), t2[k]);
+    }
+  }
+  // the score of candidate entry c for pixel k (kSearchQuad)
+  auto quad_score = [&](int k, int cY, int cCb, const SEnt& c) -> int {
+    uint32_t acc = (uint32_t)__mul24(q0[k], cY) + (uint32_t)c.x;
+    acc += (uint32_t)__mul24(q1[k], cCb);
+    acc += (uint32_t)__mul24(q2[k], c.z);
+    return (int)(acc + qT[k]);
+  };
 
   int bk[4], bi[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     bk[k] = INT_MAX;
-    bi[k] = 0;
+    bi[k] = KEYED || MODE == kSearchQuad ? 0 : -1;  // kSearchQuad: the best candidate row; -1: none taken
   }
+  if (MODE == kSearchQuad) {
+    // one candidate row per iteration (a rolled loop bounds the live registers)
+#pragma unroll 1
+    for (int dy = 0; dy < D; dy++) {
+      SEnt e[NE];
+      lds_run<NE>(&lds[iy + dy][ix], e);
+      int eY[NE], eCb[NE];
 #pragma unroll
-  for (int dy = 0; dy < D; dy++) {
+      for (int j = 0; j < NE; j++) {
+        eY[j] = (int)((uint32_t)e[j].y & 0xffffu);
+        eCb[j] = (int)((uint32_t)e[j].y >> 16);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int rm = quad_score(k, eY[k], eCb[k], e[k]);
+#pragma unroll
+        for (int dx = 1; dx < D; dx++) rm = min(rm, quad_score(k, eY[k + dx], eCb[k + dx], e[k + dx]));
+        if (rm < bk[k]) {
+          bk[k] = rm;
+          bi[k] = dy;
+        }
+      }
+    }
+  }
+  // keyed forms: the candidate rows unrolled; reference forms: a rolled loop (bounds the live registers)
+#pragma unroll
+  for (int dy = 0; dy < (KEYED ? D : 0); dy++) {
     SEnt e[NE];
     lds_run<NE>(&lds[iy + dy][ix], e);
     if (MODE == kSearchDot) {
@@ -734,7 +796,13 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
           acc = __mul24(m2[k], e[k + dx].z) + acc;
           bk[k] = min(bk[k], acc);
         }
-    } else {
+    }
+  }
+  if (!KEYED && MODE != kSearchQuad) {
+#pragma unroll 1
+    for (int dy = 0; dy < D; dy++) {
+      SEnt e[NE];
+      lds_run<NE>(&lds[iy + dy][ix], e);
 #pragma unroll
       for (int k = 0; k < 4; k++)
 #pragma unroll
@@ -768,8 +836,36 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     }
   }
   int idx[4];
+  if (MODE == kSearchQuad) {
+    // the first column of the best row that holds the minimum
 #pragma unroll
-  for (int k = 0; k < 4; k++) idx[k] = KEYED ? ((bk[k] - (iy * D + ix + k)) & (S - 1)) : bi[k];
+    for (int k = 0; k < 4; k++) {
+      SEnt e[D];
+      const int by = bi[k];
+      switch (k) {
+        case 0: lds_entries<0, D>(&lds[iy + by][ix], e); break;
+        case 1: lds_entries<1, D>(&lds[iy + by][ix], e); break;
+        case 2: lds_entries<2, D>(&lds[iy + by][ix], e); break;
+        default: lds_entries<3, D>(&lds[iy + by][ix], e); break;
+      }
+      int bx = 0;
+#pragma unroll
+      for (int dx = D - 1; dx >= 0; dx--) {
+        const int v = quad_score(k, (int)((uint32_t)e[dx].y & 0xffffu), (int)((uint32_t)e[dx].y >> 16), e[dx]);
+        bx = v == bk[k] ? dx : bx;
+      }
+      idx[k] = by * D + bx;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) idx[k] = KEYED ? ((bk[k] - (iy * D + ix + k)) & (S - 1)) : max(bi[k], 0);
+  }
+  // Reference-form scores can all equal INT_MAX (wrapped values): then no candidate is taken and the
+  // reference keeps zero distances and a zero winner sample (xCorrespPixelShiftSSE.cpp:50-66, :109-124).
+  // Keyed scores stay far below INT_MAX (host-checked ranges).
+  bool nowin[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) nowin[k] = MODE == kSearchQuad ? bk[k] == INT_MAX : (!KEYED && bi[k] < 0);
 
   // Epilogue: the winners' samples, squared differences, row sums, SCP, winner map.
   // The winner of pixel k, candidate idx = ddy * D + ddx, is the entry at byte
@@ -826,7 +922,11 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       sent_raw<MODE, KB>(winner(k), rr[0][k], rr[1][k], rr[2][k]);
+      if (nowin[k]) [[unlikely]]
+        rr[0][k] = rr[1][k] = rr[2][k] = 0;
       int e0 = t0[k] - rr[0][k], e1 = t1[k] - rr[1][k], e2 = t2[k] - rr[2][k];
+      if (nowin[k]) [[unlikely]]
+        e0 = e1 = e2 = 0;
       if (edge) {
         const bool ok = row_ok && x + k < W;
         e0 = ok ? e0 : 0;
@@ -937,7 +1037,14 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   }
 }
 
-template <int R>
+// One kernel per form of the search arithmetic, so no form's register allocation is imposed on another:
+// FORM 0 the keyed modes (kSearchNarrow, kSearchDot), 1 the reference forms (kSearchWide, kSearchWide24,
+// kSearchWide24D), 2 kSearchQuad. Each walks all tiles of the batch and returns at once from tiles of
+// pairs of another form (block-uniform).
+__host__ __device__ constexpr int search_form(int mode) {
+  return mode == kSearchNarrow || mode == kSearchDot ? 0 : (mode == kSearchQuad ? 2 : 1);
+}
+template <int R, int FORM>
 __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pairs, int npairs, int ntiles, int wY,
                                                 int wCb, int wCr, const int32_t* __restrict__ gcd,
                                                 uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
@@ -952,33 +1059,35 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
   if (tile >= ntiles) return;  // padding of the last XCD's run (uniform per block)
   const int pi = find_entry(pairs, npairs, tile, [](const PairDev& p) { return p.tile0; });
   const PairDev P = pairs[pi];
+  if (search_form(P.mode) != FORM) return;
   const int lt = tile - P.tile0;
   const int ntx = (P.w + TT::TW - 1) / TT::TW;
   const int ty = lt / ntx, tx = lt - ty * ntx;
   const int x0 = tx * TT::TW, y0 = ty * TT::TH;
-  if (P.p0.bps == 2) {
-    if (P.mode == kSearchDot)
-      search_tile<uint16_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchNarrow)
-      search_tile<uint16_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchWide24D)
-      search_tile<uint16_t, R, kSearchWide24D>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchWide24)
-      search_tile<uint16_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else
-      search_tile<uint16_t, R, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+#define IVQM_TILE(T, MODE) search_tile<T, R, MODE>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best)
+  if (FORM == 0) {
+    if (P.p0.bps == 2) {
+      if (P.mode == kSearchDot) IVQM_TILE(uint16_t, kSearchDot);
+      else IVQM_TILE(uint16_t, kSearchNarrow);
+    } else {
+      if (P.mode == kSearchDot) IVQM_TILE(uint8_t, kSearchDot);
+      else IVQM_TILE(uint8_t, kSearchNarrow);
+    }
+  } else if (FORM == 2) {
+    if (P.p0.bps == 2) IVQM_TILE(uint16_t, kSearchQuad);
+    else IVQM_TILE(uint8_t, kSearchQuad);
   } else {
-    if (P.mode == kSearchDot)
-      search_tile<uint8_t, R, kSearchDot>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchNarrow)
-      search_tile<uint8_t, R, kSearchNarrow>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchWide24D)
-      search_tile<uint8_t, R, kSearchWide24D>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else if (P.mode == kSearchWide24)
-      search_tile<uint8_t, R, kSearchWide24>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
-    else
-      search_tile<uint8_t, R, kSearchWide>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    if (P.p0.bps == 2) {
+      if (P.mode == kSearchWide24D) IVQM_TILE(uint16_t, kSearchWide24D);
+      else if (P.mode == kSearchWide24) IVQM_TILE(uint16_t, kSearchWide24);
+      else IVQM_TILE(uint16_t, kSearchWide);
+    } else {
+      if (P.mode == kSearchWide24D) IVQM_TILE(uint8_t, kSearchWide24D);
+      else if (P.mode == kSearchWide24) IVQM_TILE(uint8_t, kSearchWide24);
+      else IVQM_TILE(uint8_t, kSearchWide);
+    }
   }
+#undef IVQM_TILE
 }
 
 // ---------------------------------------------------------------------------
@@ -1438,21 +1547,29 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 
 template <int R>
 static hipError_t search_r(const PairDev* pairs, int npairs, int ntiles, const int32_t cw[4], const int32_t* gcd,
-                           uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
+                           uint32_t* rows, uint16_t* scp, uint8_t* best, uint32_t forms, hipStream_t st) {
   const int per = (ntiles + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_search<R>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1], cw[2],
-                     gcd, rows, scp, best);
+  if (forms & 1u)
+    hipLaunchKernelGGL((k_search<R, 0>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
+                       cw[2], gcd, rows, scp, best);
+  if (forms & 2u)
+    hipLaunchKernelGGL((k_search<R, 1>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
+                       cw[2], gcd, rows, scp, best);
+  if (forms & 4u)
+    hipLaunchKernelGGL((k_search<R, 2>), dim3(kXcds * per), dim3(512), 0, st, pairs, npairs, ntiles, cw[0], cw[1],
+                       cw[2], gcd, rows, scp, best);
   return hipGetLastError();
 }
 
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
-                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st) {
+                         const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, uint32_t forms,
+                         hipStream_t st) {
   if (npairs <= 0 || ntiles <= 0) return hipSuccess;
   switch (R) {
-    case 1: return search_r<1>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 2: return search_r<2>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 3: return search_r<3>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
-    case 4: return search_r<4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, st);
+    case 1: return search_r<1>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, forms, st);
+    case 2: return search_r<2>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, forms, st);
+    case 3: return search_r<3>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, forms, st);
+    case 4: return search_r<4>(pairs, npairs, ntiles, cw, gcd, rows, scp, best, forms, st);
     default: return hipErrorInvalidValue;
   }
 }

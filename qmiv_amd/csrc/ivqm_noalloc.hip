@@ -121,6 +121,7 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
   // pair table: the batch engine's plan of an unmasked pair without SSIM metrics
   const int TH = kSearchTileH, TH420 = search420_tile_rows(cfg.search_tile_rows, 32);
   int stat_blocks = 0, tiles = 0, tiles420 = 0;
+  uint32_t sforms = 0;  // k_search kernels needed (launch_search)
   int64_t rowsd_off = 0, rows_off = 0;
   for (int i = 0; i < n; i++) {
     const ivqm_pair& p = pairs[i];
@@ -148,8 +149,10 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
       d.mode = kSearchDot420;
     if ((m & IVQM_M_IVPSNR) && d.mode == kSearchDot420)
       tiles420 += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH420 - 1) / TH420);
-    else if (m & IVQM_M_IVPSNR)
+    else if (m & IVQM_M_IVPSNR) {
       tiles += ((p.width + kSearchTileW - 1) / kSearchTileW) * ((p.height + TH - 1) / TH);
+      sforms |= search_form_bit(d.mode);
+    }
     d.rowsd_off = rowsd_off;
     rowsd_off += 3 * (int64_t)p.height;
     d.rows_off = rows_off;
@@ -168,7 +171,7 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
   NA_HIPCHK(launch_gcd_finalize(d_pairs, n, d_row_sd, d_row_ssd, nullptr, d_gcd, d_sd, d_ssd, nullptr, st));
   if (tiles)
     NA_HIPCHK(launch_search(d_pairs, n, tiles, cfg.search_range, cfg.cmp_weights_search, d_gcd, d_rows, nullptr,
-                            nullptr, st));
+                            nullptr, sforms, st));
   if (tiles420)
     NA_HIPCHK(launch_search420(d_pairs, n, tiles420, cfg.search_range, TH420, cfg.cmp_weights_search, d_gcd, d_rows,
                                nullptr, nullptr, st));
