@@ -488,6 +488,57 @@ __device__ __forceinline__ void lds_entries(const SEnt* p, SEnt (&e)[N]) {
   }
 }
 
+// kSearchQuad entries {Q(r), Y, Cb, Cr}: 16 bytes, every operand a whole register (no unpacking in
+// the candidate loop).
+struct SEnt4 {
+  int x, y, z, w;
+};
+// Entries A .. A + N - 1 of a run that starts at a group boundary of the padded layout (see lds_col).
+template <int A, int N>
+__device__ __forceinline__ void lds_entries(const SEnt4* p, SEnt4 (&e)[N]) {
+  const i32x4_t* q = reinterpret_cast<const i32x4_t*>(p);
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const i32x4_t v = q[(A + j) + ((A + j) >> 2)];
+    e[j] = SEnt4{v.x, v.y, v.z, v.w};
+  }
+}
+template <int NE>
+__device__ __forceinline__ void lds_run(const SEnt4* p, SEnt4 (&e)[NE]) {
+  lds_entries<0, NE>(p, e);
+}
+template <int MODE, int KB>
+__device__ __forceinline__ void sent_raw(const SEnt4& e, int& Y, int& Cb, int& Cr) {
+  Y = e.y;
+  Cb = e.z;
+  Cr = e.w;
+}
+template <int MODE> struct EntryOf {
+  using type = SEnt;
+};
+template <> struct EntryOf<kSearchQuad> {
+  using type = SEnt4;
+};
+// LDS column of region column x. 16-byte entries get one pad entry after every 4, so the lanes of a
+// wave, whose runs start 4 entries apart, read 80 bytes apart: 8 consecutive lanes' 16-byte reads
+// cover all 32 banks once (64 bytes apart they would meet in pairs of banks). 12-byte entries are
+// 48 bytes apart per lane already.
+template <typename E> __device__ __forceinline__ constexpr int lds_col(int x) {
+  return std::is_same<E, SEnt4>::value ? x + (x >> 2) : x;
+}
+template <typename E, int LP> struct LdsPitch {
+  static constexpr int v = std::is_same<E, SEnt4>::value ? LP + LP / 4 : LP;
+};
+
+// a * b + c with 24-bit signed operands a, b: the low 32 bits of the product plus c (v_mad_i32_i24).
+// Written out because instruction selection otherwise splits the sums of products into separate
+// multiplies and three-input adds (five instructions for three terms instead of three).
+__device__ __forceinline__ int mad_i24(int a, int b, int c) {
+  int d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 // One step of an inclusive scan of three values within each 16-lane row.
 template <int CTRL>
 __device__ __forceinline__ void dpp3_add(uint32_t v[3]) {
@@ -524,7 +575,10 @@ __device__ __forceinline__ uint32_t swap_sum16(uint32_t x, uint32_t y) {
 
 template <typename T, int R, int MODE>
 __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
-                                            SEnt (*lds2)[SearchTile<R>::LH][SearchTile<R>::LP], int wY, int wCb,
+                                            typename EntryOf<MODE>::type (
+                                                *lds2)[SearchTile<R>::LH]
+                                                      [LdsPitch<typename EntryOf<MODE>::type, SearchTile<R>::LP>::v],
+                                            int wY, int wCb,
                                             int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
                                             uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
   constexpr int D = 2 * R + 1, NE = 4 + 2 * R;
@@ -540,8 +594,10 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   const int dir = wv >> 2, wd = wv & 3;  // direction, wave within the direction
   const int tl = (int)threadIdx.x & 255;  // thread within the direction
   const PicDev& Ref = dir ? Pr.p0 : Pr.p1;
-  SEnt (*lds)[LP] = lds2[dir];
-  SEnt (*tlds)[LP] = lds2[1 - dir];
+  using E = typename EntryOf<MODE>::type;
+  constexpr int LPE = LdsPitch<E, LP>::v;
+  E (*lds)[LPE] = lds2[dir];
+  E (*tlds)[LPE] = lds2[1 - dir];
   const int32_t* g4 = gcd + 4 * Pr.res_idx;
   const int g0 = dir ? -g4[0] : g4[0];
   const int g1 = dir ? -g4[1] : g4[1];
@@ -612,29 +668,30 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
 
   // Keyed entries. Host-checked ranges: w * Max fits 16 bits (DOT), S * Q(r)
   // fits int32 and w * r < 2^24, so the 16- and 24-bit multiplies are exact.
-  auto entry = [&](int rY, int rCb, int rCr, int off) -> SEnt {
-    if (MODE == kSearchDot) {
+  auto entry = [&](int rY, int rCb, int rCr, int off) -> E {
+    if constexpr (MODE == kSearchDot) {
       const u16x2_t pr = {(unsigned short)rY, (unsigned short)rCb};
       const u16x2_t wp = {(unsigned short)wY, (unsigned short)wCb};
       const uint32_t qc = __umul24(__umul24((uint32_t)wCr & 0xffffu, (uint32_t)rCr), (uint32_t)rCr);
       const uint32_t q = __builtin_amdgcn_udot2(pr * wp, pr, qc, false);
       const u16x2_t ps = pr << (unsigned short)KB;
-      return SEnt{(int)((q << KB) + (uint32_t)off), __builtin_bit_cast(int, ps), rCr};
+      return E{(int)((q << KB) + (uint32_t)off), __builtin_bit_cast(int, ps), rCr};
     }
     if (MODE == kSearchNarrow) {
       const uint32_t uY = (uint32_t)wY & 0xffffu, uCb = (uint32_t)wCb & 0xffffu, uCr = (uint32_t)wCr & 0xffffu;
       const uint32_t q = __umul24(__umul24(uY, (uint32_t)rY), (uint32_t)rY) +
                          __umul24(__umul24(uCb, (uint32_t)rCb), (uint32_t)rCb) +
                          __umul24(__umul24(uCr, (uint32_t)rCr), (uint32_t)rCr);
-      return SEnt{(int)((q << KB) + (uint32_t)off), (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
+      return E{(int)((q << KB) + (uint32_t)off), (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
     }
-    if (MODE == kSearchQuad) {
+    if constexpr (MODE == kSearchQuad) {
       // |w r| < 2^23 (host-checked): 24-bit operands, products mod 2^32
       const uint32_t q = (uint32_t)__mul24(__mul24(wY, rY), rY) + (uint32_t)__mul24(__mul24(wCb, rCb), rCb) +
                          (uint32_t)__mul24(__mul24(wCr, rCr), rCr);
-      return SEnt{(int)q, (int)((uint32_t)rY | ((uint32_t)rCb << 16)), rCr};
+      return E{(int)q, rY, rCb, rCr};
+    } else {
+      return E{MODE == kSearchWide24D ? 2 * rY : rY, rCb, rCr};
     }
-    return SEnt{MODE == kSearchWide24D ? 2 * rY : rY, rCb, rCr};
   };
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
@@ -642,7 +699,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int ly = 2 * j - PAR + h;
-      if (j < NPT && ly >= 0 && ly < LH) lds[ly][lane] = entry(ra[k][h][0], ra[k][h][1], ra[k][h][2], ly * D + lane);
+      if (j < NPT && ly >= 0 && ly < LH) lds[ly][lds_col<E>(lane)] = entry(ra[k][h][0], ra[k][h][1], ra[k][h][2], ly * D + lane);
     }
   }
 #pragma unroll
@@ -650,7 +707,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     const int e = tl + 256 * k;
     if (e < NH) {
       const int hy = e / (2 * R), hx = e - hy * (2 * R);
-      lds[hy][64 + hx] = entry(rh[k][0], rh[k][1], rh[k][2], hy * D + 64 + hx);
+      lds[hy][lds_col<E>(64 + hx)] = entry(rh[k][0], rh[k][1], rh[k][2], hy * D + 64 + hx);
     }
   }
   __syncthreads();
@@ -669,8 +726,8 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   int t0d[4];  // kSearchWide24D: 2 t0, against the doubled luma entries
   uint32_t tp[3][2];
   int m0[4], m1[4], m2[4];
-  SEnt te[4];
-  lds_entries<R, 4>(&tlds[iy + R][ix], te);
+  E te[4];
+  lds_entries<R, 4>(&tlds[iy + R][lds_col<E>(ix)], te);
   if (KEYED) {
     const int KY = -2 * S * wY, KCb = -2 * S * wCb, KCr = -2 * S * wCr;
     uint32_t yc[4];  // raw {Y, Cb} per pixel
@@ -726,11 +783,11 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     }
   }
   // the score of candidate entry c for pixel k (kSearchQuad)
-  auto quad_score = [&](int k, int cY, int cCb, const SEnt& c) -> int {
-    uint32_t acc = (uint32_t)__mul24(q0[k], cY) + (uint32_t)c.x;
-    acc += (uint32_t)__mul24(q1[k], cCb);
-    acc += (uint32_t)__mul24(q2[k], c.z);
-    return (int)(acc + qT[k]);
+  auto quad_score = [&](int k, const E& c) -> int {
+    int cr = 0;
+    if constexpr (MODE == kSearchQuad) cr = c.w;
+    const int acc = mad_i24(q2[k], cr, mad_i24(q1[k], c.z, mad_i24(q0[k], c.y, c.x)));
+    return (int)((uint32_t)acc + qT[k]);
   };
 
   int bk[4], bi[4];
@@ -743,19 +800,13 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     // one candidate row per iteration (a rolled loop bounds the live registers)
 #pragma unroll 1
     for (int dy = 0; dy < D; dy++) {
-      SEnt e[NE];
-      lds_run<NE>(&lds[iy + dy][ix], e);
-      int eY[NE], eCb[NE];
-#pragma unroll
-      for (int j = 0; j < NE; j++) {
-        eY[j] = (int)((uint32_t)e[j].y & 0xffffu);
-        eCb[j] = (int)((uint32_t)e[j].y >> 16);
-      }
+      E e[NE];
+      lds_run<NE>(&lds[iy + dy][lds_col<E>(ix)], e);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        int rm = quad_score(k, eY[k], eCb[k], e[k]);
+        int rm = quad_score(k, e[k]);
 #pragma unroll
-        for (int dx = 1; dx < D; dx++) rm = min(rm, quad_score(k, eY[k + dx], eCb[k + dx], e[k + dx]));
+        for (int dx = 1; dx < D; dx++) rm = min(rm, quad_score(k, e[k + dx]));
         if (rm < bk[k]) {
           bk[k] = rm;
           bi[k] = dy;
@@ -766,14 +817,14 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   // keyed forms: the candidate rows unrolled; reference forms: a rolled loop (bounds the live registers)
 #pragma unroll
   for (int dy = 0; dy < (KEYED ? D : 0); dy++) {
-    SEnt e[NE];
-    lds_run<NE>(&lds[iy + dy][ix], e);
+    E e[NE];
+    lds_run<NE>(&lds[iy + dy][lds_col<E>(ix)], e);
     if (MODE == kSearchDot) {
 #pragma unroll
       for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
-          const SEnt& c = e[k + dx];
+          const E& c = e[k + dx];
           // multiply-add into a fresh register first: v_dot2c accumulates in place
           int acc = __mul24(m2[k], c.z) + c.x;
           acc = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, m0[k]), __builtin_bit_cast(short2_t, c.y), acc,
@@ -801,13 +852,13 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   if (!KEYED && MODE != kSearchQuad) {
 #pragma unroll 1
     for (int dy = 0; dy < D; dy++) {
-      SEnt e[NE];
-      lds_run<NE>(&lds[iy + dy][ix], e);
+      E e[NE];
+      lds_run<NE>(&lds[iy + dy][lds_col<E>(ix)], e);
 #pragma unroll
       for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int dx = 0; dx < D; dx++) {
-          const SEnt& c = e[k + dx];
+          const E& c = e[k + dx];
           int v;
           if (MODE == kSearchWide24D) {
             // |2 dY| < 2^23: exact 24-bit operands, products mod 2^32 as the reference's int32
@@ -835,23 +886,36 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
         }
     }
   }
+  if constexpr (MODE == kSearchQuad) {
+    // the test samples again (not kept in registers through the search)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    E tq[4];
+    lds_entries<R, 4>(&tlds[iy + R][lds_col<E>(ix)], tq);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      t0[k] = tq[k].y + g0;
+      t1[k] = tq[k].z + g1;
+      t2[k] = tq[k].w + g2;
+    }
+  }
   int idx[4];
   if (MODE == kSearchQuad) {
     // the first column of the best row that holds the minimum
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      SEnt e[D];
+      E e[D];
       const int by = bi[k];
       switch (k) {
-        case 0: lds_entries<0, D>(&lds[iy + by][ix], e); break;
-        case 1: lds_entries<1, D>(&lds[iy + by][ix], e); break;
-        case 2: lds_entries<2, D>(&lds[iy + by][ix], e); break;
-        default: lds_entries<3, D>(&lds[iy + by][ix], e); break;
+        case 0: lds_entries<0, D>(&lds[iy + by][lds_col<E>(ix)], e); break;
+        case 1: lds_entries<1, D>(&lds[iy + by][lds_col<E>(ix)], e); break;
+        case 2: lds_entries<2, D>(&lds[iy + by][lds_col<E>(ix)], e); break;
+        default: lds_entries<3, D>(&lds[iy + by][lds_col<E>(ix)], e); break;
       }
       int bx = 0;
 #pragma unroll
       for (int dx = D - 1; dx >= 0; dx--) {
-        const int v = quad_score(k, (int)((uint32_t)e[dx].y & 0xffffu), (int)((uint32_t)e[dx].y >> 16), e[dx]);
+        const int v = quad_score(k, e[dx]);
         bx = v == bk[k] ? dx : bx;
       }
       idx[k] = by * D + bx;
@@ -868,13 +932,17 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
   for (int k = 0; k < 4; k++) nowin[k] = MODE == kSearchQuad ? bk[k] == INT_MAX : (!KEYED && bi[k] < 0);
 
   // Epilogue: the winners' samples, squared differences, row sums, SCP, winner map.
-  // The winner of pixel k, candidate idx = ddy * D + ddx, is the entry at byte
-  // offset idx * 12 + ddy * (LP - D) * 12 from the pixel's own tile position.
-  auto winner = [&](int k) -> SEnt {
+  // The winner of pixel k, candidate idx = ddy * D + ddx, is region entry (iy + ddy, ix + k + ddx): for
+  // 12-byte entries the byte offset (idx + ddy * (LP - D)) * 12 from the pixel's own tile position.
+  auto winner = [&](int k) -> E {
     const uint32_t u = (uint32_t)idx[k];
     const uint32_t ddy = u / (uint32_t)D;
-    const uint32_t off = __umul24(u, 12u) + __umul24(ddy, (uint32_t)((LP - D) * 12));
-    return *reinterpret_cast<const SEnt*>(reinterpret_cast<const char*>(&lds[iy][ix + k]) + off);
+    if constexpr (std::is_same<E, SEnt4>::value) {
+      return lds[iy + (int)ddy][lds_col<E>(ix + k + (int)(u - ddy * (uint32_t)D))];
+    } else {
+      const uint32_t off = __umul24(u, 12u) + __umul24(ddy, (uint32_t)((LP - D) * 12));
+      return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(&lds[iy][ix + k]) + off);
+    }
   };
   // Pixels outside the picture (tiles on the right and bottom edges) add zero.
   const bool edge = x0 + kSearchTileW > W || y0 + 16 > H;  // block-uniform
@@ -887,7 +955,7 @@ __device__ __forceinline__ void search_tile(const PairDev& Pr, int x0, int y0,
     uint32_t yc[4], cr[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const SEnt e = winner(k);
+      const E e = winner(k);
       yc[k] = MODE == kSearchDot ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, e.y) >> (unsigned short)KB)
                                  : (uint32_t)e.y;
       cr[k] = (uint32_t)e.z;
@@ -1050,7 +1118,8 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
                                                 uint32_t* __restrict__ rows, uint16_t* __restrict__ scp,
                                                 uint8_t* __restrict__ best) {
   using TT = SearchTile<R>;
-  __shared__ __attribute__((aligned(16))) SEnt lds[2][TT::LH][TT::LP];
+  using E = typename std::conditional<FORM == 2, SEnt4, SEnt>::type;
+  __shared__ __attribute__((aligned(16))) E lds[2][TT::LH][LdsPitch<E, TT::LP>::v];
   // XCD-aware order: workgroups are handed to the kXcds XCDs in turn, so XCD g
   // runs tiles g * per, g * per + 1, ...: each XCD walks a contiguous run of
   // tiles, whose halos overlap in its L2.
@@ -1065,7 +1134,7 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
   const int ty = lt / ntx, tx = lt - ty * ntx;
   const int x0 = tx * TT::TW, y0 = ty * TT::TH;
 #define IVQM_TILE(T, MODE) search_tile<T, R, MODE>(P, x0, y0, lds, wY, wCb, wCr, gcd, rows, scp, best)
-  if (FORM == 0) {
+  if constexpr (FORM == 0) {
     if (P.p0.bps == 2) {
       if (P.mode == kSearchDot) IVQM_TILE(uint16_t, kSearchDot);
       else IVQM_TILE(uint16_t, kSearchNarrow);
@@ -1073,7 +1142,7 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
       if (P.mode == kSearchDot) IVQM_TILE(uint8_t, kSearchDot);
       else IVQM_TILE(uint8_t, kSearchNarrow);
     }
-  } else if (FORM == 2) {
+  } else if constexpr (FORM == 2) {
     if (P.p0.bps == 2) IVQM_TILE(uint16_t, kSearchQuad);
     else IVQM_TILE(uint8_t, kSearchQuad);
   } else {
