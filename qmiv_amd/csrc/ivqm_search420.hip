@@ -129,8 +129,9 @@ __device__ __forceinline__ uint32_t swap16_sum(uint32_t x, uint32_t y) {
 }
 
 template <typename T, int R, int TH>
-__device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0, Lds420<R, TH>& L, int wY, int wCb,
-                                               int wCr, const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
+__device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev* __restrict__ pe, int x0, int y0,
+                                               Lds420<R, TH>& L, int wY, int wCb, int wCr,
+                                               const int32_t* __restrict__ gcd, uint32_t* __restrict__ rows,
                                                uint16_t* __restrict__ scp, uint8_t* __restrict__ best) {
   using P = S420<R, TH>;
   constexpr int D = P::D, NE = P::NE, NC = P::NC, LH = P::LH, KB = P::KB, YB = P::YB, FB = P::FB, R1 = P::R1;
@@ -312,15 +313,17 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
   // keeping them in registers through the search (bounds the live registers)
   __builtin_amdgcn_sched_barrier(0);
   asm volatile("" ::: "memory");
+  // The epilogue reads its pair fields (pe) here, after the search: loaded with the fields the fill needs
+  // they would stay live through it and be spilled to vector-register lanes (VALU reloads).
   // ---- epilogue, one pixel row of the lane at a time: winners, squared differences, row sums, winner map,
   // SCP / fused moments ----
   // The winner of pixel (iy0 + r, ix + k) is region entry (iy0 + r + ddy, ix + k + ddx) with
   // u = ddy * DP + ddx = (key - pixel constant) mod Sk; its Y and chroma words sit at the entry's offset
   // 4 (u + (LPK - DP) (u >> LOGDP)) from the pixel's own entry.
   const bool edge = x0 + 64 > W || y0 + TH > H;  // block-uniform
-  const bool want_scp = Pr.scp_off >= 0, want_mom = Pr.mom_off >= 0;  // block-uniform
+  const bool want_scp = pe->scp_off >= 0, want_mom = pe->mom_off >= 0;  // block-uniform
   const int gc[3] = {g0, g1, g2};
-  const u16x2 mx = (unsigned short)Pr.maxv;
+  const u16x2 mx = (unsigned short)pe->maxv;
   // test chroma + g of pixel pair h, per component, in both halves (shared by both rows of the lane)
   uint32_t tcp[2][2];
 #pragma unroll
@@ -386,15 +389,15 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
     }
     dpp_row16_add3(q);
     if ((lane & 15) == 15 && y < H) {
-      uint32_t* rsp = rows + Pr.rows_off + (size_t)dir * 3 * H + y;
+      uint32_t* rsp = rows + pe->rows_off + (size_t)dir * 3 * H + y;
       atomicAdd(rsp, q[0]);
       atomicAdd(rsp + H, q[1]);
       atomicAdd(rsp + 2 * H, q[2]);
     }
-    if (Pr.best_off >= 0 && y < H) {
+    if (pe->best_off >= 0 && y < H) {
       typedef __attribute__((address_space(1))) uint8_t GU8;
       const int x = x0 + ix;
-      GU8* bm = (GU8*)best + Pr.best_off + (size_t)dir * W * H + (size_t)y * W + x;
+      GU8* bm = (GU8*)best + pe->best_off + (size_t)dir * W * H + (size_t)y * W + x;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const uint32_t u = min((uint32_t)(bk[r][k] - ((iy0 + r) * DP + ix + k)) & ((1u << KB) - 1u), (uint32_t)UMAX);
@@ -416,9 +419,9 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
       if (want_scp && y < H) {
         typedef __attribute__((address_space(1))) uint8_t GU8;
         typedef __attribute__((address_space(1))) u32x2 GU32x2;
-        const size_t plane_bytes = (size_t)Pr.scp_pitch * H * 2;
-        GU8* pic = (GU8*)(scp + Pr.scp_off) + (size_t)(1 - dir) * 3 * plane_bytes;
-        const uint32_t eo = ((uint32_t)y * (uint32_t)Pr.scp_pitch + (uint32_t)(x0 + ix)) * 2u;
+        const size_t plane_bytes = (size_t)pe->scp_pitch * H * 2;
+        GU8* pic = (GU8*)(scp + pe->scp_off) + (size_t)(1 - dir) * 3 * plane_bytes;
+        const uint32_t eo = ((uint32_t)y * (uint32_t)pe->scp_pitch + (uint32_t)(x0 + ix)) * 2u;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const u32x2 o = {__builtin_bit_cast(uint32_t, sv[c][0]), __builtin_bit_cast(uint32_t, sv[c][1])};
@@ -461,12 +464,12 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, int x0, int y0
     e2[1] = swap16_sum(m[2][2], m[2][3]);
     const int odd = rp & 1;
     const int sby = (y0 >> 2) + 2 * wd + (rp >> 1), sbx = (x0 >> 2) + (lane & 15);
-    if (sby < Pr.mom_rows && sbx < Pr.mom_cols) {
+    if (sby < pe->mom_rows && sbx < pe->mom_cols) {
       typedef __attribute__((address_space(1))) u32x4 GU32x4;
       typedef __attribute__((address_space(1))) uint32_t GU32;
-      GU32x4* base = (GU32x4*)(scp + Pr.mom_off);
-      const int64_t cstride = (int64_t)Pr.mom_rows * Pr.mom_pitch;
-      const int64_t sb = (int64_t)sby * Pr.mom_pitch + sbx;
+      GU32x4* base = (GU32x4*)(scp + pe->mom_off);
+      const int64_t cstride = (int64_t)pe->mom_rows * pe->mom_pitch;
+      const int64_t sb = (int64_t)sby * pe->mom_pitch + sbx;
       const u32x4 fin = {f[0], f[1], f[2], f[3]};
       __builtin_nontemporal_store(fin, base + ((int64_t)dir * 3 + odd) * cstride + sb);  // read once, by the SSIM pass
       GU32* c2p = (GU32*)(base + ((int64_t)dir * 3 + 2) * cstride + sb);
@@ -493,9 +496,9 @@ __global__ __launch_bounds__(16 * TH) void k_search420(const PairDev* __restrict
   const int ntx = (P.w + 63) / 64;
   const int ty = lt / ntx, tx = lt - ty * ntx;
   if (P.p0.bps == 2)
-    search420_tile<uint16_t, R, TH>(P, tx * 64, ty * TH, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    search420_tile<uint16_t, R, TH>(P, pairs + pi, tx * 64, ty * TH, lds, wY, wCb, wCr, gcd, rows, scp, best);
   else
-    search420_tile<uint8_t, R, TH>(P, tx * 64, ty * TH, lds, wY, wCb, wCr, gcd, rows, scp, best);
+    search420_tile<uint8_t, R, TH>(P, pairs + pi, tx * 64, ty * TH, lds, wY, wCb, wCr, gcd, rows, scp, best);
 }
 
 template <int R, int TH>
