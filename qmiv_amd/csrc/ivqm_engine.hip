@@ -626,8 +626,22 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
           }
         }
         if (mp & IVQM_M_IVMSSSIM) {
+          // Level 0 of IV-MS-SSIM reads the IV-SSIM jobs' windows, (P0, SCP1) and (P1, SCP0): twins of
+          // those jobs as above (the ERP weights of IV-SSIM apply at the frame totals, not to partials).
+          auto twin_iv = [&](int twin_job, int iv_job) {
+            if (l != 0 || cl != 0 || iv_job < 0 || P.ssim_gen) return;
+            SsimJobDev& T = P.jobs[twin_job];
+            SsimJobDev& S = P.jobs[iv_job];
+            if (S.mom_off < 0 && S.twin_part_off < 0 && T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk &&
+                T.wsq == S.wsq && T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
+              P.ssim_blocks = T.blk0;
+              S.twin_part_off = T.part_off;
+            }
+          };
           RCCHK(add_job(i, &pp.ivms_t2r[l], ref_for(i, 0, l), ref_for(i, 3, l), w, h, cl, 0));
+          twin_iv(pp.ivms_t2r[l], pp.iv_t2r);
           RCCHK(add_job(i, &pp.ivms_r2t[l], ref_for(i, 1, l), ref_for(i, 2, l), w, h, cl, 0));
+          twin_iv(pp.ivms_r2t[l], pp.iv_r2t);
         }
       }
     }
