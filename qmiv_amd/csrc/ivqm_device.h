@@ -63,7 +63,11 @@ struct PairDev {
   // considered candidate's key without wrapping), or 0 for the reference-form masked search.
   int32_t mkey;
   int32_t mdot;  // keyed masked search with the packed int16 {Y, Cb} key (host-checked ranges)
-  int32_t pad3;
+  int32_t scp1_pitch;
+  // Fused IV-MS-SSIM (k_search420 with the moments): the search also writes MS-SSIM level 1 of both SCP
+  // pictures, [SCP0, SCP1][component][h >> 1][scp1_pitch] uint16 at scp1_off in the SCP arena
+  // (direction d writes SCP 1 - d); -1: off.
+  int64_t scp1_off;
 };
 // Largest bit depth of the fused IV-SSIM moments: 16 * 4095 fits the 16-bit halves of S_a | S_b << 16
 // and 64 * 4095^2 the 32-bit window sums.
@@ -97,8 +101,8 @@ struct SsimJobDev {
   const uint32_t* mom;
   int64_t mom_off;                // offset in the SCP arena (uint16 units) the host turns into `mom`
   int32_t mom_pitch, mom_rows;
-  // Block-averaged 8 / 4 picture jobs: partials of a twin job over the same windows without the
-  // luminance term (MS-SSIM level 0 beside SSIM), written by this job; -1: none.
+  // Block-averaged 8 / 4 jobs: partials of a twin job over the same windows without the luminance term
+  // (MS-SSIM level 0 beside SSIM, IV-MS-SSIM level 0 beside IV-SSIM), written by this job; -1: none.
   int64_t twin_part_off;
 };
 

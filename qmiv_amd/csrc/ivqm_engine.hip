@@ -344,7 +344,10 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
   // Fused IV-SSIM (block-averaged 8 / 4): the search writes 4x4 sub-block moments
   // instead of the SCP pictures when nothing else reads them -- batches of two or
   // more pairs (a single pair keeps its SCP for ivqm_get_scp and the IV-SSIM
-  // entry points that reuse it), no IV-MS-SSIM pyramid, 32-bit sums.
+  // entry points that reuse it), 32-bit sums. With IV-MS-SSIM (k_search420 pairs)
+  // the search also writes MS-SSIM level 1 of the SCP pictures: level 0 of
+  // IV-MS-SSIM reads the IV-SSIM windows (a twin of the moment job), the pyramid
+  // continues from level 1, and no full-size SCP picture is written.
   const bool fuse_ok = n > 1 && !reuse && !scp_only && !P.ssim_gen && !cfg_.ssim_wide_sums;
   for (int i = 0; i < n; i++) {
     const PairSpec& s = pairs[i];
@@ -423,12 +426,23 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
     }
     if ((int64_t)d.scp_pitch * s.h * 2 >= (1ll << 32)) return set_error(IVQM_EINVAL, "picture plane of 4 GiB or more");
-    // the keyed search variants (1 narrow, 2 dot) write the moments
-    const bool fused = fuse_ok && (mp & IVQM_M_IVSSIM) && !(mp & IVQM_M_IVMSSSIM) && !s.msk &&
+    // the keyed search variants (1 narrow, 2 dot) write the moments; k_search420 also the SCP level 1
+    const bool fused = fuse_ok && (mp & IVQM_M_IVSSIM) &&
+                       (!(mp & IVQM_M_IVMSSSIM) || (d.mode == kSearchDot420 && ms_num_scales(s.w, s.h) > 1)) &&
+                       !s.msk &&
                        s.bd <= kMomMaxBits && (d.mode == 1 || d.mode == 2 || d.mode == kSearchDot420) &&
                        num_blocks(s.w, 8, 4) > 0 &&
                        num_blocks(s.h, 8, 4) > 0;
     d.mom_off = -1;
+    d.scp1_off = -1;
+    d.scp1_pitch = 0;
+    pp.ns = ms_num_scales(s.w, s.h);
+    if (fused && (mp & IVQM_M_IVMSSSIM) && pp.ns > 1) {
+      d.scp1_pitch = scp_pitch(s.w >> 1);
+      P.scp_total = (P.scp_total + 127) & ~(int64_t)127;
+      d.scp1_off = P.scp_total;
+      P.scp_total += 2 * 3 * (int64_t)d.scp1_pitch * (s.h >> 1);
+    }
     if (fused) {
       // sub-blocks 0 .. num_blocks of each axis: those the 8 / 4 windows cover, all inside the picture
       d.scp_off = -1;
@@ -447,7 +461,6 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     pp.scp_off = d.scp_off;
     pp.fused = fused;
     d.best_off = (best_on_ && n == 1) ? 0 : -1;
-    pp.ns = ms_num_scales(s.w, s.h);
     if (cfg_.erp) pp.erp_h = s.h;
     std::snprintf(sigbuf, sizeof sigbuf, "%dx%d:%d:%d:%d:%x:%d|", s.w, s.h, s.bd, chroma_format(s.p0), s.p0.bps, mp,
                   s.msk ? s.msk_bps : 0);
@@ -463,6 +476,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     for (int which = 0; which < 4; which++) {
       if (which >= 2 && !(mp & IVQM_M_IVMSSSIM)) continue;
       for (int l = 1; l < pp.ns; l++) {
+        if (which >= 2 && l == 1 && P.pairs[i].scp1_off >= 0) continue;  // written by the search
         pyr[i][which][l] = P.pyr_total;
         P.pyr_total += 3 * (int64_t)scp_pitch(pp.w >> l) * (pp.h >> l);
       }
@@ -479,9 +493,13 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       r.arena = 0;
       r.user = which == 0 ? d.p0 : d.p1;
     } else if (l == 0) {
-      r.arena = 1;
+      r.arena = d.scp_off >= 0 ? 1 : 3;  // fused pairs have no full-size SCP (their jobs read moments)
       r.pitch = d.scp_pitch;
       r.off = d.scp_off + (int64_t)(which - 2) * 3 * d.scp_pitch * d.h;
+    } else if (l == 1 && which >= 2 && d.scp1_off >= 0) {  // SCP level 1 from the search
+      r.arena = 1;
+      r.pitch = d.scp1_pitch;
+      r.off = d.scp1_off + (int64_t)(which - 2) * 3 * d.scp1_pitch * (d.h >> 1);
     } else {
       r.arena = 2;
       r.pitch = scp_pitch(d.w >> l);
@@ -507,6 +525,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     for (int which = 0; which < 4; which++) {
       if (which >= 2 && !(mp & IVQM_M_IVMSSSIM)) continue;
       for (int l = 1; l < pp.ns; l++) {
+        if (which >= 2 && l == 1 && P.pairs[i].scp1_off >= 0) continue;  // written by the search
         DownJob dj;
         std::memset(&dj, 0, sizeof(dj));
         dj.w = pp.w >> (l - 1);
@@ -628,20 +647,25 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         if (mp & IVQM_M_IVMSSSIM) {
           // Level 0 of IV-MS-SSIM reads the IV-SSIM jobs' windows, (P0, SCP1) and (P1, SCP0): twins of
           // those jobs as above (the ERP weights of IV-SSIM apply at the frame totals, not to partials).
+          bool twin_fail = false;
           auto twin_iv = [&](int twin_job, int iv_job) {
             if (l != 0 || cl != 0 || iv_job < 0 || P.ssim_gen) return;
             SsimJobDev& T = P.jobs[twin_job];
             SsimJobDev& S = P.jobs[iv_job];
-            if (S.mom_off < 0 && S.twin_part_off < 0 && T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk &&
+            if (S.twin_part_off < 0 && T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk &&
                 T.wsq == S.wsq && T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
               P.ssim_blocks = T.blk0;
               S.twin_part_off = T.part_off;
+            } else if (S.mom_off >= 0) {
+              twin_fail = true;  // a fused pair has no SCP picture to read
             }
           };
           RCCHK(add_job(i, &pp.ivms_t2r[l], ref_for(i, 0, l), ref_for(i, 3, l), w, h, cl, 0));
           twin_iv(pp.ivms_t2r[l], pp.iv_t2r);
           RCCHK(add_job(i, &pp.ivms_r2t[l], ref_for(i, 1, l), ref_for(i, 2, l), w, h, cl, 0));
           twin_iv(pp.ivms_r2t[l], pp.iv_r2t);
+          if (twin_fail || (l == 0 && pp.fused && cl != 0))
+            return set_error(IVQM_EINVAL, "internal: fused IV-MS-SSIM level 0 without its IV-SSIM twin");
         }
       }
     }

@@ -1305,7 +1305,7 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         float num = 2.0f * i64_to_f32(cov) + c2n;
         float den = i64_to_f32(va) + i64_to_f32(vb) + c2n;
         // the twin's score is this job's without the luminance term: the same operations as its own job
-        if (!MOM && J.twin_part_off >= 0) s_cs = (double)(num * __builtin_amdgcn_rcpf(den));
+        if (J.twin_part_off >= 0) s_cs = (double)(num * __builtin_amdgcn_rcpf(den));
         if (J.calc_l) {  // luminance term: positive terms only, FP32 products suffice
           const float fa = (float)sA, fb = (float)sB;
           num *= 2.0f * fa * fb + c1n;
@@ -1315,7 +1315,7 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
       }
       s = wave_sum_f64_dpp(s);
       if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
-      if (!MOM && J.twin_part_off >= 0) {  // block-uniform
+      if (J.twin_part_off >= 0) {  // block-uniform
         s_cs = wave_sum_f64_dpp(s_cs);
         if (lane == 0) partial[J.twin_part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s_cs;
       }

@@ -161,6 +161,7 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
     d.scp_off = -1;
     d.best_off = -1;
     d.mom_off = -1;
+    d.scp1_off = -1;
   }
   // every offset stays inside its scratch region
   if (rowsd_off > 3 * (int64_t)kMaxRows || rows_off > 6 * (int64_t)kMaxRows)

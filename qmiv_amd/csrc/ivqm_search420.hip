@@ -337,10 +337,15 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   // 2h, 2h + 1), S_aa, S_bb, S_ab; a = the test picture's sample (t - g), b = the SCP sample
   u16x2 sa[3], sb[3];
   uint32_t saa[3], sbb[3], sab[3];
+  // MS-SSIM level 1 of the SCP (pe->scp1_off >= 0, with the moments): each 2 x 2 quad of the lane's
+  // pixels (rows iy0, iy0 + 1, columns ix + 2h, ix + 2h + 1) summed per component as a u16 pair
+  const bool want_scp1 = pe->scp1_off >= 0;  // block-uniform
+  u16x2 qs[3][2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     sa[c] = sb[c] = (unsigned short)0;
     saa[c] = sbb[c] = sab[c] = 0u;
+    qs[c][0] = qs[c][1] = (unsigned short)0;
   }
   constexpr int UMAX = DP * 2 * R + 2 * R;
 #pragma unroll
@@ -441,6 +446,32 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
             sbb[c] = __builtin_amdgcn_udot2(sv[c][h], sv[c][h], sbb[c], false);
             sab[c] = __builtin_amdgcn_udot2(ta, sv[c][h], sab[c], false);
           }
+        if (want_scp1) {
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) qs[c][h] += sv[c][h];
+        }
+      }
+    }
+  }
+  if (want_scp1) {
+    // level 1 = 2 x 2 rounded mean (xPixelOpsSTD.cpp:50-67, odd sizes truncated) at (x1 + h, y1)
+    const int W1 = W >> 1, H1 = H >> 1;
+    const int x1 = (x0 + ix) >> 1, y1 = (y0 + iy0) >> 1;
+    if (y1 < H1 && x1 < W1) {
+      typedef __attribute__((address_space(1))) uint16_t GU16;
+      typedef __attribute__((address_space(1))) uint32_t GU32;
+      const int64_t plane = (int64_t)pe->scp1_pitch * H1;
+      GU16* pic = (GU16*)(scp + pe->scp1_off) + (int64_t)(1 - dir) * 3 * plane + (int64_t)y1 * pe->scp1_pitch + x1;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const uint32_t v0 = ((uint32_t)qs[c][0].x + qs[c][0].y + 2u) >> 2;
+        const uint32_t v1 = ((uint32_t)qs[c][1].x + qs[c][1].y + 2u) >> 2;
+        if (x1 + 1 < W1)  // x1 even, the pitch even: a 4-byte aligned pair
+          __builtin_nontemporal_store(v0 | (v1 << 16), (GU32*)(pic + c * plane));
+        else
+          pic[c * plane] = (uint16_t)v0;
       }
     }
   }

@@ -102,3 +102,41 @@ def test_fused_ivssim_against_oracle(oracle):
         err = max(abs(r["ivssim"] - iv), abs(r["ivssim_r2t"] - ir), abs(r["ivssim_t2r"] - it))
         print(wl.width, wl.height, bd, "fused IV-SSIM |err| vs FP64 oracle", f"{err:.2e}", flush=True)
         assert err <= SSIM_TOL
+
+
+# IV-MS-SSIM in fused batches (k_search420 pairs: 4:2:0 up to 12-bit, keyed): the search writes the moments
+# and MS-SSIM level 1 of both SCP pictures; level 0 is the IV-SSIM jobs' twin. Odd sizes from level 1 on
+# truncate (xSSIM.cpp:65).
+PAIRS_MS = PAIRS + [synth.Workload(cfg=606, width=204, height=130, block_shift=2),
+                    synth.Workload(cfg=607, width=588, height=530, bit_depth=8, global_shift=1)]
+IVMS = IV | Q.IVQM_M_IVMSSSIM
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("metrics", [IVMS, Q.IVQM_M_ALL], ids=["ivms", "all"])
+def test_fused_ivmsssim_equals_single_pair_path(metrics, use_graph):
+    host = _host(PAIRS_MS)
+    want = [_single(wl, p0, p1, metrics) for wl, p0, p1 in host]
+    for rep, got in enumerate(_batch(host, metrics, use_graph=use_graph, reps=2)):
+        for i, (g, w) in enumerate(zip(got, want)):
+            print(PAIRS_MS[i].width, PAIRS_MS[i].height, PAIRS_MS[i].bit_depth, "IV-MS-SSIM", g["ivmsssim"],
+                  flush=True)
+            assert g == w, (rep, i, PAIRS_MS[i])
+
+
+def test_fused_ivmsssim_against_oracle(oracle):
+    host = _host([PAIRS_MS[1], PAIRS_MS[3], PAIRS_MS[6], PAIRS_MS[7]])
+    got = _batch(host, IVMS, erp=1)[0]
+    for (wl, p0n, p1n), r in zip(host, got):
+        bd = wl.bit_depth
+        p0 = oracle.to444([np.asarray(p, np.uint16) for p in p0n], wl.chroma_420)
+        p1 = oracle.to444([np.asarray(p, np.uint16) for p in p1n], wl.chroma_420)
+        g = oracle.gcd(p0, p1, bd, wrap32=False)
+        _, scp1 = oracle.cps(p0, p1, bd, g, R=2, want_scp=True)[:2]
+        _, scp0 = oracle.cps(p1, p0, bd, [-x for x in g], R=2, want_scp=True)[:2]
+        iv = oracle.ivssim(p0, p1, scp0, scp1, bd, erp=True)[0]
+        ivms = oracle.ivmsssim(p0, p1, scp0, scp1, bd)
+        errs = (abs(r["ivssim"] - iv), abs(r["ivmsssim"] - ivms))
+        print(wl.width, wl.height, bd, "fused IV-SSIM / IV-MS-SSIM |err| vs FP64 oracle",
+              " ".join(f"{e:.2e}" for e in errs), flush=True)
+        assert max(errs) <= SSIM_TOL
