@@ -141,7 +141,8 @@ __host__ __device__ inline uint32_t search_form_bit(int mode) {
 hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                          const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, uint32_t forms,
                          hipStream_t st);
-// forms: bit 0 when jobs read pictures, bit 1 when jobs read fused moments (one launch each).
+// forms: bit 0 when jobs read pictures with 32-bit sums (wsq 0), bit 2 when jobs read pictures with wider
+// sums, bit 1 when jobs read fused moments (one launch each).
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
                        hipStream_t st);
 // Any window mode / size / stride / border extension: one wave per 64 window columns and

@@ -310,7 +310,13 @@ int Engine::erp_offset(int h, int64_t* off) {
 // Which k_ssim launches a plan needs: bit 0 jobs that read pictures, bit 1 jobs that read fused moments.
 static uint32_t ssim_forms(const BatchPlan& P) {
   uint32_t f = 0;
-  for (const SsimJobDev& J : P.jobs) f |= J.mom_off >= 0 ? 2u : 1u;
+  // jobs own consecutive block ranges in table order; twins own none and launch nothing
+  const size_t nj = P.jobs.size();
+  for (size_t j = 0; j < nj; j++) {
+    const SsimJobDev& J = P.jobs[j];
+    const int nb = (j + 1 < nj ? P.jobs[j + 1].blk0 : P.ssim_blocks) - J.blk0;
+    if (nb > 0) f |= J.mom_off >= 0 ? 2u : (J.wsq == 0 ? 1u : 4u);
+  }
   return f;
 }
 

@@ -1353,11 +1353,13 @@ __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, i
 #undef IVQM_SSIM
 }
 
-// MOM: this launch runs the jobs that read fused moments (20 VGPRs, full occupancy), the other launch
-// the jobs that read pictures; blocks of the other kind exit at once.
-template <int NB, bool MOM>
+// One launch per job form, so no form's register allocation is imposed on another: FORM 0 the jobs
+// that read pictures with 32-bit sums (up to 12-bit samples), 1 those with wider sums, 2 the jobs that
+// read fused moments; blocks of another form exit at once.
+template <int NB, int FORM>
 __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
                                               double* __restrict__ partial) {
+  constexpr bool MOM = FORM == 2;
   // XCD-aware order (as in k_search): XCD g runs blocks g * per, g * per + 1, ...,
   // so neighbouring chunks and band groups, which share cache lines of their
   // rows, run on one XCD close together and meet in its L2.
@@ -1367,14 +1369,15 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
   const SsimJobDev J = jobs[ji];
   if ((J.mom != nullptr) != MOM) return;  // block-uniform
+  if (!MOM && (J.wsq == 0) != (FORM == 0)) return;
   const int lb = b - J.blk0;
   const int per_c = J.nchunk * J.nbg;
   const int c = lb / per_c, rem = lb - c * per_c;
   const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
   // band is wave-uniform: row pointers stay scalar and every load adds only the lane's column
   const int lane = threadIdx.x & 63, band = bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (MOM) ssim_wave<uint16_t, 0, uint16_t, 0, 0, NB, true>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
-  else if (J.wsq == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  if constexpr (FORM == 2) ssim_wave<uint16_t, 0, uint16_t, 0, 0, NB, true>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  else if constexpr (FORM == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
   else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
 }
@@ -1643,13 +1646,13 @@ hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, co
   }
 }
 
-template <bool MOM>
+template <int FORM>
 static hipError_t ssim_launch(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st) {
   const int grid = kXcds * ((nblk + kXcds - 1) / kXcds);
   switch (nb) {
-    case 4: hipLaunchKernelGGL((k_ssim<4, MOM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
-    case 8: hipLaunchKernelGGL((k_ssim<8, MOM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
-    case 16: hipLaunchKernelGGL((k_ssim<16, MOM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 4: hipLaunchKernelGGL((k_ssim<4, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 8: hipLaunchKernelGGL((k_ssim<8, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 16: hipLaunchKernelGGL((k_ssim<16, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1659,10 +1662,14 @@ hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, doub
                        hipStream_t st) {
   if (njobs <= 0 || nblk <= 0) return hipSuccess;
   if (forms & 1u) {
-    const hipError_t e = ssim_launch<false>(jobs, njobs, nblk, nb, partial, st);
+    const hipError_t e = ssim_launch<0>(jobs, njobs, nblk, nb, partial, st);
     if (e != hipSuccess) return e;
   }
-  if (forms & 2u) return ssim_launch<true>(jobs, njobs, nblk, nb, partial, st);
+  if (forms & 4u) {
+    const hipError_t e = ssim_launch<1>(jobs, njobs, nblk, nb, partial, st);
+    if (e != hipSuccess) return e;
+  }
+  if (forms & 2u) return ssim_launch<2>(jobs, njobs, nblk, nb, partial, st);
   return hipSuccess;
 }
 
