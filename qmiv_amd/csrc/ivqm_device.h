@@ -146,8 +146,20 @@ hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, co
 hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
                        hipStream_t st);
 // Any window mode / size / stride / border extension: one wave per 64 window columns and
-// kSsimGenRows window rows, 4 waves per block.
-constexpr int kSsimGenRows = 8;
+// kSsimGenRows window rows, 4 waves per block. Stride-4 windows of 8, 11 or 16 samples without a border
+// extension (regular: window 11; block: BlockGaussianInt and BlockAveraged other than 8 / 4) run the
+// rolling-row kernel instead: 64 - (N - 1) / 4 window columns and kSsimRollRows window rows per wave.
+constexpr int kSsimGenRows = 8, kSsimRollRows = 16;
+struct SsimGenShape {
+  int cols, rows;  // window columns and window rows of one wave
+  bool roll;
+};
+__host__ __device__ inline SsimGenShape ssim_gen_shape(int mode, int wnd, int ext, int stride) {
+  const bool regular = mode <= 2;
+  const int n = regular ? 11 : wnd;
+  if (stride == 4 && (regular ? ext == 0 : (n == 8 || n == 16))) return SsimGenShape{64 - (n - 1) / 4, kSsimRollRows, true};
+  return SsimGenShape{64, kSsimGenRows, false};
+}
 hipError_t launch_ssim_gen(const SsimJobDev* jobs, int njobs, int nblk, int mode, int wnd, int ext, int stride,
                            double* partial, hipStream_t st);
 hipError_t launch_ssim_finalize(const SsimJobDev* jobs, int njobs, const double* partial, const double* erp_w,

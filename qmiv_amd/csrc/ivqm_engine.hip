@@ -574,8 +574,9 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     if (J.nux <= 0 || J.nuy <= 0)
       return set_error(IVQM_EINVAL, "picture (or MS-SSIM scale) smaller than the SSIM window");
     if (P.ssim_gen) {
-      J.nchunk = (J.nux + 63) / 64;
-      J.nbg = (J.nuy + 4 * kSsimGenRows - 1) / (4 * kSsimGenRows);
+      const SsimGenShape sh = ssim_gen_shape(cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, str);
+      J.nchunk = (J.nux + sh.cols - 1) / sh.cols;
+      J.nbg = (J.nuy + 4 * sh.rows - 1) / (4 * sh.rows);
     } else {
       J.nchunk = (J.nux + 62) / 63;
       J.nbg = ((J.nuy + P.nb - 1) / P.nb + 3) / 4;

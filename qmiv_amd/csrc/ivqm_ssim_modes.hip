@@ -247,6 +247,165 @@ __global__ __launch_bounds__(256) void k_ssim_static(const SsimJobDev* __restric
     ssim_static_wave<uint8_t, uint16_t, S, N, WEIGHTED, EXT>(J, J.b, J.a, c, chunk, wr0, lane, partial);
 }
 
+// ---------------------------------------------------------------------------
+// k_ssim_roll: stride-4 windows of N = 8, 11 or 16 samples without a border extension (regular 11 x 11
+// centred windows from column 5, block windows from column 0: window column wx starts at sample 4 wx).
+// Lane l reads the 4 samples 4 l .. 4 l + 3 of a row with one vector load and takes the next
+// N - 4 samples from lanes l + 1, l + 2, ... with DPP wave shifts, so the wave's windows are 64 - (N - 1) / 4
+// columns (the last lanes only supply samples; ssim_gen_shape). The wave walks kSsimRollRows window rows
+// top to bottom, 4 sample rows per step: a step's rows feed the JW = ceil(N / 4) windows that overlap them,
+// held as rolling accumulators (window q - j in slot j at step q), and after each step the oldest window is
+// complete: its score joins its window row's partial sum. Each sample row is read once per wave.
+// Moments in FP64 as in k_ssim_gen (integer weights keep every sum exact; averaged windows add with weight
+// 1, and a row outside a window adds its samples with weight 0, which leaves the sums unchanged).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_next(uint32_t v) {  // lane l + 1's value (wave_shl:1)
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+// Samples 4 g .. 4 g + 3 of a row (luma-grid columns; chroma at x >> cs), clamped to the row's last sample.
+template <typename T>
+__device__ __forceinline__ void roll_load4(const __attribute__((address_space(1))) T* row, int g4, int cs, int wlast,
+                                           uint32_t v[4]) {
+  typedef __attribute__((address_space(1))) const T GT;
+  if (g4 + 3 <= wlast) {
+    if (cs == 0) {
+      if (sizeof(T) == 2) {
+        typedef unsigned short u16x4r __attribute__((ext_vector_type(4)));
+        const u16x4r u = *(const __attribute__((address_space(1))) u16x4r*)(row + g4);
+        v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+      } else {
+        typedef unsigned char u8x4r __attribute__((ext_vector_type(4)));
+        const u8x4r u = *(const __attribute__((address_space(1))) u8x4r*)(row + g4);
+        v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+      }
+    } else {
+      const uint32_t a = row[g4 >> cs], b = row[(g4 >> cs) + 1];
+      v[0] = a; v[1] = cs == 1 ? a : b; v[2] = cs == 1 ? b : row[(g4 >> cs) + 2]; v[3] = cs == 1 ? b : row[(g4 >> cs) + 3];
+    }
+  } else {  // the right edge: samples past the row repeat its last one (only lanes past the windows read them)
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[t] = row[min(g4 + t, wlast) >> cs];
+  }
+}
+
+template <typename TA, typename TB, int N, bool WEIGHTED>
+__device__ void ssim_roll_wave(const SsimJobDev& J, const PicDev& A, const PicDev& B, int c, int chunk, int wr0,
+                               int lane, const double* wl, double* __restrict__ partial) {
+  constexpr int JW = (N + 3) / 4, NG = JW, WPW = 64 - (N - 1) / 4, KR = kSsimRollRows;
+  static_assert(N == 8 || N == 11 || N == 16, "window");
+  const int nk = min(KR, J.nuy - wr0);
+  const int wx = chunk * WPW + lane;
+  const bool active = lane < WPW && wx < J.nux;
+  const int g4 = 4 * wx;  // this lane's first sample column (windows start at 4 wx: see the launch rule)
+  const int ys = J.gy0 + wr0 * 4 - (J.mode <= 2 ? 5 : 0);
+  const int ylast = ys + (nk - 1) * 4 + N - 1;  // last row a live window reads; later rows are clamped to it
+  typedef __attribute__((address_space(1))) const TA GTA;
+  typedef __attribute__((address_space(1))) const TB GTB;
+  GTA* pa = (GTA*)plane_ptr(A, c);
+  GTB* pb = (GTB*)plane_ptr(B, c);
+  const int sa = plane_stride(A, c), sb = plane_stride(B, c);
+  const int csa = c ? A.cs : 0, csb = c ? B.cs : 0;
+  const int rsa = c ? A.csy : 0, rsb = c ? B.csy : 0;
+  const int wlast = J.w - 1;
+  const double K = J.kscale;
+  double mA[JW], mB[JW], mAA[JW], mBB[JW], mAB[JW];
+#pragma unroll
+  for (int j = 0; j < JW; j++) mA[j] = mB[j] = mAA[j] = mBB[j] = mAB[j] = 0.0;
+  const int nq = nk + JW - 1;  // steps: window nk - 1 completes in step nk - 1 + JW - 1
+#pragma unroll 1
+  for (int q = 0; q < nq; q++) {
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+      // one sample row at a time: the weights stay in LDS (read where used, not hoisted into registers)
+      asm volatile("" ::: "memory");
+      const int y = min(ys + 4 * q + r, ylast);
+      uint32_t va[4 * NG], vb[4 * NG];
+      roll_load4<TA>(pa + (size_t)(uint32_t)(y >> rsa) * (uint32_t)sa, g4, csa, wlast, va);
+      roll_load4<TB>(pb + (size_t)(uint32_t)(y >> rsb) * (uint32_t)sb, g4, csb, wlast, vb);
+#pragma unroll
+      for (int m = 1; m < NG; m++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          va[4 * m + t] = wave_next(va[4 * (m - 1) + t]);
+          vb[4 * m + t] = wave_next(vb[4 * (m - 1) + t]);
+        }
+#pragma unroll
+      for (int dx = 0; dx < N; dx++) {
+        const double a = (double)va[dx], b = (double)vb[dx];
+        const double aa = a * a, bb = b * b, ab = a * b;  // exact: integers below 2^32
+#pragma unroll
+        for (int j = 0; j < JW; j++) {
+          // slot j holds window q - j, whose row dy = r + 4 j this is; rows dy >= N carry zero weights
+          const double w = wl[(r + 4 * j) * N + dx];
+          mA[j] = fma(a, w, mA[j]);
+          mB[j] = fma(b, w, mB[j]);
+          mAA[j] = fma(aa, w, mAA[j]);
+          mBB[j] = fma(bb, w, mBB[j]);
+          mAB[j] = fma(ab, w, mAB[j]);
+        }
+        // one column at a time (15 independent accumulations per column keep the FP64 pipe busy; letting
+        // the scheduler hoist the conversions and products of every column raises the live registers)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // window k = q - (JW - 1) is complete
+    const int k = q - (JW - 1);  // wave-uniform
+    if (k >= 0) {
+      double v = 0.0;
+      if (active) {
+        // xStructSimSTD.cpp:104-108 / :176-180 / :399-414: means, variances, covariance, L * CS
+        const double ar = mA[JW - 1] * K, at = mB[JW - 1] * K;
+        const double vr = mAA[JW - 1] * K - ar * ar, vt = mBB[JW - 1] * K - at * at, cv = mAB[JW - 1] * K - ar * at;
+        v = (2 * cv + J.c2) / (vr + vt + J.c2);
+        if (J.calc_l) v = ((2 * ar * at + J.c1) / (ar * ar + at * at + J.c1)) * v;
+      }
+      v = wave_sum_f64_dpp(v);
+      if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wr0 + k) * J.nchunk + chunk] = v;
+    }
+#pragma unroll
+    for (int j = JW - 1; j > 0; j--) {
+      mA[j] = mA[j - 1];
+      mB[j] = mB[j - 1];
+      mAA[j] = mAA[j - 1];
+      mBB[j] = mBB[j - 1];
+      mAB[j] = mAB[j - 1];
+    }
+    mA[0] = mB[0] = mAA[0] = mBB[0] = mAB[0] = 0.0;
+  }
+}
+
+template <int N, bool WEIGHTED>
+__global__ __launch_bounds__(256) void k_ssim_roll(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
+                                                   double* __restrict__ partial) {
+  const int per = (nblk + kXcds - 1) / kXcds;  // XCD-aware order, as in k_ssim
+  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (b >= nblk) return;
+  const int ji = find_entry(jobs, njobs, b, [](const SsimJobDev& j) { return j.blk0; });
+  const SsimJobDev J = jobs[ji];
+  const int lb = b - J.blk0;
+  const int per_c = J.nchunk * J.nbg;
+  const int c = lb / per_c, rem = lb - c * per_c;
+  const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
+  const int lane = threadIdx.x & 63;
+  // The window weights [dy][dx] in LDS (the same table for every job; 1 for averaged windows), padded with
+  // zero rows to 4 ceil(N / 4): a step's row r feeds slot j with weight row r + 4 j, zero past the window.
+  constexpr int NR = 4 * ((N + 3) / 4);
+  __shared__ double wl[NR * N];
+  for (int i = (int)threadIdx.x; i < NR * N; i += 256) wl[i] = i < N * N ? (WEIGHTED ? J.wt[i] : 1.0) : 0.0;
+  __syncthreads();
+  const int wr0 = (bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * kSsimRollRows;
+  if (wr0 >= J.nuy) return;  // whole wave
+  if (J.a.bps == 2 && J.b.bps == 2)
+    ssim_roll_wave<uint16_t, uint16_t, N, WEIGHTED>(J, J.a, J.b, c, chunk, wr0, lane, wl, partial);
+  else if (J.a.bps == 1 && J.b.bps == 1)
+    ssim_roll_wave<uint8_t, uint8_t, N, WEIGHTED>(J, J.a, J.b, c, chunk, wr0, lane, wl, partial);
+  else if (J.a.bps == 1)
+    ssim_roll_wave<uint8_t, uint16_t, N, WEIGHTED>(J, J.a, J.b, c, chunk, wr0, lane, wl, partial);
+  else  // SSIM is symmetric in its two pictures
+    ssim_roll_wave<uint8_t, uint16_t, N, WEIGHTED>(J, J.b, J.a, c, chunk, wr0, lane, wl, partial);
+}
+
 template <int NW, bool WEIGHTED, bool EXT>
 __global__ __launch_bounds__(256) void k_ssim_gen(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
                                                   double* __restrict__ partial) {
@@ -314,6 +473,17 @@ hipError_t launch_ssim_gen(const SsimJobDev* jobs, int njobs, int nblk, int mode
   const dim3 g(kXcds * ((nblk + kXcds - 1) / kXcds)), t(256);
   const bool regular = mode <= 2, weighted = mode == 0 || mode == 1 || mode == 3, ext_on = regular && ext != 0;
   const int n = regular ? 11 : wnd;
+  if (ssim_gen_shape(mode, wnd, ext, stride).roll) {
+#define IVQM_ROLL(N)                                                                                         \
+  if (n == N) {                                                                                              \
+    if (weighted) hipLaunchKernelGGL((k_ssim_roll<N, true>), g, t, 0, st, jobs, njobs, nblk, partial);        \
+    else hipLaunchKernelGGL((k_ssim_roll<N, false>), g, t, 0, st, jobs, njobs, nblk, partial);               \
+    return hipGetLastError();                                                                                \
+  }
+    IVQM_ROLL(8) IVQM_ROLL(11) IVQM_ROLL(16)
+#undef IVQM_ROLL
+    return hipErrorInvalidValue;
+  }
   // static rows pay at stride 1 and 2, and for weighted windows (measured at 4096 x 2048: averaged 16 / 4
   // runs faster in the general kernel, whose lanes share fewer cache lines per load at larger strides)
   if ((stride <= 2 || weighted) && launch_static_any(stride, n, weighted, ext_on, g, t, st, jobs, njobs, nblk, partial))

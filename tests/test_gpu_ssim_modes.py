@@ -27,12 +27,15 @@ CONFIGS = [
     (2, 0, 11, 4), (2, 5, 11, 1), (2, 1, 11, 11),
     (3, 0, 8, 4), (3, 0, 16, 1), (3, 0, 8, 8),
     (4, 0, 16, 4), (4, 0, 32, 8), (4, 0, 8, 1), (4, 0, 8, 2), (4, 0, 5, 3),
+    (0, 0, 11, 4), (1, 0, 11, 4),
 ]
 WORKLOADS = {
     "10b420": synth.Workload(cfg=700, width=200, height=136, global_shift=1),
     "8b420": synth.Workload(cfg=701, width=160, height=128, bit_depth=8),
     "16b444": synth.Workload(cfg=702, width=144, height=112, bit_depth=16, chroma_420=False, sigma_r=12,
                              sigma_t=12, block_shift=2),
+    # wider than one wave of stride-4 windows (62 columns of 11-wide windows) and taller than one wave's rows
+    "10b420w": synth.Workload(cfg=704, width=532, height=290, global_shift=1, block_shift=2),
 }
 
 
@@ -70,9 +73,19 @@ def test_ssim_modes_8bit(oracle, cfg):
     _check(oracle, WORKLOADS["8b420"], *cfg)
 
 
-@pytest.mark.parametrize("cfg", [(1, 4, 11, 2), (2, 3, 11, 4), (3, 0, 8, 2), (4, 0, 32, 4)])
+@pytest.mark.parametrize("cfg", [(1, 4, 11, 2), (2, 3, 11, 4), (3, 0, 8, 2), (4, 0, 32, 4), (1, 0, 11, 4),
+                                 (3, 0, 16, 4)])
 def test_ssim_modes_16bit_444(oracle, cfg):
     _check(oracle, WORKLOADS["16b444"], *cfg)
+
+
+@pytest.mark.parametrize("cfg", [(1, 0, 11, 4), (0, 0, 11, 4), (2, 0, 11, 4), (3, 0, 8, 4), (3, 0, 16, 4),
+                                 (4, 0, 16, 4)])
+def test_ssim_modes_stride4_several_waves(oracle, cfg):
+    """Stride-4 windows without a border extension run the rolling-row kernel (one vector load per sample
+    row and lane, neighbours by DPP, window rows walked with rolling accumulators): a picture several
+    waves wide and tall, with a ragged right edge."""
+    _check(oracle, WORKLOADS["10b420w"], *cfg)
 
 
 def test_picture_smaller_than_window_is_rejected():
