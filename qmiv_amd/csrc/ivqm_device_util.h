@@ -57,6 +57,21 @@ __device__ __forceinline__ int ldpix(const PicDev& P, int c, int x, int y) {
   return (int)p[__umul24((uint32_t)(y >> sy), (uint32_t)plane_stride(P, c)) + (uint32_t)(x >> sx)];
 }
 
+// a * b + c with 24-bit signed operands a, b: the low 32 bits of the product plus c (v_mad_i32_i24).
+// Written out because instruction selection otherwise splits the sums of products into separate
+// multiplies and three-input adds (five instructions for three terms instead of three).
+__device__ __forceinline__ int mad_i24(int a, int b, int c) {
+  int d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+// The same with a wave-uniform b read from a scalar register (no copy of it into a vector register).
+__device__ __forceinline__ int mad_i24_s(int a, int b, int c) {
+  int d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // v_med3_i32
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {

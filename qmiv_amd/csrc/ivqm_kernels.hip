@@ -530,15 +530,6 @@ template <typename E, int LP> struct LdsPitch {
   static constexpr int v = std::is_same<E, SEnt4>::value ? LP + LP / 4 : LP;
 };
 
-// a * b + c with 24-bit signed operands a, b: the low 32 bits of the product plus c (v_mad_i32_i24).
-// Written out because instruction selection otherwise splits the sums of products into separate
-// multiplies and three-input adds (five instructions for three terms instead of three).
-__device__ __forceinline__ int mad_i24(int a, int b, int c) {
-  int d;
-  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-  return d;
-}
-
 // One step of an inclusive scan of three values within each 16-lane row.
 template <int CTRL>
 __device__ __forceinline__ void dpp3_add(uint32_t v[3]) {

@@ -190,35 +190,34 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
       rh[k][2] = ldpix<T>(Ref, 2, x, y);
     }
   }
-  // Entries: key = Sk * Q'(r) + ly * DP + lx, Q'(r) = sum_c w_c r_c (r_c - 2 C0), Y, chroma word
-  // {Cb * Sy, Cr * Sy} written by one lane per chroma column. Host-checked: weights below 128, samples
-  // below 2^12 (so 2 C0 < 2^14); the masks below state those ranges, so plain multiplies become the 24-bit
-  // multiply instructions.
-  const uint32_t uwY = (uint32_t)wY & 0x7fu, uwCb = (uint32_t)wCb & 0x7fu, uwCr = (uint32_t)wCr & 0x7fu;
-  const int c2 = (2 * C0) & 0x3fff;
-  auto q_of = [&](uint32_t w, int v) -> int {
-    const int vm = (int)((uint32_t)v & 0xfffu);
-    return (int)(w * (uint32_t)vm) * (vm - c2);
+  // Entries: key = Sk * Q'(r) + ly * DP + lx, Y, chroma pair {Cb, Cr} and chroma word {Cb * Sy, Cr * Sy}
+  // (the pair shifted; Sy * Max < 2^16 host-checked) written by one lane per chroma column.
+  // Q'(r) = sum_c r_c t_c with t_c = w_c (r_c - 2 C0) = r_c w_c - 2 C0 w_c: 24-bit multiply-adds (host-checked:
+  // samples below 2^12, weights below 128, 2 C0 below 2^14, so |t_c| < 2^21), exact mod 2^32 as the keys are.
+  // The chroma part is shared by the two rows of a pair; each row adds its luma part in one multiply-add.
+  const int c2 = 2 * C0;
+  const int kY = -wY * c2, kCb = -wCb * c2, kCr = -wCr * c2;
+  auto qc_of = [&](int cb, int cr) -> int {
+    return mad_i24(cb, mad_i24_s(cb, wCb, kCb), __mul24(cr, mad_i24_s(cr, wCr, kCr)));
   };
-  auto chroma_word = [&](int cb, int cr) -> int {
-    return (int)(((uint32_t)cb << YB) | ((uint32_t)cr << (16 + YB)));
-  };
+  auto q_of = [&](int y, int qc) -> int { return mad_i24(y, mad_i24_s(y, wY, kY), qc); };
   auto cwriter = [&](int lx) -> bool { return R1 ? (lx == 0 || (lx & 1)) : !(lx & 1); };
 #pragma unroll
   for (int k = 0; k < NPW; k++) {
     const int j = wd + P::WPD * k;
     if (j < NPT) {
-      const int qc = q_of(uwCb, ra[k][2]) + q_of(uwCr, ra[k][3]);
+      const int qc = qc_of(ra[k][2], ra[k][3]);
+      const uint32_t cu = (uint32_t)ra[k][2] | ((uint32_t)ra[k][3] << 16);
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int ly = 2 * j - R1 + h;
         if (ly >= 0 && ly < LH) {
-          L.key[dir][ly][lane] = (int)(((uint32_t)(q_of(uwY, ra[k][h]) + qc) << KB) + (uint32_t)(ly * DP + lane));
+          L.key[dir][ly][lane] = (int)(((uint32_t)q_of(ra[k][h], qc) << KB) + (uint32_t)(ly * DP + lane));
           L.ys[dir][ly][lane] = (uint32_t)ra[k][h];
-          L.cu[dir][ly][lane] = (uint32_t)ra[k][2] | ((uint32_t)ra[k][3] << 16);
+          L.cu[dir][ly][lane] = cu;
         }
       }
-      if (cwriter(lane)) L.cw[dir][j][(lane + R1) >> 1] = chroma_word(ra[k][2], ra[k][3]);
+      if (cwriter(lane)) L.cw[dir][j][(lane + R1) >> 1] = (int)(cu << YB);
     }
   }
 #pragma unroll
@@ -226,11 +225,11 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
     const int e = tl + P::WPD * 64 * k;
     if (e < NH) {
       const int hy = e / (2 * R), hx = e - hy * (2 * R), lx = 64 + hx;
-      const int q = q_of(uwY, rh[k][0]) + q_of(uwCb, rh[k][1]) + q_of(uwCr, rh[k][2]);
-      L.key[dir][hy][lx] = (int)(((uint32_t)q << KB) + (uint32_t)(hy * DP + lx));
+      const uint32_t cu = (uint32_t)rh[k][1] | ((uint32_t)rh[k][2] << 16);
+      L.key[dir][hy][lx] = (int)(((uint32_t)q_of(rh[k][0], qc_of(rh[k][1], rh[k][2])) << KB) + (uint32_t)(hy * DP + lx));
       L.ys[dir][hy][lx] = (uint32_t)rh[k][0];
-      L.cu[dir][hy][lx] = (uint32_t)rh[k][1] | ((uint32_t)rh[k][2] << 16);
-      if (cwriter(lx)) L.cw[dir][(hy + R1) >> 1][(lx + R1) >> 1] = chroma_word(rh[k][1], rh[k][2]);
+      L.cu[dir][hy][lx] = cu;
+      if (cwriter(lx)) L.cw[dir][(hy + R1) >> 1][(lx + R1) >> 1] = (int)(cu << YB);
     }
   }
   __syncthreads();
@@ -243,17 +242,27 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   auto test_y = [&](int r, int k) -> int { return (int)L.ys[1 - dir][iy0 + r + R][ix + k + R]; };
   auto test_c = [&](int h) -> uint32_t { return (uint32_t)L.cw[1 - dir][(iy0 + R + R1) >> 1][(ix + 2 * h + R + R1) >> 1]; };
   // Multipliers (host-checked): luma 2 Sk wY t' < 2^24, chroma {2 F wCb t'_Cb, 2 F wCr t'_Cr} u16.
-  const uint32_t kmY = ((uint32_t)(2 * wY) << KB) & 0xffffu;
+  // (t'_c = C0 - g_c - s_c >= 0, below 2^14; the masks state the u16 ranges for the 24-bit multiplies; the
+  // chroma pair is packed with a byte permute)
+  const uint32_t kmY = ((uint32_t)(2 * wY) << KB) & 0xffffu, kmCb = ((uint32_t)(2 * wCb) << FB) & 0xffffu,
+                 kmCr = ((uint32_t)(2 * wCr) << FB) & 0xffffu;
+  // (uniform constants read as such, so that they are folded once rather than per sample)
+  const int c0y = __builtin_amdgcn_readfirstlane(C0 - g0), c0b = __builtin_amdgcn_readfirstlane(C0 - g1),
+            c0r = __builtin_amdgcn_readfirstlane(C0 - g2);
   uint32_t mY[2][4], mc[2];
 #pragma unroll
   for (int r = 0; r < 2; r++)
 #pragma unroll
-    for (int k = 0; k < 4; k++) mY[r][k] = kmY * ((uint32_t)(C0 - test_y(r, k) - g0) & 0xffffu);
+    for (int k = 0; k < 4; k++) {
+      mY[r][k] = kmY * ((uint32_t)(c0y - test_y(r, k)) & 0xffffu);
+      __builtin_assume(mY[r][k] < (1u << 24));  // host-checked: 2 Sk wY 2 A < 2^24
+    }
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const uint32_t w = test_c(h);
     const int tcb = (int)((w & 0xffffu) >> YB), tcr = (int)(w >> (16 + YB));
-    mc[h] = (uint32_t)((2 * wCb << FB) * (C0 - tcb - g1)) | ((uint32_t)((2 * wCr << FB) * (C0 - tcr - g2)) << 16);
+    mc[h] = __builtin_amdgcn_perm(kmCr * ((uint32_t)(c0r - tcr) & 0xffffu), kmCb * ((uint32_t)(c0b - tcb) & 0xffffu),
+                                  0x05040100u);
   }
 
   int bk[2][4];
@@ -324,14 +333,17 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   const bool want_scp = pe->scp_off >= 0, want_mom = pe->mom_off >= 0;  // block-uniform
   const int gc[3] = {g0, g1, g2};
   const u16x2 mx = (unsigned short)pe->maxv;
-  // test chroma + g of pixel pair h, per component, in both halves (shared by both rows of the lane)
+  // test chroma of pixel pair h, per component, in both halves (shared by both rows of the lane): the
+  // samples (tca, the a samples of the moments) and plus g (tcp)
   uint32_t tcp[2][2];
+  u16x2 tca[2][2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const uint32_t w = L.cu[1 - dir][iy0 + R][ix + 2 * h + R];
-    const u16x2 cb = (unsigned short)((w & 0xffffu) + (uint32_t)g1), cr = (unsigned short)((w >> 16) + (uint32_t)g2);
-    tcp[0][h] = __builtin_bit_cast(uint32_t, cb);
-    tcp[1][h] = __builtin_bit_cast(uint32_t, cr);
+    tca[0][h] = (unsigned short)(w & 0xffffu);
+    tca[1][h] = (unsigned short)(w >> 16);
+    tcp[0][h] = __builtin_bit_cast(uint32_t, tca[0][h] + (unsigned short)g1);
+    tcp[1][h] = __builtin_bit_cast(uint32_t, tca[1][h] + (unsigned short)g2);
   }
   // fused IV-SSIM partials over the lane's 8 pixels: per component sum a and sum b as u16 pairs (pixels
   // 2h, 2h + 1), S_aa, S_bb, S_ab; a = the test picture's sample (t - g), b = the SCP sample
@@ -354,22 +366,24 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
     const int y = y0 + iy0 + r;
     const unsigned int* ybase = &L.ys[dir][iy0 + r][ix];
     const unsigned int* cbase = &L.cu[dir][iy0 + r][ix];
+    const uint32_t nb = (uint32_t)(-((iy0 + r) * DP + ix));  // minus the row's pixel constant at k = 0
     uint32_t wy[4], wc[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       // exact keys give u <= UMAX; the clamp keeps the reads inside the region regardless
-      const uint32_t u = min((uint32_t)(bk[r][k] - ((iy0 + r) * DP + ix + k)) & ((1u << KB) - 1u), (uint32_t)UMAX);
+      const uint32_t u = min(((uint32_t)bk[r][k] + nb + (uint32_t)(-k)) & ((1u << KB) - 1u), (uint32_t)UMAX);
       const uint32_t o = u + (uint32_t)(P::LPK - DP) * (u >> LOGDP);  // in entries
       wy[k] = ybase[k + o];
       wc[k] = cbase[k + o];
     }
     // u16 pairs of pixels (2h, 2h + 1): test + g and winner samples per component
     uint32_t tp[3][2], wp[3][2];
+    u16x2 tya[2];  // test luma samples (the a samples of the moments)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const unsigned int* ty = &L.ys[1 - dir][iy0 + r + R][ix + 2 * h + R];
-      const u16x2 tyv = {(unsigned short)ty[0], (unsigned short)ty[1]};
-      tp[0][h] = __builtin_bit_cast(uint32_t, tyv + (unsigned short)g0);
+      tya[h] = u16x2{(unsigned short)ty[0], (unsigned short)ty[1]};
+      tp[0][h] = __builtin_bit_cast(uint32_t, tya[h] + (unsigned short)g0);
       tp[1][h] = tcp[0][h];
       tp[2][h] = tcp[1][h];
       wp[0][h] = __builtin_amdgcn_perm(wy[2 * h + 1], wy[2 * h], 0x05040100u);
@@ -434,15 +448,18 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
         }
       }
       if (want_mom) {
-        // at most 12-bit samples (host-checked): no u16 half overflows over 16 pixels
+        // at most 12-bit samples (host-checked): no u16 half overflows over 16 pixels. The chroma a samples
+        // are the same in both rows: their S_a and S_aa are taken once, after the rows.
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
           for (int h = 0; h < 2; h++) {
-            const u16x2 ta = __builtin_bit_cast(u16x2, tp[c][h]) - (unsigned short)gc[c];
-            sa[c] += ta;
+            const u16x2 ta = c ? tca[c - 1][h] : tya[h];
+            if (c == 0) {
+              sa[c] += ta;
+              saa[c] = __builtin_amdgcn_udot2(ta, ta, saa[c], false);
+            }
             sb[c] += sv[c][h];
-            saa[c] = __builtin_amdgcn_udot2(ta, ta, saa[c], false);
             sbb[c] = __builtin_amdgcn_udot2(sv[c][h], sv[c][h], sbb[c], false);
             sab[c] = __builtin_amdgcn_udot2(ta, sv[c][h], sab[c], false);
           }
@@ -479,6 +496,12 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
     // Fused IV-SSIM: a 4 x 4 sub-block's rows (0, 1) and (2, 3) are lanes l and l ^ 16. Per lane and
     // component {S_a | S_b << 16, S_aa, S_bb, S_ab}; the lane-pair reduction leaves component 0 and words
     // 0, 2 of component 2 in even 16-lane rows, component 1 and words 1, 3 of component 2 in odd rows.
+#pragma unroll
+    for (int c = 1; c < 3; c++) {  // the chroma a samples: both rows alike
+      sa[c] = (tca[c - 1][0] + tca[c - 1][1]) << (unsigned short)1;
+      saa[c] = __builtin_amdgcn_udot2(tca[c - 1][1], tca[c - 1][1],
+                                      __builtin_amdgcn_udot2(tca[c - 1][0], tca[c - 1][0], 0u, false), false) << 1;
+    }
     uint32_t m[3][4];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
