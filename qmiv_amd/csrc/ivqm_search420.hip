@@ -38,6 +38,7 @@
 #include "ivqm_device_util.h"
 
 #include <limits.h>
+#include <stddef.h>
 
 namespace ivqm {
 
@@ -237,6 +238,7 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   // ---- this lane's pixels: tile rows iy0, iy0 + 1 (iy0 even), tile columns ix .. ix + 3 ----
   const int rp = lane >> 4;
   const int iy0 = 8 * wd + 2 * rp, ix = 4 * (lane & 15);
+  __builtin_assume(iy0 >= 0 && iy0 < TH);  // (small non-negative operands: 24-bit multiplies in the addressing)
   // Test samples: the other direction's region at (iy + R, ix + k + R) holds the test picture at the
   // clamped position; both rows and each pixel pair (2h, 2h + 1) share one chroma sample.
   auto test_y = [&](int r, int k) -> int { return (int)L.ys[1 - dir][iy0 + r + R][ix + k + R]; };
@@ -278,10 +280,15 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   struct Run {
     int K[NE], Y[NE], C[NC];
   };
-  auto load_run = [&](int cr, Run& u) {
-    lds_read16<NE>(&L.key[dir][iy0 + cr][ix], u.K);
-    lds_read16<NE>(reinterpret_cast<const int*>(&L.ys[dir][iy0 + cr][ix]), u.Y);
-    lds_read8<NC>(&L.cw[dir][(iy0 + cr + R1) >> 1][ix >> 1], u.C);
+  // The run of region row iy0 + cr from its key entry kp (the Y entries follow at a constant distance) and of
+  // chroma row ci = (iy0 + cr + R1) >> 1 = (iy0 >> 1) + ((cr + R1) >> 1) (iy0 is even) from cp.
+  constexpr int YD = 2 * LH * P::LPK;  // ints from a key entry to its Y entry
+  using LT = Lds420<R, TH>;
+  static_assert(offsetof(LT, ys) - offsetof(LT, key) == YD * sizeof(int), "layout");
+  auto load_run = [&](const int* kp, const int* cp, Run& u) {
+    lds_read16<NE>(kp, u.K);
+    lds_read16<NE>(kp + YD, u.Y);
+    lds_read8<NC>(cp, u.C);
   };
   auto score = [&](const Run& u, bool row0, bool row1) {
 #pragma unroll
@@ -305,15 +312,21 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
       }
     }
   };
+  // (iteration i scores rows cr = 2i + 1, 2i + 2 and loads rows 2i + 2, 2i + 3, chroma rows
+  // (iy0 >> 1) + 1 + i and (iy0 >> 1) + 1 + i + R1: the loop's addresses advance by constant steps)
   Run ua, ub;
-  load_run(0, ua);
-  load_run(1, ub);
+  const int* kp = &L.key[dir][iy0][ix];
+  const int* cp = &L.cw[dir][iy0 >> 1][ix >> 1];
+  load_run(kp, cp, ua);
+  load_run(kp + P::LPK, cp + R1 * P::LPC, ub);
   score(ua, true, false);
 #pragma unroll 1
-  for (int cr = 1; cr < 2 * R + 1; cr += 2) {
-    load_run(cr + 1, ua);
+  for (int i = 0; i < R; i++) {
+    kp += 2 * P::LPK;
+    cp += P::LPC;
+    load_run(kp, cp, ua);
     score(ub, true, true);
-    load_run(cr + 2, ub);
+    load_run(kp + P::LPK, cp + R1 * P::LPC, ub);
     score(ua, true, true);
   }
   score(ub, false, true);
@@ -332,7 +345,20 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
   const bool edge = x0 + 64 > W || y0 + TH > H;  // block-uniform
   const bool want_scp = pe->scp_off >= 0, want_mom = pe->mom_off >= 0;  // block-uniform
   const int gc[3] = {g0, g1, g2};
-  const u16x2 mx = (unsigned short)pe->maxv;
+  // clip(r - g, 0, Max) = (r +sat ca) -sat cb with bias = 2^16 - 1 - Max and g clamped to [-Max, Max]
+  // (which leaves the clip unchanged for r in [0, Max]): g >= 0: ca = bias (no saturation as r <= Max),
+  // cb = bias + g <= 2^16 - 1, giving max(r - g, 0); g < 0: ca = bias - g <= 2^16 - 1 (the sum saturates
+  // where r - g > Max), cb = bias, giving min(r - g, Max)
+  u16x2 ca[3], cb[3];
+  {
+    const int maxv = pe->maxv, bias = 0xffff - maxv;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int g = clampi(gc[c], -maxv, maxv);
+      ca[c] = (unsigned short)(g < 0 ? bias - g : bias);
+      cb[c] = (unsigned short)(g < 0 ? bias : bias + g);
+    }
+  }
   // test chroma of pixel pair h, per component, in both halves (shared by both rows of the lane): the
   // samples (tca, the a samples of the moments) and plus g (tcp)
   uint32_t tcp[2][2];
@@ -424,17 +450,15 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
       }
     }
     if (want_scp || want_mom) {
-      // SCP samples clip(r - g, 0, Max) for r in [0, Max]: r - g saturated at 0 when g >= 0,
-      // min(r + |g|, Max) otherwise (g is uniform)
+      // SCP samples clip(r - g, 0, Max) for r in [0, Max] as two saturating u16 operations with uniform
+      // operands: (r +sat ca) -sat cb (see ca, cb)
       u16x2 sv[3][2];
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const u16x2 w = __builtin_bit_cast(u16x2, wp[c][h]);
-          sv[c][h] = gc[c] >= 0 ? __builtin_elementwise_sub_sat(w, (u16x2)(unsigned short)gc[c])
-                                : __builtin_elementwise_min(w + (u16x2)(unsigned short)(-gc[c]), mx);
-        }
+        for (int h = 0; h < 2; h++)
+          sv[c][h] = __builtin_elementwise_sub_sat(
+              __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, wp[c][h]), ca[c]), cb[c]);
       if (want_scp && y < H) {
         typedef __attribute__((address_space(1))) uint8_t GU8;
         typedef __attribute__((address_space(1))) u32x2 GU32x2;
