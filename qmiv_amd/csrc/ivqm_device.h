@@ -81,7 +81,7 @@ struct SsimJobDev {
   int32_t calc_l, erp, wsq;       // luminance term, ERP row weights, sum widths (0: 32-bit, 1: 64-bit
                                   // windows, 2: 64-bit throughout)
   int32_t blk0;                   // first block of this job (all 3 components)
-  int64_t part_off;               // [3][nuy][nchunk] doubles in the partial buffer
+  int64_t part_off;               // [3][ceil(nuy / grp)][nchunk] doubles in the partial buffer
   int64_t erp_off;                // offset of this height's weights, or -1
   int32_t out_idx;                // [3][2] doubles at out[6 * out_idx]
   float c1n, c2n;                 // C1 * n^2, C2 * n^2 of this job's bit depth (xSSIM.cpp:50-52)
@@ -91,7 +91,9 @@ struct SsimJobDev {
   int32_t gx0, gy0;               // first window position: regular = centre (5, or 0 with an extension),
                                   // block = top-left (0)
   int32_t erp_base;               // ERP weight row of window row j = erp_base + j * stride
-  int32_t pad;
+  int32_t grp;                    // window rows per partial row: 1 (k_ssim_gen; k_ssim_finalize applies the
+                                  // ERP weights) or 4 (k_ssim: the partial of rows 4i .. 4i + 3 is their
+                                  // ERP-weighted sum, the weights applied in the kernel)
   double c1, c2;                  // xSSIM.cpp:50-52
   double kscale;                  // sums -> moments: 1 (float weights), 2^-18 (Q18), 1/area (averaged)
   const double* wt;               // wnd x wnd weights (row pitch wnd) followed by their transpose, or null
@@ -143,8 +145,8 @@ hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, co
                          hipStream_t st);
 // forms: bit 0 when jobs read pictures with 32-bit sums (wsq 0), bit 2 when jobs read pictures with wider
 // sums, bit 1 when jobs read fused moments (one launch each).
-hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
-                       hipStream_t st);
+hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, const double* erp_w, double* partial,
+                       uint32_t forms, hipStream_t st);
 // Any window mode / size / stride / border extension: one wave per 64 window columns and
 // kSsimGenRows window rows, 4 waves per block. Stride-4 windows of 8, 11 or 16 samples without a border
 // extension (regular: window 11; block: BlockGaussianInt and BlockAveraged other than 8 / 4) run the

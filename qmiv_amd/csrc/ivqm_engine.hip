@@ -590,8 +590,9 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     J.blk0 = P.ssim_blocks;
     P.ssim_blocks += 3 * J.nchunk * J.nbg;
     J.twin_part_off = -1;
+    J.grp = P.ssim_gen ? 1 : 4;
     J.part_off = P.partial_total;
-    P.partial_total += 3 * (int64_t)J.nuy * J.nchunk;
+    P.partial_total += 3 * (int64_t)((J.nuy + J.grp - 1) / J.grp) * J.nchunk;
     J.out_idx = (int)P.jobs.size();
     const double maxv = (double)((1 << P.pp[i].bd) - 1);
     const double c1 = (0.01 * maxv) * (0.01 * maxv), c2 = (0.03 * maxv) * (0.03 * maxv);  // xSSIM.cpp:50-52
@@ -644,7 +645,7 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
           if (l == 0 && cl == 0 && pp.ssim >= 0 && !P.ssim_gen) {
             SsimJobDev& T = P.jobs[pp.ms[0]];
             SsimJobDev& S = P.jobs[pp.ssim];
-            if (T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk && T.wsq == S.wsq &&
+            if (T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk && T.wsq == S.wsq && !T.erp &&
                 T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
               P.ssim_blocks = T.blk0;
               S.twin_part_off = T.part_off;
@@ -653,14 +654,14 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
         }
         if (mp & IVQM_M_IVMSSSIM) {
           // Level 0 of IV-MS-SSIM reads the IV-SSIM jobs' windows, (P0, SCP1) and (P1, SCP0): twins of
-          // those jobs as above (the ERP weights of IV-SSIM apply at the frame totals, not to partials).
+          // those jobs as above (a twin has no ERP weights: the IV-SSIM job weights only its own partials).
           bool twin_fail = false;
           auto twin_iv = [&](int twin_job, int iv_job) {
             if (l != 0 || cl != 0 || iv_job < 0 || P.ssim_gen) return;
             SsimJobDev& T = P.jobs[twin_job];
             SsimJobDev& S = P.jobs[iv_job];
             if (S.twin_part_off < 0 && T.nux == S.nux && T.nuy == S.nuy && T.nchunk == S.nchunk &&
-                T.wsq == S.wsq && T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
+                T.wsq == S.wsq && !T.erp && T.blk0 + 3 * T.nchunk * T.nbg == P.ssim_blocks) {
               P.ssim_blocks = T.blk0;
               S.twin_part_off = T.part_off;
             } else if (S.mom_off >= 0) {
@@ -883,7 +884,7 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
       HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
                              d_partial_, st));
     else
-      HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, ssim_forms(P), st));
+      HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_erp_, d_partial_, ssim_forms(P), st));
     HIPCHK(launch_ssim_finalize(d_jobs_, nj, d_partial_, d_erp_, d_ssim_out_, st));
   }
   if (P.need_stats) {
@@ -1131,7 +1132,7 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
           HIPCHK(launch_ssim_gen(d_jobs_, nj, P.ssim_blocks, cfg_.ssim_mode, cfg_.ssim_window, cfg_.ssim_border, cfg_.ssim_stride,
                                  d_partial_, stream_));
         else
-          HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_partial_, ssim_forms(P), stream_));
+          HIPCHK(launch_ssim(d_jobs_, nj, P.ssim_blocks, P.nb, d_erp_, d_partial_, ssim_forms(P), stream_));
         break;
       case 3:
         HIPCHK(launch_search_masked(d_pairs_, P.n, P.mtiles, cfg_.search_range, cfg_.cmp_weights_search, d_gcd_,

@@ -1216,9 +1216,12 @@ __device__ __forceinline__ float i64_to_f32(int64_t n) {
 // uint32 (64 * 4095^2 < 2^32); SM 1 up to 14-bit, sub-blocks uint32, windows uint64; SM 2 uint64.
 // MOM: the sub-block sums come from the search's fused moments (J.mom; SM 0) instead of the pictures --
 // the same integers, so the same scores bit for bit.
+// Each lane sums its windows' scores over a group of 4 window rows (bands start at multiples of 4, so a group
+// lies in one band), weighted by their rows' ERP weights (weight 1 without), and the group's wave total is its
+// partial: the grouping does not depend on the band height, so every band height gives the same doubles.
 template <typename TA, int CSA, typename TB, int CSB, int SM, int NB, bool MOM = false>
 __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
-                          double* __restrict__ partial) {
+                          const double* __restrict__ erp_w, double* __restrict__ partial) {
   static_assert(!MOM || SM == 0, "fused moments are 32-bit");
   using SQ = typename std::conditional<SM == 2, uint64_t, uint32_t>::type;  // sub-block squared sums
   using WQ = typename std::conditional<SM == 0, uint32_t, uint64_t>::type;  // window squared sums
@@ -1233,6 +1236,11 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
   const int sa = plane_stride(J.a, c), sb = plane_stride(J.b, c);
   uint32_t pA = 0, pB = 0;
   SQ pAA = 0, pBB = 0, pAB = 0;
+  static_assert(NB % 4 == 0, "bands hold whole groups of 4 window rows");
+  const int ngrp = (J.nuy + 3) >> 2;
+  const bool weighted = J.erp && J.erp_off >= 0;
+  const double* wrow = erp_w + (weighted ? J.erp_off + J.erp_base : 0);
+  double acc = 0.0, acc_cs = 0.0;
 #pragma unroll 1
   for (int k = 0; k <= NB; k++) {
     const int by = wy0 + k;
@@ -1304,11 +1312,18 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         }
         s = (double)(num * __builtin_amdgcn_rcpf(den));
       }
-      s = wave_sum_f64_dpp(s);
-      if (lane == 0) partial[J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s;
-      if (J.twin_part_off >= 0) {  // block-uniform
-        s_cs = wave_sum_f64_dpp(s_cs);
-        if (lane == 0) partial[J.twin_part_off + ((int64_t)c * J.nuy + wy) * J.nchunk + chunk] = s_cs;
+      acc = weighted ? fma(s, wrow[(int64_t)wy * J.stride], acc) : acc + s;
+      acc_cs += s_cs;  // the twin (MS-SSIM level 0) is never ERP-weighted
+      if ((wy & 3) == 3 || wy == J.nuy - 1) {  // wave-uniform: the group's last row
+        const int64_t pi = ((int64_t)c * ngrp + (wy >> 2)) * J.nchunk + chunk;
+        acc = wave_sum_f64_dpp(acc);
+        if (lane == 0) partial[J.part_off + pi] = acc;
+        acc = 0.0;
+        if (J.twin_part_off >= 0) {  // block-uniform
+          acc_cs = wave_sum_f64_dpp(acc_cs);
+          if (lane == 0) partial[J.twin_part_off + pi] = acc_cs;
+          acc_cs = 0.0;
+        }
       }
     }
     pA = A; pB = B; pAA = AA; pBB = BB; pAB = AB;
@@ -1319,9 +1334,9 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
 // pairs two caller pictures of one format, or a caller picture with a 4:4:4 SCP picture (either order).
 template <int SM, int NB>
 __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, int lane, float c1n, float c2n,
-                              double* partial) {
+                              const double* erp_w, double* partial) {
   const int csa = c ? chroma_format(J.a) : 0, csb = c ? chroma_format(J.b) : 0;
-#define IVQM_SSIM(TA, A, TB, B) ssim_wave<TA, A, TB, B, SM, NB>(J, c, chunk, band, lane, c1n, c2n, partial)
+#define IVQM_SSIM(TA, A, TB, B) ssim_wave<TA, A, TB, B, SM, NB>(J, c, chunk, band, lane, c1n, c2n, erp_w, partial)
   if (J.a.bps == 2 && J.b.bps == 2) {
     if (csa == 0 && csb == 0) IVQM_SSIM(uint16_t, 0, uint16_t, 0);
     else if (csa == 1 && csb == 1) IVQM_SSIM(uint16_t, 1, uint16_t, 1);
@@ -1349,7 +1364,7 @@ __device__ void ssim_dispatch(const SsimJobDev& J, int c, int chunk, int band, i
 // read fused moments; blocks of another form exit at once.
 template <int NB, int FORM>
 __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ jobs, int njobs, int nblk,
-                                              double* __restrict__ partial) {
+                                              const double* __restrict__ erp_w, double* __restrict__ partial) {
   constexpr bool MOM = FORM == 2;
   // XCD-aware order (as in k_search): XCD g runs blocks g * per, g * per + 1, ...,
   // so neighbouring chunks and band groups, which share cache lines of their
@@ -1367,10 +1382,11 @@ __global__ __launch_bounds__(256) void k_ssim(const SsimJobDev* __restrict__ job
   const int bg = rem / J.nchunk, chunk = rem - bg * J.nchunk;
   // band is wave-uniform: row pointers stay scalar and every load adds only the lane's column
   const int lane = threadIdx.x & 63, band = bg * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if constexpr (FORM == 2) ssim_wave<uint16_t, 0, uint16_t, 0, 0, NB, true>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
-  else if constexpr (FORM == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
-  else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
-  else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, partial);
+  if constexpr (FORM == 2)
+    ssim_wave<uint16_t, 0, uint16_t, 0, 0, NB, true>(J, c, chunk, band, lane, J.c1n, J.c2n, erp_w, partial);
+  else if constexpr (FORM == 0) ssim_dispatch<0, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, erp_w, partial);
+  else if (J.wsq == 1) ssim_dispatch<1, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, erp_w, partial);
+  else ssim_dispatch<2, NB>(J, c, chunk, band, lane, J.c1n, J.c2n, erp_w, partial);
 }
 
 // One block per (job, component): FP64 frame totals of the window-row sums.
@@ -1384,17 +1400,17 @@ __global__ __launch_bounds__(256) void k_ssim_finalize(const SsimJobDev* __restr
   const SsimJobDev J = jobs[j];
   const int tid = threadIdx.x;
   double acc = 0.0, accw = 0.0;
-  for (int wy = tid; wy < J.nuy; wy += 256) {
-    const double* p = partial + J.part_off + ((int64_t)c * J.nuy + wy) * J.nchunk;
+  const bool weighted = J.erp && J.erp_off >= 0;
+  const int nrow = (J.nuy + J.grp - 1) / J.grp;  // partial rows
+  for (int r = tid; r < nrow; r += 256) {
+    const double* p = partial + J.part_off + ((int64_t)c * nrow + r) * J.nchunk;
     double rs = 0.0;
     for (int k = 0; k < J.nchunk; k++) rs += p[k];
-    if (J.erp && J.erp_off >= 0) {
-      const double wt = erp_w[J.erp_off + J.erp_base + (int64_t)wy * J.stride];
-      rs *= wt;
-      accw += wt;
-    }
+    if (weighted && J.grp == 1) rs *= erp_w[J.erp_off + J.erp_base + (int64_t)r * J.stride];
     acc += rs;
   }
+  if (weighted)  // the weight total
+    for (int wy = tid; wy < J.nuy; wy += 256) accw += erp_w[J.erp_off + J.erp_base + (int64_t)wy * J.stride];
   __shared__ double s_a[4], s_w[4];
   acc = wave_sum_f64(acc);
   accw = wave_sum_f64(accw);
@@ -1638,29 +1654,32 @@ hipError_t launch_search(const PairDev* pairs, int npairs, int ntiles, int R, co
 }
 
 template <int FORM>
-static hipError_t ssim_launch(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, hipStream_t st) {
+static hipError_t ssim_launch(const SsimJobDev* jobs, int njobs, int nblk, int nb, const double* erp_w, double* partial,
+                              hipStream_t st) {
   const int grid = kXcds * ((nblk + kXcds - 1) / kXcds);
   switch (nb) {
-    case 4: hipLaunchKernelGGL((k_ssim<4, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
-    case 8: hipLaunchKernelGGL((k_ssim<8, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
-    case 16: hipLaunchKernelGGL((k_ssim<16, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, partial); break;
+    case 4: hipLaunchKernelGGL((k_ssim<4, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, erp_w, partial); break;
+    case 8: hipLaunchKernelGGL((k_ssim<8, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, erp_w, partial); break;
+    case 16:
+      hipLaunchKernelGGL((k_ssim<16, FORM>), dim3(grid), dim3(256), 0, st, jobs, njobs, nblk, erp_w, partial);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, double* partial, uint32_t forms,
-                       hipStream_t st) {
+hipError_t launch_ssim(const SsimJobDev* jobs, int njobs, int nblk, int nb, const double* erp_w, double* partial,
+                       uint32_t forms, hipStream_t st) {
   if (njobs <= 0 || nblk <= 0) return hipSuccess;
   if (forms & 1u) {
-    const hipError_t e = ssim_launch<0>(jobs, njobs, nblk, nb, partial, st);
+    const hipError_t e = ssim_launch<0>(jobs, njobs, nblk, nb, erp_w, partial, st);
     if (e != hipSuccess) return e;
   }
   if (forms & 4u) {
-    const hipError_t e = ssim_launch<1>(jobs, njobs, nblk, nb, partial, st);
+    const hipError_t e = ssim_launch<1>(jobs, njobs, nblk, nb, erp_w, partial, st);
     if (e != hipSuccess) return e;
   }
-  if (forms & 2u) return ssim_launch<2>(jobs, njobs, nblk, nb, partial, st);
+  if (forms & 2u) return ssim_launch<2>(jobs, njobs, nblk, nb, erp_w, partial, st);
   return hipSuccess;
 }
 
