@@ -1157,7 +1157,8 @@ __global__ __launch_bounds__(512) void k_search(const PairDev* __restrict__ pair
 // (63 windows) and a band of NB window rows; it walks the band top to bottom,
 // carrying the previous 4x4 sub-block row so every input row is read once per
 // band. Window moments are exact integers; the ratio is FP32 on the exact
-// int64 numerators n*S_ab - S_a*S_b; window values are summed in FP64.
+// numerators n*S_ab - S_a*S_b (formed in FP64, where they are exact, and
+// rounded once); window values are summed in FP64.
 // ---------------------------------------------------------------------------
 // Four samples of a picture row (luma grid x .. x + 3, x a multiple of 4) from
 // a wave-uniform row pointer: one vector load; 4:2:0 chroma reads 2 samples
@@ -1199,17 +1200,14 @@ __device__ __forceinline__ void load4(const __attribute__((address_space(1))) T*
   }
 }
 
-// Value of lane + 1 (DPP wave shift; no LDS traffic). Lane 63 reads 0.
+// Value of lane + 1 (DPP wave shift; no LDS traffic). Lane 63 reads 0: its source lane is out of range and
+// bound_ctrl makes such a read 0, so no old value is set up (and an add of the result takes the shift as
+// its own DPP operand).
 __device__ __forceinline__ uint32_t next_lane32(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ uint64_t next_lane64(uint64_t v) {
   return (uint64_t)next_lane32((uint32_t)v) | ((uint64_t)next_lane32((uint32_t)(v >> 32)) << 32);
-}
-// int64 -> float for |n| < 2^51: high part times 2^20 plus the exact low 20 bits, one rounding
-// when |n| < 2^44 (every bit depth up to 14), at most 1.5 ulp above.
-__device__ __forceinline__ float i64_to_f32(int64_t n) {
-  return fmaf((float)(int32_t)(n >> 20), 1048576.0f, (float)(int32_t)(n & 0xFFFFF));
 }
 
 // Sum widths of the window moments (host picks by bit depth): SM 0 up to 12-bit, everything in
@@ -1292,26 +1290,28 @@ __device__ void ssim_wave(const SsimJobDev& J, int c, int chunk, int band, int l
         nBB = (WQ)next_lane64((uint64_t)vBB);
         nAB = (WQ)next_lane64((uint64_t)vAB);
       }
-      double s = 0.0, s_cs = 0.0;
-      if (lane < 63 && bx < J.nux) {
-        const uint32_t sA = vA + nA, sB = vB + nB;  // < 2^23
-        const int64_t sAA = (int64_t)(uint64_t)(vAA + nAA), sBB = (int64_t)(uint64_t)(vBB + nBB);
-        const int64_t sAB = (int64_t)(uint64_t)(vAB + nAB);
-        // exact numerators n S_ab - S_a S_b (the cancellation happens in integers)
-        const int64_t cov = 64 * sAB - (int64_t)((uint64_t)sA * sB);
-        const int64_t va = 64 * sAA - (int64_t)((uint64_t)sA * sA);
-        const int64_t vb = 64 * sBB - (int64_t)((uint64_t)sB * sB);
-        float num = 2.0f * i64_to_f32(cov) + c2n;
-        float den = i64_to_f32(va) + i64_to_f32(vb) + c2n;
-        // the twin's score is this job's without the luminance term: the same operations as its own job
-        if (J.twin_part_off >= 0) s_cs = (double)(num * __builtin_amdgcn_rcpf(den));
-        if (J.calc_l) {  // luminance term: positive terms only, FP32 products suffice
-          const float fa = (float)sA, fb = (float)sB;
-          num *= 2.0f * fa * fb + c1n;
-          den *= fa * fa + fb * fb + c1n;
-        }
-        s = (double)(num * __builtin_amdgcn_rcpf(den));
+      // Every lane scores (the sums of lane 63 and of columns past the last window are finite, the
+      // denominators positive) and the scores of windows that do not exist are dropped: the window sums
+      // take the lane shift as a DPP operand of their adds.
+      const bool valid = lane < 63 && bx < J.nux;
+      const uint32_t sA = vA + nA, sB = vB + nB;  // < 2^23
+      // exact numerators n S_ab - S_a S_b, each rounded once to FP32: in FP64 every term is an integer
+      // below 2^53 (S_a S_b < 2^46, n S_xy < 2^45), so the products and the difference are exact
+      const double dA = (double)sA, dB = (double)sB;
+      const float fcov = (float)fma(64.0, (double)(WQ)(vAB + nAB), -(dA * dB));
+      const float fva = (float)fma(64.0, (double)(WQ)(vAA + nAA), -(dA * dA));
+      const float fvb = (float)fma(64.0, (double)(WQ)(vBB + nBB), -(dB * dB));
+      float num = 2.0f * fcov + c2n;
+      float den = fva + fvb + c2n;
+      // the twin's score is this job's without the luminance term: the same operations as its own job
+      double s_cs = 0.0;
+      if (J.twin_part_off >= 0) s_cs = valid ? (double)(num * __builtin_amdgcn_rcpf(den)) : 0.0;
+      if (J.calc_l) {  // luminance term: positive terms only, FP32 products suffice
+        const float fa = (float)sA, fb = (float)sB;
+        num *= 2.0f * fa * fb + c1n;
+        den *= fa * fa + fb * fb + c1n;
       }
+      const double s = valid ? (double)(num * __builtin_amdgcn_rcpf(den)) : 0.0;
       acc = weighted ? fma(s, wrow[(int64_t)wy * J.stride], acc) : acc + s;
       acc_cs += s_cs;  // the twin (MS-SSIM level 0) is never ERP-weighted
       if ((wy & 3) == 3 || wy == J.nuy - 1) {  // wave-uniform: the group's last row

@@ -259,8 +259,9 @@ __global__ __launch_bounds__(256) void k_ssim_static(const SsimJobDev* __restric
 // Moments in FP64 as in k_ssim_gen (integer weights keep every sum exact; averaged windows add with weight
 // 1, and a row outside a window adds its samples with weight 0, which leaves the sums unchanged).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_next(uint32_t v) {  // lane l + 1's value (wave_shl:1)
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+// Lane l + 1's value (wave_shl:1); lane 63 reads 0 (bound_ctrl: a source lane out of range reads 0).
+__device__ __forceinline__ uint32_t wave_next(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 
 // Samples 4 g .. 4 g + 3 of a row (luma-grid columns; chroma at x >> cs), clamped to the row's last sample.
