@@ -209,27 +209,39 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
   const int y = y0 + iy, x = x0 + ix;
   const int sg = dir ? -1 : 1;
   const int g0 = sg * gcd[4 * P.res_idx], g1 = sg * gcd[4 * P.res_idx + 1], g2 = sg * gcd[4 * P.res_idx + 2];
+  // Loads through buffer resources over the planes and the mask (row byte offsets scalar where the row is
+  // wave-uniform, 32-bit vector offsets otherwise; no 64-bit address arithmetic).
+  const int csx = Rp.cs, csy = Rp.csy;
+  const __amdgpu_buffer_rsrc_t rY = plane_rsrc<T>(Rp, 0, H), rCb = plane_rsrc<T>(Rp, 1, H), rCr = plane_rsrc<T>(Rp, 2, H);
+  const uint32_t sY = (uint32_t)plane_stride(Rp, 0) * (uint32_t)sizeof(T), sC = (uint32_t)plane_stride(Rp, 1) * (uint32_t)sizeof(T);
+  const __amdgpu_buffer_rsrc_t rM =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<TM*>(msk), (short)0, H * ms * (int)sizeof(TM), 0x00020000);
+  const uint32_t sM = (uint32_t)ms * (uint32_t)sizeof(TM);
   // test samples t = s + sg * GCD and weights of the lane's pixels (weight 0 outside the picture),
   // loaded before the region so both sets of loads are in flight together
   int t[4][3];
   uint32_t m[4];
+  {
+    const __amdgpu_buffer_rsrc_t tY = plane_rsrc<T>(Tp, 0, H), tCb = plane_rsrc<T>(Tp, 1, H), tCr = plane_rsrc<T>(Tp, 2, H);
+    const uint32_t yc = (uint32_t)min(y, H - 1);  // the lane's row (per lane: vector offsets)
+    const uint32_t oY = yc * (uint32_t)plane_stride(Tp, 0) * (uint32_t)sizeof(T);
+    const uint32_t oC = (yc >> Tp.csy) * (uint32_t)plane_stride(Tp, 1) * (uint32_t)sizeof(T);
+    const uint32_t oM = yc * sM;
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    t[k][0] = t[k][1] = t[k][2] = 0;
-    m[k] = 0;
-    if (y < H && x + k < W) {
-      t[k][0] = ldpix<T>(Tp, 0, x + k, y) + g0;
-      t[k][1] = ldpix<T>(Tp, 1, x + k, y) + g1;
-      t[k][2] = ldpix<T>(Tp, 2, x + k, y) + g2;
-      m[k] = (uint32_t)msk[(size_t)y * ms + x + k];
+    for (int k = 0; k < 4; k++) {
+      const uint32_t xk = (uint32_t)min(x + k, W - 1);
+      const uint32_t xc = (xk >> Tp.cs) * (uint32_t)sizeof(T);
+      t[k][0] = ld_buf<T>(tY, xk * (uint32_t)sizeof(T) + oY, 0) + g0;
+      t[k][1] = ld_buf<T>(tCb, xc + oC, 0) + g1;
+      t[k][2] = ld_buf<T>(tCr, xc + oC, 0) + g2;
+      m[k] = (uint32_t)ld_buf<TM>(rM, xk * (uint32_t)sizeof(TM) + oM, 0);
+      if (!(y < H && x + k < W)) m[k] = 0;
     }
   }
+  // Region entries: lane = region column (x0 - R + lane, clamped), wave wv stages rows wv, wv + 4, ...; the 2R
+  // halo columns 64 .. 64 + 2R - 1 are staged by the first 2R * LH threads.
   const uint32_t uY = (uint32_t)cw.x, uCb = (uint32_t)cw.y, uCr = (uint32_t)cw.z;
-  for (int e = (int)threadIdx.x; e < LW * LH; e += 256) {
-    const int ry_ = e / LW, rx_ = e - ry_ * LW;
-    const int rx = clampi(x0 - R + rx_, 0, W - 1), ry = clampi(y0 - R + ry_, 0, H - 1);
-    const uint32_t Y = (uint32_t)ldpix<T>(Rp, 0, rx, ry), Cb = (uint32_t)ldpix<T>(Rp, 1, rx, ry);
-    const uint32_t Cr = (uint32_t)ldpix<T>(Rp, 2, rx, ry), M = (uint32_t)msk[(size_t)ry * ms + rx];
+  auto put = [&](int ry_, int rx_, uint32_t Y, uint32_t Cb, uint32_t Cr, uint32_t M) {
     // Q(r) = sum w r^2 exactly: w * r < 2^24 and S * Q + off < 2^31 (host-checked)
     const uint32_t q = __umul24(__umul24(uY, Y), Y) + __umul24(__umul24(uCb, Cb), Cb) + __umul24(__umul24(uCr, Cr), Cr);
     lk[ry_][rx_] = M ? (int)(q * (uint32_t)S + (uint32_t)(ry_ * D + rx_)) : P.mkey;
@@ -240,6 +252,38 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
       lb[ry_][rx_] = (int)Cb;
     }
     lr[ry_][rx_] = (int)Cr;
+  };
+  {
+    constexpr int NRW = (LH + 3) / 4;
+    const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);
+    const uint32_t oYx = cx * (uint32_t)sizeof(T), oCx = (cx >> csx) * (uint32_t)sizeof(T), oMx = cx * (uint32_t)sizeof(TM);
+    uint32_t ra[NRW][4];
+#pragma unroll
+    for (int k = 0; k < NRW; k++) {
+      const int ry_ = wv + 4 * k;
+      if (ry_ < LH) {
+        const uint32_t ry = (uint32_t)clampi(y0 - R + ry_, 0, H - 1);  // wave-uniform
+        ra[k][0] = (uint32_t)ld_buf<T>(rY, oYx, ry * sY);
+        ra[k][1] = (uint32_t)ld_buf<T>(rCb, oCx, (ry >> csy) * sC);
+        ra[k][2] = (uint32_t)ld_buf<T>(rCr, oCx, (ry >> csy) * sC);
+        ra[k][3] = (uint32_t)ld_buf<TM>(rM, oMx, ry * sM);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NRW; k++) {
+      const int ry_ = wv + 4 * k;
+      if (ry_ < LH) put(ry_, lane, ra[k][0], ra[k][1], ra[k][2], ra[k][3]);
+    }
+    const int e = (int)threadIdx.x;
+    if (e < 2 * R * LH) {
+      const int ry_ = e / (2 * R), rx_ = 64 + (e - ry_ * (2 * R));
+      const uint32_t hx = (uint32_t)clampi(x0 - R + rx_, 0, W - 1), hy = (uint32_t)clampi(y0 - R + ry_, 0, H - 1);
+      const uint32_t Y = (uint32_t)ld_buf<T>(rY, hx * (uint32_t)sizeof(T) + hy * sY, 0);
+      const uint32_t oc = (hx >> csx) * (uint32_t)sizeof(T) + (hy >> csy) * sC;
+      const uint32_t Cb = (uint32_t)ld_buf<T>(rCb, oc, 0), Cr = (uint32_t)ld_buf<T>(rCr, oc, 0);
+      const uint32_t M = (uint32_t)ld_buf<TM>(rM, hx * (uint32_t)sizeof(TM) + hy * sM, 0);
+      put(ry_, rx_, Y, Cb, Cr, M);
+    }
   }
   __syncthreads();
   // multipliers -2 S w t (|value| < 2^23, host-checked)
