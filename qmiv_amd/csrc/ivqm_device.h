@@ -62,7 +62,8 @@ struct PairDev {
   // Masked search in keyed form: the key of a masked-out candidate's entry (host-checked to exceed every
   // considered candidate's key without wrapping), or 0 for the reference-form masked search.
   int32_t mkey;
-  int32_t mdot;  // keyed masked search with the packed int16 {Y, Cb} key (host-checked ranges)
+  int32_t mdot;  // keyed masked search with the packed int16 {Y, Cb} key (1) or the 4:2:0 form with packed
+                 // int16 {Cb, Cr} chroma words on 64 x 32 tiles (2) (host-checked ranges); 0: narrow key
   int32_t scp1_pitch;
   // Fused IV-MS-SSIM (k_search420 with the moments): the search also writes MS-SSIM level 1 of both SCP
   // pictures, [SCP0, SCP1][component][h >> 1][scp1_pitch] uint16 at scp1_off in the SCP arena
@@ -132,6 +133,7 @@ hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* 
 // set when a pair of form f (0 reference form, 1 keyed narrow, 2 keyed dot) is in the batch. Pairs with
 // scp_off >= 0 also write their SCP pictures into scp.
 constexpr int kMaskTileW = 64, kMaskTileH = 16;
+constexpr int kMask420TileH = 32;  // tile height of the 4:2:0 keyed masked search (PairDev::mdot == 2)
 hipError_t launch_search_masked(const PairDev* pairs, int npairs, int ntiles, int R, const int32_t cw[4],
                                 const int32_t* gcd, uint64_t* rows64, uint16_t* scp, uint32_t forms, hipStream_t st);
 // forms: one k_search kernel per form of the search arithmetic present among the batch's k_search pairs,

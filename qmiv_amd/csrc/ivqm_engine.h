@@ -129,7 +129,8 @@ struct BatchPlan {
   bool ssim_gen = false;  // SSIM jobs run k_ssim_gen (any configuration but block-averaged 8 / 4)
   bool any_mask = false;
   int mtiles = 0;                              // k_search_masked tiles
-  uint32_t mforms = 0;                         // masked search forms present (bit 0 reference, 1 narrow, 2 dot)
+  uint32_t mforms = 0;                         // masked search forms present (bit 0 reference, 1 narrow, 2 dot,
+                                               // 3 4:2:0)
   uint32_t sforms = 0;                         // k_search kernels needed (search_form_bit of each pair's mode)
   int64_t aux_total = 0, rows64_total = 0;     // masked pairs' aux rows and uint64 search rows
   int64_t rows_total = 0, rowsd_total = 0, scp_total = 0, pyr_total = 0, partial_total = 0;
@@ -221,7 +222,7 @@ class Engine {
   int erp_offset(int h, int64_t* off);
   int masked_epilogue(const RingSlot& S, const PairPlan& pp, int i, ivqm_frame_result& r);
   int search_mode(int maxv, int32_t gcd_abs_max) const;
-  int32_t masked_key_base(int maxv, int32_t gcd_abs_max) const;
+  int32_t masked_key_base(int maxv, int32_t gcd_abs_max, int tile_h) const;  // tile_h: the form's tile height
   bool masked_dot_ok(int maxv, int32_t gcd_abs_max) const;
 
   ivqm_config cfg_;

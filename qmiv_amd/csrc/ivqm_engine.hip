@@ -224,7 +224,7 @@ int pick_search_mode(const ivqm_config& cfg, int maxv, int32_t gmax) {
 // r <= Max and T = 2 S wsum A Max bounding the per-pixel term, B = 2^31 - 1 - T never wraps and exceeds
 // every considered key (<= S wsum Max^2 + T + offset) when 2^31 - 1 - 2 T > that bound. Returns B, or 0
 // for the reference-form masked search.
-int32_t Engine::masked_key_base(int maxv, int32_t gmax) const {
+int32_t Engine::masked_key_base(int maxv, int32_t gmax, int tile_h) const {
   if (cfg_.search_mode_cap == 1) return 0;
   const int32_t* w = cfg_.cmp_weights_search;
   for (int k = 0; k < 3; k++)
@@ -235,7 +235,7 @@ int32_t Engine::masked_key_base(int maxv, int32_t gmax) const {
   const int64_t wsum = (int64_t)w[0] + w[1] + w[2];
   const int64_t wmax = std::max<int64_t>(w[0], std::max<int64_t>(w[1], w[2]));
   if (A >= 32768 || 2 * S * wmax * A >= (1ll << 23) || wmax * maxv >= (1ll << 24)) return 0;
-  const int64_t off_max = (int64_t)(kMaskTileH + 2 * R) * D + kMaskTileW + 2 * R;
+  const int64_t off_max = (int64_t)(tile_h + 2 * R) * D + kMaskTileW + 2 * R;
   const int64_t T = 2 * S * wsum * A * maxv;
   const int64_t vmax = S * wsum * (int64_t)maxv * maxv + T + off_max;
   const int64_t B = (1ll << 31) - 1 - T;
@@ -392,8 +392,12 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
       P.rows64_total += 6 * (int64_t)s.h;
       pp.masked = true;
       pp.bdm = s.msk_bd;
-      d.mkey = masked_key_base(d.maxv, gmax);
+      d.mkey = masked_key_base(d.maxv, gmax, kMaskTileH);
       d.mdot = (d.mkey != 0 && masked_dot_ok(d.maxv, gmax)) ? 1 : 0;
+      // 4:2:0 pairs: the 4:2:0 form (the packed form's operand ranges, keys over its taller tiles)
+      if (d.mdot && chroma_format(s.p0) == 1 && chroma_format(s.p1) == 1 && cfg_.search_mode_cap == 0 &&
+          masked_key_base(d.maxv, gmax, kMask420TileH) != 0)
+        d.mdot = 2;
       P.any_mask = true;
     }
     d.stat_blk0 = P.stat_blocks;
@@ -406,8 +410,9 @@ int Engine::plan(const PairSpec* pairs, int n, uint32_t metrics, BatchPlan& P) {
     d.mtile0 = P.mtiles;
     if (s.msk) {
       if ((mp & IVQM_M_IVPSNR) || scp_only) {  // the masked search also writes the SCP pictures
-        P.mtiles += 2 * ((s.w + kMaskTileW - 1) / kMaskTileW) * ((s.h + kMaskTileH - 1) / kMaskTileH);
-        P.mforms |= 1u << (d.mkey == 0 ? 0 : (d.mdot ? 2 : 1));
+        const int th = d.mdot == 2 ? kMask420TileH : kMaskTileH;
+        P.mtiles += 2 * ((s.w + kMaskTileW - 1) / kMaskTileW) * ((s.h + th - 1) / th);
+        P.mforms |= 1u << (d.mkey == 0 ? 0 : (d.mdot == 2 ? 3 : (d.mdot ? 2 : 1)));
       }
     } else if ((mp & IVQM_M_IVPSNR) || (P.need_scp && ((mp & ivs) || scp_only))) {
       if (d.mode == kSearchDot420)

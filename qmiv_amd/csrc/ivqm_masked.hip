@@ -274,6 +274,7 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
       const int ry_ = wv + 4 * k;
       if (ry_ < LH) put(ry_, lane, ra[k][0], ra[k][1], ra[k][2], ra[k][3]);
     }
+    static_assert(2 * R * LH <= 256, "one pass over the halo entries");
     const int e = (int)threadIdx.x;
     if (e < 2 * R * LH) {
       const int ry_ = e / (2 * R), rx_ = 64 + (e - ry_ * (2 * R));
@@ -363,8 +364,255 @@ __device__ void masked_tile_keyed(const PairDev& P, int dir, int x0, int y0, int
   }
 }
 
-// Form of a pair's masked search: 0 reference form, 1 keyed (narrow), 2 keyed (packed dot).
-__device__ __forceinline__ int masked_form(const PairDev& P) { return P.mkey == 0 ? 0 : (P.mdot ? 2 : 1); }
+// 4:2:0 keyed form (PairDev::mdot == 2; host-checked ranges as the packed form): the key of the keyed forms,
+// scored as in k_search420. A chroma sample covers a 2 x 2 luma block, so the pixels (2m, 2m + 1) of two rows
+// (2n, 2n + 1) share their test chroma and the chroma part of a candidate's score,
+//   X = key + dot2({-2 wCb t_Cb, -2 wCr t_Cr}, {Cb * S, Cr * S}),
+// computed once per candidate entry and pixel pair (v_dot2 over int16 pairs); a candidate then costs one
+// 24-bit multiply-add (luma, -2 S wY t_Y times Y) and half a min3. A lane owns 2 rows x 4 columns, so each
+// candidate row's LDS runs and X serve both rows. Block = 4 waves on one 64 x 32 tile of one direction.
+// LDS: keys and Y per luma entry, raw {Cb, Cr} per luma entry (the epilogue's winner samples), chroma words
+// {Cb * S, Cr * S} at chroma resolution.
+template <int R> struct Mask420 {
+  static constexpr int D = 2 * R + 1, TW = 64, TH = kMask420TileH, LW = TW + 2 * R, LH = TH + 2 * R;
+  static constexpr int R1 = R & 1;
+  static constexpr int LPK = (LW + 3) & ~3;          // luma plane pitch (ints): lane runs 16-byte aligned
+  static constexpr int LWC = (LW - 1 + R1) / 2 + 1;  // chroma entries of a region row
+  static constexpr int LHC = (LH - 1 + R1) / 2 + 1;  // chroma rows of the region
+  static constexpr int LPC = (LWC + 1) & ~1;         // chroma plane pitch (ints): lane runs 8-byte aligned
+  static constexpr int NE = 4 + 2 * R;               // luma entries of a lane's candidate run
+  static constexpr int NC = (NE - 1 + R1) / 2 + 1;   // chroma entries of the run
+  static constexpr int KB = MaskTile<R>::KB;
+  static constexpr int LDS_INTS = 3 * LH * LPK + LHC * LPC;
+  static_assert(TH == 32, "4 waves x 8 rows");
+  static_assert(NE % 2 == 0 && NC % 2 == 0, "run lengths");
+};
+
+template <typename T, typename TM, int R>
+__device__ void masked420_tile(const PairDev& P, int dir, int x0, int y0, int* lds, int4 cw,
+                               const int32_t* __restrict__ gcd, uint64_t* __restrict__ rows64,
+                               uint16_t* __restrict__ scp) {
+  using MT = Mask420<R>;
+  constexpr int D = MT::D, LH = MT::LH, LPK = MT::LPK, LPC = MT::LPC, NE = MT::NE, NC = MT::NC, R1 = MT::R1;
+  constexpr int S = 1 << MT::KB, NHC = 2 * R;
+  int (*lk)[LPK] = reinterpret_cast<int (*)[LPK]>(lds);
+  int (*lyv)[LPK] = reinterpret_cast<int (*)[LPK]>(lds + LH * LPK);
+  uint32_t (*lcu)[LPK] = reinterpret_cast<uint32_t (*)[LPK]>(lds + 2 * LH * LPK);
+  int (*lcw)[LPC] = reinterpret_cast<int (*)[LPC]>(lds + 3 * LH * LPK);
+  const int W = P.w, H = P.h;
+  const PicDev& Tp = dir ? P.p1 : P.p0;  // direction 0: P0 + GCD searched in P1; 1: P1 - GCD in P0
+  const PicDev& Rp = dir ? P.p0 : P.p1;
+  const TM* msk = (const TM*)P.msk;
+  const int ms = P.msk_stride;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int rp = lane >> 4;
+  const int iy0 = 8 * wv + 2 * rp, ix = 4 * (lane & 15);  // the lane's rows iy0, iy0 + 1 and columns ix .. ix + 3
+  __builtin_assume(iy0 >= 0 && iy0 < 32);
+  const int sg = dir ? -1 : 1;
+  const int g0 = sg * gcd[4 * P.res_idx], g1 = sg * gcd[4 * P.res_idx + 1], g2 = sg * gcd[4 * P.res_idx + 2];
+  const __amdgpu_buffer_rsrc_t rY = plane_rsrc<T>(Rp, 0, H), rCb = plane_rsrc<T>(Rp, 1, H), rCr = plane_rsrc<T>(Rp, 2, H);
+  const uint32_t sY = (uint32_t)plane_stride(Rp, 0) * (uint32_t)sizeof(T), sC = (uint32_t)plane_stride(Rp, 1) * (uint32_t)sizeof(T);
+  const __amdgpu_buffer_rsrc_t rM =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<TM*>(msk), (short)0, H * ms * (int)sizeof(TM), 0x00020000);
+  const uint32_t sM = (uint32_t)ms * (uint32_t)sizeof(TM);
+  // ---- test samples t = s + sg * GCD of the lane's 8 pixels (chroma: one sample per pixel pair, both rows)
+  // and their weights (0 outside the picture); loaded before the region so both sets of loads are in flight
+  int tY[2][4], tC[2][2];
+  uint32_t m[2][4];
+  {
+    const __amdgpu_buffer_rsrc_t qY = plane_rsrc<T>(Tp, 0, H), qCb = plane_rsrc<T>(Tp, 1, H), qCr = plane_rsrc<T>(Tp, 2, H);
+    const uint32_t tsY = (uint32_t)plane_stride(Tp, 0) * (uint32_t)sizeof(T), tsC = (uint32_t)plane_stride(Tp, 1) * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int y = y0 + iy0 + r;
+      const uint32_t yc = (uint32_t)min(y, H - 1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t xk = (uint32_t)min(x0 + ix + k, W - 1);
+        tY[r][k] = ld_buf<T>(qY, xk * (uint32_t)sizeof(T) + yc * tsY, 0) + g0;
+        m[r][k] = (y < H && x0 + ix + k < W) ? (uint32_t)ld_buf<TM>(rM, xk * (uint32_t)sizeof(TM) + yc * sM, 0) : 0u;
+      }
+    }
+    const uint32_t ycc = ((uint32_t)min(y0 + iy0, H - 1) >> 1) * tsC;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const uint32_t xc = ((uint32_t)min(x0 + ix + 2 * h, W - 1) >> 1) * (uint32_t)sizeof(T);
+      tC[0][h] = ld_buf<T>(qCb, xc + ycc, 0) + g1;
+      tC[1][h] = ld_buf<T>(qCr, xc + ycc, 0) + g2;
+    }
+  }
+  // ---- region: rows y0 - R + ly, columns x0 - R + lx (clamped). Region rows go in pairs (2j - R1, 2j + 1 - R1)
+  // of one chroma row j; wave wv stages pairs wv, wv + 4, ...; lane = region column, the 2R halo columns
+  // 64 .. 64 + 2R - 1 by the first threads.
+  const uint32_t uY = (uint32_t)cw.x, uCb = (uint32_t)cw.y, uCr = (uint32_t)cw.z;
+  auto entry = [&](int ly, int lx, uint32_t Y, uint32_t Cb, uint32_t Cr, uint32_t M) {
+    // Q(r) = sum w r^2 exactly: w * r < 2^24 and S * Q + off < 2^31 (host-checked)
+    const uint32_t q = __umul24(__umul24(uY, Y), Y) + __umul24(__umul24(uCb, Cb), Cb) + __umul24(__umul24(uCr, Cr), Cr);
+    lk[ly][lx] = M ? (int)(q * (uint32_t)S + (uint32_t)(ly * D + lx)) : P.mkey;
+    lyv[ly][lx] = (int)Y;
+    lcu[ly][lx] = Cb | (Cr << 16);
+  };
+  auto cwriter = [&](int lx) -> bool { return R1 ? (lx == 0 || (lx & 1)) : !(lx & 1); };
+  {
+    constexpr int NPW = (MT::LHC + 3) / 4;  // row pairs per wave
+    const uint32_t cx = (uint32_t)clampi(x0 - R + lane, 0, W - 1);
+    const uint32_t oYx = cx * (uint32_t)sizeof(T), oCx = (cx >> 1) * (uint32_t)sizeof(T), oMx = cx * (uint32_t)sizeof(TM);
+    uint32_t ra[NPW][6];  // Y, M of the pair's two rows, Cb, Cr
+#pragma unroll
+    for (int k = 0; k < NPW; k++) {
+      const int j = wv + 4 * k;
+      if (j < MT::LHC) {
+        const int ly0 = 2 * j - R1, ly1 = ly0 + 1;
+        const uint32_t ya = (uint32_t)clampi(y0 - R + ly0, 0, H - 1), yb = (uint32_t)clampi(y0 - R + ly1, 0, H - 1);
+        if (ly0 >= 0) {
+          ra[k][0] = (uint32_t)ld_buf<T>(rY, oYx, ya * sY);
+          ra[k][1] = (uint32_t)ld_buf<TM>(rM, oMx, ya * sM);
+        }
+        if (ly1 < LH) {
+          ra[k][2] = (uint32_t)ld_buf<T>(rY, oYx, yb * sY);
+          ra[k][3] = (uint32_t)ld_buf<TM>(rM, oMx, yb * sM);
+        }
+        const uint32_t yc = (ly0 >= 0 ? ya : yb) >> 1;
+        ra[k][4] = (uint32_t)ld_buf<T>(rCb, oCx, yc * sC);
+        ra[k][5] = (uint32_t)ld_buf<T>(rCr, oCx, yc * sC);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NPW; k++) {
+      const int j = wv + 4 * k;
+      if (j < MT::LHC) {
+        const int ly0 = 2 * j - R1;
+        if (ly0 >= 0) entry(ly0, lane, ra[k][0], ra[k][4], ra[k][5], ra[k][1]);
+        if (ly0 + 1 < LH) entry(ly0 + 1, lane, ra[k][2], ra[k][4], ra[k][5], ra[k][3]);
+        if (cwriter(lane)) lcw[j][(lane + R1) >> 1] = (int)((ra[k][4] * (uint32_t)S) | ((ra[k][5] * (uint32_t)S) << 16));
+      }
+    }
+    for (int e = (int)threadIdx.x; e < NHC * LH; e += 256) {  // (more than one pass for R = 4)
+      const int hy = e / NHC, lx = 64 + (e - hy * NHC);
+      const uint32_t hx = (uint32_t)clampi(x0 - R + lx, 0, W - 1), hyy = (uint32_t)clampi(y0 - R + hy, 0, H - 1);
+      const uint32_t Y = (uint32_t)ld_buf<T>(rY, hx * (uint32_t)sizeof(T) + hyy * sY, 0);
+      const uint32_t oc = (hx >> 1) * (uint32_t)sizeof(T) + (hyy >> 1) * sC;
+      const uint32_t Cb = (uint32_t)ld_buf<T>(rCb, oc, 0), Cr = (uint32_t)ld_buf<T>(rCr, oc, 0);
+      const uint32_t M = (uint32_t)ld_buf<TM>(rM, hx * (uint32_t)sizeof(TM) + hyy * sM, 0);
+      entry(hy, lx, Y, Cb, Cr, M);
+      if (cwriter(lx)) lcw[(hy + R1) >> 1][(lx + R1) >> 1] = (int)((Cb * (uint32_t)S) | ((Cr * (uint32_t)S) << 16));
+    }
+  }
+  __syncthreads();
+  // ---- multipliers: luma -2 S wY t_Y (|value| < 2^23), chroma {-2 wCb t_Cb, -2 wCr t_Cr} as an int16 pair
+  // (|value| <= 32767), host-checked
+  int kY[2][4];
+  uint32_t mc[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) kY[r][k] = -2 * S * cw.x * tY[r][k];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+    mc[h] = ((uint32_t)(-2 * cw.y * tC[0][h]) & 0xffffu) | ((uint32_t)(-2 * cw.z * tC[1][h]) << 16);
+  int bk[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) bk[r][k] = INT_MAX;
+  // Candidate rows cr = 0 .. 2R + 1 (region rows iy0 + cr): pixel row 0 takes dy = cr, row 1 dy = cr - 1.
+  struct Run {
+    int K[NE], Y[NE], C[NC];
+  };
+  // the run of region row iy0 + cr from its key entry kp (the Y entries follow at a constant distance) and of
+  // chroma row (iy0 + cr + R1) >> 1 = (iy0 >> 1) + ((cr + R1) >> 1) (iy0 is even) from cp
+  auto load_run = [&](const int* kp, const int* cp, Run& u) {
+    lds_ints<NE>(kp, u.K);
+    lds_ints<NE>(kp + LH * LPK, u.Y);
+#pragma unroll
+    for (int i = 0; i < NC / 2; i++) {
+      typedef int i32x2m_t __attribute__((ext_vector_type(2)));
+      const i32x2m_t w = reinterpret_cast<const i32x2m_t*>(cp)[i];
+      u.C[2 * i] = w.x;
+      u.C[2 * i + 1] = w.y;
+    }
+  };
+  auto score = [&](const Run& u, bool row0, bool row1) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // pixel pair h = pixels 2h, 2h + 1 and its entries 2h .. 2h + 1 + 2R
+      int X[NE];
+#pragma unroll
+      for (int e = 2 * h; e < 2 * h + 2 + 2 * R; e++)
+        X[e] = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2m_t, mc[h]), __builtin_bit_cast(s16x2m_t, u.C[(e + R1) >> 1]),
+                                      u.K[e], false);
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        if (r == 0 ? !row0 : !row1) continue;
+#pragma unroll
+        for (int k = 2 * h; k < 2 * h + 2; k++)
+#pragma unroll
+          for (int dx = 0; dx < D; dx++) bk[r][k] = min(bk[r][k], mad_i24(kY[r][k], u.Y[k + dx], X[k + dx]));
+      }
+    }
+  };
+  // (iteration i scores rows 2i + 1, 2i + 2 and loads rows 2i + 2, 2i + 3: the addresses advance by constant
+  // steps)
+  Run ua, ub;
+  const int* kp = &lk[iy0][ix];
+  const int* cp = &lcw[iy0 >> 1][ix >> 1];
+  load_run(kp, cp, ua);
+  load_run(kp + LPK, cp + R1 * LPC, ub);
+  score(ua, true, false);
+#pragma unroll 1
+  for (int i = 0; i < R; i++) {
+    kp += 2 * LPK;
+    cp += LPC;
+    load_run(kp, cp, ua);
+    score(ub, true, true);
+    load_run(kp + LPK, cp + R1 * LPC, ub);
+    score(ua, true, true);
+  }
+  score(ub, false, true);
+  // ---- epilogue, one pixel row at a time: winners, weighted squared differences, row sums, SCP
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int iy = iy0 + r, y = y0 + iy;
+    uint64_t v[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = x0 + ix + k;
+      if (m[r][k] != 0) {
+        const uint32_t idx = (uint32_t)(bk[r][k] - (iy * D + ix + k)) & (uint32_t)(S - 1);
+        const uint32_t wdy = min(idx / (uint32_t)D, (uint32_t)(2 * R)), wdx = min(idx - (idx / (uint32_t)D) * (uint32_t)D, (uint32_t)(2 * R));
+        const int ey = iy + (int)wdy, ex = ix + k + (int)wdx;
+        const int rYv = lyv[ey][ex];
+        const uint32_t cu = lcu[ey][ex];
+        const int rCb = (int)(cu & 0xffffu), rCr = (int)(cu >> 16);
+        const int d0 = tY[r][k] - rYv, d1 = tC[0][k >> 1] - rCb, d2 = tC[1][k >> 1] - rCr;
+        // |d| < 2^15: the squares are positive int32
+        v[0] += (uint64_t)(int64_t)(d0 * d0) * (uint64_t)m[r][k];
+        v[1] += (uint64_t)(int64_t)(d1 * d1) * (uint64_t)m[r][k];
+        v[2] += (uint64_t)(int64_t)(d2 * d2) * (uint64_t)m[r][k];
+        if (P.scp_off >= 0) {
+          const int r3[3] = {rYv, rCb, rCr};
+          const int g3[3] = {g0, g1, g2};
+          masked_scp_store(P, dir, x, y, true, r3, g3, scp);
+        }
+      } else if (P.scp_off >= 0 && y < H && x < W) {
+        const int z3[3] = {0, 0, 0};
+        masked_scp_store(P, dir, x, y, false, z3, z3, scp);
+      }
+    }
+    const uint64_t s0 = row16_sum_u64_dpp(v[0]), s1 = row16_sum_u64_dpp(v[1]), s2 = row16_sum_u64_dpp(v[2]);
+    if ((lane & 15) == 15 && y < H) {
+      uint64_t* rsp = rows64 + P.rows64_off + (int64_t)dir * 3 * H + y;
+      atomicAdd((unsigned long long*)rsp, (unsigned long long)s0);
+      atomicAdd((unsigned long long*)(rsp + H), (unsigned long long)s1);
+      atomicAdd((unsigned long long*)(rsp + 2 * H), (unsigned long long)s2);
+    }
+  }
+}
+
+// Form of a pair's masked search: 0 reference form, 1 keyed (narrow), 2 keyed (packed dot), 3 keyed 4:2:0.
+__device__ __forceinline__ int masked_form(const PairDev& P) {
+  return P.mkey == 0 ? 0 : (P.mdot == 2 ? 3 : (P.mdot ? 2 : 1));
+}
 
 // One launch per form present in the batch (the forms' register needs differ: one kernel would run
 // every form at the largest); tiles of pairs of other forms exit at once.
@@ -373,19 +621,29 @@ __global__ __launch_bounds__(256) void k_search_masked(const PairDev* __restrict
                                                        int4 cw, const int32_t* __restrict__ gcd,
                                                        uint64_t* __restrict__ rows64, uint16_t* __restrict__ scp) {
   using MT = MaskTile<R>;
-  // keyed forms: int planes of LH x LWP; the reference form: LH x LW 8-byte entries
-  __shared__ __attribute__((aligned(16))) int ldsi[(F == 2 ? 3 : 4) * MT::LH * MT::LWP];
+  // keyed forms: int planes of LH x LWP (4:2:0 form: its planes); the reference form: LH x LW 8-byte entries
+  constexpr int NI = F == 3 ? Mask420<R>::LDS_INTS : (F == 2 ? 3 : 4) * MT::LH * MT::LWP;
+  __shared__ __attribute__((aligned(16))) int ldsi[NI];
   const int b = blockIdx.x;
   if (b >= ntiles) return;
   const int pi = find_entry(pairs, npairs, b, [](const PairDev& p) { return p.mtile0; });
   const PairDev& P = pairs[pi];
   if (masked_form(P) != F) return;  // block-uniform
-  const int nx = (P.w + kMaskTileW - 1) / kMaskTileW, ny = (P.h + kMaskTileH - 1) / kMaskTileH;
+  constexpr int TH = F == 3 ? kMask420TileH : kMaskTileH;
+  const int nx = (P.w + kMaskTileW - 1) / kMaskTileW, ny = (P.h + TH - 1) / TH;
   const int lt = b - P.mtile0;
   const int dir = lt / (nx * ny), rem = lt - dir * nx * ny;
   const int ty = rem / nx, tx = rem - ty * nx;
-  const int x0 = tx * kMaskTileW, y0 = ty * kMaskTileH;
-  if (F == 0) {
+  const int x0 = tx * kMaskTileW, y0 = ty * TH;
+  if (F == 3) {
+    if (P.p0.bps == 2) {
+      if (P.msk_bps == 2) masked420_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+      else masked420_tile<uint16_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+    } else {
+      if (P.msk_bps == 2) masked420_tile<uint8_t, uint16_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+      else masked420_tile<uint8_t, uint8_t, R>(P, dir, x0, y0, ldsi, cw, gcd, rows64, scp);
+    }
+  } else if (F == 0) {
     auto lds = reinterpret_cast<u32x2m_t (*)[MT::LW]>(ldsi);
     if (P.p0.bps == 2) {
       if (P.msk_bps == 2) masked_tile<uint16_t, uint16_t, R>(P, dir, x0, y0, lds, cw, gcd, rows64, scp);
@@ -412,6 +670,7 @@ hipError_t masked_r(const PairDev* pairs, int npairs, int ntiles, int4 w, const 
   if (forms & 1u) hipLaunchKernelGGL((k_search_masked<R, 0>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
   if (forms & 2u) hipLaunchKernelGGL((k_search_masked<R, 1>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
   if (forms & 4u) hipLaunchKernelGGL((k_search_masked<R, 2>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
+  if (forms & 8u) hipLaunchKernelGGL((k_search_masked<R, 3>), dim3(ntiles), dim3(256), 0, st, pairs, npairs, ntiles, w, gcd, rows64, scp);
   return hipGetLastError();
 }
 

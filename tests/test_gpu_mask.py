@@ -71,11 +71,12 @@ def test_mask_against_oracle_640x360(oracle, c420, bdm, R):
     assert list(r.ssd[:3]) == [oracle.weighted_ssd(p0[c], p1[c], m) for c in range(3)]
 
 
-@pytest.mark.parametrize("bd,R,c420", [(10, 1, True), (10, 2, True), (8, 3, False), (8, 2, True), (10, 2, False)])
+@pytest.mark.parametrize("bd,R,c420", [(10, 1, True), (10, 2, True), (8, 3, False), (8, 2, True), (10, 2, False),
+                                       (8, 3, True), (10, 3, True), (10, 4, True)])
 def test_masked_search_keyed_form_equals_reference_form(oracle, bd, R, c420):
-    """The keyed masked search (packed dot key where the ranges allow, search_mode_cap 0; narrow key,
-    cap 2; masked candidates keyed above every considered one) and the reference-form masked search
-    (cap 1) give the same row sums, and they match the oracle."""
+    """The keyed masked search (search_mode_cap 0: the 4:2:0 form for 4:2:0 pairs, else the packed dot
+    key, where the ranges allow; narrow key, cap 2; masked candidates keyed above every considered one)
+    and the reference-form masked search (cap 1) give the same row sums, and they match the oracle."""
     wl = synth.Workload(cfg=170 + R, width=328, height=200, bit_depth=bd, chroma_420=c420, global_shift=1)
     p0n, p1n = synth.make_pair(wl, 0, as_u8=bd == 8)
     m = synth.make_mask(wl.width, wl.height, 10, seed=11 + R)
@@ -118,9 +119,9 @@ def test_masked_scp_matches_reference_golden(case):
     assert [_sha1(p) for p in s1] == case["scp1_sha1"]
 
 
-@pytest.mark.parametrize("bd,R,c420", [(10, 2, True), (8, 3, False), (10, 4, True)])
+@pytest.mark.parametrize("bd,R,c420", [(10, 2, True), (8, 3, False), (10, 4, True), (8, 3, True), (10, 1, True)])
 def test_masked_scp_every_search_form_against_oracle(oracle, bd, R, c420):
-    """Masked SCP generation in each masked search form (packed dot key, reference form, narrow key)
+    """Masked SCP generation in each masked search form (4:2:0 or packed dot key, reference form, narrow key)
     at 328x200: identical pictures, equal to the oracle's GenShftCompPicsM with 0 at masked pixels;
     a masked IV-PSNR call on the same context afterwards is unchanged."""
     wl = synth.Workload(cfg=190 + R, width=328, height=200, bit_depth=bd, chroma_420=c420, global_shift=1)
