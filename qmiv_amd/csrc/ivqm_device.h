@@ -194,7 +194,7 @@ int search420_key_bits(int R);
 int search420_sample_bits(int R);
 hipError_t launch_search420(const PairDev* pairs, int npairs, int ntiles, int R, int th, const int32_t cw[4],
                             const int32_t* gcd, uint32_t* rows, uint16_t* scp, uint8_t* best, hipStream_t st);
-constexpr int kDownBlkW = 256, kDownBlkH = 4;  // k_downsample: outputs per block row (4 per lane), rows
+constexpr int kDownBlkW = 512, kDownBlkH = 4;  // k_downsample: outputs per block row (8 per lane), rows
 // MI355X: 8 XCDs, each with its own L2; consecutive workgroups go to consecutive XCDs.
 constexpr int kXcds = 8;
 

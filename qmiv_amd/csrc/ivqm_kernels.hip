@@ -1430,68 +1430,81 @@ __device__ __forceinline__ void down_px(const PicDev& S, int c, int x, int y, ui
   dst[(size_t)c * pitch * dh + (size_t)y * pitch + x] = (uint16_t)(v >> 2);
 }
 
-// Four adjacent outputs x4 .. x4 + 3 of row y (x4 + 3 < dw): the two source rows' 8 samples as two
-// vector loads each where the row is aligned for them (wave-uniform: one row per wave), one 8-byte
-// store (pyramid rows are 128-byte aligned). A 4:2:0 source's chroma is the level-1 chroma itself
-// (the 2x2 block replicates one sample), so it is copied; a 4:2:2 source's 2x2 block holds two
-// samples twice each, one above the other.
-template <typename T>
-__device__ __forceinline__ void down_x4(const PicDev& S, int c, int x4, int y, uint16_t* dst, int pitch, int dh) {
-  typedef __attribute__((address_space(1))) const T GT;
-  typedef unsigned short u16x4d_t __attribute__((ext_vector_type(4)));
-  uint32_t o[4];
-  const int cf = c ? chroma_format(S) : 0;
-  if (cf == 1) {
-    GT* row = (GT*)plane_ptr(S, c) + (size_t)y * plane_stride(S, c);
-#pragma unroll
-    for (int k = 0; k < 4; k++) o[k] = row[x4 + k];
-  } else if (cf == 2) {
-    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c);
-    GT* r1 = r0 + plane_stride(S, c);
-#pragma unroll
-    for (int k = 0; k < 4; k++) o[k] = (2 * (uint32_t)r0[x4 + k] + 2 * (uint32_t)r1[x4 + k] + 2) >> 2;
-  } else {
-    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c);
-    GT* r1 = r0 + plane_stride(S, c);
-    uint32_t a[8], b[8];
-    constexpr int AL = 4 * (int)sizeof(T);  // a 4-sample vector
-    if (((reinterpret_cast<uintptr_t>(r0) | reinterpret_cast<uintptr_t>(r1)) & (AL - 1)) == 0) {
-      if (sizeof(T) == 2) {
-        typedef __attribute__((address_space(1))) const u16x4d_t GV;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const u16x4d_t va = *(GV*)(r0 + 2 * x4 + 4 * h), vb = *(GV*)(r1 + 2 * x4 + 4 * h);
-          a[4 * h] = va.x; a[4 * h + 1] = va.y; a[4 * h + 2] = va.z; a[4 * h + 3] = va.w;
-          b[4 * h] = vb.x; b[4 * h + 1] = vb.y; b[4 * h + 2] = vb.z; b[4 * h + 3] = vb.w;
-        }
-      } else {
-        typedef __attribute__((address_space(1))) const uint32_t GW;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const uint32_t wa = *(GW*)(r0 + 2 * x4 + 4 * h), wb = *(GW*)(r1 + 2 * x4 + 4 * h);
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            a[4 * h + k] = (wa >> (8 * k)) & 0xffu;
-            b[4 * h + k] = (wb >> (8 * k)) & 0xffu;
-          }
-        }
-      }
+// M consecutive samples from p with the widest loads the address allows (16, 8 or 4 bytes; single
+// samples otherwise). The alignment of p is the row's: every lane's offset is a multiple of M samples, so
+// the branch is wave-uniform.
+template <typename T, int M>
+__device__ __forceinline__ void load_samples(const __attribute__((address_space(1))) T* p, uint32_t (&v)[M]) {
+  typedef __attribute__((address_space(1))) const uint8_t GB;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  constexpr int BYTES = M * (int)sizeof(T);
+  auto unpack = [&](uint32_t w, int i0) {  // one 32-bit word: 2 (u16) or 4 (u8) samples
+    if (sizeof(T) == 2) {
+      v[i0] = w & 0xffffu;
+      v[i0 + 1] = w >> 16;
     } else {
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        a[k] = r0[2 * x4 + k];
-        b[k] = r1[2 * x4 + k];
-      }
+      for (int k = 0; k < 4; k++) v[i0 + k] = (w >> (8 * k)) & 0xffu;
     }
+  };
+  constexpr int SPW = 4 / (int)sizeof(T);  // samples per 32-bit word
+  if (BYTES % 16 == 0 && (a & 15) == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) o[k] = (a[2 * k] + a[2 * k + 1] + b[2 * k] + b[2 * k + 1] + 2) >> 2;
+    for (int i = 0; i < BYTES / 16; i++) {
+      const u32x4_t w = *(const __attribute__((address_space(1))) u32x4_t*)((GB*)p + 16 * i);
+      unpack(w.x, (4 * i) * SPW);
+      unpack(w.y, (4 * i + 1) * SPW);
+      unpack(w.z, (4 * i + 2) * SPW);
+      unpack(w.w, (4 * i + 3) * SPW);
+    }
+  } else if (BYTES % 8 == 0 && (a & 7) == 0) {
+#pragma unroll
+    for (int i = 0; i < BYTES / 8; i++) {
+      typedef unsigned int u32x2d_t __attribute__((ext_vector_type(2)));
+      const u32x2d_t w = *(const __attribute__((address_space(1))) u32x2d_t*)((GB*)p + 8 * i);
+      unpack(w.x, (2 * i) * SPW);
+      unpack(w.y, (2 * i + 1) * SPW);
+    }
+  } else if (BYTES % 4 == 0 && (a & 3) == 0) {
+#pragma unroll
+    for (int i = 0; i < BYTES / 4; i++) unpack(*(const __attribute__((address_space(1))) uint32_t*)((GB*)p + 4 * i), i * SPW);
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; k++) v[k] = p[k];
   }
-  typedef __attribute__((address_space(1))) u16x4d_t GO;
-  const u16x4d_t v = {(unsigned short)o[0], (unsigned short)o[1], (unsigned short)o[2], (unsigned short)o[3]};
-  *(GO*)(dst + (size_t)c * pitch * dh + (size_t)y * pitch + x4) = v;
 }
 
-// Block = 4 rows x 256 outputs (a lane makes 4 adjacent outputs); the last partial group of a row
+// Eight adjacent outputs x8 .. x8 + 7 of row y (x8 + 7 < dw): the two source rows' 16 samples each with the
+// widest loads their alignment allows (wave-uniform: one row per wave), one 16-byte store (pyramid rows are
+// 256-byte aligned). A 4:2:0 source's chroma is the level-1 chroma itself (the 2x2 block replicates one
+// sample), so it is copied; a 4:2:2 source's 2x2 block holds two samples twice each, one above the other.
+template <typename T>
+__device__ __forceinline__ void down_x8(const PicDev& S, int c, int x8, int y, uint16_t* dst, int pitch, int dh) {
+  typedef __attribute__((address_space(1))) const T GT;
+  uint32_t o[8];
+  const int cf = c ? chroma_format(S) : 0;
+  if (cf == 1) {
+    load_samples<T, 8>((GT*)plane_ptr(S, c) + (size_t)y * plane_stride(S, c) + x8, o);
+  } else if (cf == 2) {
+    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c) + x8;
+    uint32_t a[8], b[8];
+    load_samples<T, 8>(r0, a);
+    load_samples<T, 8>(r0 + plane_stride(S, c), b);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (2 * a[k] + 2 * b[k] + 2) >> 2;
+  } else {
+    GT* r0 = (GT*)plane_ptr(S, c) + (size_t)(2 * y) * plane_stride(S, c) + 2 * x8;
+    uint32_t a[16], b[16];
+    load_samples<T, 16>(r0, a);
+    load_samples<T, 16>(r0 + plane_stride(S, c), b);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (a[2 * k] + a[2 * k + 1] + b[2 * k] + b[2 * k + 1] + 2) >> 2;
+  }
+  const u32x4_t v = {o[0] | (o[1] << 16), o[2] | (o[3] << 16), o[4] | (o[5] << 16), o[6] | (o[7] << 16)};
+  *(__attribute__((address_space(1))) u32x4_t*)(dst + (size_t)c * pitch * dh + (size_t)y * pitch + x8) = v;
+}
+
+// Block = 4 rows x 512 outputs (a lane makes 8 adjacent outputs); the last partial group of a row
 // takes the one-output form.
 __global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ jobs, int njobs) {
   const int b = blockIdx.x;
@@ -1504,14 +1517,14 @@ __global__ __launch_bounds__(256) void k_downsample(const DownJob* __restrict__ 
   if (c >= J.ncomp) return;  // the job writes luma only (level 1 with the caller's chroma planes)
   lb -= c * nbx * nby;
   const int by = lb / nbx, bx = lb - by * nbx;
-  const int x4 = bx * kDownBlkW + 4 * (threadIdx.x & 63);
+  const int x8 = bx * kDownBlkW + 8 * (threadIdx.x & 63);
   const int y = by * kDownBlkH + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (x4 >= dw || y >= dh) return;
-  if (x4 + 3 < dw) {
-    if (J.src.bps == 2) down_x4<uint16_t>(J.src, c, x4, y, J.dst, J.pitch, dh);
-    else down_x4<uint8_t>(J.src, c, x4, y, J.dst, J.pitch, dh);
+  if (x8 >= dw || y >= dh) return;
+  if (x8 + 7 < dw) {
+    if (J.src.bps == 2) down_x8<uint16_t>(J.src, c, x8, y, J.dst, J.pitch, dh);
+    else down_x8<uint8_t>(J.src, c, x8, y, J.dst, J.pitch, dh);
   } else {
-    for (int x = x4; x < dw; x++) {
+    for (int x = x8; x < dw; x++) {
       if (J.src.bps == 2) down_px<uint16_t>(J.src, c, x, y, J.dst, J.pitch, dh);
       else down_px<uint8_t>(J.src, c, x, y, J.dst, J.pitch, dh);
     }
