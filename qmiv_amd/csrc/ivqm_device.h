@@ -123,8 +123,15 @@ struct DownJob {
 // Launchers (ivqm_kernels.hip). All are asynchronous on `st`; tables live in
 // device memory. `nblk` is the total block count of the flattened grid.
 // Masked pairs take 3 x h stats rows (every component at 4:4:4), unmasked pairs h + 2 x (h >> cs).
+// k_stats form of a pair: 0 unmasked 16-bit containers with samples below 2^12 (32-bit partial sums,
+// a kernel of few registers), 1 everything else (8-bit samples, wider sums, masks). One launch per form
+// present; `forms` is the bitmask of stats_form_bit over the batch's pairs with statistics.
+__host__ __device__ inline int stats_form(const PairDev& p) {
+  return (p.msk == nullptr && p.p0.bps == 2 && p.maxv < 4096) ? 0 : 1;
+}
+__host__ __device__ inline uint32_t stats_form_bit(const PairDev& p) { return 1u << stats_form(p); }
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st);
+                        uint32_t* rows, int64_t* aux, uint64_t* rows64, uint32_t forms, hipStream_t st);
 // mcount: per pair {non-zero mask samples, differing samples of Y, Cb, Cr} (masked pairs only).
 hipError_t launch_gcd_finalize(const PairDev* pairs, int npairs, const int64_t* row_sd, const uint64_t* row_ssd,
                                const int64_t* aux, int32_t* gcd, int64_t* sd, uint64_t* ssd, int64_t* mcount,

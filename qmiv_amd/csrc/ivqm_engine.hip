@@ -307,6 +307,13 @@ int Engine::erp_offset(int h, int64_t* off) {
   return IVQM_OK;
 }
 
+// Which k_stats launches a plan needs (stats_form_bit over its pairs).
+static uint32_t stats_forms(const BatchPlan& P) {
+  uint32_t f = 0;
+  for (int i = 0; i < P.n; i++) f |= stats_form_bit(P.pairs[i]);
+  return f;
+}
+
 // Which k_ssim launches a plan needs: bit 0 jobs that read pictures, bit 1 jobs that read fused moments.
 static uint32_t ssim_forms(const BatchPlan& P) {
   uint32_t f = 0;
@@ -873,7 +880,8 @@ int Engine::issue(const BatchPlan& P, RingSlot& S) {
   if (nj) HIPCHK(hipMemcpyAsync(d_jobs_, S.t_jobs, sizeof(SsimJobDev) * nj, hipMemcpyHostToDevice, st));
   if (nd) HIPCHK(hipMemcpyAsync(d_down_, S.t_down, sizeof(DownJob) * nd, hipMemcpyHostToDevice, st));
   if (P.need_stats) {
-    HIPCHK(launch_stats(d_pairs_, n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, st));
+    HIPCHK(launch_stats(d_pairs_, n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, stats_forms(P),
+                        st));
     HIPCHK(launch_gcd_finalize(d_pairs_, n, d_row_sd_, d_row_ssd_, d_aux_, d_gcd_, d_sd_, d_ssd_, d_mcount_, st));
   }
   if (P.mtiles)
@@ -1127,7 +1135,8 @@ int Engine::profile(int kernel, int iters, double* avg_ms) {
     if (i == 0) HIPCHK(hipEventRecord(e0, stream_));
     switch (kernel) {
       case 0:
-        HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_, stream_));
+        HIPCHK(launch_stats(d_pairs_, P.n, P.stat_blocks, d_row_sd_, d_row_ssd_, d_rows_, d_aux_, d_rows64_,
+                            stats_forms(P), stream_));
         break;
       case 1:
         HIPCHK(launch_search_all(P, stream_));

@@ -116,8 +116,9 @@ __device__ __forceinline__ void stats_vec64(const uint4& va, const uint4& vr, in
   ssd += q;
 }
 
-// Sums of one plane row (lb counts the rows of Y, then Cb, then Cr) by one wave.
-template <typename T>
+// Sums of one plane row (lb counts the rows of Y, then Cb, then Cr) by one wave. NARROW: the caller
+// guarantees samples below 2^12 (the 32-bit partial form only).
+template <typename T, bool NARROW = false>
 __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, int64_t* __restrict__ row_sd,
                                           uint64_t* __restrict__ row_ssd) {
   const int H = P.h, W = P.w, Hc = H >> P.p0.csy, Wc = W >> P.p0.cs;
@@ -160,7 +161,7 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
         }
         sd += s;
         ssd += q;
-      } else if (P.maxv < 4096) {
+      } else if (NARROW || P.maxv < 4096) {
         int32_t s = 0;
         uint32_t q = 0;
 #pragma unroll
@@ -225,6 +226,9 @@ __device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64
 
 // nrows = sum over pairs of H + 2 * Hc (masked pairs: 3 H); a pair's rows start at stat_blk0.
 // Each luma row y also clears the six search row sums of y.
+// FORM (stats_form): 0 the narrow 16-bit form alone, 46 VGPRs (10 waves per SIMD, so more rows' loads in
+// flight); 1 every other pair. Rows of pairs of the other form are skipped.
+template <int FORM>
 __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs, int nrows,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
                                                uint32_t* __restrict__ rows, int64_t* __restrict__ aux,
@@ -237,6 +241,12 @@ __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs
     const int pi = find_entry(pairs, npairs, rw, [](const PairDev& p) { return p.stat_blk0; });
     const PairDev& P = pairs[pi];
     const int lb = rw - P.stat_blk0;
+    if (stats_form(P) != FORM) continue;  // wave-uniform
+    if constexpr (FORM == 0) {
+      if (lb < P.h && lane < 6) rows[P.rows_off + (int64_t)lane * P.h + lb] = 0u;
+      stats_row<uint16_t, true>(P, lb, lane, row_sd, row_ssd);
+      continue;
+    }
     if (P.msk) {
       const int c = lb / P.h, y = lb - c * P.h;
       if (c == 0 && lane < 6) rows64[P.rows64_off + (int64_t)lane * P.h + y] = 0u;
@@ -1619,12 +1629,18 @@ hipError_t launch_colour(const PicDev& src, int w, int h, int bit_depth, int dir
 }
 
 hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* rows, int64_t* aux, uint64_t* rows64, hipStream_t st) {
+                        uint32_t* rows, int64_t* aux, uint64_t* rows64, uint32_t forms, hipStream_t st) {
   if (npairs <= 0 || nblk <= 0) return hipSuccess;
-  // nblk = total plane rows; one wave per row, grid-stride. 73 VGPRs: 6 waves per SIMD, so
-  // 256 CUs x 24 waves = 1536 blocks of 4 waves fill the chip once
-  const int grid = std::min((nblk + 3) / 4, 1536);
-  hipLaunchKernelGGL(k_stats, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
+  // nblk = total plane rows; one wave per row, grid-stride. Each form's grid fills the chip once:
+  // form 0 at 10 waves per SIMD (256 CUs x 40 waves = 2560 blocks of 4 waves), form 1 (73 VGPRs) at 6
+  if (forms & 1u) {
+    const int grid = std::min((nblk + 3) / 4, 2560);
+    hipLaunchKernelGGL(k_stats<0>, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
+  }
+  if (forms & 2u) {
+    const int grid = std::min((nblk + 3) / 4, 1536);
+    hipLaunchKernelGGL(k_stats<1>, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
+  }
   return hipGetLastError();
 }
 

@@ -168,7 +168,9 @@ int run(const ivqm_config& cfg, const ivqm_pair* pairs, int n, uint32_t m, ivqm_
     return set_error(IVQM_EINVAL, "internal: row partials outside the scratch array");
 
   NA_HIPCHK(hipMemcpyAsync(d_pairs, h_pairs, sizeof(PairDev) * n, hipMemcpyHostToDevice, st));
-  NA_HIPCHK(launch_stats(d_pairs, n, stat_blocks, d_row_sd, d_row_ssd, d_rows, nullptr, nullptr, st));
+  uint32_t stforms = 0;  // k_stats kernels needed
+  for (int i = 0; i < n; i++) stforms |= stats_form_bit(h_pairs[i]);
+  NA_HIPCHK(launch_stats(d_pairs, n, stat_blocks, d_row_sd, d_row_ssd, d_rows, nullptr, nullptr, stforms, st));
   NA_HIPCHK(launch_gcd_finalize(d_pairs, n, d_row_sd, d_row_ssd, nullptr, d_gcd, d_sd, d_ssd, nullptr, st));
   if (tiles)
     NA_HIPCHK(launch_search(d_pairs, n, tiles, cfg.search_range, cfg.cmp_weights_search, d_gcd, d_rows, nullptr,
