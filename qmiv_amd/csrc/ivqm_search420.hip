@@ -386,6 +386,7 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
     qs[c][0] = qs[c][1] = (unsigned short)0;
   }
   constexpr int UMAX = DP * 2 * R + 2 * R;
+  uint32_t qr[2][3];  // the lane's squared differences per row and component (row sums after the rows)
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     __builtin_amdgcn_sched_barrier(0);
@@ -432,13 +433,8 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
       }
       q[c] = (uint32_t)__builtin_amdgcn_sdot2(d[1], d[1], __builtin_amdgcn_sdot2(d[0], d[0], 0, false), false);
     }
-    dpp_row16_add3(q);
-    if ((lane & 15) == 15 && y < H) {
-      uint32_t* rsp = rows + pe->rows_off + (size_t)dir * 3 * H + y;
-      atomicAdd(rsp, q[0]);
-      atomicAdd(rsp + H, q[1]);
-      atomicAdd(rsp + 2 * H, q[2]);
-    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) qr[r][c] = q[c];
     if (pe->best_off >= 0 && y < H) {
       typedef __attribute__((address_space(1))) uint8_t GU8;
       const int x = x0 + ix;
@@ -494,6 +490,34 @@ __device__ __forceinline__ void search420_tile(const PairDev& Pr, const PairDev*
             for (int h = 0; h < 2; h++) qs[c][h] += sv[c][h];
         }
       }
+    }
+  }
+  // Row sums of both rows in one reduction: even lanes carry row iy0, odd lanes row iy0 + 1 (each lane hands
+  // its partner the other row's value), then sums over the lanes of one parity within each 16-lane row:
+  // lane 14 ends with row iy0's total, lane 15 with row iy0 + 1's.
+  {
+    const bool odd = lane & 1;
+    uint32_t v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const uint32_t own = odd ? qr[1][c] : qr[0][c], give = odd ? qr[0][c] : qr[1][c];
+      v[c] = own + (uint32_t)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xf, 0xf, true);  // quad_perm [1, 0, 3, 2]
+    }
+#define IVQM_R16P(ctrl)                                                                    \
+    {                                                                                      \
+      const uint32_t a0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v[0], ctrl, 0xf, 0xf, true); \
+      const uint32_t a1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v[1], ctrl, 0xf, 0xf, true); \
+      const uint32_t a2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v[2], ctrl, 0xf, 0xf, true); \
+      v[0] += a0; v[1] += a1; v[2] += a2;                                                  \
+    }
+    IVQM_R16P(0x112) IVQM_R16P(0x114) IVQM_R16P(0x118)  // row_shr 2, 4, 8
+#undef IVQM_R16P
+    const int yr = y0 + iy0 + (odd ? 1 : 0);
+    if ((lane & 15) >= 14 && yr < H) {
+      uint32_t* rsp = rows + pe->rows_off + (size_t)dir * 3 * H + yr;
+      atomicAdd(rsp, v[0]);
+      atomicAdd(rsp + H, v[1]);
+      atomicAdd(rsp + 2 * H, v[2]);
     }
   }
   if (want_scp1) {
