@@ -569,11 +569,13 @@ __device__ void masked420_tile(const PairDev& P, int dir, int x0, int y0, int* l
     score(ua, true, true);
   }
   score(ub, false, true);
-  // ---- epilogue, one pixel row at a time: winners, weighted squared differences, row sums, SCP
+  // ---- epilogue, one pixel row at a time: winners, weighted squared differences, SCP; then the row sums
+  uint64_t vr[2][3];
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     const int iy = iy0 + r, y = y0 + iy;
-    uint64_t v[3] = {0, 0, 0};
+    uint64_t* v = vr[r];
+    v[0] = v[1] = v[2] = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int x = x0 + ix + k;
@@ -599,13 +601,33 @@ __device__ void masked420_tile(const PairDev& P, int dir, int x0, int y0, int* l
         masked_scp_store(P, dir, x, y, false, z3, z3, scp);
       }
     }
-    const uint64_t s0 = row16_sum_u64_dpp(v[0]), s1 = row16_sum_u64_dpp(v[1]), s2 = row16_sum_u64_dpp(v[2]);
-    if ((lane & 15) == 15 && y < H) {
-      uint64_t* rsp = rows64 + P.rows64_off + (int64_t)dir * 3 * H + y;
-      atomicAdd((unsigned long long*)rsp, (unsigned long long)s0);
-      atomicAdd((unsigned long long*)(rsp + H), (unsigned long long)s1);
-      atomicAdd((unsigned long long*)(rsp + 2 * H), (unsigned long long)s2);
-    }
+  }
+  // Row sums of both rows in one reduction: even lanes carry row iy0, odd lanes row iy0 + 1 (each lane hands
+  // its partner the other row's value), then 64-bit sums over the lanes of one parity within each 16-lane row
+  // (row_shr 2, 4, 8): lane 14 ends with row iy0's total, lane 15 with row iy0 + 1's.
+  const bool odd = lane & 1;
+  uint64_t s[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const uint64_t own = odd ? vr[1][c] : vr[0][c], give = odd ? vr[0][c] : vr[1][c];
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)give, 0xB1, 0xf, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(give >> 32), 0xB1, 0xf, 0xf, true);
+    s[c] = own + ((uint64_t)lo | ((uint64_t)hi << 32));
+  }
+#define IVQM_ROWP(ctrl)                                                                                     \
+  _Pragma("unroll") for (int c = 0; c < 3; c++) {                                                          \
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)s[c], ctrl, 0xf, 0xf, true);         \
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(s[c] >> 32), ctrl, 0xf, 0xf, true); \
+    s[c] += (uint64_t)lo | ((uint64_t)hi << 32);                                                           \
+  }
+  IVQM_ROWP(0x112) IVQM_ROWP(0x114) IVQM_ROWP(0x118)
+#undef IVQM_ROWP
+  const int yr = y0 + iy0 + (odd ? 1 : 0);
+  if ((lane & 15) >= 14 && yr < H) {
+    uint64_t* rsp = rows64 + P.rows64_off + (int64_t)dir * 3 * H + yr;
+    atomicAdd((unsigned long long*)rsp, (unsigned long long)s[0]);
+    atomicAdd((unsigned long long*)(rsp + H), (unsigned long long)s[1]);
+    atomicAdd((unsigned long long*)(rsp + 2 * H), (unsigned long long)s[2]);
   }
 }
 
