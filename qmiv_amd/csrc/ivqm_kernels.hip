@@ -141,10 +141,13 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
         __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(a), (short)0, nv * 16, 0x00020000);
     const __amdgpu_buffer_rsrc_t br =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(r), (short)0, nv * 16, 0x00020000);
-    for (int i0 = 0; i0 < nv; i0 += 256) {
-      uint4 va[4], vr[4];
+    // the narrow form keeps a whole 4096-sample 16-bit row (8 vectors per picture and lane) in flight;
+    // its 64 squares per lane and pass are < 2^30
+    constexpr int NV = NARROW ? 8 : 4;
+    for (int i0 = 0; i0 < nv; i0 += 64 * NV) {
+      uint4 va[NV], vr[NV];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < NV; k++) {
         const uint32_t off = (uint32_t)(i0 + 64 * k + lane) * 16u;
         va[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ba, off, 0, 0));
         vr[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(br, off, 0, 0));
@@ -165,12 +168,12 @@ __device__ __forceinline__ void stats_row(const PairDev& P, int lb, int lane, in
         int32_t s = 0;
         uint32_t q = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) stats_vec32<T>(va[k], vr[k], s, q);
+        for (int k = 0; k < NV; k++) stats_vec32<T>(va[k], vr[k], s, q);
         sd += s;
         ssd += q;
       } else {
 #pragma unroll
-        for (int k = 0; k < 4; k++) stats_vec64(va[k], vr[k], sd, ssd);
+        for (int k = 0; k < NV; k++) stats_vec64(va[k], vr[k], sd, ssd);
       }
     }
     x_done = nv * V;
@@ -226,8 +229,9 @@ __device__ void stats_row_masked(const PairDev& P, int c, int y, int lane, int64
 
 // nrows = sum over pairs of H + 2 * Hc (masked pairs: 3 H); a pair's rows start at stat_blk0.
 // Each luma row y also clears the six search row sums of y.
-// FORM (stats_form): 0 the narrow 16-bit form alone, 46 VGPRs (10 waves per SIMD, so more rows' loads in
-// flight); 1 every other pair. Rows of pairs of the other form are skipped.
+// FORM (stats_form): 0 the narrow 16-bit form alone, 64 VGPRs with a whole 4096-sample row of both
+// pictures in flight per wave (8 waves per SIMD); 1 every other pair. Rows of pairs of the other form
+// are skipped.
 template <int FORM>
 __global__ __launch_bounds__(256) void k_stats(const PairDev* __restrict__ pairs, int npairs, int nrows,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
@@ -1632,9 +1636,9 @@ hipError_t launch_stats(const PairDev* pairs, int npairs, int nblk, int64_t* row
                         uint32_t* rows, int64_t* aux, uint64_t* rows64, uint32_t forms, hipStream_t st) {
   if (npairs <= 0 || nblk <= 0) return hipSuccess;
   // nblk = total plane rows; one wave per row, grid-stride. Each form's grid fills the chip once:
-  // form 0 at 10 waves per SIMD (256 CUs x 40 waves = 2560 blocks of 4 waves), form 1 (73 VGPRs) at 6
+  // form 0 (64 VGPRs) at 8 waves per SIMD (256 CUs x 32 waves = 2048 blocks of 4 waves), form 1 (73 VGPRs) at 6
   if (forms & 1u) {
-    const int grid = std::min((nblk + 3) / 4, 2560);
+    const int grid = std::min((nblk + 3) / 4, 2048);
     hipLaunchKernelGGL(k_stats<0>, dim3(grid), dim3(256), 0, st, pairs, npairs, nblk, row_sd, row_ssd, rows, aux, rows64);
   }
   if (forms & 2u) {
