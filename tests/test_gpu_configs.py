@@ -6,6 +6,8 @@
   pair by pair, to one pair per launch.
 * The fused IV-SSIM batch (two or more pairs: the search writes sub-block
   moments instead of SCP pictures) at 4096x2048 ERP against the oracle.
+* Rows wider than one pass of the row-statistics kernel (widths 4104 and 8200,
+  10-bit 4:2:0) against the oracle.
 * The GCD of a pair whose component sums pass 2^31 (8192x4096, 14-bit, both
   chroma formats), with the exact sum (default) and with the int32 sum of the
   x86-64-v4 reference build (gcd_sd_wrap32, xDistortionAVX512.cpp:14-73).
@@ -105,6 +107,28 @@ def test_fused_ivssim_batch_erp_4096x2048_matches_oracle(oracle):
     for f, ((p0, p1), r) in enumerate(zip(host, got)):
         merge_worst(worst, check_pair(oracle, r, p0, p1, wl.bit_depth, True, IVS, erp=True, label=f"frame {f}"))
     print("fused IV-SSIM 4096x2048 ERP: max |err| vs FP64 oracle:", worst, flush=True)
+
+
+def test_rows_wider_than_one_stats_pass_match_oracle(oracle):
+    """10-bit 4:2:0 pairs whose luma rows take more than one pass of the row-statistics kernel's
+    narrow form (4096 samples of both pictures per wave and pass): widths 4104 (a second pass of one
+    vector) and 8200 (three passes; its chroma rows of 4100 samples are not 16-byte aligned on odd
+    rows, so they take the scalar path), in one batch; GCD, PSNR, WS-PSNR and IV-PSNR bit for bit."""
+    import torch
+    m = Q.IVQM_M_PSNR | Q.IVQM_M_WSPSNR | Q.IVQM_M_IVPSNR
+    host, items = [], []
+    for k, w in enumerate((4104, 8200)):
+        wl = synth.Workload(cfg=760 + k, width=w, height=24, bit_depth=10)
+        p0, p1 = synth.make_pair(wl, k)
+        host.append((wl, p0, p1))
+        items.append((_dev(torch, p0), _dev(torch, p1), w, wl.height, 10, True))
+    torch.cuda.synchronize()
+    with IvqmBatch() as b:
+        got = [r.as_dict() for r in b.run(b.make_pairs(items), m)]
+    for (wl, p0, p1), r in zip(host, got):
+        label = f"{wl.width}x{wl.height} 10-bit"
+        print(label, flush=True)
+        check_pair(oracle, r, p0, p1, 10, True, m, label=label)
 
 
 @pytest.mark.parametrize("chroma_420", [True, False])
