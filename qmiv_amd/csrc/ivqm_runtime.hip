@@ -652,19 +652,25 @@ int ivqm_batch_autotune(ivqm_batch* b, const ivqm_pair* pairs, int32_t n, uint32
   if (!b) return set_error(IVQM_EINVAL, "NULL argument");
   std::vector<PairSpec> specs;
   RCCHK(batch_specs(pairs, n, specs));
-  double best_s = 1e30, best_j = 1e30;
+  double best_j = 1e30;
   int py_best = b->eng->tuning_py(), nb_best = b->eng->tuning_nb();
   // the keyed 4:2:0 search's tile rows (the other search variants run 64 x 16 tiles)
   const int ths[2] = {16, 32}, nbs[3] = {4, 8, 16};
-  for (int pi = 0; pi < 2; pi++) {
+  // A warm-up trial, then each tile height twice in alternating order, keeping each one's best: the
+  // first trials after an idle device run at lower clocks and would favour whichever height goes last.
+  double tth[2] = {1e30, 1e30};
+  bool search_tuned = true;
+  for (int it = 0; it < 5; it++) {
+    const int pi = it == 0 ? 0 : (it - 1) & 1;
     b->eng->set_tuning(ths[pi], nb_best);
     RCCHK(b->eng->enqueue(specs.data(), n, metrics & 0xFFu, 0, false));
     RCCHK(b->eng->collect(0, nullptr));
-    if (!b->eng->last_plan().need_search || !b->eng->last_plan().tiles420) break;
+    if (!b->eng->last_plan().need_search || !b->eng->last_plan().tiles420) { search_tuned = false; break; }
     double t = 0;
     RCCHK(b->eng->profile(1, 5, &t));
-    if (t < best_s) { best_s = t; py_best = ths[pi]; }
+    if (it > 0) tth[pi] = std::min(tth[pi], t);
   }
+  if (search_tuned) py_best = tth[1] < tth[0] ? ths[1] : ths[0];
   for (int ni = 0; ni < 3; ni++) {
     b->eng->set_tuning(py_best, nbs[ni]);
     RCCHK(b->eng->enqueue(specs.data(), n, metrics & 0xFFu, 0, false));
